@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Grid + degrid throughput of the w-projection workload on one MI355X.
+
+Workload ``wproj2048`` (the default): N=2048 grid, theta=0.1, lam=20480,
+2e6 visibilities with surface density ~ r^-1.5 out to 0.45 lam, w uniform in
++-1500 wavelengths, one w-kernel per 20-wavelength w-bin (151 bins, support
+16..32 growing with |w|, oversampling 8, far field 64 px), visibilities from
+20 point sources plus noise.  One step = w_cache_imaging's gridding of all
+visibilities into a cleared grid followed by w_cache_predict's degridding of
+that grid (per-visibility kernel of its w-bin, crocodile/synthesis.py:757-828),
+each a call through the C ABI with inputs resident in HBM.  Kernel generation
+is done once before timing (it is reported separately).
+
+Prints one JSON line (metric visibilities/s, higher is better) with the
+roofline of the dominant kernel measured by HIP events over the timed steps
+and the CPU baseline (the FP64 NumPy restatement of the same per-visibility
+loops, one core, on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='wproj2048', choices=['wproj2048', 'plain512'])
+    ap.add_argument('--cpu-sample', type=int, default=4000,
+                    help='visibilities in the CPU-baseline sample (0 disables it)')
+    ap.add_argument('--tile', type=int, default=0)
+    ap.add_argument('--chunk', type=int, default=4096)
+    ap.add_argument('--batch', type=int, default=256)
+    return ap.parse_args()
+
+
+def build_wproj(G, W, torch, dev, nv=None):
+    c = W.SPEC['wproj2048']
+    uvw, vis = W.wproj2048(nv=nv)
+    wb = numpy_round(uvw[:, 2] / c['wstep'])
+    lo, hi = int(wb.min()), int(wb.max())
+    ws = [c['wstep'] * b for b in range(lo, hi + 1)]
+    S = [W.wproj_support(w, c['wmax'], c['S_min'], c['S_max']) for w in ws]
+    scale = 1.0 / c['NpixFF'] ** 2
+    t0 = time.perf_counter()
+    bank_img = G.w_kernel_bank(c['theta'], ws, c['NpixFF'], S, c['Q'], True, scale)
+    bank_pred = G.w_kernel_bank(c['theta'], ws, c['NpixFF'], S, c['Q'], False, scale)
+    torch.cuda.synchronize()
+    kgen_ms = 1e3 * (time.perf_counter() - t0)
+    return dict(c=c, uvw=uvw, vis=vis, ws=ws, S=S, lo=lo, scale=scale,
+                bank_img=bank_img, bank_pred=bank_pred, kgen_ms=kgen_ms,
+                kernel_bytes=8 * int(bank_img.kern.numel()))
+
+
+def numpy_round(x):
+    import numpy
+    return numpy.round(x).astype(int)
+
+
+def cpu_baseline(G, W, torch, dev, n):
+    """FP64 NumPy restatement (oracle) of w_cache_imaging + w_cache_predict,
+    single core, on an ``n``-visibility sample of the workload; also the
+    device result on the same sample for the error figures."""
+    import numpy
+    from oracle import synthesis_ref as R
+
+    c = W.SPEC['wproj2048']
+    uvw, vis = W.wproj2048(nv=n, seed=c['seed'] + 77)
+    wb = numpy.round(uvw[:, 2] / c['wstep']).astype(int)
+    tables_img, tables_pred = {}, {}
+    for b in numpy.unique(wb):
+        w = c['wstep'] * b
+        S = W.wproj_support(w, c['wmax'], c['S_min'], c['S_max'])
+        tables_img[w] = R.w_kernel(c['theta'], w, c['NpixFF'], S, c['Q']) / c['NpixFF'] ** 2
+        tables_pred[w] = R.w_kernel(c['theta'], w, c['NpixFF'], S, c['Q'],
+                                    invert=False) / c['NpixFF'] ** 2
+    t0 = time.perf_counter()
+    g64 = R.w_cache_imaging(c['theta'], c['lam'], uvw, None, vis, wstep=c['wstep'],
+                            kernel_cache=lambda th, w: tables_img[w])
+    t1 = time.perf_counter()
+    v64 = R.w_cache_predict(c['theta'], c['lam'], uvw, None, g64, wstep=c['wstep'],
+                            kernel_cache=lambda th, w: tables_pred[w])
+    t2 = time.perf_counter()
+    # device result on the same sample (kernels generated on the device)
+    kw = dict(NpixFF=c['NpixFF'], Qpx=c['Q'])
+    g32 = G.w_cache_imaging(c['theta'], c['lam'], uvw, None, vis, wstep=c['wstep'],
+                            kernel_fn=lambda th, w: G.w_kernel(
+                                th, w, kw['NpixFF'],
+                                W.wproj_support(w, c['wmax'], c['S_min'], c['S_max']),
+                                kw['Qpx']) / c['NpixFF'] ** 2)
+    v32 = G.w_cache_predict(c['theta'], c['lam'], uvw, None, g64, wstep=c['wstep'],
+                            kernel_fn=lambda th, w: G.w_kernel(
+                                th, w, kw['NpixFF'],
+                                W.wproj_support(w, c['wmax'], c['S_min'], c['S_max']),
+                                kw['Qpx'], invert=False) / c['NpixFF'] ** 2)
+
+    def err(x, ref):
+        d = x - ref
+        return (float(numpy.linalg.norm(d) / numpy.linalg.norm(ref)),
+                float(numpy.abs(d).max() / numpy.abs(ref).max()))
+
+    ge, ve = err(g32, g64), err(v32, v64)
+    base = {
+        'value': n / (t2 - t0), 'unit': 'visibilities/s', 'cores': 1, 'kind': 'port',
+        'sample': '%d visibilities of wproj2048 (seed %d), FP64 NumPy per-visibility '
+                  'w_cache_imaging + w_cache_predict loops (oracle/synthesis_ref.py), '
+                  'kernels precomputed; grid %.2fs + degrid %.2fs'
+                  % (n, c['seed'] + 77, t1 - t0, t2 - t1),
+        'host_cpus': os.cpu_count(),
+    }
+    parity = {'sample_nv': n, 'grid_rms_rel': ge[0], 'grid_max_rel': ge[1],
+              'vis_rms_rel': ve[0], 'vis_max_rel': ve[1]}
+    return base, parity
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU path '
+              '(--gpus %d requested); run with --gpus 1' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from crocodile_amd import _lib, workloads as W, synthesis as G
+
+    dev = _lib.device()
+    _lib.set_tuning(args.tile, args.chunk, args.batch)
+    print('[bench] building workload %s' % args.workload, file=sys.stderr, flush=True)
+    if args.workload == 'wproj2048':
+        wl = build_wproj(G, W, torch, dev)
+        c = wl['c']
+        N, lam = c['N'], c['lam']
+        bank_img, bank_pred = wl['bank_img'], wl['bank_pred']
+        mode, wstep, wbin0 = _lib.PLANE_WBIN, c['wstep'], wl['lo']
+        config = {'workload': 'wproj2048', 'N': N, 'theta': c['theta'], 'lam': lam,
+                  'nv': c['nv'], 'w_planes': len(wl['ws']), 'wstep': wstep,
+                  'support': [min(wl['S']), max(wl['S'])], 'oversampling': c['Q'],
+                  'npix_ff': c['NpixFF'], 'kernel_gen_ms': round(wl['kgen_ms'], 1)}
+        uvw, vis = wl['uvw'], wl['vis']
+        kernel_bytes = wl['kernel_bytes']
+    else:
+        c = W.SPEC['plain512']
+        uvw, vis = W.plain512()
+        N, lam = c['N'], c['lam']
+        aaf = G.anti_aliasing_function(N, 0, c['S'])
+        kv = G.kernel_oversample(aaf, c['Q'], c['S'])
+        kv = kv / kv[0].sum()
+        bank_img = bank_pred = G.KernelBank.from_tables([kv], separable=True)
+        mode, wstep, wbin0 = _lib.PLANE_SINGLE, 1.0, 0
+        config = {'workload': 'plain512', 'N': N, 'nv': c['nv'], 'support': c['S'],
+                  'oversampling': c['Q'], 'kernel': 'separable PSWF'}
+        kernel_bytes = 8 * int(bank_img.kern.numel())
+    nv = uvw.shape[0]
+    uvw_d = torch.as_tensor(uvw, device=dev).contiguous()
+    vis_d = torch.as_tensor(vis, device=dev).to(torch.complex64).contiguous()
+    guv = torch.zeros((N, N), dtype=torch.complex64, device=dev)
+    out = torch.empty(nv, dtype=torch.complex64, device=dev)
+
+    def step():
+        guv.zero_()
+        G.grid_device(bank_img, guv, uvw_d, lam, vis_d, mode, None, wstep, wbin0)
+        g_ms, _ = _lib.last_kernel_ms()
+        G.degrid_device(bank_pred, guv, uvw_d, lam, out, mode, None, wstep, wbin0)
+        _, d_ms = _lib.last_kernel_ms()
+        return g_ms, d_ms
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print('[bench] timing %d steps' % args.steps, file=sys.stderr, flush=True)
+    gms, dms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g, d = step()
+        gms.append(g)
+        dms.append(d)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = 1e3 * (t1 - t0) / args.steps
+    value = nv * args.steps / (t1 - t0)
+
+    g_avg = sum(gms) / len(gms)
+    d_avg = sum(dms) / len(dms)
+    # Algorithmic bytes per launch: uvw (24 B) + visibility (8 B) per
+    # visibility, the grid written (or read) once, the kernel bank read once.
+    alg_bytes = nv * (24 + 8) + N * N * 8 + kernel_bytes
+    name, t_ms = ('k_grid_tiles', g_avg) if g_avg >= d_avg else ('k_degrid_tiles', d_avg)
+    achieved = alg_bytes / (t_ms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                       '%s_%s_pmc.json' % (name, config['workload']))
+    if os.path.exists(pmc):
+        traffic = json.load(open(pmc)).get('hbm_bytes_per_launch')
+    roofline = {'bound': 'hbm', 'kernel': name, 'achieved': round(achieved, 2),
+                'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 5),
+                'traffic': traffic, 'algorithmic_bytes': alg_bytes,
+                'kernel_ms': {'k_grid_tiles': round(g_avg, 4), 'k_degrid_tiles': round(d_avg, 4)}}
+
+    cpu, parity = None, None
+    if args.cpu_sample > 0 and config['workload'] == 'wproj2048':
+        print('[bench] cpu baseline on %d visibilities' % args.cpu_sample, file=sys.stderr,
+              flush=True)
+        cpu, parity = cpu_baseline(G, W, torch, dev, args.cpu_sample)
+
+    line = {
+        'metric': 'visibilities/s grid+degrid', 'value': round(value, 1),
+        'unit': 'visibilities/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(ms_per_step, 4), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'c64',
+        'data': 'synthetic', 'config': config, 'roofline': roofline,
+        'cpu_baseline': cpu, 'parity_vs_fp64_oracle': parity,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

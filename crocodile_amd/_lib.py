@@ -1,0 +1,124 @@
+"""ctypes binding of libcrocgpu.so (the C ABI declared in include/crocgpu.h).
+
+The library is required: importing the GPU entry points without a built
+library, or calling them without a ROCm device, raises immediately.  There is
+no CPU fallback.
+"""
+
+import ctypes
+import os
+import threading
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libcrocgpu.so')
+
+OK = 0
+EINVAL, EDOMAIN, EBOUNDS, EHIP, ENOMEM, EPLANE = -1, -2, -3, -4, -5, -6
+PLANE_SINGLE, PLANE_INDEX, PLANE_WBIN = 0, 1, 2
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+
+# name -> argtypes, in the order of include/crocgpu.h
+SIGNATURES = {
+    'croc_version': [],
+    'croc_last_error': [],
+    'croc_ctx_create': [_I, ctypes.POINTER(_P)],
+    'croc_ctx_destroy': [_P],
+    'croc_ctx_set_stream': [_P, _P],
+    'croc_ctx_set_tuning': [_P, _I, _I, _I],
+    'croc_last_kernel_ms': [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    'croc_frac_coords': [_P, _P, _I64, _I64, _D, _I, _I, _I, _P, _P, _P, _P],
+    'croc_convgrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _P, _I, _I, _I, _P, _P,
+                      _I, _I, _P, _D, _I],
+    'croc_convdegrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _P, _I, _I, _I, _P, _P,
+                        _I, _I, _P, _D, _I],
+    'croc_grid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
+    'croc_degrid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
+    'croc_w_kernels': [_P, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P, _I],
+    'croc_kernel_oversample': [_P, _P, _I, _I, _I, _I, _P],
+    'croc_make_hermitian': [_P, _P, _I, _I],
+}
+
+_lib = None
+_lock = threading.Lock()
+_ctx = {}
+
+
+class CrocError(RuntimeError):
+    """Failure reported by libcrocgpu that has no reference exception type."""
+
+
+def load():
+    """Load libcrocgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    'libcrocgpu.so is not built (%s); run python -m crocodile_amd.build'
+                    % LIB_PATH)
+            L = ctypes.CDLL(LIB_PATH)
+            for name, args in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.argtypes = args
+                fn.restype = ctypes.c_char_p if name == 'croc_last_error' else ctypes.c_int
+            _lib = L
+    return _lib
+
+
+def check(rc):
+    """Map a libcrocgpu status to the exception the reference raises."""
+    if rc == OK:
+        return
+    msg = load().croc_last_error().decode(errors='replace')
+    if rc == EDOMAIN:
+        raise AssertionError(msg)
+    if rc in (EBOUNDS, EPLANE):
+        raise ValueError(msg)
+    if rc == EINVAL:
+        raise ValueError(msg)
+    if rc == ENOMEM:
+        raise MemoryError(msg)
+    raise CrocError('libcrocgpu error %d: %s' % (rc, msg))
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError('crocodile_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+def ctx():
+    """Per-device library context, bound to torch's current stream."""
+    dev = device()
+    L = load()
+    c = _ctx.get(dev.index)
+    if c is None:
+        h = _P()
+        check(L.croc_ctx_create(dev.index, ctypes.byref(h)))
+        c = h
+        _ctx[dev.index] = c
+    check(L.croc_ctx_set_stream(c, _P(torch.cuda.current_stream(dev).cuda_stream)))
+    return c
+
+
+def last_kernel_ms():
+    a, b = ctypes.c_float(), ctypes.c_float()
+    check(load().croc_last_kernel_ms(ctx(), ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def set_tuning(tile=0, chunk=4096, batch=256):
+    check(load().croc_ctx_set_tuning(ctx(), tile, chunk, batch))
+
+
+def ptr(t):
+    """Device address of a tensor (None stays NULL)."""
+    return None if t is None else _P(t.data_ptr())

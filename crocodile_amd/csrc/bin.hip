@@ -1,0 +1,271 @@
+// Tile binning of visibilities (shared by gridding and degridding).
+//
+// Each visibility is located in FP64 exactly as crocodile's frac_coords does
+// (synthesis.py:419-449), its kernel plane is selected (make_gcf :451-461,
+// w-bin of w_cache_imaging :793), and it is assigned to the grid tile that
+// holds the first cell of its S x S window.  A counting sort then lays the
+// visibilities out tile by tile and cuts every tile into work items of at
+// most `chunk` visibilities, which the persistent tile kernels pull from a
+// queue.  Block-local LDS histograms keep global atomics to one per
+// (block, tile) pair, which matters for the strongly centre-peaked uv
+// densities of real arrays.
+#include "internal.h"
+
+namespace {
+
+constexpr int BIN_NT = 1024;
+constexpr int BIN_PER = 4;
+constexpr int BIN_VPB = BIN_NT * BIN_PER;
+constexpr int HIST_MAX = 16384;
+constexpr uint32_t INVALID = 0xffffffffu;
+
+__device__ __forceinline__ uint4 locate(const BinArgs &a, int T, int ntx,
+                                        int64_t i, uint32_t *st) {
+  const double *pu = a.uvw + i * a.stride;
+  const double px = grid_fraction(pu[0], a.lam);
+  const double py = grid_fraction(pu[1], a.lam);
+  int ix, xf, iy, yf;
+  const bool okx = frac_coord_dev(a.N, a.Q, px, &ix, &xf);
+  const bool oky = frac_coord_dev(a.N, a.Q, py, &iy, &yf);
+  if (!(okx && oky)) {
+    *st |= ST_DOMAIN;
+    return make_uint4(0, 0, 0, INVALID);
+  }
+  int64_t plane = 0;
+  if (a.plane_mode == CROC_PLANE_INDEX) {
+    plane = a.vplane[i];
+  } else if (a.plane_mode == CROC_PLANE_WBIN) {
+    plane = wbin_dev(pu[2], a.wstep) - a.wbin0;
+  }
+  if (plane < 0 || plane >= a.nplanes) {
+    *st |= ST_PLANE;
+    return make_uint4(0, 0, 0, INVALID);
+  }
+  const int S = a.plane_s[plane];
+  if (S < 1 || S > a.max_support) {  // LDS regions are sized for max_support
+    *st |= ST_SUPPORT;
+    return make_uint4(0, 0, 0, INVALID);
+  }
+  const int x0 = ix - S / 2, y0 = iy - S / 2;
+  if (x0 < 0 || y0 < 0 || x0 + S > a.N || y0 + S > a.N) {
+    *st |= ST_BOUNDS;
+    return make_uint4(0, 0, 0, INVALID);
+  }
+  const uint32_t tile = (uint32_t)((y0 / T) * ntx + x0 / T);
+  return make_uint4((uint32_t)x0 | ((uint32_t)y0 << 16),
+                    (uint32_t)xf | ((uint32_t)yf << 16), (uint32_t)plane, tile);
+}
+
+__global__ __launch_bounds__(BIN_NT) void k_bin_count(
+    BinArgs a, int T, int ntx, int ntiles, uint4 *__restrict__ loc,
+    uint32_t *__restrict__ count, uint32_t *__restrict__ status) {
+  __shared__ uint32_t hist[HIST_MAX];
+  const bool lds = ntiles <= HIST_MAX;
+  const int tid = threadIdx.x;
+  if (lds)
+    for (int t = tid; t < ntiles; t += BIN_NT) hist[t] = 0;
+  __syncthreads();
+  uint32_t st = 0;
+  const int64_t base = (int64_t)blockIdx.x * BIN_VPB;
+#pragma unroll
+  for (int k = 0; k < BIN_PER; k++) {
+    const int64_t i = base + k * BIN_NT + tid;
+    if (i < a.nv) {
+      const uint4 L = locate(a, T, ntx, i, &st);
+      loc[i] = L;
+      if (L.w != INVALID) {
+        if (lds)
+          atomicAdd(&hist[L.w], 1u);
+        else
+          atomicAdd(&count[L.w], 1u);
+      }
+    }
+  }
+  if (st) atomicOr(status, st);
+  __syncthreads();
+  if (lds)
+    for (int t = tid; t < ntiles; t += BIN_NT)
+      if (hist[t]) atomicAdd(&count[t], hist[t]);
+}
+
+// One block: exclusive scans of visibility counts and of work-item counts,
+// then the item list.  An error status empties the item list so that no
+// output is modified.
+__global__ __launch_bounds__(BIN_NT) void k_bin_scan(
+    int ntiles, int chunk, const uint32_t *__restrict__ count,
+    uint32_t *__restrict__ offset, uint32_t *__restrict__ cursor,
+    int4 *__restrict__ items, uint32_t *__restrict__ nitems,
+    const uint32_t *__restrict__ status, uint32_t *__restrict__ qheads) {
+  __shared__ uint32_t sc[BIN_NT], si[BIN_NT];
+  const int tid = threadIdx.x;
+  const int per = (ntiles + BIN_NT - 1) / BIN_NT;
+  const int t0 = min(ntiles, tid * per), t1 = min(ntiles, t0 + per);
+  uint32_t cs = 0, is = 0;
+  for (int t = t0; t < t1; t++) {
+    const uint32_t c = count[t];
+    cs += c;
+    is += (c + chunk - 1) / chunk;
+  }
+  sc[tid] = cs;
+  si[tid] = is;
+  __syncthreads();
+  for (int off = 1; off < BIN_NT; off <<= 1) {
+    uint32_t a = 0, b = 0;
+    if (tid >= off) {
+      a = sc[tid - off];
+      b = si[tid - off];
+    }
+    __syncthreads();
+    sc[tid] += a;
+    si[tid] += b;
+    __syncthreads();
+  }
+  uint32_t co = sc[tid] - cs, io = si[tid] - is;
+  for (int t = t0; t < t1; t++) {
+    const uint32_t c = count[t];
+    offset[t] = co;
+    cursor[t] = co;
+    for (uint32_t s = 0; s < c; s += chunk)
+      items[io++] = make_int4(t, (int)(co + s), (int)(co + min(c, s + chunk)), 0);
+    co += c;
+  }
+  if (tid == BIN_NT - 1) {
+    offset[ntiles] = co;
+    *nitems = (*status != 0) ? 0u : io;
+    for (int q = 0; q < 8; q++) qheads[q] = 0;
+  }
+}
+
+__global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
+    int64_t nv, int ntiles, const uint4 *__restrict__ loc,
+    uint32_t *__restrict__ cursor, uint4 *__restrict__ rec,
+    const float2 *__restrict__ vis, float2 *__restrict__ svis) {
+  __shared__ uint32_t hist[HIST_MAX];
+  const int tid = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * BIN_VPB;
+  uint4 L[BIN_PER];
+  uint32_t rank[BIN_PER];
+  if (ntiles <= HIST_MAX) {
+    for (int t = tid; t < ntiles; t += BIN_NT) hist[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BIN_PER; k++) {
+      const int64_t i = base + k * BIN_NT + tid;
+      L[k] = (i < nv) ? loc[i] : make_uint4(0, 0, 0, INVALID);
+      rank[k] = (L[k].w != INVALID) ? atomicAdd(&hist[L[k].w], 1u) : 0u;
+    }
+    __syncthreads();
+    for (int t = tid; t < ntiles; t += BIN_NT)
+      if (hist[t]) hist[t] = atomicAdd(&cursor[t], hist[t]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BIN_PER; k++) {
+      if (L[k].w == INVALID) continue;
+      const int64_t i = base + k * BIN_NT + tid;
+      const uint32_t pos = hist[L[k].w] + rank[k];
+      rec[pos] = make_uint4(L[k].x, L[k].y, L[k].z, (uint32_t)i);
+      if (vis) svis[pos] = vis[i];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < BIN_PER; k++) {
+      const int64_t i = base + k * BIN_NT + tid;
+      if (i >= nv) continue;
+      const uint4 l = loc[i];
+      if (l.w == INVALID) continue;
+      const uint32_t pos = atomicAdd(&cursor[l.w], 1u);
+      rec[pos] = make_uint4(l.x, l.y, l.z, (uint32_t)i);
+      if (vis) svis[pos] = vis[i];
+    }
+  }
+}
+
+}  // namespace
+
+int choose_tile(croc_ctx *ctx, int max_support) {
+  if (ctx->tile) return ctx->tile;
+  const int s16 = (max_support + 15) / 16 * 16;
+  return TILE_R + 1 - s16;  // keeps T + S - 1 <= TILE_R
+}
+
+int bin_visibilities(croc_ctx *ctx, const BinArgs &a, int max_support,
+                     BinPlan *plan) {
+  CROC_REQUIRE(max_support >= 1 && max_support <= 64, CROC_EINVAL,
+               "kernel support %d outside [1, 64]", max_support);
+  CROC_REQUIRE(a.N >= 1 && a.N <= 65535, CROC_EINVAL, "grid size %d outside "
+               "[1, 65535]", a.N);
+  CROC_REQUIRE(a.Q >= 1 && a.Q <= 65535, CROC_EINVAL, "oversampling %d outside"
+               " [1, 65535]", a.Q);
+  CROC_REQUIRE(a.nv >= 0 && a.nv < (int64_t)INT32_MAX, CROC_EINVAL,
+               "visibility count %lld outside [0, 2^31)", (long long)a.nv);
+  CROC_REQUIRE(a.nplanes >= 1, CROC_EINVAL, "empty kernel bank");
+  CROC_REQUIRE(a.max_support == max_support, CROC_EINVAL, "support mismatch");
+  const int T = choose_tile(ctx, max_support);
+  CROC_REQUIRE(T + max_support - 1 <= TILE_R, CROC_EINVAL,
+               "tile %d too large for support %d", T, max_support);
+  BinPlan &p = *plan;
+  p.T = T;
+  p.R = TILE_R;
+  p.ntx = (a.N + T - 1) / T;
+  p.nty = p.ntx;
+  p.ntiles = p.ntx * p.nty;
+  p.chunk = ctx->chunk;
+  p.max_items = a.nv / p.chunk + p.ntiles + 1;
+
+  p.loc = (uint4 *)ws_get(ctx, WS_LOC, sizeof(uint4) * a.nv);
+  p.rec = (uint4 *)ws_get(ctx, WS_REC, sizeof(uint4) * a.nv);
+  if (a.vis) p.svis = (float2 *)ws_get(ctx, WS_SVIS, sizeof(float2) * a.nv);
+  uint32_t *tiles = (uint32_t *)ws_get(ctx, WS_TILES,
+                                       sizeof(uint32_t) * (3 * p.ntiles + 1));
+  p.items = (int4 *)ws_get(ctx, WS_ITEMS, sizeof(int4) * p.max_items);
+  uint32_t *small = (uint32_t *)ws_get(ctx, WS_SMALL, 256);
+  if (!p.loc || !p.rec || (a.vis && !p.svis) || !tiles || !p.items || !small)
+    return CROC_ENOMEM;
+  p.count = tiles;
+  p.offset = tiles + p.ntiles;
+  p.cursor = tiles + 2 * p.ntiles + 1;
+  p.status = small;
+  p.nitems = small + 1;
+  p.qhead = small + 8;  // 8 queue heads, one per XCD
+
+  hipStream_t s = ctx->stream;
+  int rc = zero_u32_async(ctx, p.count, p.ntiles);
+  if (rc) return rc;
+  rc = zero_u32_async(ctx, small, 16);
+  if (rc) return rc;
+  const int nblk = (int)((a.nv + BIN_VPB - 1) / BIN_VPB);
+  if (nblk > 0)
+    k_bin_count<<<nblk, BIN_NT, 0, s>>>(a, T, p.ntx, p.ntiles, p.loc, p.count,
+                                        p.status);
+  k_bin_scan<<<1, BIN_NT, 0, s>>>(p.ntiles, p.chunk, p.count, p.offset,
+                                  p.cursor, p.items, p.nitems, p.status,
+                                  p.qhead);
+  if (nblk > 0)
+    k_bin_scatter<<<nblk, BIN_NT, 0, s>>>(a.nv, p.ntiles, p.loc, p.cursor,
+                                          p.rec, a.vis, p.svis);
+  CROC_HIP(hipGetLastError());
+  return CROC_OK;
+}
+
+namespace {
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+}  // namespace
+
+// A fill kernel rather than hipMemsetAsync, so the call sequence stays valid
+// under stream capture.
+int zero_u32_async(croc_ctx *ctx, uint32_t *p, int64_t n) {
+  if (n <= 0) return CROC_OK;
+  k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(p, n, 0u);
+  CROC_HIP(hipGetLastError());
+  return CROC_OK;
+}
+
+int read_status(croc_ctx *ctx, const BinPlan &plan) {
+  CROC_HIP(hipMemcpyAsync(ctx->host_status, plan.status, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+  CROC_HIP(hipStreamSynchronize(ctx->stream));
+  return croc_status_to_code(*ctx->host_status);
+}

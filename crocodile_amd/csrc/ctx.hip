@@ -1,0 +1,142 @@
+// Context, workspace and error plumbing of libcrocgpu.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "internal.h"
+
+static thread_local char g_err[512] = "";
+
+void croc_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char *croc_last_error(void) { return g_err; }
+
+extern "C" int croc_version(void) { return 1; }
+
+int croc_status_to_code(uint32_t st) {
+  if (st & ST_DOMAIN) {
+    croc_set_error("grid coordinate outside [-0.5, 0.5)");
+    return CROC_EDOMAIN;
+  }
+  if (st & ST_PLANE) {
+    croc_set_error("w-bin / plane index outside the kernel bank");
+    return CROC_EPLANE;
+  }
+  if (st & ST_SUPPORT) {
+    croc_set_error("a kernel plane's support exceeds max_support");
+    return CROC_EINVAL;
+  }
+  if (st & ST_BOUNDS) {
+    croc_set_error("kernel window extends beyond the grid");
+    return CROC_EBOUNDS;
+  }
+  return CROC_OK;
+}
+
+int ctx_enter(croc_ctx *ctx) {
+  CROC_REQUIRE(ctx != nullptr, CROC_EINVAL, "null context");
+  CROC_HIP(hipSetDevice(ctx->device));
+  return CROC_OK;
+}
+
+void *ws_get(croc_ctx *ctx, WsSlot slot, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (ctx->ws_size[slot] >= bytes) return ctx->ws[slot];
+  if (ctx->ws[slot]) {
+    // The stream may still use the old buffer.
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      croc_set_error("stream synchronisation failed while growing workspace");
+      return nullptr;
+    }
+    (void)hipFree(ctx->ws[slot]);
+    ctx->ws[slot] = nullptr;
+    ctx->ws_size[slot] = 0;
+  }
+  size_t want = bytes + bytes / 4;  // headroom against repeated growth
+  void *p = nullptr;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    croc_set_error("hipMalloc of %zu bytes failed", want);
+    return nullptr;
+  }
+  ctx->ws[slot] = p;
+  ctx->ws_size[slot] = want;
+  return p;
+}
+
+extern "C" int croc_ctx_create(int device, croc_ctx **out) {
+  CROC_REQUIRE(out != nullptr, CROC_EINVAL, "null output pointer");
+  int ndev = 0;
+  CROC_HIP(hipGetDeviceCount(&ndev));
+  CROC_REQUIRE(device >= 0 && device < ndev, CROC_EINVAL,
+               "device %d not present (%d devices)", device, ndev);
+  CROC_HIP(hipSetDevice(device));
+  croc_ctx *ctx = new croc_ctx();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    ctx->num_cu = prop.multiProcessorCount;
+  if (hipHostMalloc((void **)&ctx->host_status, 64, hipHostMallocDefault) !=
+      hipSuccess) {
+    delete ctx;
+    croc_set_error("hipHostMalloc failed");
+    return CROC_ENOMEM;
+  }
+  for (int i = 0; i < 4; i++) {
+    if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
+      croc_set_error("hipEventCreate failed");
+      return CROC_EHIP;
+    }
+  }
+  *out = ctx;
+  return CROC_OK;
+}
+
+extern "C" int croc_ctx_destroy(croc_ctx *ctx) {
+  if (!ctx) return CROC_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (int i = 0; i < WS_COUNT; i++)
+    if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
+  for (int i = 0; i < 4; i++)
+    if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  if (ctx->host_status) (void)hipHostFree(ctx->host_status);
+  delete ctx;
+  return CROC_OK;
+}
+
+extern "C" int croc_ctx_set_stream(croc_ctx *ctx, void *stream) {
+  CROC_REQUIRE(ctx != nullptr, CROC_EINVAL, "null context");
+  ctx->stream = (hipStream_t)stream;
+  return CROC_OK;
+}
+
+extern "C" int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk,
+                                   int batch) {
+  CROC_REQUIRE(ctx != nullptr, CROC_EINVAL, "null context");
+  CROC_REQUIRE(tile == 0 || (tile >= 16 && tile <= 80 && tile % 16 == 0),
+               CROC_EINVAL, "tile must be 0 or a multiple of 16 in [16, 80]");
+  CROC_REQUIRE(chunk >= 64, CROC_EINVAL, "chunk must be >= 64");
+  CROC_REQUIRE(batch >= 8, CROC_EINVAL, "batch must be >= 8");
+  ctx->tile = tile;
+  ctx->chunk = chunk;
+  ctx->batch = batch;
+  return CROC_OK;
+}
+
+extern "C" int croc_last_kernel_ms(croc_ctx *ctx, float *grid_ms,
+                                   float *degrid_ms) {
+  CROC_REQUIRE(ctx != nullptr, CROC_EINVAL, "null context");
+  if (grid_ms) *grid_ms = ctx->grid_ms;
+  if (degrid_ms) *degrid_ms = ctx->degrid_ms;
+  return CROC_OK;
+}
+
+int persistent_blocks(croc_ctx *ctx, int per_cu) {
+  // A multiple of 8 so that every XCD receives the same number of blocks.
+  int g = ctx->num_cu * per_cu;
+  return (g + 7) / 8 * 8;
+}

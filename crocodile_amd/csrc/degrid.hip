@@ -1,0 +1,136 @@
+// Convolutional degridding: crocodile/synthesis.py:484-503 (convdegrid) on
+// CDNA4.
+//
+// Same tile work items as gridding.  A workgroup stages the 95 x 95 cell
+// grid region of its tile in LDS once (coalesced row reads from HBM), then
+// each wavefront takes one visibility at a time: lanes walk the S*S taps,
+// gather grid cells from LDS and kernel taps from the (cache-resident)
+// kernel bank, accumulate complex products in FP32 and the wavefront
+// reduces the 64 partial sums with cross-lane shuffles.
+#include "internal.h"
+
+namespace {
+
+constexpr int DEGRID_NT = 512;
+
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+template <int NT, bool SEPARABLE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_degrid_tiles(
+    const float2 *__restrict__ grid, int N, int T, int ntx,
+    const uint4 *__restrict__ rec, float2 *__restrict__ vis_out,
+    const float2 *__restrict__ kern, int Q,
+    const int64_t *__restrict__ plane_off, const int32_t *__restrict__ plane_s,
+    const int4 *__restrict__ items, const uint32_t *__restrict__ nitems,
+    uint32_t *__restrict__ qhead) {
+  constexpr int R = TILE_R, CELLS = R * R;
+  constexpr int NW = NT / 64;
+  __shared__ float2 sg[CELLS];
+  __shared__ int s_item;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nit = (int)*nitems;
+
+  for (;;) {
+    if (tid == 0) s_item = (int)atomicAdd(qhead, 1u);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= nit) break;
+    const int4 item = items[it];
+    const int tx0 = (item.x % ntx) * T, ty0 = (item.x / ntx) * T;
+    for (int c = tid; c < CELLS; c += NT) {
+      const int cy = c / R, cx = c - cy * R;
+      const int gy = ty0 + cy, gx = tx0 + cx;
+      sg[c] = (gy < N && gx < N) ? grid[(int64_t)gy * N + gx] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+
+    for (int j = item.y + wave; j < item.z; j += NW) {
+      const uint4 r = rec[j];
+      const int S = plane_s[r.z];
+      const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+      const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
+      const int SS = S * S;
+      const float invS = 1.0f / (float)S;
+      float ar = 0.f, ai = 0.f;
+      if (!SEPARABLE) {
+        const float2 *k = kern + plane_off[r.z] + (int64_t)(yf * Q + xf) * SS;
+        for (int t = lane; t < SS; t += 64) {
+          const int i = (int)(((float)t + 0.5f) * invS);
+          const int jj = t - i * S;
+          const float2 g = sg[base + i * R + jj];
+          const float2 kv = k[t];
+          ar += g.x * kv.x - g.y * kv.y;
+          ai += g.x * kv.y + g.y * kv.x;
+        }
+      } else {
+        const float2 *ky = kern + plane_off[r.z] + (int64_t)yf * S;
+        const float2 *kx = kern + plane_off[r.z] + (int64_t)xf * S;
+        for (int t = lane; t < SS; t += 64) {
+          const int i = (int)(((float)t + 0.5f) * invS);
+          const int jj = t - i * S;
+          const float2 g = sg[base + i * R + jj];
+          const float2 a = ky[i], b = kx[jj];
+          const float kr = a.x * b.x - a.y * b.y, ki = a.x * b.y + a.y * b.x;
+          ar += g.x * kr - g.y * ki;
+          ai += g.x * ki + g.y * kr;
+        }
+      }
+      ar = wave_sum(ar);
+      ai = wave_sum(ai);
+      if (lane == 0) vis_out[r.w] = make_float2(ar, ai);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+extern "C" int croc_convdegrid(croc_ctx *ctx, const void *grid, int N,
+                               const double *uvw, int64_t nv, int64_t stride,
+                               double lam, void *vis_out, const void *kern,
+                               int Q, int separable, int nplanes,
+                               const int64_t *plane_off,
+                               const int32_t *plane_s, int max_support,
+                               int plane_mode, const int32_t *vplane,
+                               double wstep, int wbin0) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  CROC_REQUIRE(grid && kern && plane_off && plane_s, CROC_EINVAL,
+               "null array argument");
+  CROC_REQUIRE(nv == 0 || (uvw && vis_out), CROC_EINVAL, "null uvw / vis_out");
+  CROC_REQUIRE(stride >= 2, CROC_EINVAL, "uvw stride must be >= 2");
+  CROC_REQUIRE(plane_mode != CROC_PLANE_INDEX || vplane, CROC_EINVAL,
+               "CROC_PLANE_INDEX needs vplane");
+  CROC_REQUIRE(plane_mode != CROC_PLANE_WBIN || (stride >= 3 && wstep > 0),
+               CROC_EINVAL, "CROC_PLANE_WBIN needs a w column and wstep > 0");
+  CROC_REQUIRE(lam > 0, CROC_EINVAL, "lam must be positive");
+  if (nv == 0) return CROC_OK;
+
+  BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_mode, vplane,
+            wstep, wbin0, nullptr, max_support};
+  BinPlan plan;
+  rc = bin_visibilities(ctx, a, max_support, &plan);
+  if (rc) return rc;
+  const int G = persistent_blocks(ctx, 2);
+  CROC_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+  if (separable)
+    k_degrid_tiles<DEGRID_NT, true><<<G, DEGRID_NT, 0, ctx->stream>>>(
+        (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
+        (const float2 *)kern, Q, plane_off, plane_s, plan.items, plan.nitems,
+        plan.qhead);
+  else
+    k_degrid_tiles<DEGRID_NT, false><<<G, DEGRID_NT, 0, ctx->stream>>>(
+        (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
+        (const float2 *)kern, Q, plane_off, plane_s, plan.items, plan.nitems,
+        plan.qhead);
+  CROC_HIP(hipGetLastError());
+  CROC_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+  rc = read_status(ctx, plan);
+  (void)hipEventElapsedTime(&ctx->degrid_ms, ctx->ev[2], ctx->ev[3]);
+  return rc;
+}

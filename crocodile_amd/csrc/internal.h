@@ -1,0 +1,151 @@
+// Internal declarations shared by the libcrocgpu translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/crocgpu.h"
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+void croc_set_error(const char *fmt, ...);
+
+#define CROC_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      croc_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                     __FILE__, __LINE__);                                     \
+      return CROC_EHIP;                                                       \
+    }                                                                         \
+  } while (0)
+
+#define CROC_REQUIRE(cond, code, ...)                                         \
+  do {                                                                        \
+    if (!(cond)) {                                                            \
+      croc_set_error(__VA_ARGS__);                                            \
+      return (code);                                                          \
+    }                                                                         \
+  } while (0)
+
+// Status bits raised by device kernels (OR-ed into a device word).
+enum : uint32_t {
+  ST_DOMAIN = 1u,  // coordinate outside [-0.5, 0.5)
+  ST_BOUNDS = 2u,  // kernel window outside the grid
+  ST_PLANE = 4u,   // plane index outside the bank
+  ST_SUPPORT = 8u, // a plane's support exceeds the declared maximum
+};
+
+int croc_status_to_code(uint32_t st);
+
+// ---------------------------------------------------------------------------
+// Context: stream, reusable device workspace, pinned status word, events.
+// ---------------------------------------------------------------------------
+enum WsSlot {
+  WS_LOC = 0,   // per-visibility located record (uint4)
+  WS_REC,       // tile-sorted record (uint4)
+  WS_SVIS,      // tile-sorted visibilities (float2)
+  WS_TILES,     // count / offset / cursor per tile
+  WS_ITEMS,     // work items (int4)
+  WS_SMALL,     // status, item count, queue heads
+  WS_FF,        // w-kernel far fields (double2)
+  WS_TMP,       // oversampling intermediate (double2)
+  WS_COUNT
+};
+
+struct croc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void *ws[WS_COUNT] = {};
+  size_t ws_size[WS_COUNT] = {};
+  uint32_t *host_status = nullptr;  // pinned
+  hipEvent_t ev[4] = {};
+  float grid_ms = 0.f, degrid_ms = 0.f;
+  int num_cu = 256;
+  int tile = 0;     // 0: automatic
+  int chunk = 4096; // visibilities per work item
+  int batch = 256;  // visibilities per compensated flush
+};
+
+// Grow-only workspace; returns nullptr on failure (error already set).
+void *ws_get(croc_ctx *ctx, WsSlot slot, size_t bytes);
+
+// Make the context's device current.
+int ctx_enter(croc_ctx *ctx);
+
+// ---------------------------------------------------------------------------
+// FP64 coordinate arithmetic.  Written with explicit round-to-nearest
+// intrinsics so that no FMA contraction changes a rounding: the results must
+// equal NumPy's bit for bit, ties included.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t floordiv64(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1;
+  return q;
+}
+
+// p = u / lam (IEEE division, or u itself when lam == 1).
+__device__ __forceinline__ double grid_fraction(double u, double lam) {
+  return lam == 1.0 ? u : __ddiv_rn(u, lam);
+}
+
+// synthesis.py:419-435: x = N//2 + p*N; fx = around(-x*Q) (half-even);
+// returns false when p is outside [-0.5, 0.5).
+__device__ __forceinline__ bool frac_coord_dev(int N, int Q, double p, int *ix,
+                                               int *xf) {
+  if (!(p >= -0.5 && p < 0.5)) return false;
+  double x = __dadd_rn((double)(N / 2), __dmul_rn(p, (double)N));
+  double fx = rint(__dmul_rn(-x, (double)Q));
+  int64_t f = (int64_t)fx;
+  int64_t q = floordiv64(f, Q);
+  *ix = (int)(-q);
+  *xf = (int)(f - q * Q);
+  return true;
+}
+
+// numpy.round(w / wstep) (half-even).
+__device__ __forceinline__ int64_t wbin_dev(double w, double wstep) {
+  return (int64_t)rint(__ddiv_rn(w, wstep));
+}
+
+// ---------------------------------------------------------------------------
+// Binning of visibilities by the grid tile holding their kernel window's
+// first cell (shared by gridding and degridding).
+// ---------------------------------------------------------------------------
+struct BinArgs {
+  int N;
+  const double *uvw;
+  int64_t nv, stride;
+  double lam;
+  int Q;
+  int nplanes;
+  const int32_t *plane_s;
+  int plane_mode;
+  const int32_t *vplane;
+  double wstep;
+  int wbin0;
+  const float2 *vis;  // non-null: gather visibilities into tile order
+  int max_support;    // every plane_s must lie in [1, max_support]
+};
+
+struct BinPlan {
+  int T = 64, R = 95, ntx = 0, nty = 0, ntiles = 0, chunk = 4096;
+  int64_t max_items = 0;
+  uint4 *loc = nullptr, *rec = nullptr;
+  float2 *svis = nullptr;
+  uint32_t *count = nullptr, *offset = nullptr, *cursor = nullptr;
+  int4 *items = nullptr;
+  uint32_t *status = nullptr, *nitems = nullptr, *qhead = nullptr;
+};
+
+// Region edge held in LDS by the tile kernels.
+constexpr int TILE_R = 95;
+
+int choose_tile(croc_ctx *ctx, int max_support);
+int bin_visibilities(croc_ctx *ctx, const BinArgs &a, int max_support,
+                     BinPlan *plan);
+int read_status(croc_ctx *ctx, const BinPlan &plan);
+int persistent_blocks(croc_ctx *ctx, int per_cu);
+int zero_u32_async(croc_ctx *ctx, uint32_t *p, int64_t n);

@@ -1,0 +1,872 @@
+"""GPU (MI355X / gfx950) drop-in for crocodile's gridding path.
+
+Same function names, positional order, defaults and exceptions as
+crocodile/synthesis.py for the convolutional gridding / degridding path:
+coordinate helpers, kernel generation (``kernel_oversample``, ``w_kernel``),
+``frac_coord(s)``, ``convgrid`` / ``convdegrid``, nearest-cell
+``grid`` / ``degrid`` and the imaging / predict drivers (imgfn / predfn
+protocol of synthesis.py:883-927).  Every function accepts NumPy arrays
+(copied to the device; ``convgrid`` still updates the caller's array in
+place) or torch tensors, and returns NumPy for NumPy input and torch device
+tensors for torch input.
+
+All arithmetic on the path runs in libcrocgpu (hand-written HIP):
+coordinates and w-kernel phases in FP64, gridding / degridding in complex
+FP32 with compensated accumulation.  Nothing here falls back to the CPU; a
+missing library or GPU raises.
+"""
+
+import math
+import numbers
+
+import numpy
+import scipy.special
+import torch
+
+from . import _lib
+
+# --------------------------------------------------------------------------
+# helpers: array plumbing between NumPy / torch and device buffers
+# --------------------------------------------------------------------------
+
+
+def _is_torch(x):
+    return isinstance(x, torch.Tensor)
+
+
+def _dev(x, dtype):
+    """Contiguous device tensor of ``dtype`` holding ``x``."""
+    dev = _lib.device()
+    if _is_torch(x):
+        return x.to(device=dev, dtype=dtype).contiguous()
+    arr = numpy.asarray(x)
+    if arr.dtype == object:
+        arr = arr.astype(numpy.complex128 if dtype.is_complex else numpy.float64)
+    return torch.as_tensor(numpy.ascontiguousarray(arr)).to(device=dev, dtype=dtype).contiguous()
+
+
+def _uvw(p):
+    """[nv, k>=2] float64 device tensor of coordinates."""
+    t = _dev(p, torch.float64)
+    if t.ndim != 2 or t.shape[1] < 2:
+        raise ValueError('coordinates must have shape [nv, >=2]')
+    return t
+
+
+def _out(t, like_torch, dtype=numpy.complex128):
+    """Return device tensor ``t`` as torch (torch callers) or NumPy."""
+    if like_torch:
+        return t
+    a = t.cpu().numpy()
+    return a.astype(dtype) if dtype is not None else a
+
+
+def _torch_in(*xs):
+    return any(_is_torch(x) for x in xs)
+
+
+# --------------------------------------------------------------------------
+# L0: coordinates, FFT, padding                      (synthesis.py:38-197)
+# --------------------------------------------------------------------------
+
+
+def coordinateBounds(N):
+    """Lowest and highest coordinate, step 1/N, zero at N//2.
+    synthesis.py:38-55."""
+    if N % 2 == 0:
+        return -0.5, 0.5 * (N - 2) / N
+    return -0.5 * (N - 1) / N, 0.5 * (N - 1) / N
+
+
+def coordinates(N):
+    """``(i - N//2)/N`` for i in [0, N).  synthesis.py:57-63."""
+    return (numpy.arange(N) - N // 2) / N
+
+
+def coordinates2(N):
+    """(cx, cy): cx varies along axis 1, cy along axis 0.
+    synthesis.py:66-80."""
+    c = coordinates(N)
+    return numpy.array([numpy.tile(c, (N, 1)), numpy.tile(c[:, None], (1, N))])
+
+
+def _fft2c(a, inverse):
+    was_torch = _is_torch(a)
+    t = _dev(a, torch.complex128) if not (was_torch and a.dtype == torch.complex64) \
+        else a.to(_lib.device()).contiguous()
+    dims = tuple(range(t.ndim))
+    if t.ndim not in (1, 2):
+        raise AssertionError('Unsupported shape for FFT!')
+    t = torch.fft.ifftshift(t, dim=dims)
+    t = torch.fft.ifftn(t, dim=dims) if inverse else torch.fft.fftn(t, dim=dims)
+    t = torch.fft.fftshift(t, dim=dims)
+    return t if was_torch else t.cpu().numpy()
+
+
+def fft(a):
+    """Centred forward FFT (unnormalised), on the device (rocFFT through
+    torch.fft).  synthesis.py:83-93."""
+    return _fft2c(a, False)
+
+
+def ifft(a):
+    """Centred inverse FFT (1/N^2), on the device.  synthesis.py:96-106."""
+    return _fft2c(a, True)
+
+
+def pad_mid(ff, N):
+    """Zero-pad around N//2.  synthesis.py:109-129."""
+    N0 = ff.shape[0]
+    if N == N0:
+        return ff
+    assert N > N0
+    lo, hi = N // 2 - N0 // 2, (N + 1) // 2 - (N0 + 1) // 2
+    if _is_torch(ff):
+        pads = [lo, hi] * ff.ndim
+        if ff.ndim == 2:
+            assert N0 == ff.shape[1]
+        return torch.nn.functional.pad(ff, pads)
+    if ff.ndim == 2:
+        assert N0 == ff.shape[1]
+        return numpy.pad(ff, [(lo, hi), (lo, hi)], mode='constant')
+    return numpy.pad(ff, [(lo, hi)], mode='constant')
+
+
+def extract_mid(a, N):
+    """Centre N (x N) section.  synthesis.py:131-158."""
+    assert N <= a.shape[0]
+    h, odd, c0 = N // 2, N % 2, a.shape[0] // 2
+    if a.ndim == 2:
+        assert N <= a.shape[1]
+        c1 = a.shape[1] // 2
+        return a[c0 - h:c0 + h + odd, c1 - h:c1 + h + odd]
+    if a.ndim == 1:
+        return a[c0 - h:c0 + h + odd]
+    raise AssertionError('Unsupported grid shape for extract_mid!')
+
+
+def extract_oversampled(a, Qpx, N):
+    """[Qpx,Qpx,N,N] (2D) / [Qpx,N] (1D) sub-sampled kernels.
+    synthesis.py:160-197."""
+    Na = a.shape[0]
+    start = Na // 2 - Qpx * (N // 2)
+    lib = torch if _is_torch(a) else numpy
+    if a.ndim == 1:
+        return lib.stack([a[start + f:start + f + Qpx * N:Qpx] for f in range(Qpx)])
+    if a.ndim == 2:
+        return lib.stack([lib.stack([a[start + fy:start + fy + Qpx * N:Qpx,
+                                       start + fx:start + fx + Qpx * N:Qpx]
+                                     for fx in range(Qpx)]) for fy in range(Qpx)])
+    raise AssertionError('Unsupported grid shape for extract_oversampled!')
+
+
+# --------------------------------------------------------------------------
+# L1: kernels                                         (synthesis.py:199-387)
+# --------------------------------------------------------------------------
+
+
+def anti_aliasing_function(shape, m, c):
+    """Prolate-spheroidal anti-aliasing function, evaluated once on the host
+    with scipy.special.pro_ang1 exactly as the reference does (there is no
+    device routine for PSWFs; the result only seeds kernel tables).
+    synthesis.py:199-218."""
+    if numpy.ndim(shape) == 0:
+        mult = 2 - 1 / shape / 4
+        return scipy.special.pro_ang1(m, m, c, mult * coordinates(shape))[0]
+    return numpy.outer(anti_aliasing_function(shape[0], m, c),
+                       anti_aliasing_function(shape[1], m, c))
+
+
+def kernel_transform(dl, dm):
+    """synthesis.py:220-248."""
+    dn = numpy.sqrt(1 - dl ** 2 - dm ** 2)
+    f = (dn - 1) / (dl ** 2 + dm ** 2) if (dl != 0 or dm != 0) else 0
+    return numpy.array([[dn - dm * dm * f, dl * dm * f],
+                        [dl * dm * f, dn - dl * dl * f]])
+
+
+def kernel_coordinates(N, theta, dl=0, dm=0, T=None):
+    """synthesis.py:250-270."""
+    l, m = coordinates2(N) * theta
+    if T is not None:
+        l, m = T[0, 0] * l + T[1, 0] * m, T[0, 1] * l + T[1, 1] * m
+    return l + dl, m + dm
+
+
+def w_kernel_function(l, m, w):
+    """exp(2 pi i w (1 - sqrt(1 - l^2 - m^2))).  synthesis.py:273-293."""
+    lib = torch if _is_torch(l) else numpy
+    r2 = l ** 2 + m ** 2
+    assert bool((r2 < 1.0).all()), 'Error in image coordinate system'
+    return lib.exp(2j * numpy.pi * w * (1 - lib.sqrt(1.0 - r2)))
+
+
+def kernel_recentre(cp, theta, w, dl, dm):
+    """synthesis.py:296-311."""
+    l, m = coordinates2(cp.shape[0]) * theta
+    ramp = numpy.exp(-2j * numpy.pi * w * (dl * l + dm * m))
+    if _is_torch(cp):
+        ramp = torch.as_tensor(ramp, device=cp.device)
+    return cp * ramp
+
+
+def visibility_recentre(uvw, dl, dm):
+    """synthesis.py:314-327."""
+    out = uvw.clone() if _is_torch(uvw) else numpy.array(uvw, dtype=float)
+    out[:, 0] = uvw[:, 0] - uvw[:, 2] * dl
+    out[:, 1] = uvw[:, 1] - uvw[:, 2] * dm
+    return out
+
+
+def kernel_oversample(ff, Qpx, s=None):
+    """Oversampled convolution kernels [Qpx,Qpx,s,s] (2D far field) or
+    [Qpx,s] (1D), computed on the device in FP64 as the zoom DFT equal to
+    pad_mid + fft + extract_oversampled.  synthesis.py:330-354."""
+    was_torch = _is_torch(ff)
+    N = ff.shape[0]
+    if s is None:
+        s = N
+    if ff.ndim == 2:
+        assert N == ff.shape[1]
+    if ff.ndim not in (1, 2):
+        raise AssertionError('Unsupported grid shape for extract_oversampled!')
+    assert N * Qpx >= N
+    if Qpx * N < s:
+        raise ValueError('support larger than the oversampled far field')
+    src = _dev(ff, torch.complex128)
+    shape = (Qpx, Qpx, s, s) if ff.ndim == 2 else (Qpx, s)
+    out = torch.empty(shape, dtype=torch.complex128, device=src.device)
+    _lib.check(_lib.load().croc_kernel_oversample(
+        _lib.ctx(), _lib.ptr(src), N, ff.ndim, Qpx, s, _lib.ptr(out)))
+    return out if was_torch else out.cpu().numpy()
+
+
+def _w_kernel_args(kwargs):
+    dl = float(kwargs.pop('dl', 0))
+    dm = float(kwargs.pop('dm', 0))
+    T = kwargs.pop('T', None)
+    if kwargs:
+        raise TypeError('kernel_coordinates() got unexpected keyword argument(s) %s'
+                        % ', '.join(sorted(kwargs)))
+    Tv = None
+    if T is not None:
+        Tv = numpy.ascontiguousarray(numpy.asarray(T, dtype=numpy.float64).reshape(4))
+    return dl, dm, Tv
+
+
+def w_kernel_bank(theta, ws, NpixFF, supports, Qpx, invert=True, scale=1.0,
+                  double=False, **kwargs):
+    """Generate the w-kernels of many planes in one device launch.
+
+    Returns a :class:`KernelBank` (complex64 unless ``double``) whose plane k
+    is ``scale * w_kernel(theta, ws[k], NpixFF, supports[k], Qpx, invert,
+    **kwargs)``.  Same arithmetic as :func:`w_kernel` (synthesis.py:357-377).
+    """
+    ws = numpy.ascontiguousarray(numpy.asarray(ws, dtype=numpy.float64).reshape(-1))
+    S = numpy.ascontiguousarray(numpy.broadcast_to(
+        numpy.asarray(supports, dtype=numpy.int32), ws.shape))
+    for s in S:
+        assert NpixFF > s or (NpixFF == s and Qpx == 1)
+    dl, dm, Tv = _w_kernel_args(dict(kwargs))
+    sizes = (Qpx * Qpx * S.astype(numpy.int64) ** 2)
+    off = numpy.concatenate([[0], numpy.cumsum(sizes)[:-1]]).astype(numpy.int64)
+    dtype = torch.complex128 if double else torch.complex64
+    out = torch.empty(int(sizes.sum()), dtype=dtype, device=_lib.device())
+    L = _lib.load()
+    _lib.check(L.croc_w_kernels(
+        _lib.ctx(), float(theta), ws.ctypes.data_as(_lib._P), len(ws), int(NpixFF),
+        S.ctypes.data_as(_lib._P), int(Qpx), int(bool(invert)), dl, dm,
+        None if Tv is None else Tv.ctypes.data_as(_lib._P),
+        off.ctypes.data_as(_lib._P), _lib.ptr(out), int(bool(double))))
+    if scale != 1.0:
+        out.mul_(scale)
+    return KernelBank(out, off, S, Qpx, separable=False)
+
+
+def w_kernel(theta, w, NpixFF, NpixKern, Qpx, invert=True, **kwargs):
+    """Oversampled w-kernel [Qpx,Qpx,NpixKern,NpixKern] (complex128,
+    unnormalised), generated on the device in FP64.  ``kwargs`` are
+    kernel_coordinates' ``dl``, ``dm``, ``T``.  synthesis.py:357-377."""
+    bank = w_kernel_bank(theta, [w], NpixFF, [NpixKern], Qpx, invert, double=True,
+                         **kwargs)
+    return bank.kern.reshape(Qpx, Qpx, NpixKern, NpixKern).cpu().numpy()
+
+
+def invert_kernel(a):
+    """Pseudo-inverse conj(a)/|a|^2 (unused by the reference, kept for the
+    API).  synthesis.py:380-386."""
+    return (a.conj() if _is_torch(a) else numpy.conj(a)) / (abs(a) ** 2)
+
+
+def w_conj_kernel_fn(kernel_fn):
+    """Fold k(-w) = conj(k(w)).  synthesis.py:682-699."""
+    def fn(theta, w, *args, **kw):
+        if w < 0:
+            k = kernel_fn(theta, -w, *args, **kw)
+            return k.conj() if _is_torch(k) else numpy.conj(k)
+        return kernel_fn(theta, w, *args, **kw)
+    return fn
+
+
+def aw_kernel_fn(a_kernel_fn, w_kernel_fn=w_kernel):
+    """AW kernels: the two antennas' A-kernels convolved (mode 'same') and
+    then convolved with every oversampled w sub-kernel, on the device in
+    FP64.  synthesis.py:701-734."""
+    from .awkernel import convolve_same, convolve_same_batched
+
+    def fn(theta, w, a1, a2, *src):
+        akern = convolve_same(a_kernel_fn(theta, a1, *src), a_kernel_fn(theta, a2, *src))
+        return convolve_same_batched(akern, w_kernel_fn(theta, w))
+    return fn
+
+
+# --------------------------------------------------------------------------
+# kernel banks (the device form of make_gcf's kernel tables)
+# --------------------------------------------------------------------------
+
+
+class KernelBank:
+    """Kernel tables on the device: complex tables concatenated, plane k at
+    element ``plane_off[k]`` with support ``plane_s[k]``; layout
+    [Q][Q][S][S] (2D) or [Q][S] (separable)."""
+
+    def __init__(self, kern, plane_off, plane_s, Q, separable):
+        dev = _lib.device()
+        self.kern = kern
+        self.plane_off_h = numpy.asarray(plane_off, dtype=numpy.int64)
+        self.plane_s_h = numpy.asarray(plane_s, dtype=numpy.int32)
+        self.plane_off = torch.as_tensor(self.plane_off_h, device=dev)
+        self.plane_s = torch.as_tensor(self.plane_s_h, device=dev)
+        self.Q = int(Q)
+        self.separable = bool(separable)
+        self.nplanes = len(self.plane_s_h)
+        self.max_support = int(self.plane_s_h.max())
+
+    def as_complex64(self):
+        if self.kern.dtype == torch.complex64:
+            return self
+        return KernelBank(self.kern.to(torch.complex64), self.plane_off_h, self.plane_s_h,
+                          self.Q, self.separable)
+
+    @classmethod
+    def from_tables(cls, tables, separable=False):
+        """Bank from a list of equally-oversampled square tables."""
+        dev = _lib.device()
+        ts = [_dev(t, torch.complex64).reshape(-1) for t in tables]
+        shapes = [tuple(t.shape) for t in tables]
+        Q = shapes[0][0]
+        S = [s[-1] for s in shapes]
+        for s in shapes:
+            if s[0] != Q:
+                raise ValueError('all kernel tables must share the oversampling')
+            if not separable and s[-1] != s[-2]:
+                raise NotImplementedError('non-square 2D kernel tables')
+        sizes = numpy.array([t.numel() for t in ts], dtype=numpy.int64)
+        off = numpy.concatenate([[0], numpy.cumsum(sizes)[:-1]]).astype(numpy.int64)
+        kern = torch.cat(ts) if len(ts) > 1 else ts[0]
+        return cls(kern.to(dev), off, S, Q, separable)
+
+
+def make_gcf(src, gcf):
+    """Reference accessor (Q, gw, gh, fn) of synthesis.py:451-461, for API
+    compatibility; the device path uses :func:`kernel_bank`."""
+    src = numpy.array(src)
+    k = src.shape[1]
+    if gcf.ndim - k == 2:
+        Qpx, gw = gcf.shape[-2:]
+        return Qpx, gw, gw, lambda s, xf, yf: numpy.outer(gcf[(*s, yf)], gcf[(*s, xf)])
+    if gcf.ndim - k == 4:
+        Qpx, gh, gw = gcf.shape[-3:]
+        return Qpx, gw, gh, lambda s, xf, yf: gcf[(*s, yf, xf)]
+    raise AssertionError('unsupported kernel table rank')
+
+
+def kernel_bank(gcf, src, nv, for_grid):
+    """Device bank and per-visibility plane index for ``gcf`` indexed by
+    ``src`` rows (make_gcf semantics, synthesis.py:451-461).  Returns
+    (bank, plane_mode, vplane)."""
+    if isinstance(gcf, KernelBank):
+        return gcf.as_complex64(), _lib.PLANE_SINGLE, None
+    if src is None:
+        src_a = numpy.zeros((nv, 0), dtype=numpy.int64)
+    elif _is_torch(src):
+        src_a = src.cpu().numpy()
+    else:
+        src_a = numpy.array(src)
+    if src_a.ndim != 2:
+        src_a = src_a.reshape(len(src_a), -1)
+    k = src_a.shape[1]
+    d = gcf.ndim - k
+    if d == 2:
+        separable = True
+        Q, S = gcf.shape[-2:]
+    elif d == 4:
+        separable = False
+        Q, gh, gw = gcf.shape[-3:]
+        if gh != gw:
+            if for_grid:
+                # convgrid unpacks make_gcf's (Q, gw, gh) as (Q, gh, gw), so
+                # the window is transposed against the table and NumPy's
+                # broadcasting fails (synthesis.py:458, :477, :480-481).
+                raise ValueError('operands could not be broadcast: convgrid '
+                                 'with a non-square %dx%d kernel' % (gh, gw))
+            raise NotImplementedError('non-square 2D kernel tables')
+        S = gw
+    else:
+        raise AssertionError('unsupported kernel table rank')
+    prefix = tuple(gcf.shape[:k])
+    nplanes = int(numpy.prod(prefix)) if k else 1
+    table = (Q, S) if separable else (Q, Q, S, S)
+    tsize = int(numpy.prod(table))
+    kern = _dev(gcf, torch.complex64).reshape(-1)
+    off = numpy.arange(nplanes, dtype=numpy.int64) * tsize
+    bank = KernelBank(kern, off, numpy.full(nplanes, S, dtype=numpy.int32), Q, separable)
+    if k == 0:
+        return bank, _lib.PLANE_SINGLE, None
+    idx = src_a.astype(numpy.int64)
+    dims = numpy.array(prefix, dtype=numpy.int64)
+    idx = numpy.where(idx < 0, idx + dims, idx)
+    if (idx < 0).any() or (idx >= dims).any():
+        raise IndexError('src index out of bounds for the kernel table')
+    vplane = numpy.ravel_multi_index(tuple(idx.T), prefix).astype(numpy.int32)
+    return bank, _lib.PLANE_INDEX, torch.as_tensor(vplane, device=_lib.device())
+
+
+# --------------------------------------------------------------------------
+# L2: gridding core                                   (synthesis.py:389-503)
+# --------------------------------------------------------------------------
+
+
+def _grid_call(fn, grid_t, N, uvw_t, lam, data_t, bank, mode, vplane, wstep=1.0, wbin0=0):
+    _lib.check(fn(_lib.ctx(), _lib.ptr(grid_t), int(N), _lib.ptr(uvw_t), uvw_t.shape[0],
+                  uvw_t.shape[1], float(lam), _lib.ptr(data_t), _lib.ptr(bank.kern),
+                  bank.Q, int(bank.separable), bank.nplanes, _lib.ptr(bank.plane_off),
+                  _lib.ptr(bank.plane_s), bank.max_support, int(mode), _lib.ptr(vplane),
+                  float(wstep), int(wbin0)))
+
+
+def grid_device(bank, guv, uvw, lam, vis, mode=_lib.PLANE_SINGLE, vplane=None,
+                wstep=1.0, wbin0=0):
+    """Accumulate ``vis`` (complex64 device [nv]) at ``uvw / lam`` into the
+    complex64 device grid ``guv`` with ``bank`` (device-resident fast path of
+    convgrid / w_cache_imaging)."""
+    _grid_call(_lib.load().croc_convgrid, guv, guv.shape[0], uvw, lam, vis, bank,
+               mode, vplane, wstep, wbin0)
+
+
+def degrid_device(bank, guv, uvw, lam, out, mode=_lib.PLANE_SINGLE, vplane=None,
+                  wstep=1.0, wbin0=0):
+    """Degrid complex64 device grid ``guv`` into ``out`` (complex64 [nv])."""
+    _grid_call(_lib.load().croc_convdegrid, guv, guv.shape[0], uvw, lam, out, bank,
+               mode, vplane, wstep, wbin0)
+
+
+def _square(a):
+    if a.ndim != 2 or a.shape[0] != a.shape[1]:
+        raise NotImplementedError('only square grids are supported on the device')
+    return a.shape[0]
+
+
+def frac_coord(N, Qpx, p):
+    """(ix, xf) with ix - xf/Qpx the nearest 1/Qpx approximation of
+    N//2 + p N, half-to-even at ties; int64 results.  synthesis.py:419-435."""
+    was_torch = _is_torch(p)
+    pt = _dev(p, torch.float64).reshape(-1)
+    uv = torch.stack([pt, pt], dim=1).contiguous()
+    n = uv.shape[0]
+    outs = [torch.empty(n, dtype=torch.int32, device=uv.device) for _ in range(4)]
+    _lib.check(_lib.load().croc_frac_coords(_lib.ctx(), _lib.ptr(uv), n, 2, 1.0, int(N),
+                                            int(N), int(Qpx), *[_lib.ptr(o) for o in outs]))
+    ix, xf = outs[0].to(torch.int64), outs[1].to(torch.int64)
+    if was_torch:
+        return ix, xf
+    shape = numpy.shape(p)
+    return ix.cpu().numpy().reshape(shape), xf.cpu().numpy().reshape(shape)
+
+
+def frac_coords(shape, Qpx, p):
+    """(x, xf, y, yf) from p[:, 0] (width) and p[:, 1] (height).
+    synthesis.py:438-449."""
+    was_torch = _is_torch(p)
+    h, w = shape
+    uv = _uvw(p)
+    n = uv.shape[0]
+    outs = [torch.empty(n, dtype=torch.int32, device=uv.device) for _ in range(4)]
+    _lib.check(_lib.load().croc_frac_coords(_lib.ctx(), _lib.ptr(uv), n, uv.shape[1], 1.0,
+                                            int(h), int(w), int(Qpx),
+                                            *[_lib.ptr(o) for o in outs]))
+    outs = [o.to(torch.int64) for o in outs]
+    return tuple(outs) if was_torch else tuple(o.cpu().numpy() for o in outs)
+
+
+def convgrid(gcf, a, p, src, v):
+    """Grid visibilities ``v`` at ``p`` (fraction of the grid) into ``a`` in
+    place: a[y-S//2 : y+(S+1)//2, x-S//2 : x+(S+1)//2] += k(s, xf, yf) v.
+    No conjugation.  synthesis.py:464-481."""
+    N = _square(a)
+    uv = _uvw(p)
+    bank, mode, vplane = kernel_bank(gcf, src, uv.shape[0], True)
+    vt = _dev(v, torch.complex64).reshape(-1)
+    if _is_torch(a) and a.is_cuda and a.dtype == torch.complex64 and a.is_contiguous():
+        grid_device(bank, a, uv, 1.0, vt, mode, vplane)
+        return
+    acc = torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
+    grid_device(bank, acc, uv, 1.0, vt, mode, vplane)
+    if _is_torch(a):
+        a += acc.to(device=a.device, dtype=a.dtype)
+    else:
+        a += acc.cpu().numpy()
+
+
+def convdegrid(gcf, a, p, src=None):
+    """v_i = sum(a[window_i] * k(s, xf, yf)), no conjugation.
+    synthesis.py:484-503."""
+    N = _square(a)
+    uv = _uvw(p)
+    bank, mode, vplane = kernel_bank(gcf, src, uv.shape[0], False)
+    gt = _dev(a, torch.complex64)
+    out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
+    degrid_device(bank, gt, uv, 1.0, out, mode, vplane)
+    return _out(out, _torch_in(a, p))
+
+
+def grid(a, p, v):
+    """Nearest-cell gridding, cell N//2 + floor(0.5 + N p).
+    synthesis.py:389-401."""
+    N = _square(a)
+    uv = _uvw(p)
+    vt = _dev(v, torch.complex64).reshape(-1)
+    direct = _is_torch(a) and a.is_cuda and a.dtype == torch.complex64 and a.is_contiguous()
+    acc = a if direct else torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
+    _lib.check(_lib.load().croc_grid_nearest(_lib.ctx(), _lib.ptr(acc), N, _lib.ptr(uv),
+                                             uv.shape[0], uv.shape[1], 1.0, _lib.ptr(vt)))
+    if not direct:
+        if _is_torch(a):
+            a += acc.to(device=a.device, dtype=a.dtype)
+        else:
+            a += acc.cpu().numpy()
+
+
+def degrid(a, p):
+    """Nearest-cell degridding.  synthesis.py:404-416."""
+    N = _square(a)
+    uv = _uvw(p)
+    gt = _dev(a, torch.complex64)
+    out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
+    _lib.check(_lib.load().croc_degrid_nearest(_lib.ctx(), _lib.ptr(gt), N, _lib.ptr(uv),
+                                               uv.shape[0], uv.shape[1], 1.0, _lib.ptr(out)))
+    return _out(out, _torch_in(a, p))
+
+
+# --------------------------------------------------------------------------
+# L3: imaging drivers                                 (synthesis.py:506-927)
+# --------------------------------------------------------------------------
+
+
+def sort_vis_w(p, v=None):
+    """Order by w (stable).  synthesis.py:506-515."""
+    if _is_torch(p):
+        zs = torch.sort(p[:, 2], stable=True).indices
+    else:
+        zs = numpy.argsort(p[:, 2], kind='stable')
+    if v is not None:
+        return p[zs], v[zs]
+    return p[zs]
+
+
+def slice_vis(step, uvw, v=None):
+    """Chunks of ``step`` visibilities.  synthesis.py:518-532."""
+    starts = range(0, len(uvw), step)
+    if v is None:
+        return [uvw[i:i + step] for i in starts]
+    return [(uvw[i:i + step], v[i:i + step]) for i in starts]
+
+
+def _grid_size(theta, lam):
+    N = int(round(theta * lam))
+    assert N > 1
+    return N
+
+
+def doweight(theta, lam, p, v):
+    """Uniform weights: divide by the visibility count of each cell
+    (frac_coords with Qpx=1; the reference indexes its count grid as
+    gw[x, y]).  synthesis.py:535-549."""
+    N = _grid_size(theta, lam)
+    was_torch = _torch_in(p, v)
+    uv = _uvw(p)
+    n = uv.shape[0]
+    outs = [torch.empty(n, dtype=torch.int32, device=uv.device) for _ in range(4)]
+    _lib.check(_lib.load().croc_frac_coords(_lib.ctx(), _lib.ptr(uv), n, uv.shape[1],
+                                            float(lam), N, N, 1,
+                                            *[_lib.ptr(o) for o in outs]))
+    x, y = outs[0].to(torch.int64), outs[2].to(torch.int64)
+    if ((x < 0) | (x >= N) | (y < 0) | (y >= N)).any():
+        raise IndexError('index out of bounds for the weight grid')
+    key = x * N + y
+    counts = torch.bincount(key, minlength=N * N).to(torch.float64)
+    vt = v.to(uv.device) if _is_torch(v) else torch.as_tensor(numpy.asarray(v), device=uv.device)
+    res = vt / counts[key]
+    return res if was_torch else res.cpu().numpy()
+
+
+def _vis_dev(vis):
+    return _dev(vis, torch.complex64).reshape(-1)
+
+
+def simple_imaging(theta, lam, uvw, src, vis):
+    """Nearest-cell imaging.  synthesis.py:552-562."""
+    N = _grid_size(theta, lam)
+    uv = _uvw(uvw)
+    guv = torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
+    vt = _vis_dev(vis)
+    _lib.check(_lib.load().croc_grid_nearest(_lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uv),
+                                             uv.shape[0], uv.shape[1], float(lam),
+                                             _lib.ptr(vt)))
+    return _out(guv, _torch_in(uvw, vis))
+
+
+def simple_predict(theta, lam, uvw, src, guv):
+    """Nearest-cell predict.  synthesis.py:565-580."""
+    N = _grid_size(theta, lam)
+    uv = _uvw(uvw)
+    gt = _dev(guv, torch.complex64)
+    out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
+    _lib.check(_lib.load().croc_degrid_nearest(_lib.ctx(), _lib.ptr(gt), _square(gt),
+                                               _lib.ptr(uv), uv.shape[0], uv.shape[1],
+                                               float(lam), _lib.ptr(out)))
+    return _out(out, _torch_in(uvw, guv))
+
+
+def conv_imaging(theta, lam, uvw, src, vis, kv):
+    """Convolutional imaging with a user kernel.  synthesis.py:583-598."""
+    N = _grid_size(theta, lam)
+    uv = _uvw(uvw)
+    bank, mode, vplane = kernel_bank(kv, src, uv.shape[0], True)
+    guv = torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
+    grid_device(bank, guv, uv, lam, _vis_dev(vis), mode, vplane)
+    return _out(guv, _torch_in(uvw, vis))
+
+
+def conv_predict(theta, lam, uvw, src, guv, kv):
+    """Convolutional predict with a user kernel.  synthesis.py:600-613."""
+    uv = _uvw(uvw)
+    bank, mode, vplane = kernel_bank(kv, src, uv.shape[0], False)
+    gt = _dev(guv, torch.complex64)
+    _square(gt)
+    out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
+    degrid_device(bank, gt, uv, lam, out, mode, vplane)
+    return _out(out, _torch_in(uvw, guv))
+
+
+def _native_w_kernel(kernel_fn):
+    return kernel_fn is w_kernel
+
+
+def _bank_from_fn(kernel_fn, theta, ws, src_rows, kwargs):
+    """One plane per key: kernel_fn(theta, w, *src_row, **kwargs).  The
+    device generator is used directly when kernel_fn is :func:`w_kernel`
+    and no source columns are present."""
+    if _native_w_kernel(kernel_fn) and (src_rows is None or src_rows.shape[1] == 0):
+        kw = dict(kwargs)
+        NpixFF = kw.pop('NpixFF')
+        NpixKern = kw.pop('NpixKern')
+        Qpx = kw.pop('Qpx')
+        invert = kw.pop('invert', True)
+        return w_kernel_bank(theta, ws, NpixFF, NpixKern, Qpx, invert, **kw)
+    tables = []
+    for i, w in enumerate(ws):
+        s = () if src_rows is None else tuple(src_rows[i])
+        tables.append(kernel_fn(theta, w, *s, **kwargs))
+    return KernelBank.from_tables(tables)
+
+
+def _slice_planes(uv, wstep):
+    """Stable w-order, plane index per visibility and mean w per slice."""
+    order = torch.sort(uv[:, 2], stable=True).indices
+    nv = uv.shape[0]
+    nsl = (nv + wstep - 1) // wstep
+    sw = uv[order, 2].cpu().numpy()
+    means = [float(numpy.mean(sw[i * wstep:(i + 1) * wstep])) for i in range(nsl)]
+    vplane = torch.empty(nv, dtype=torch.int32, device=uv.device)
+    vplane[order] = (torch.arange(nv, device=uv.device) // wstep).to(torch.int32)
+    return means, vplane, order
+
+
+def w_slice_imaging(theta, lam, uvw, src, vis, wstep=2000, kernel_fn=w_kernel,
+                    **kwargs):
+    """W-projection by slices of ``wstep`` visibilities sorted by w, kernel at
+    each slice's mean w.  The reference refers to undefined ``p, v``
+    (NameError, synthesis.py:638); the intended ``uvw, vis`` are used, with
+    a stable sort.  synthesis.py:616-644."""
+    N = _grid_size(theta, lam)
+    uv = _uvw(uvw)
+    means, vplane, order = _slice_planes(uv, wstep)
+    src_rows = _src_rows(src, uv.shape[0])
+    if src_rows is not None and src_rows.shape[1]:
+        raise NotImplementedError('w_slice_imaging with source columns')
+    bank = _bank_from_fn(kernel_fn, theta, means, None, kwargs)
+    guv = torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
+    grid_device(bank.as_complex64(), guv, uv, lam, _vis_dev(vis), _lib.PLANE_INDEX, vplane)
+    return _out(guv, _torch_in(uvw, vis))
+
+
+def w_slice_predict(theta, lam, uvw, src, guv, wstep=2000, kernel_fn=w_kernel,
+                    **kwargs):
+    """Predict counterpart of w_slice_imaging, results in input order.  The
+    reference uses undefined ``p`` (NameError, synthesis.py:672).
+    synthesis.py:647-679."""
+    _grid_size(theta, lam)
+    uv = _uvw(uvw)
+    means, vplane, order = _slice_planes(uv, wstep)
+    bank = _bank_from_fn(kernel_fn, theta, means, None, kwargs)
+    gt = _dev(guv, torch.complex64)
+    _square(gt)
+    out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
+    degrid_device(bank.as_complex64(), gt, uv, lam, out, _lib.PLANE_INDEX, vplane)
+    return _out(out, _torch_in(uvw, guv))
+
+
+def _src_rows(src, nv):
+    if src is None:
+        return None
+    a = src.cpu().numpy() if _is_torch(src) else numpy.asarray(src)
+    if a.ndim == 1 and a.size == 0:
+        return None
+    return a.reshape(nv, -1) if a.size else None
+
+
+def w_cache_planes(theta, uv, src, wstep, fn, kwargs):
+    """Kernel bank with one plane per distinct (w-bin, src row) key and the
+    plane-selection arguments for the gridder.  w-bin = round-half-even of
+    w / wstep (numpy.round in synthesis.py:793)."""
+    wb = torch.round(uv[:, 2] / wstep).to(torch.int64)
+    src_rows = _src_rows(src, uv.shape[0])
+    if src_rows is None or src_rows.shape[1] == 0:
+        lo, hi = int(wb.min()), int(wb.max())
+        present = torch.zeros(hi - lo + 1, dtype=torch.bool, device=uv.device)
+        present[wb - lo] = True
+        present = present.cpu().numpy()
+        if present.all() or (hi - lo + 1) <= 4 * int(present.sum()):
+            # dense bin range: the gridder derives the plane from w itself
+            bins = numpy.arange(lo, hi + 1)
+            bank = _bank_from_fn(fn, theta, [wstep * float(b) for b in bins], None, kwargs)
+            return bank.as_complex64(), _lib.PLANE_WBIN, None, lo
+        bins, inv = torch.unique(wb, return_inverse=True)
+        bins = bins.cpu().numpy()
+        bank = _bank_from_fn(fn, theta, [wstep * float(b) for b in bins], None, kwargs)
+        return bank.as_complex64(), _lib.PLANE_INDEX, inv.to(torch.int32), 0
+    keys = torch.cat([wb[:, None].to(torch.float64),
+                      torch.as_tensor(src_rows, dtype=torch.float64, device=uv.device)], 1)
+    ukeys, inv = torch.unique(keys, dim=0, return_inverse=True)
+    ukeys = ukeys.cpu().numpy()
+    ws = [wstep * float(k[0]) for k in ukeys]
+    srcs = numpy.array([[_as_index(x) for x in k[1:]] for k in ukeys], dtype=object)
+    bank = _bank_from_fn(fn, theta, ws, srcs, kwargs)
+    return bank.as_complex64(), _lib.PLANE_INDEX, inv.to(torch.int32), 0
+
+
+def _as_index(x):
+    return int(x) if float(x).is_integer() else float(x)
+
+
+def w_cache_imaging(theta, lam, uvw, src, vis, wstep=2000, kernel_cache=None,
+                    kernel_fn=w_kernel, **kwargs):
+    """W-projection with the kernel of each visibility's w-bin
+    (wstep * round(w / wstep)); ``kernel_cache`` replaces ``kernel_fn`` when
+    given.  Kernels are made once per distinct key and all visibilities are
+    gridded in one device pass.  synthesis.py:757-796."""
+    fn = kernel_cache if kernel_cache is not None else kernel_fn
+    N = int(round(theta * lam))
+    uv = _uvw(uvw)
+    bank, mode, vplane, wbin0 = w_cache_planes(theta, uv, src, wstep, fn, kwargs)
+    guv = torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
+    grid_device(bank, guv, uv, lam, _vis_dev(vis), mode, vplane, wstep, wbin0)
+    return _out(guv, _torch_in(uvw, vis))
+
+
+def w_cache_predict(theta, lam, uvw, src, guv, wstep=2000, kernel_cache=None,
+                    kernel_fn=w_kernel, **kwargs):
+    """Predict counterpart of w_cache_imaging, results in input order.  The
+    reference uses undefined ``nv`` / ``ixs`` (NameError,
+    synthesis.py:823/827).  synthesis.py:799-828."""
+    fn = kernel_cache if kernel_cache is not None else kernel_fn
+    uv = _uvw(uvw)
+    bank, mode, vplane, wbin0 = w_cache_planes(theta, uv, src, wstep, fn, kwargs)
+    gt = _dev(guv, torch.complex64)
+    _square(gt)
+    out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
+    degrid_device(bank, gt, uv, lam, out, mode, vplane, wstep, wbin0)
+    return _out(out, _torch_in(uvw, guv))
+
+
+def mirror_uvw(uvw, vis):
+    """Flip baselines with v < 0 and conjugate them.  synthesis.py:831-852."""
+    if _is_torch(uvw):
+        neg = uvw[:, 1] < 0
+        uvw = uvw.clone()
+        vis = vis.clone()
+        uvw[neg] = -uvw[neg]
+        vis[neg] = vis[neg].conj()
+        return uvw, vis
+    neg = uvw[:, 1] < 0
+    uvw = numpy.array(uvw, dtype=float, copy=True)
+    vis = numpy.array(vis, copy=True)
+    uvw[neg] = -uvw[neg]
+    vis[neg] = numpy.conj(vis[neg])
+    return uvw, vis
+
+
+def make_grid_hermitian(guv):
+    """In place: add the conjugated point mirror about N//2 (even N: rows and
+    columns 1..N-1).  synthesis.py:855-880."""
+    N = _square(guv)
+    if _is_torch(guv) and guv.is_cuda and guv.is_contiguous() and \
+            guv.dtype in (torch.complex64, torch.complex128):
+        _lib.check(_lib.load().croc_make_hermitian(
+            _lib.ctx(), _lib.ptr(guv), N, int(guv.dtype == torch.complex128)))
+        return guv
+    t = _dev(guv, torch.complex128)
+    _lib.check(_lib.load().croc_make_hermitian(_lib.ctx(), _lib.ptr(t), N, 1))
+    if _is_torch(guv):
+        guv.copy_(t.to(guv.device, guv.dtype))
+    else:
+        guv[...] = t.cpu().numpy()
+    return guv
+
+
+def do_imaging(theta, lam, uvw, src, vis, imgfn, **kwargs):
+    """Mirror, weight, image, hermitian, ifft; PSF likewise; normalise.
+    synthesis.py:883-911."""
+    if src is None:
+        src = numpy.ndarray((len(vis), 0))
+    uvw, vis = mirror_uvw(uvw, vis)
+    ones = torch.ones(len(uvw), dtype=torch.float64) if _is_torch(uvw) else numpy.ones(len(uvw))
+    wt = doweight(theta, lam, uvw, ones)
+    cdrt = imgfn(theta, lam, uvw, src, wt * vis, **kwargs)
+    drt = ifft(make_grid_hermitian(cdrt)).real
+    c = imgfn(theta, lam, uvw, src, wt, **kwargs)
+    psf = ifft(make_grid_hermitian(c)).real
+    pmax = psf.max()
+    assert pmax > 0.0
+    return drt / pmax, psf / pmax, pmax
+
+
+def do_predict(theta, lam, uvw, src, modelimage, predfn, **kwargs):
+    """fft(model image) then predfn.  synthesis.py:914-927."""
+    m = modelimage.to(torch.complex128) if _is_torch(modelimage) else modelimage.astype(complex)
+    return predfn(theta, lam, uvw, src, fft(m), **kwargs)
+
+
+def sample_image(image, x, y):
+    """Bilinear sample relative to the image middle.  synthesis.py:736-755."""
+    x = x + image.shape[0] / 2
+    y = y + image.shape[1] / 2
+    ix = numpy.floor(x).astype(int)
+    dx = x - ix
+    iy = numpy.floor(y).astype(int)
+    dy = y - iy
+    return numpy.sum([image[iy, ix] * ((1 - dx) * (1 - dy)),
+                      image[iy, ix + 1] * (dx * (1 - dy)),
+                      image[iy + 1, ix] * ((1 - dx) * dy),
+                      image[iy + 1, ix + 1] * (dx * dy)], axis=0)

@@ -1,0 +1,124 @@
+"""Seeded synthetic inputs for the five benchmark workloads.
+
+Visibilities come from point sources through the phase-tracked direct
+Fourier transform of crocodile/simulate.py:210-230 (simulate_point), with
+(u, v, w) either drawn from analytic densities or from antenna layouts rotated
+by xyz_to_uvw (simulate.py:74-100).  Everything here is input generation, off
+the gridding path; it runs on the host in NumPy, or on the device through
+torch when ``device`` is given for the large layouts.
+
+Workload names:
+
+* ``plain512``   N=512, theta=0.1, lam=5120, 1e5 vis in a disk, w=0,
+                 separable PSWF kernel S=8 Q=8.
+* ``wproj2048``  N=2048, theta=0.1, lam=20480, 2e6 vis with surface density
+                 r^-1.5, w uniform +-1500, w-kernels every 20 wavelengths with
+                 support 16..32 growing with |w|, Q=8.
+* ``towers4096`` N=4096, theta=0.08, lam=51200, 1e7 vis from 128 antennas.
+* ``diff1024``   1024^2 image of Gaussian blobs, 1e6 vis, w +-500.
+* ``towers8192`` N=8192, theta=0.05, lam=163840, 5e7 vis from 256 antennas.
+"""
+
+import math
+
+import numpy
+
+SPEC = {
+    'plain512': dict(N=512, theta=0.1, lam=5120.0, nv=100_000, S=8, Q=8,
+                     nsrc=10, seed=1001),
+    'wproj2048': dict(N=2048, theta=0.1, lam=20480.0, nv=2_000_000, Q=8,
+                      wmax=1500.0, wstep=20.0, S_min=16, S_max=32, NpixFF=64,
+                      nsrc=20, seed=1002),
+    'towers4096': dict(N=4096, theta=0.08, lam=51200.0, nv=10_000_000, Q=16,
+                       S=16, nant=128, hours=6.0, freq=1.4e9, dec=-45.0,
+                       subgrid=128, nsrc=20, seed=1003),
+    'diff1024': dict(N=1024, theta=0.1, lam=10240.0, nv=1_000_000, Q=8,
+                     wmax=500.0, nblob=50, seed=1004),
+    'towers8192': dict(N=8192, theta=0.05, lam=163840.0, nv=50_000_000, Q=32,
+                       S=16, nant=256, hours=8.0, freq=150e6, dec=-45.0,
+                       subgrid=128, nsrc=20, seed=1005),
+}
+
+
+def wproj_support(w, wmax=1500.0, S_min=16, S_max=32):
+    """Even kernel support growing linearly from S_min at w=0 to S_max at
+    |w| = wmax (the wproj2048 w-kernel size rule)."""
+    steps = (S_max - S_min) // 2
+    k = int(math.ceil(steps * min(abs(float(w)) / wmax, 1.0) - 1e-9))
+    return S_min + 2 * k
+
+
+def point_sources(rng, n, theta):
+    """n sources with l, m uniform in +-theta/2 and lognormal(0,1) flux."""
+    l = rng.uniform(-theta / 2, theta / 2, n)
+    m = rng.uniform(-theta / 2, theta / 2, n)
+    flux = rng.lognormal(0.0, 1.0, n)
+    return l, m, flux
+
+
+def dft_vis(uvw, l, m, flux, chunk=1 << 20):
+    """sum_k F_k exp(-2 pi i (u l_k + v m_k + w (n_k - 1))) in FP64, the
+    phase-tracked point-source DFT of simulate.py:228-230."""
+    nv = uvw.shape[0]
+    out = numpy.zeros(nv, dtype=numpy.complex128)
+    n1 = numpy.sqrt(1.0 - l ** 2 - m ** 2) - 1.0
+    for s in range(0, nv, chunk):
+        u = uvw[s:s + chunk]
+        acc = numpy.zeros(u.shape[0], dtype=numpy.complex128)
+        for k in range(len(l)):
+            ph = u[:, 0] * l[k] + u[:, 1] * m[k] + u[:, 2] * n1[k]
+            acc += flux[k] * numpy.exp(-2j * numpy.pi * ph)
+        out[s:s + chunk] = acc
+    return out
+
+
+def add_noise(rng, vis, sigma=0.01):
+    return vis + sigma * (rng.standard_normal(vis.shape) + 1j * rng.standard_normal(vis.shape))
+
+
+def plain512(nv=None, seed=None):
+    """uvw [nv,3] (wavelengths, w=0), vis [nv] complex128, sources."""
+    c = SPEC['plain512']
+    rng = numpy.random.default_rng(c['seed'] if seed is None else seed)
+    nv = c['nv'] if nv is None else nv
+    R = 0.45 * c['lam']
+    r = R * numpy.sqrt(rng.uniform(0, 1, nv))
+    phi = rng.uniform(0, 2 * numpy.pi, nv)
+    uvw = numpy.stack([r * numpy.cos(phi), r * numpy.sin(phi), numpy.zeros(nv)], axis=1)
+    l, m, f = point_sources(rng, c['nsrc'], c['theta'])
+    vis = add_noise(rng, dft_vis(uvw, l, m, f))
+    return uvw, vis
+
+
+def wproj2048(nv=None, seed=None):
+    """uvw [nv,3] with surface density ~ r^-1.5 out to 0.45 lam, w uniform
+    in +-wmax; vis from 20 point sources plus noise."""
+    c = SPEC['wproj2048']
+    rng = numpy.random.default_rng(c['seed'] if seed is None else seed)
+    nv = c['nv'] if nv is None else nv
+    R = 0.45 * c['lam']
+    r = R * rng.uniform(0, 1, nv) ** 2
+    phi = rng.uniform(0, 2 * numpy.pi, nv)
+    w = rng.uniform(-c['wmax'], c['wmax'], nv)
+    uvw = numpy.stack([r * numpy.cos(phi), r * numpy.sin(phi), w], axis=1)
+    l, m, f = point_sources(rng, c['nsrc'], c['theta'])
+    vis = add_noise(rng, dft_vis(uvw, l, m, f))
+    return uvw, vis
+
+
+def antenna_layout(rng, nant, sigma_m=1500.0, arms_to_m=None):
+    """Antenna (x, y, z) in metres: a 2D Gaussian core (z small), optionally
+    plus three log-spiral arms reaching ``arms_to_m``."""
+    if arms_to_m is None:
+        xy = rng.normal(0.0, sigma_m, (nant, 2))
+    else:
+        ncore = nant // 2
+        core = rng.normal(0.0, sigma_m, (ncore, 2))
+        narm = nant - ncore
+        t = numpy.linspace(0.0, 1.0, narm)
+        rad = sigma_m * (arms_to_m / sigma_m) ** t
+        ang = 2.0 * numpy.pi * (numpy.arange(narm) % 3) / 3 + 1.5 * numpy.log(rad / sigma_m)
+        arms = numpy.stack([rad * numpy.cos(ang), rad * numpy.sin(ang)], axis=1)
+        xy = numpy.concatenate([core, arms])
+    z = rng.normal(0.0, 5.0, (nant, 1))
+    return numpy.concatenate([xy, z], axis=1)

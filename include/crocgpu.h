@@ -1,0 +1,136 @@
+/*
+ * crocgpu.h -- C ABI of the MI355X (gfx950) gridding / degridding library
+ * libcrocgpu.so.
+ *
+ * Every entry point takes plain device pointers and sizes, returns an int
+ * status (CROC_OK or a negative CROC_E* code; croc_last_error() gives the
+ * message for the calling thread) and runs on the context's HIP stream.
+ * Entry points that report per-visibility errors (coordinate domain, window
+ * bounds) synchronise the stream before returning; on error the output
+ * arrays are left unmodified.
+ *
+ * Array conventions (all device memory):
+ *   cf32  complex64 stored as interleaved float pairs (re, im)
+ *   cf64  complex128 stored as interleaved double pairs
+ *   uvw   double [nv][stride], columns u, v, w (wavelengths); the grid
+ *         coordinate is p = uvw / lam computed in IEEE FP64 as NumPy does
+ *   grid  cf32 [N][N] row-major, rows = v, columns = u, centre at N/2
+ *   kernel bank: all kernel tables concatenated; plane k starts at complex
+ *         element plane_off[k] and has support plane_s[k] (square);
+ *         2D table layout [Q][Q][S][S] = [yf][xf][y][x]; separable table
+ *         layout [Q][S] (the kernel is the outer product of rows yf and xf)
+ *
+ * Plane selection (which table a visibility uses):
+ *   CROC_PLANE_SINGLE  every visibility uses plane 0
+ *   CROC_PLANE_INDEX   plane = vplane[i]            (src-indexed tables)
+ *   CROC_PLANE_WBIN    plane = rint(w_i / wstep) - wbin0   (w-cache bins,
+ *                      rint is round-half-even like numpy.round)
+ */
+#ifndef CROCGPU_H
+#define CROCGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CROC_OK 0
+#define CROC_EINVAL (-1)   /* invalid argument                                   */
+#define CROC_EDOMAIN (-2)  /* grid coordinate outside [-0.5, 0.5): AssertionError */
+#define CROC_EBOUNDS (-3)  /* kernel window leaves the grid: ValueError          */
+#define CROC_EHIP (-4)     /* HIP runtime failure                                 */
+#define CROC_ENOMEM (-5)   /* device allocation failed                            */
+#define CROC_EPLANE (-6)   /* w-bin / plane index outside the kernel bank         */
+
+#define CROC_PLANE_SINGLE 0
+#define CROC_PLANE_INDEX 1
+#define CROC_PLANE_WBIN 2
+
+typedef struct croc_ctx croc_ctx;
+
+/* Library / context management (no reference counterpart: the reference is
+ * a set of pure NumPy functions; state it keeps implicitly -- the pylru
+ * kernel cache of synthesis.py:788 -- lives in a context here). */
+int croc_version(void);
+const char *croc_last_error(void);
+int croc_ctx_create(int device, croc_ctx **out);
+int croc_ctx_destroy(croc_ctx *ctx);
+int croc_ctx_set_stream(croc_ctx *ctx, void *hip_stream);
+/* Tuning knobs: tile edge (0 = automatic), visibilities per work item,
+ * visibilities per compensated flush batch. */
+int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk, int batch);
+/* Device time in milliseconds of the last gridding / degridding kernel
+ * (HIP events around its launch on the context stream). */
+int croc_last_kernel_ms(croc_ctx *ctx, float *grid_ms, float *degrid_ms);
+
+/* frac_coords: replaces crocodile/synthesis.py:438-449 (frac_coords) and
+ * :419-435 (frac_coord) -- x = N//2 + p*N, fx = round-half-even(-x*Q),
+ * ix = -floor(fx/Q), xf = fx mod Q; p = uvw/lam in FP64.  Outputs int32
+ * [nv].  Returns CROC_EDOMAIN if any p lies outside [-0.5, 0.5).
+ * Native counterpart: examples/grid/grid.h:177-178 (frac_coord). */
+int croc_frac_coords(croc_ctx *ctx, const double *uvw, int64_t nv,
+                     int64_t stride, double lam, int Nh, int Nw, int Q,
+                     int32_t *x, int32_t *xf, int32_t *y, int32_t *yf);
+
+/* convgrid: replaces crocodile/synthesis.py:464-481 (convgrid, in-place
+ * accumulation a[window] += k(s, xf, yf) * v, no conjugation) with the
+ * kernel selection of make_gcf :451-461 and of w_cache_imaging :792-795
+ * (plane_mode CROC_PLANE_WBIN).  grid is updated in place.
+ * Native counterparts: examples/grid/grid.h:192-193 (grid_wprojection),
+ * grid.c:750-792 (grid_awprojection). */
+int croc_convgrid(croc_ctx *ctx, void *grid, int N, const double *uvw,
+                  int64_t nv, int64_t stride, double lam, const void *vis,
+                  const void *kern, int Q, int separable, int nplanes,
+                  const int64_t *plane_off, const int32_t *plane_s,
+                  int max_support, int plane_mode, const int32_t *vplane,
+                  double wstep, int wbin0);
+
+/* convdegrid: replaces crocodile/synthesis.py:484-503 (convdegrid,
+ * v = sum(a[window] * k(s, xf, yf))) and the per-visibility loop of
+ * w_cache_predict :799-828.  vis_out cf32 [nv].
+ * Native counterpart: examples/grid/grid.h:183-186 (degrid_conv_uv). */
+int croc_convdegrid(croc_ctx *ctx, const void *grid, int N, const double *uvw,
+                    int64_t nv, int64_t stride, double lam, void *vis_out,
+                    const void *kern, int Q, int separable, int nplanes,
+                    const int64_t *plane_off, const int32_t *plane_s,
+                    int max_support, int plane_mode, const int32_t *vplane,
+                    double wstep, int wbin0);
+
+/* grid / degrid (nearest cell): replace crocodile/synthesis.py:389-401 and
+ * :404-416, cell N//2 + floor(0.5 + N*p) (round-half-up). */
+int croc_grid_nearest(croc_ctx *ctx, void *grid, int N, const double *uvw,
+                      int64_t nv, int64_t stride, double lam, const void *vis);
+int croc_degrid_nearest(croc_ctx *ctx, const void *grid, int N,
+                        const double *uvw, int64_t nv, int64_t stride,
+                        double lam, void *vis_out);
+
+/* w-kernel bank generation: replaces crocodile/synthesis.py:357-377
+ * (w_kernel) = :250-270 (kernel_coordinates) + :273-293
+ * (w_kernel_function, with 1/k when invert) + :330-354 (kernel_oversample)
+ * for nw planes in one launch.  w: host array [nw]; S: host array [nw]
+ * (NpixKern per plane); T: host 2x2 matrix row-major or NULL.  Output
+ * written at out + plane_off[k] (host offsets, complex elements), layout
+ * [Q][Q][S][S]; out_double selects cf64 (1) or cf32 (0) output.
+ * Computed in FP64 (phase and the oversampled DFT). */
+int croc_w_kernels(croc_ctx *ctx, double theta, const double *w, int nw,
+                   int npix_ff, const int32_t *S, int Q, int invert,
+                   double dl, double dm, const double *T,
+                   const int64_t *plane_off, void *out, int out_double);
+
+/* kernel_oversample: replaces crocodile/synthesis.py:330-354 for a device
+ * far field ff (cf64, [n][n] when ndim == 2, [n] when ndim == 1):
+ * pad to n*Q, centred FFT, extract -- computed as the equivalent direct
+ * zoom DFT in FP64.  out cf64 [Q][Q][s][s] or [Q][s]. */
+int croc_kernel_oversample(croc_ctx *ctx, const void *ff, int n, int ndim,
+                           int Q, int s, void *out);
+
+/* make_grid_hermitian: replaces crocodile/synthesis.py:855-880 (in place,
+ * [N][N], cf64 when is_double else cf32).
+ * Native counterpart: examples/grid/grid.c:794-817. */
+int croc_make_hermitian(croc_ctx *ctx, void *grid, int N, int is_double);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CROCGPU_H */

@@ -1,0 +1,329 @@
+"""GPU parity of the core path against the FP64 oracle.
+
+Tolerances: integer outputs (frac_coord) bit-exact; FP64 kernel generation
+within 1e-11 relative of the NumPy FFT route; complex64 gridding and
+degridding within 1e-5 RMS relative (the stated target) and 1e-4 max
+relative of the complex128 oracle.
+"""
+import itertools
+import json
+import os
+
+import numpy
+import pytest
+import torch
+from numpy.testing import assert_allclose
+
+import crocodile_amd.synthesis as G
+from crocodile_amd import _lib
+from oracle import cgrid
+from oracle import synthesis_ref as R
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+
+RMS_TOL = 1e-5
+MAX_TOL = 1e-4
+
+
+def rel_err(x, ref):
+    x = numpy.asarray(x)
+    ref = numpy.asarray(ref)
+    d = x - ref
+    rms = numpy.linalg.norm(d) / max(numpy.linalg.norm(ref), 1e-300)
+    mx = numpy.abs(d).max() / max(numpy.abs(ref).max(), 1e-300)
+    return rms, mx
+
+
+def assert_close(x, ref, rms_tol=RMS_TOL, max_tol=MAX_TOL, what=''):
+    rms, mx = rel_err(x, ref)
+    print('%s rms_rel=%.3e max_rel=%.3e' % (what, rms, mx), flush=True)
+    assert rms <= rms_tol and mx <= max_tol, (what, rms, mx)
+
+
+# ---------------------------------------------------------------- frac_coord
+
+def test_frac_coord_golden(gpu):
+    g = json.load(open(os.path.join(GOLDEN, 't05_frac_coord.json')))
+    p = numpy.array(g['u']) / g['grid_size']
+    x, xf = G.frac_coord(g['grid_size'], g['oversample'], p)
+    assert x.tolist() == g['x'] and xf.tolist() == g['fx']
+
+
+@pytest.mark.parametrize('N,Q', [(2, 1), (2, 3), (512, 8), (2048, 8), (4095, 16), (8192, 32)])
+def test_frac_coord_bit_exact(gpu, N, Q):
+    rng = numpy.random.default_rng(N + Q)
+    p = rng.uniform(-0.5, 0.5, 20000)
+    # exact ties of -x*Q at every half-integer the grid reaches
+    ties = ((numpy.arange(-N * Q, N * Q) + 0.5) / Q - N // 2) / N
+    ties = ties[(ties >= -0.5) & (ties < 0.5)]
+    p = numpy.concatenate([p, ties, [-0.5, 0.0, numpy.nextafter(0.5, 0)]])
+    x, xf = G.frac_coord(N, Q, p)
+    rx, rxf = R.frac_coord(N, Q, p)
+    assert (x == rx).all() and (xf == rxf).all()
+
+
+def test_frac_coords_shape_order(gpu):
+    rng = numpy.random.default_rng(1)
+    p = rng.uniform(-0.5, 0.5, (1000, 3))
+    got = G.frac_coords((64, 32), 4, p)
+    ref = R.frac_coords((64, 32), 4, p)
+    for a, b in zip(got, ref):
+        assert (a == b).all()
+
+
+def test_frac_coord_domain_error(gpu):
+    with pytest.raises(AssertionError):
+        G.frac_coord(8, 2, numpy.array([0.1, 0.5]))
+
+
+# ------------------------------------------------------------------ kernels
+
+@pytest.mark.parametrize('theta,w,F,S,Q,kw', [
+    (0.1, 0.0, 16, 8, 4, {}),
+    (0.1, 137.0, 64, 32, 8, {}),
+    (0.1, -1500.0, 64, 32, 8, {}),
+    (0.08, 620.0, 33, 16, 16, {}),
+    (0.2, 55.5, 20, 9, 3, {'dl': 0.01, 'dm': -0.02}),
+    (0.05, 300.0, 24, 12, 2, {'T': numpy.array([[1.0, 0.02], [-0.01, 0.98]])}),
+])
+@pytest.mark.parametrize('invert', [True, False])
+def test_w_kernel_matches_oracle(gpu, theta, w, F, S, Q, kw, invert):
+    got = G.w_kernel(theta, w, F, S, Q, invert=invert, **kw)
+    ref = R.w_kernel(theta, w, F, S, Q, invert=invert, **kw)
+    assert got.shape == ref.shape and got.dtype == numpy.complex128
+    assert_allclose(got, ref, rtol=0, atol=1e-11 * numpy.abs(ref).max())
+
+
+def test_w_kernel_perfect_sampling(gpu):
+    for N in range(1, 6):
+        for w in [0.0, 3.0, -7.5]:
+            assert_allclose(G.w_kernel(N / 1000, w, N, N, 1),
+                            R.w_kernel(N / 1000, w, N, N, 1), atol=1e-12)
+
+
+def test_w_kernel_coordinate_error(gpu):
+    with pytest.raises(AssertionError):
+        G.w_kernel(2.5, 10.0, 16, 8, 2)
+
+
+@pytest.mark.parametrize('N,Q,s', [(5, 1, 3), (16, 4, 14), (31, 8, 29), (12, 3, 10)])
+def test_kernel_oversample_2d(gpu, N, Q, s):
+    pat = R.coordinates2(N)[1] + R.coordinates2(N)[0] * 1j
+    assert_allclose(G.kernel_oversample(pat, Q, s), R.kernel_oversample(pat, Q, s),
+                    atol=1e-12 * N * N)
+
+
+def test_kernel_oversample_1d_pswf(gpu):
+    aaf = R.anti_aliasing_function(512, 0, 8)
+    got = G.kernel_oversample(aaf, 8, 8)
+    ref = R.kernel_oversample(aaf, 8, 8)
+    assert_allclose(got, ref, atol=1e-12 * numpy.abs(ref).max())
+
+
+def test_kernel_oversample_subgrid_property(gpu):
+    """tests/test_synthesis.py:77-91 on the device implementation."""
+    for N in range(3, 20):
+        pat = R.coordinates2(N)[1] + R.coordinates2(N)[0] * 1j
+        k1 = G.kernel_oversample(pat, 1, N - 2)
+        k2 = G.kernel_oversample(pat, 2, N - 2)
+        k4 = G.kernel_oversample(pat, 4, N - 2)
+        assert_allclose(k1[0, 0], k2[0, 0], atol=1e-12)
+        for ux, uy in itertools.product(range(2), range(2)):
+            assert_allclose(k2[uy, ux], k4[2 * uy, 2 * ux], atol=1e-12)
+
+
+# ------------------------------------------------------------ gridding core
+
+def random_case(rng, N, S, Q, nv, planes=1, separable=False, margin=None):
+    shape = (Q, S) if separable else (Q, Q, S, S)
+    tables = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+              for _ in range(planes)]
+    lim = 0.5 - (S // 2 + 2) / N if margin is None else margin
+    p = numpy.stack([rng.uniform(-lim, lim, nv), rng.uniform(-lim, lim, nv),
+                     rng.uniform(-1, 1, nv)], axis=1)
+    v = rng.standard_normal(nv) + 1j * rng.standard_normal(nv)
+    return tables, p, v
+
+
+CASES = [
+    # N, S, Q, nv, planes, separable
+    (64, 4, 2, 3000, 1, False),
+    (128, 8, 8, 20000, 1, True),
+    (256, 16, 4, 20000, 3, False),
+    (512, 32, 8, 30000, 2, False),
+    (300, 7, 5, 10000, 1, False),
+    (200, 24, 3, 15000, 4, True),
+    (96, 64, 1, 500, 1, False),
+    (2048, 20, 8, 50000, 5, False),
+]
+
+
+@pytest.mark.parametrize('N,S,Q,nv,planes,separable', CASES)
+def test_convgrid_convdegrid_parity(gpu, N, S, Q, nv, planes, separable):
+    print('case N=%d S=%d Q=%d nv=%d planes=%d sep=%d' % (N, S, Q, nv, planes, separable),
+          flush=True)
+    rng = numpy.random.default_rng(N * 7 + S)
+    tables, p, v = random_case(rng, N, S, Q, nv, planes, separable)
+    vplane = rng.integers(0, planes, nv)
+    gcf = numpy.array(tables) if planes > 1 else tables[0]
+    src = vplane[:, None] if planes > 1 else None
+    bank = cgrid.Bank(tables, separable)
+    ref = numpy.zeros((N, N), dtype=complex)
+    cgrid.convgrid(bank, ref, p, v, vplane.astype(numpy.int32) if planes > 1 else None)
+    a = numpy.zeros((N, N), dtype=complex)
+    G.convgrid(gcf, a, p, src, v)
+    assert_close(a, ref, what='grid')
+    vref = cgrid.convdegrid(bank, ref, p, vplane.astype(numpy.int32) if planes > 1 else None)
+    vd = G.convdegrid(gcf, ref, p, src)
+    assert_close(vd, vref, what='degrid')
+
+
+def test_convgrid_accumulates_in_place(gpu):
+    rng = numpy.random.default_rng(2)
+    (gcf,), p, v = random_case(rng, 64, 6, 4, 500)
+    a = rng.standard_normal((64, 64)) + 1j * rng.standard_normal((64, 64))
+    ref = a.copy()
+    R.convgrid(gcf, ref, p, None, v)
+    G.convgrid(gcf, a, p, None, v)
+    assert_close(a, ref, what='in-place')
+
+
+def test_torch_device_inputs(gpu):
+    rng = numpy.random.default_rng(3)
+    (gcf,), p, v = random_case(rng, 128, 8, 4, 4000)
+    ref = numpy.zeros((128, 128), dtype=complex)
+    R.convgrid(gcf, ref, p, None, v)
+    a = torch.zeros((128, 128), dtype=torch.complex64, device=gpu)
+    G.convgrid(torch.as_tensor(gcf, device=gpu), a, torch.as_tensor(p, device=gpu), None,
+               torch.as_tensor(v, device=gpu))
+    assert isinstance(a, torch.Tensor)
+    assert_close(a.cpu().numpy(), ref, what='torch grid')
+    out = G.convdegrid(gcf, a, torch.as_tensor(p, device=gpu))
+    assert isinstance(out, torch.Tensor) and out.is_cuda
+
+
+def test_reference_exact_delta_image(gpu):
+    """tests/test_synthesis.py:109-138 through the device path."""
+    for lam in [6, 1e15]:
+        gcf = numpy.conj(G.w_kernel(1 / lam, 0, 1, 1, 1))
+        for N in range(2, 6):
+            u, v = R.coordinates2(N)
+            uvw = numpy.transpose([numpy.hstack(u), numpy.hstack(v), 0 * numpy.hstack(u)]) * lam
+            for x, y in itertools.product(range(-(N // 2), (N + 1) // 2), repeat=2):
+                vis = R.simulate_point(uvw, x / lam, y / lam)
+                a = numpy.zeros((N, N), dtype=complex)
+                G.grid(a, uvw / lam, vis)
+                G.convgrid(gcf, a, uvw / lam, None, vis)
+                a /= 2
+                img = numpy.real(R.ifft(a))
+                assert abs(img[N // 2 + y, N // 2 + x] - 1) < 1e-6
+                img[N // 2 + y, N // 2 + x] = 0
+                assert_allclose(img, 0, atol=1e-6)
+                assert_allclose(G.degrid(a, uvw / lam), vis, rtol=1e-6, atol=1e-6)
+                assert_allclose(G.convdegrid(gcf, a, uvw / lam), vis, rtol=1e-6, atol=1e-6)
+
+
+def test_reference_grid_degrid_w(gpu):
+    """tests/test_synthesis.py:242-281 through the device path (kernels of
+    size N on a 2N grid, perfect sampling)."""
+    lam = 1000
+    for uw, vw in [(.5, 0), (0, .5), (-1, 0), (0, -1)]:
+        for N in range(1, 6):
+            u, v = R.coordinates2(N)
+            u, v = numpy.hstack(u), numpy.hstack(v)
+            uvw_all = numpy.transpose([u, v, uw * u + vw * v]) * lam
+            slices = R.slice_vis(N, R.sort_vis_w(uvw_all))
+            gcfs = [(G.w_kernel(N / lam, numpy.mean(s[:, 2]), N, N, 1) / N ** 2,
+                     G.w_kernel(N / lam, numpy.mean(s[:, 2]), N, N, 1, invert=False) / N ** 2)
+                    for s in slices]
+            for x, y in itertools.product(range(-(N // 2), (N + 1) // 2), repeat=2):
+                img_ref = numpy.zeros((N, N))
+                img_ref[N // 2 + y, N // 2 + x] = 1
+                a_ref = numpy.fft.ifftshift(R.fft(img_ref))
+                a_ref = numpy.vstack(2 * [numpy.hstack(2 * [a_ref])])
+                a = numpy.zeros((2 * N, 2 * N), dtype=complex)
+                for s, (gcf, gcf_p) in zip(slices, gcfs):
+                    vis = R.simulate_point(s, x / lam, y / lam)
+                    assert_allclose(G.convdegrid(gcf_p, a_ref, s / lam / 2), vis,
+                                    rtol=2e-6, atol=2e-6)
+                    G.convgrid(gcf, a, s / lam / 2, None, vis)
+                a = numpy.fft.fftshift(a[:N, :N] + a[:N, N:] + a[N:, :N] + a[N:, N:])
+                img = numpy.real(R.ifft(a))
+                assert_allclose(img[N // 2 + y, N // 2 + x], 1, rtol=1e-5)
+                assert_allclose(img, img_ref, atol=2e-3)
+
+
+def test_window_outside_grid_raises_and_leaves_grid(gpu):
+    gcf = numpy.ones((2, 2, 4, 4), dtype=complex)
+    a = numpy.zeros((16, 16), dtype=complex)
+    p = numpy.array([[0.0, 0.0], [0.49, 0.0]])
+    with pytest.raises(ValueError):
+        G.convgrid(gcf, a, p, None, numpy.array([1.0 + 0j, 1.0 + 0j]))
+    assert (a == 0).all()
+    with pytest.raises(ValueError):
+        G.convdegrid(gcf, a, p)
+
+
+def test_domain_error(gpu):
+    gcf = numpy.ones((2, 2, 4, 4), dtype=complex)
+    a = numpy.zeros((16, 16), dtype=complex)
+    with pytest.raises(AssertionError):
+        G.convgrid(gcf, a, numpy.array([[0.0, -0.6]]), None, numpy.array([1.0 + 0j]))
+
+
+def test_non_square_kernel_grid_error(gpu):
+    gcf = numpy.ones((2, 2, 4, 6), dtype=complex)
+    with pytest.raises(ValueError):
+        G.convgrid(gcf, numpy.zeros((32, 32), dtype=complex), numpy.zeros((1, 2)), None,
+                   numpy.ones(1, dtype=complex))
+
+
+def test_empty_input(gpu):
+    gcf = numpy.ones((2, 2, 4, 4), dtype=complex)
+    a = numpy.zeros((16, 16), dtype=complex)
+    G.convgrid(gcf, a, numpy.zeros((0, 3)), None, numpy.zeros(0, dtype=complex))
+    assert (a == 0).all()
+    assert G.convdegrid(gcf, a, numpy.zeros((0, 3))).shape == (0,)
+
+
+def test_nearest_grid_degrid(gpu):
+    rng = numpy.random.default_rng(9)
+    N = 100
+    p = rng.uniform(-0.5, 0.49, (5000, 3))
+    v = rng.standard_normal(5000) + 1j * rng.standard_normal(5000)
+    a = numpy.zeros((N, N), dtype=complex)
+    ref = numpy.zeros((N, N), dtype=complex)
+    G.grid(a, p, v)
+    R.grid(ref, p, v)
+    assert_close(a, ref, what='nearest grid')
+    assert_close(G.degrid(ref, p), R.degrid(ref, p), what='nearest degrid')
+
+
+def test_make_grid_hermitian(gpu):
+    rng = numpy.random.default_rng(10)
+    for N in [2, 5, 8, 33, 64]:
+        g = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+        ref = R.make_grid_hermitian(g.copy())
+        got = G.make_grid_hermitian(g.copy())
+        assert_allclose(got, ref, rtol=1e-15, atol=1e-15)
+        t = torch.as_tensor(g, device=gpu).to(torch.complex64)
+        G.make_grid_hermitian(t)
+        assert_close(t.cpu().numpy(), ref, what='hermitian c64')
+
+
+def test_tiling_knobs_do_not_change_results(gpu):
+    rng = numpy.random.default_rng(12)
+    (gcf,), p, v = random_case(rng, 256, 12, 4, 20000)
+    ref = numpy.zeros((256, 256), dtype=complex)
+    R.convgrid(gcf, ref, p, None, v)
+    try:
+        for tile, chunk, batch in [(16, 64, 8), (48, 1000, 64), (80, 100000, 100000)]:
+            _lib.set_tuning(tile, chunk, batch)
+            a = numpy.zeros((256, 256), dtype=complex)
+            G.convgrid(gcf, a, p, None, v)
+            assert_close(a, ref, what='tile=%d chunk=%d batch=%d' % (tile, chunk, batch))
+            assert_close(G.convdegrid(gcf, ref, p), R.convdegrid(gcf, ref, p),
+                         what='degrid tile=%d' % tile)
+    finally:
+        _lib.set_tuning(0, 4096, 256)
