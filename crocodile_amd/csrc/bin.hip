@@ -82,7 +82,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_count(
     }
   }
   if (st) atomicOr(status, st);
-  __syncthreads();
+  barrier_after_lds_atomics();
   if (lds)
     for (int t = tid; t < ntiles; t += BIN_NT)
       if (hist[t]) atomicAdd(&count[t], hist[t]);

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
           }
         }
       }
-      __syncthreads();
+      barrier_after_lds_atomics();
 #pragma unroll
       for (int k = 0; k < PER; k++) {
         const int c = tid + k * NT;

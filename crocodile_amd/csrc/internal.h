@@ -76,8 +76,8 @@ void *ws_get(croc_ctx *ctx, WsSlot slot, size_t bytes);
 int ctx_enter(croc_ctx *ctx);
 
 // ---------------------------------------------------------------------------
-// FP64 coordinate arithmetic.  Written with explicit round-to-nearest
-// intrinsics so that no FMA contraction changes a rounding: the results must
+// FP64 coordinate arithmetic.  Every product and sum is rounded separately
+// (FMA contraction is switched off in these functions): the results must
 // equal NumPy's bit for bit, ties included.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int64_t floordiv64(int64_t a, int64_t b) {
@@ -88,16 +88,18 @@ __device__ __forceinline__ int64_t floordiv64(int64_t a, int64_t b) {
 
 // p = u / lam (IEEE division, or u itself when lam == 1).
 __device__ __forceinline__ double grid_fraction(double u, double lam) {
-  return lam == 1.0 ? u : __ddiv_rn(u, lam);
+#pragma clang fp contract(off)
+  return lam == 1.0 ? u : u / lam;
 }
 
 // synthesis.py:419-435: x = N//2 + p*N; fx = around(-x*Q) (half-even);
 // returns false when p is outside [-0.5, 0.5).
 __device__ __forceinline__ bool frac_coord_dev(int N, int Q, double p, int *ix,
                                                int *xf) {
+#pragma clang fp contract(off)
   if (!(p >= -0.5 && p < 0.5)) return false;
-  double x = __dadd_rn((double)(N / 2), __dmul_rn(p, (double)N));
-  double fx = rint(__dmul_rn(-x, (double)Q));
+  double x = (double)(N / 2) + p * (double)N;
+  double fx = rint(-x * (double)Q);
   int64_t f = (int64_t)fx;
   int64_t q = floordiv64(f, Q);
   *ix = (int)(-q);
@@ -107,7 +109,17 @@ __device__ __forceinline__ bool frac_coord_dev(int N, int Q, double p, int *ix,
 
 // numpy.round(w / wstep) (half-even).
 __device__ __forceinline__ int64_t wbin_dev(double w, double wstep) {
-  return (int64_t)rint(__ddiv_rn(w, wstep));
+#pragma clang fp contract(off)
+  return (int64_t)rint(w / wstep);
+}
+
+// Workgroup barrier after LDS atomics whose results are not used
+// (ds_add_f32 / ds_add_u32 without return).  The compiler does not wait for
+// such operations before s_barrier, so another wavefront could read a cell
+// before the atomic has landed; drain the LDS counter explicitly first.
+__device__ __forceinline__ void barrier_after_lds_atomics() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------

@@ -25,7 +25,8 @@ __global__ void k_frac_coords(const double *__restrict__ uvw, int64_t nv,
 // synthesis.py:399 / :414: N//2 + floor(0.5 + N p); a negative index wraps
 // once as NumPy indexing does, anything else outside the grid is an error.
 __device__ __forceinline__ bool nearest_cell(int N, double p, int *c) {
-  const double f = floor(__dadd_rn(0.5, __dmul_rn((double)N, p)));
+#pragma clang fp contract(off)
+  const double f = floor(0.5 + (double)N * p);
   int64_t k = (int64_t)(N / 2) + (int64_t)f;
   if (k < 0) k += N;
   if (k < 0 || k >= N) return false;

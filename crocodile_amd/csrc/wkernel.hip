@@ -47,29 +47,30 @@ struct FFArgs {
 // optional transform (l, m) <- (T00 l + T10 m, T01 l + T11 m) and shift.
 __global__ void k_wfarfield(FFArgs a, const double *__restrict__ w,
                             double2 *__restrict__ ff, uint32_t *status) {
+#pragma clang fp contract(off)
   const int F = a.F;
   const int p = blockIdx.y;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= F * F) return;
   const int mi = idx / F, li = idx - mi * F;
-  double l = __dmul_rn(__ddiv_rn((double)(li - F / 2), (double)F), a.theta);
-  double m = __dmul_rn(__ddiv_rn((double)(mi - F / 2), (double)F), a.theta);
+  double l = ((double)(li - F / 2) / (double)F) * a.theta;
+  double m = ((double)(mi - F / 2) / (double)F) * a.theta;
   if (a.has_T) {
-    const double l2 = __dadd_rn(__dmul_rn(a.T00, l), __dmul_rn(a.T10, m));
-    const double m2 = __dadd_rn(__dmul_rn(a.T01, l), __dmul_rn(a.T11, m));
+    const double l2 = a.T00 * l + a.T10 * m;
+    const double m2 = a.T01 * l + a.T11 * m;
     l = l2;
     m = m2;
   }
-  l = __dadd_rn(l, a.dl);
-  m = __dadd_rn(m, a.dm);
-  const double r2 = __dadd_rn(__dmul_rn(l, l), __dmul_rn(m, m));
+  l = (l + a.dl);
+  m = (m + a.dm);
+  const double r2 = l * l + m * m;
   if (!(r2 < 1.0)) {
     atomicOr(status, 1u);
     ff[(int64_t)p * F * F + idx] = make_double2(0.0, 0.0);
     return;
   }
-  const double ph = __dadd_rn(1.0, -sqrt(__dadd_rn(1.0, -r2)));
-  const double phase = __dmul_rn(__dmul_rn(2.0 * M_PI, w[p]), ph);
+  const double ph = 1.0 - sqrt(1.0 - r2);
+  const double phase = (2.0 * M_PI * w[p]) * ph;
   double s, c;
   sincos(phase, &s, &c);
   double2 v = make_double2(c, s);
