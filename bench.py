@@ -34,7 +34,7 @@ def parse():
     ap.add_argument('--cpu-sample', type=int, default=4000,
                     help='visibilities in the CPU-baseline sample (0 disables it)')
     ap.add_argument('--tile', type=int, default=0)
-    ap.add_argument('--chunk', type=int, default=4096)
+    ap.add_argument('--chunk', type=int, default=1024)
     ap.add_argument('--batch', type=int, default=256)
     return ap.parse_args()
 

@@ -115,7 +115,7 @@ def last_kernel_ms():
     return a.value, b.value
 
 
-def set_tuning(tile=0, chunk=4096, batch=256):
+def set_tuning(tile=0, chunk=1024, batch=256):
     check(load().croc_ctx_set_tuning(ctx(), tile, chunk, batch))
 
 

@@ -19,6 +19,37 @@ __device__ __forceinline__ float wave_sum(float x) {
   return x;
 }
 
+// Partial sums of one visibility over a block of up to 64*NQ taps starting
+// at t0; loads unpredicated (clamped index) and issued before first use.
+template <int NQ, bool SEPARABLE>
+__device__ __forceinline__ void gather_taps(
+    const float2 *sg, int base, int R, int S, int SS, float invS, int t0,
+    int lane, const float2 *k2, const float2 *ky, const float2 *kx, float &ar,
+    float &ai) {
+  float2 kv[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int t = min(t0 + q * 64 + lane, SS - 1);
+    if (!SEPARABLE) {
+      kv[q] = k2[t];
+    } else {
+      const int i = (int)(((float)t + 0.5f) * invS);
+      const float2 a = ky[i], b = kx[t - i * S];
+      kv[q] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int t = t0 + q * 64 + lane;
+    if (t < SS) {
+      const int i = (int)(((float)t + 0.5f) * invS);
+      const float2 g = sg[base + i * (R - S) + t];
+      ar += g.x * kv[q].x - g.y * kv[q].y;
+      ai += g.x * kv[q].y + g.y * kv[q].x;
+    }
+  }
+}
+
 template <int NT, bool SEPARABLE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_degrid_tiles(
     const float2 *__restrict__ grid, int N, int T, int ntx,
@@ -57,28 +88,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       const int SS = S * S;
       const float invS = 1.0f / (float)S;
       float ar = 0.f, ai = 0.f;
-      if (!SEPARABLE) {
-        const float2 *k = kern + plane_off[r.z] + (int64_t)(yf * Q + xf) * SS;
-        for (int t = lane; t < SS; t += 64) {
-          const int i = (int)(((float)t + 0.5f) * invS);
-          const int jj = t - i * S;
-          const float2 g = sg[base + i * R + jj];
-          const float2 kv = k[t];
-          ar += g.x * kv.x - g.y * kv.y;
-          ai += g.x * kv.y + g.y * kv.x;
-        }
+      const float2 *kp = kern + plane_off[r.z];
+      const float2 *k2 = kp + (int64_t)(yf * Q + xf) * SS;  // 2D table
+      const float2 *ky = kp + (int64_t)yf * S;               // separable rows
+      const float2 *kx = kp + (int64_t)xf * S;
+      const int nq = (SS + 63) >> 6;
+      if (nq <= 4) {
+        gather_taps<4, SEPARABLE>(sg, base, R, S, SS, invS, 0, lane, k2, ky, kx, ar, ai);
+      } else if (nq <= 8) {
+        gather_taps<8, SEPARABLE>(sg, base, R, S, SS, invS, 0, lane, k2, ky, kx, ar, ai);
       } else {
-        const float2 *ky = kern + plane_off[r.z] + (int64_t)yf * S;
-        const float2 *kx = kern + plane_off[r.z] + (int64_t)xf * S;
-        for (int t = lane; t < SS; t += 64) {
-          const int i = (int)(((float)t + 0.5f) * invS);
-          const int jj = t - i * S;
-          const float2 g = sg[base + i * R + jj];
-          const float2 a = ky[i], b = kx[jj];
-          const float kr = a.x * b.x - a.y * b.y, ki = a.x * b.y + a.y * b.x;
-          ar += g.x * kr - g.y * ki;
-          ai += g.x * ki + g.y * kr;
-        }
+        for (int t0 = 0; t0 < SS; t0 += 64 * 16)
+          gather_taps<16, SEPARABLE>(sg, base, R, S, SS, invS, t0, lane, k2, ky, kx, ar, ai);
       }
       ar = wave_sum(ar);
       ai = wave_sum(ai);

@@ -3,123 +3,202 @@
 //
 // Work item = up to `chunk` visibilities whose kernel windows start inside
 // one T x T grid tile.  A workgroup privatises the 95 x 95 cell region
-// covering every such window in LDS (planar re / im FP32 arrays), lets each
-// wavefront spread one visibility at a time over its S*S taps with LDS
-// float atomics (ds_add_f32), and every `batch` visibilities folds the LDS
-// partial sums into per-thread Kahan-compensated register accumulators
-// (each thread owns a fixed set of cells, so no two threads touch the same
-// accumulator).  At the end of the item the compensated sums are added to
-// the HBM grid with one float atomic per touched cell component.
-// Workgroups are persistent and pull items from a device queue.
+// covering every such window in LDS and lets each wavefront spread one
+// visibility at a time over its S*S taps.
+//
+// Accumulation is exact fixed point: every FP32 contribution k*v is scaled
+// by 2^e and rounded to a 64-bit integer, then added with integer atomics
+// (ds_add_u64 in LDS, global_atomic_add_x2 into an int64 workspace grid).
+// e is chosen on the device from max|v| * max|k| and the visibility count so
+// that no partial sum can overflow; with 2M visibilities it leaves ~40
+// fractional bits relative to the largest contribution.  Integer addition is
+// associative, so the result does not depend on scheduling: gridding is
+// bit-reproducible.  (LDS float atomics, ds_add_f32, run about ten times
+// slower than the integer ones on gfx950, which is what rules them out.)
+// A final pass adds workspace / 2^e into the caller's complex64 grid.
 #include "internal.h"
 
 namespace {
 
 constexpr int GRID_NT = 1024;
 
-__device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
-  const float y = x - c;
-  const float t = s + y;
-  c = (t - s) - y;
-  s = t;
+struct ScaleState {
+  unsigned int max_vis_bits;  // max |re|,|im| of the visibilities (float bits)
+  unsigned int max_k_bits;    // max |re|,|im| of the kernel bank (float bits)
+  double scale;               // 2^e
+  double inv_scale;           // 2^-e
+};
+
+__device__ __forceinline__ float cabs_max(float2 a) {
+  return fmaxf(fabsf(a.x), fabsf(a.y));
+}
+
+// Grid-stride maxima of the visibility and kernel component magnitudes over
+// every plane table of the bank.  Non-negative floats order like their bit
+// patterns, so atomicMax on the bits is a float max.
+__global__ void k_scale_maxima(const float2 *__restrict__ vis, int64_t nv,
+                               const float2 *__restrict__ kern,
+                               const int64_t *__restrict__ plane_off,
+                               const int32_t *__restrict__ plane_s, int nplanes,
+                               int Q, int separable, ScaleState *st) {
+  float mv = 0.f, mk = 0.f;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = i0; i < nv; i += stride) mv = fmaxf(mv, cabs_max(vis[i]));
+  for (int p = 0; p < nplanes; p++) {
+    const int64_t S = plane_s[p];
+    const int64_t n = separable ? Q * S : (int64_t)Q * Q * S * S;
+    const float2 *k = kern + plane_off[p];
+    for (int64_t i = i0; i < n; i += stride) mk = fmaxf(mk, cabs_max(k[i]));
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    mv = fmaxf(mv, __shfl_xor(mv, off, 64));
+    mk = fmaxf(mk, __shfl_xor(mk, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&st->max_vis_bits, __float_as_uint(mv));
+    atomicMax(&st->max_k_bits, __float_as_uint(mk));
+  }
+}
+
+// 2^e with (nv + 1) * 2 * max|v| * max|k|^(1 or 2) * 2^e < 2^62.
+__global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
+  const double mv = (double)__uint_as_float(st->max_vis_bits);
+  double mk = (double)__uint_as_float(st->max_k_bits);
+  if (separable) mk = mk * mk * 2.0;  // product of two table entries
+  const double bound = 2.0 * mv * mk * (double)(nv + 1);
+  int e = 40;
+  if (bound > 0.0 && isfinite(bound)) {
+    int ex;
+    frexp(bound, &ex);  // bound < 2^ex
+    e = 61 - ex;
+  }
+  e = max(-1000, min(e, 1000));
+  st->scale = ldexp(1.0, e);
+  st->inv_scale = ldexp(1.0, -e);
+}
+
+__device__ __forceinline__ unsigned long long fixed(float x, double scale) {
+  return (unsigned long long)__double2ll_rn((double)x * scale);
+}
+
+// Spread one visibility over a block of up to 64*NQ taps starting at t0.
+// All NQ loads are issued before the first use; indices past the table end
+// are clamped to its last tap (a valid address, value unused), so the loads
+// need no predication and the compiler issues them back to back.
+template <int NQ, bool SEPARABLE>
+__device__ __forceinline__ void spread_taps(
+    unsigned long long *sre, unsigned long long *sim, int base, int R, int S,
+    int SS, float invS, int t0, int lane, const float2 *k2, const float2 *ky,
+    const float2 *kx, float2 v, double scale) {
+  float2 kv[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int t = min(t0 + q * 64 + lane, SS - 1);
+    if (!SEPARABLE) {
+      kv[q] = k2[t];
+    } else {
+      const int i = (int)(((float)t + 0.5f) * invS);
+      const float2 a = ky[i], b = kx[t - i * S];
+      kv[q] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int t = t0 + q * 64 + lane;
+    if (t < SS) {
+      const int i = (int)(((float)t + 0.5f) * invS);
+      const int c = base + i * (R - S) + t;  // base + i*R + (t - i*S)
+      atomicAdd(&sre[c], fixed(v.x * kv[q].x - v.y * kv[q].y, scale));
+      atomicAdd(&sim[c], fixed(v.x * kv[q].y + v.y * kv[q].x, scale));
+    }
+  }
 }
 
 template <int NT, bool SEPARABLE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_grid_tiles(
-    float2 *__restrict__ grid, int N, int T, int ntx,
+__global__ __launch_bounds__(NT) void k_grid_tiles(
+    unsigned long long *__restrict__ acc, int N, int T, int ntx,
     const uint4 *__restrict__ rec, const float2 *__restrict__ svis,
     const float2 *__restrict__ kern, int Q,
     const int64_t *__restrict__ plane_off, const int32_t *__restrict__ plane_s,
     const int4 *__restrict__ items, const uint32_t *__restrict__ nitems,
-    uint32_t *__restrict__ qhead, int batch) {
-  constexpr int R = TILE_R, CELLS = R * R, PER = (CELLS + NT - 1) / NT;
+    uint32_t *__restrict__ qhead, const ScaleState *__restrict__ st) {
+  constexpr int R = TILE_R, CELLS = R * R;
   constexpr int NW = NT / 64;
-  __shared__ float sre[CELLS];
-  __shared__ float sim[CELLS];
+  __shared__ unsigned long long sre[CELLS];
+  __shared__ unsigned long long sim[CELLS];
   __shared__ int s_item;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nit = (int)*nitems;
-  float acc_re[PER], acc_im[PER], cmp_re[PER], cmp_im[PER];
+  const double scale = st->scale;
 
+  for (int c = tid; c < CELLS; c += NT) {
+    sre[c] = 0ull;
+    sim[c] = 0ull;
+  }
   for (;;) {
     if (tid == 0) s_item = (int)atomicAdd(qhead, 1u);
-    for (int c = tid; c < CELLS; c += NT) {
-      sre[c] = 0.f;
-      sim[c] = 0.f;
-    }
-#pragma unroll
-    for (int k = 0; k < PER; k++) acc_re[k] = acc_im[k] = cmp_re[k] = cmp_im[k] = 0.f;
     __syncthreads();
     const int it = s_item;
     if (it >= nit) break;
     const int4 item = items[it];
     const int tx0 = (item.x % ntx) * T, ty0 = (item.x / ntx) * T;
 
-    for (int b0 = item.y; b0 < item.z; b0 += batch) {
-      const int b1 = min(item.z, b0 + batch);
-      for (int j = b0 + wave; j < b1; j += NW) {
-        const uint4 r = rec[j];
-        const float2 v = svis[j];
-        const int S = plane_s[r.z];
-        const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
-        const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
-        const int SS = S * S;
-        const float invS = 1.0f / (float)S;
-        if (!SEPARABLE) {
-          const float2 *k = kern + plane_off[r.z] + (int64_t)(yf * Q + xf) * SS;
-          for (int t = lane; t < SS; t += 64) {
-            const int i = (int)(((float)t + 0.5f) * invS);
-            const int jj = t - i * S;
-            const float2 kv = k[t];
-            const int c = base + i * R + jj;
-            atomicAdd(&sre[c], v.x * kv.x - v.y * kv.y);
-            atomicAdd(&sim[c], v.x * kv.y + v.y * kv.x);
-          }
-        } else {
-          const float2 *ky = kern + plane_off[r.z] + (int64_t)yf * S;
-          const float2 *kx = kern + plane_off[r.z] + (int64_t)xf * S;
-          for (int t = lane; t < SS; t += 64) {
-            const int i = (int)(((float)t + 0.5f) * invS);
-            const int jj = t - i * S;
-            const float2 a = ky[i], b = kx[jj];
-            const float kr = a.x * b.x - a.y * b.y, ki = a.x * b.y + a.y * b.x;
-            const int c = base + i * R + jj;
-            atomicAdd(&sre[c], kr * v.x - ki * v.y);
-            atomicAdd(&sim[c], kr * v.y + ki * v.x);
-          }
-        }
+    for (int j = item.y + wave; j < item.z; j += NW) {
+      const uint4 r = rec[j];
+      const float2 v = svis[j];
+      const int S = plane_s[r.z];
+      const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+      const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
+      const int SS = S * S;
+      const float invS = 1.0f / (float)S;
+      const float2 *kp = kern + plane_off[r.z];
+      const float2 *k2 = kp + (int64_t)(yf * Q + xf) * SS;  // 2D table
+      const float2 *ky = kp + (int64_t)yf * S;               // separable rows
+      const float2 *kx = kp + (int64_t)xf * S;
+      if (SS <= 256) {
+        spread_taps<4, SEPARABLE>(sre, sim, base, R, S, SS, invS, 0, lane, k2, ky, kx, v, scale);
+      } else {
+        for (int t0 = 0; t0 < SS; t0 += 512)
+          spread_taps<8, SEPARABLE>(sre, sim, base, R, S, SS, invS, t0, lane, k2, ky, kx, v,
+                                    scale);
       }
-      barrier_after_lds_atomics();
-#pragma unroll
-      for (int k = 0; k < PER; k++) {
-        const int c = tid + k * NT;
-        if (c < CELLS) {
-          kahan_add(acc_re[k], cmp_re[k], sre[c]);
-          kahan_add(acc_im[k], cmp_im[k], sim[c]);
-          sre[c] = 0.f;
-          sim[c] = 0.f;
-        }
-      }
-      __syncthreads();
     }
-
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-      const int c = tid + k * NT;
-      if (c < CELLS) {
+    barrier_after_lds_atomics();
+    // Flush the region (touched cells only) and clear it for the next item.
+    for (int c = tid; c < CELLS; c += NT) {
+      const unsigned long long re = sre[c], im = sim[c];
+      if (re | im) {
         const int cy = c / R, cx = c - cy * R;
         const int gy = ty0 + cy, gx = tx0 + cx;
         if (gy < N && gx < N) {
-          const float re = acc_re[k] - cmp_re[k];
-          const float im = acc_im[k] - cmp_im[k];
-          float *g = (float *)(grid + (int64_t)gy * N + gx);
-          if (re != 0.f) unsafeAtomicAdd(g, re);
-          if (im != 0.f) unsafeAtomicAdd(g + 1, im);
+          unsigned long long *g = acc + 2 * ((int64_t)gy * N + gx);
+          if (re) atomicAdd(g, re);
+          if (im) atomicAdd(g + 1, im);
         }
+        sre[c] = 0ull;
+        sim[c] = 0ull;
       }
     }
     __syncthreads();
+  }
+}
+
+// grid[c] += acc[c] / 2^e, clearing nothing (the workspace is zeroed per call).
+__global__ void k_grid_finish(float2 *__restrict__ grid,
+                              const long long *__restrict__ acc, int64_t n,
+                              const ScaleState *__restrict__ st) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long re = acc[2 * i], im = acc[2 * i + 1];
+  if (re | im) {
+    const double s = st->inv_scale;
+    float2 g = grid[i];
+    g.x += (float)((double)re * s);
+    g.y += (float)((double)im * s);
+    grid[i] = g;
   }
 }
 
@@ -150,20 +229,37 @@ extern "C" int croc_convgrid(croc_ctx *ctx, void *grid, int N,
   BinPlan plan;
   rc = bin_visibilities(ctx, a, max_support, &plan);
   if (rc) return rc;
+
+  const int64_t cells = (int64_t)N * N;
+  unsigned long long *acc =
+      (unsigned long long *)ws_get(ctx, WS_ACC, sizeof(unsigned long long) * 2 * cells);
+  ScaleState *st = (ScaleState *)ws_get(ctx, WS_SCALE, sizeof(ScaleState));
+  if (!acc || !st) return CROC_ENOMEM;
+  rc = zero_u32_async(ctx, (uint32_t *)acc, 4 * cells);
+  if (rc) return rc;
+  rc = zero_u32_async(ctx, (uint32_t *)st, sizeof(ScaleState) / 4);
+  if (rc) return rc;
+  k_scale_maxima<<<4 * ctx->num_cu, 256, 0, ctx->stream>>>(
+      (const float2 *)vis, nv, (const float2 *)kern, plane_off, plane_s, nplanes, Q,
+      separable, st);
+  k_scale_pick<<<1, 1, 0, ctx->stream>>>(st, nv, separable);
+  CROC_HIP(hipGetLastError());
+
   const int G = persistent_blocks(ctx, 1);
   CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   if (separable)
     k_grid_tiles<GRID_NT, true><<<G, GRID_NT, 0, ctx->stream>>>(
-        (float2 *)grid, N, plan.T, plan.ntx, plan.rec, plan.svis,
-        (const float2 *)kern, Q, plane_off, plane_s, plan.items, plan.nitems,
-        plan.qhead, ctx->batch);
+        acc, N, plan.T, plan.ntx, plan.rec, plan.svis, (const float2 *)kern, Q,
+        plane_off, plane_s, plan.items, plan.nitems, plan.qhead, st);
   else
     k_grid_tiles<GRID_NT, false><<<G, GRID_NT, 0, ctx->stream>>>(
-        (float2 *)grid, N, plan.T, plan.ntx, plan.rec, plan.svis,
-        (const float2 *)kern, Q, plane_off, plane_s, plan.items, plan.nitems,
-        plan.qhead, ctx->batch);
+        acc, N, plan.T, plan.ntx, plan.rec, plan.svis, (const float2 *)kern, Q,
+        plane_off, plane_s, plan.items, plan.nitems, plan.qhead, st);
   CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  k_grid_finish<<<(unsigned)((cells + 255) / 256), 256, 0, ctx->stream>>>(
+      (float2 *)grid, (const long long *)acc, cells, st);
+  CROC_HIP(hipGetLastError());
   rc = read_status(ctx, plan);
   (void)hipEventElapsedTime(&ctx->grid_ms, ctx->ev[0], ctx->ev[1]);
   return rc;

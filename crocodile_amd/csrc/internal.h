@@ -52,6 +52,8 @@ enum WsSlot {
   WS_SMALL,     // status, item count, queue heads
   WS_FF,        // w-kernel far fields (double2)
   WS_TMP,       // oversampling intermediate (double2)
+  WS_ACC,       // int64 fixed-point gridding accumulator [N][N][2]
+  WS_SCALE,     // fixed-point scale state
   WS_COUNT
 };
 
@@ -65,7 +67,7 @@ struct croc_ctx {
   float grid_ms = 0.f, degrid_ms = 0.f;
   int num_cu = 256;
   int tile = 0;     // 0: automatic
-  int chunk = 4096; // visibilities per work item
+  int chunk = 1024; // visibilities per work item
   int batch = 256;  // visibilities per compensated flush
 };
 

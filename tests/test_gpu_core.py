@@ -326,4 +326,35 @@ def test_tiling_knobs_do_not_change_results(gpu):
             assert_close(G.convdegrid(gcf, ref, p), R.convdegrid(gcf, ref, p),
                          what='degrid tile=%d' % tile)
     finally:
-        _lib.set_tuning(0, 4096, 256)
+        _lib.set_tuning(0, 1024, 256)
+
+
+def test_gridding_is_bit_reproducible(gpu):
+    """Fixed-point accumulation is order independent: different tilings,
+    chunkings and repeated runs give bit-identical grids."""
+    rng = numpy.random.default_rng(13)
+    (gcf,), p, v = random_case(rng, 512, 24, 8, 60000)
+    grids = []
+    try:
+        for tile, chunk in [(0, 4096), (16, 64), (48, 999), (0, 4096)]:
+            _lib.set_tuning(tile, chunk, 256)
+            a = numpy.zeros((512, 512), dtype=complex)
+            G.convgrid(gcf, a, p, None, v)
+            grids.append(a)
+    finally:
+        _lib.set_tuning(0, 1024, 256)
+    for g in grids[1:]:
+        assert (g == grids[0]).all()
+
+
+def test_gridding_dynamic_range(gpu):
+    """Visibilities spanning 12 orders of magnitude keep FP32-level relative
+    accuracy on the grid (fixed-point scale chosen from the largest)."""
+    rng = numpy.random.default_rng(14)
+    (gcf,), p, v = random_case(rng, 256, 8, 4, 5000)
+    v = v * 10.0 ** rng.uniform(-12, 0, len(v))
+    ref = numpy.zeros((256, 256), dtype=complex)
+    cgrid.convgrid(cgrid.Bank([gcf]), ref, p, v)
+    a = numpy.zeros((256, 256), dtype=complex)
+    G.convgrid(gcf, a, p, None, v)
+    assert_close(a, ref, what='dynamic range')
