@@ -33,24 +33,26 @@ __device__ __forceinline__ float cabs_max(float2 a) {
   return fmaxf(fabsf(a.x), fabsf(a.y));
 }
 
-// Grid-stride maxima of the visibility and kernel component magnitudes over
-// every plane table of the bank.  Non-negative floats order like their bit
-// patterns, so atomicMax on the bits is a float max.
+// Grid-stride maxima of the visibility and kernel component magnitudes.
+// The bank is scanned as one extent [0, max_k(plane_off[k] + table size)),
+// which the C ABI requires to be a single allocation.  Non-negative floats
+// order like their bit patterns, so atomicMax on the bits is a float max.
 __global__ void k_scale_maxima(const float2 *__restrict__ vis, int64_t nv,
                                const float2 *__restrict__ kern,
                                const int64_t *__restrict__ plane_off,
                                const int32_t *__restrict__ plane_s, int nplanes,
                                int Q, int separable, ScaleState *st) {
+  int64_t nk = 0;
+  for (int p = 0; p < nplanes; p++) {
+    const int64_t S = plane_s[p];
+    const int64_t n = separable ? Q * S : (int64_t)Q * Q * S * S;
+    nk = max(nk, plane_off[p] + n);
+  }
   float mv = 0.f, mk = 0.f;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t i = i0; i < nv; i += stride) mv = fmaxf(mv, cabs_max(vis[i]));
-  for (int p = 0; p < nplanes; p++) {
-    const int64_t S = plane_s[p];
-    const int64_t n = separable ? Q * S : (int64_t)Q * Q * S * S;
-    const float2 *k = kern + plane_off[p];
-    for (int64_t i = i0; i < n; i += stride) mk = fmaxf(mk, cabs_max(k[i]));
-  }
+  for (int64_t i = i0; i < nk; i += stride) mk = fmaxf(mk, cabs_max(kern[i]));
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     mv = fmaxf(mv, __shfl_xor(mv, off, 64));

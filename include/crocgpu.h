@@ -15,8 +15,9 @@
  *   uvw   double [nv][stride], columns u, v, w (wavelengths); the grid
  *         coordinate is p = uvw / lam computed in IEEE FP64 as NumPy does
  *   grid  cf32 [N][N] row-major, rows = v, columns = u, centre at N/2
- *   kernel bank: all kernel tables concatenated; plane k starts at complex
- *         element plane_off[k] and has support plane_s[k] (square);
+ *   kernel bank: kernel tables in one allocation of at least
+ *         max_k(plane_off[k] + table size) complex elements; plane k starts
+ *         at element plane_off[k] and has support plane_s[k] (square);
  *         2D table layout [Q][Q][S][S] = [yf][xf][y][x]; separable table
  *         layout [Q][S] (the kernel is the outer product of rows yf and xf)
  *
