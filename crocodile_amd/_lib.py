@@ -42,6 +42,7 @@ SIGNATURES = {
     'croc_w_kernels': [_P, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P, _I],
     'croc_kernel_oversample': [_P, _P, _I, _I, _I, _I, _P],
     'croc_make_hermitian': [_P, _P, _I, _I],
+    'croc_dft2d_batched': [_P, _P, _P, _I, _I64, _I],
 }
 
 _lib = None

@@ -54,6 +54,10 @@ enum WsSlot {
   WS_TMP,       // oversampling intermediate (double2)
   WS_ACC,       // int64 fixed-point gridding accumulator [N][N][2]
   WS_SCALE,     // fixed-point scale state
+  WS_DFT_FWD,   // fp16-split centred DFT matrix, forward sign
+  WS_DFT_INV,   // fp16-split centred DFT matrix, inverse sign
+  WS_SUB,       // w-towers sub-grid stack
+  WS_TOWER,     // w-towers job / chunk tables
   WS_COUNT
 };
 
@@ -163,3 +167,11 @@ int bin_visibilities(croc_ctx *ctx, const BinArgs &a, int max_support,
 int read_status(croc_ctx *ctx, const BinPlan &plan);
 int persistent_blocks(croc_ctx *ctx, int per_cu);
 int zero_u32_async(croc_ctx *ctx, uint32_t *p, int64_t n);
+
+// Batched centred 2D DFT (dft.hip): out[b] = out_scale * D in[b] D with
+// optional per-pixel modulation exp(i sign k[b] phi[pixel]) of the input
+// and / or output.
+int dft2d_launch(croc_ctx *ctx, const void *in, void *out, int n, int64_t batch,
+                 int64_t batch_stride, int sign, float out_scale, const double *phi_in,
+                 const int32_t *k_in, int sign_in, const double *phi_out,
+                 const int32_t *k_out, int sign_out);

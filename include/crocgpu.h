@@ -126,6 +126,14 @@ int croc_w_kernels(croc_ctx *ctx, double theta, const double *w, int nw,
 int croc_kernel_oversample(croc_ctx *ctx, const void *ff, int n, int ndim,
                            int Q, int s, void *out);
 
+/* Batched centred 2D DFT of n x n complex64 sub-grids (n = 128):
+ * inverse = 0: numpy.fft.fftshift(fft2(ifftshift(x))) -- crocodile fft,
+ * synthesis.py:83-93; inverse = 1: the ifft counterpart with 1/n^2,
+ * synthesis.py:96-106.  Matrix-core (fp16x3 split) evaluation; in and out
+ * may alias.  The sub-grid transform of the w-towers drivers. */
+int croc_dft2d_batched(croc_ctx *ctx, const void *in, void *out, int n,
+                       int64_t batch, int inverse);
+
 /* make_grid_hermitian: replaces crocodile/synthesis.py:855-880 (in place,
  * [N][N], cf64 when is_double else cf32).
  * Native counterpart: examples/grid/grid.c:794-817. */
