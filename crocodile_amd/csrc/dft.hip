@@ -131,7 +131,8 @@ __global__ void k_dft_matrix(_Float16 *dsplit, int n, int sign) {
 }
 
 __global__ __launch_bounds__(DFT_NT) void k_dft2d(
-    const float2 *__restrict__ in, float2 *__restrict__ out, int64_t batch_stride,
+    const float2 *in, float2 *out, int64_t batch_stride,  // may alias (in place)
+    const int32_t *__restrict__ in_idx, const int32_t *__restrict__ out_idx,
     const _Float16 *__restrict__ dsplit, float out_scale, ModArgs mod_in,
     ModArgs mod_out) {
   constexpr int n = DFT_N, NN = n * n;
@@ -140,8 +141,8 @@ __global__ __launch_bounds__(DFT_NT) void k_dft2d(
   const int b = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const float2 *X = in + (int64_t)b * batch_stride;
-  float2 *Y = out + (int64_t)b * batch_stride;
+  const float2 *X = in + (int64_t)(in_idx ? in_idx[b] : b) * batch_stride;
+  float2 *Y = out + (int64_t)(out_idx ? out_idx[b] : b) * batch_stride;
   const _Float16 *Drh = dsplit, *Drl = dsplit + NN, *Dih = dsplit + 2 * NN,
                  *Dil = dsplit + 3 * NN;
   _Float16 *Lrh = lds, *Lrl = lds + PLANE, *Lih = lds + 2 * PLANE, *Lil = lds + 3 * PLANE;
@@ -247,14 +248,16 @@ int dft_tables(croc_ctx *ctx, int n, int sign, const _Float16 **out) {
 int dft2d_launch(croc_ctx *ctx, const void *in, void *out, int n, int64_t batch,
                  int64_t batch_stride, int sign, float out_scale, const double *phi_in,
                  const int32_t *k_in, int sign_in, const double *phi_out,
-                 const int32_t *k_out, int sign_out) {
+                 const int32_t *k_out, int sign_out, const int32_t *in_idx,
+                 const int32_t *out_idx) {
   const _Float16 *d = nullptr;
   int rc = dft_tables(ctx, n, sign, &d);
   if (rc) return rc;
   if (batch <= 0) return CROC_OK;
   ModArgs mi{phi_in, k_in, sign_in}, mo{phi_out, k_out, sign_out};
   k_dft2d<<<(unsigned)batch, DFT_NT, 0, ctx->stream>>>((const float2 *)in, (float2 *)out,
-                                                        batch_stride, d, out_scale, mi, mo);
+                                                        batch_stride, in_idx, out_idx, d,
+                                                        out_scale, mi, mo);
   CROC_HIP(hipGetLastError());
   return CROC_OK;
 }
@@ -267,7 +270,7 @@ extern "C" int croc_dft2d_batched(croc_ctx *ctx, const void *in, void *out, int 
   CROC_REQUIRE(batch >= 0 && batch <= 0x7fffffff, CROC_EINVAL, "bad batch");
   const float scale = inverse ? 1.0f / ((float)n * (float)n) : 1.0f;
   rc = dft2d_launch(ctx, in, out, n, batch, (int64_t)n * n, inverse ? +1 : -1, scale, nullptr,
-                    nullptr, 0, nullptr, nullptr, 0);
+                    nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr);
   if (rc) return rc;
   CROC_HIP(hipStreamSynchronize(ctx->stream));
   return CROC_OK;

@@ -168,10 +168,11 @@ int read_status(croc_ctx *ctx, const BinPlan &plan);
 int persistent_blocks(croc_ctx *ctx, int per_cu);
 int zero_u32_async(croc_ctx *ctx, uint32_t *p, int64_t n);
 
-// Batched centred 2D DFT (dft.hip): out[b] = out_scale * D in[b] D with
-// optional per-pixel modulation exp(i sign k[b] phi[pixel]) of the input
-// and / or output.
+// Batched centred 2D DFT (dft.hip): out[oi(b)] = out_scale * D in[ii(b)] D
+// with optional per-pixel modulation exp(i sign k[b] phi[pixel]) of the
+// input and / or output; ii / oi are in_idx / out_idx or the identity.
 int dft2d_launch(croc_ctx *ctx, const void *in, void *out, int n, int64_t batch,
                  int64_t batch_stride, int sign, float out_scale, const double *phi_in,
                  const int32_t *k_in, int sign_in, const double *phi_out,
-                 const int32_t *k_out, int sign_out);
+                 const int32_t *k_out, int sign_out, const int32_t *in_idx,
+                 const int32_t *out_idx);

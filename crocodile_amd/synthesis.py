@@ -870,3 +870,67 @@ def sample_image(image, x, y):
                       image[iy, ix + 1] * (dx * (1 - dy)),
                       image[iy + 1, ix] * ((1 - dx) * dy),
                       image[iy + 1, ix + 1] * (dx * dy)], axis=0)
+
+
+# --------------------------------------------------------------------------
+# W-towers drivers (examples/notebooks/wtowers.ipynb, wtowers-predict.ipynb,
+# examples/grid/grid.c:482-710) in the imgfn / predfn protocol
+# --------------------------------------------------------------------------
+
+
+def _tower_kernel(kv):
+    """(device table or None, Q, S, separable) for the sub-grid kernel."""
+    if kv is None:
+        return None, 1, 1, 0
+    t = _dev(kv, torch.complex64)
+    if t.ndim == 2:
+        Q, S = t.shape
+        return t.reshape(-1), int(Q), int(S), 1
+    if t.ndim == 4:
+        Q, Q2, gh, gw = t.shape
+        if gh != gw or Q != Q2:
+            raise NotImplementedError('w-towers need a square [Q,Q,S,S] kernel')
+        return t.reshape(-1), int(Q), int(gw), 0
+    raise AssertionError('unsupported kernel table rank')
+
+
+def w_towers_imaging(theta, lam, uvw, src, vis, wstep=100.0, subgrid_size=128, margin=32,
+                     kv=None):
+    """W-towers imaging.  Visibilities are binned into chunks of
+    subgrid_size - margin cells and w-planes round(w / wstep); every
+    (chunk, plane) is gridded on its own sub-grid (nearest cell when ``kv``
+    is None, as the notebook's simple_imaging, else convolution with the
+    oversampled kernel ``kv``), inverse transformed, multiplied by
+    exp(-2 pi i wp wstep (1 - n)), summed per chunk, transformed back and
+    added into the grid.  wtowers.ipynb imaging cell; grid.c:482-710."""
+    N = _grid_size(theta, lam)
+    uv = _uvw(uvw)
+    if uv.shape[1] < 3:
+        raise ValueError('w-towers need uvw with a w column')
+    kern, Q, S, sep = _tower_kernel(kv)
+    guv = torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
+    vt = _vis_dev(vis)
+    _lib.check(_lib.load().croc_wtowers_grid(
+        _lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uv), uv.shape[0], uv.shape[1], float(theta),
+        _lib.ptr(vt), float(wstep), int(subgrid_size), int(margin), _lib.ptr(kern), Q, S, sep))
+    return _out(guv, _torch_in(uvw, vis))
+
+
+def w_towers_predict(theta, lam, uvw, src, guv, wstep=100.0, subgrid_size=128, margin=32,
+                     kv=None):
+    """W-towers predict: per chunk, inverse transform of the grid window;
+    per w-plane, multiply by exp(+2 pi i wp wstep (1 - n)), transform and
+    degrid (nearest cell or ``kv``).  wtowers-predict.ipynb predict cell."""
+    N = _grid_size(theta, lam)
+    uv = _uvw(uvw)
+    if uv.shape[1] < 3:
+        raise ValueError('w-towers need uvw with a w column')
+    gt = _dev(guv, torch.complex64)
+    if _square(gt) != N:
+        raise ValueError('grid size does not match round(theta * lam)')
+    kern, Q, S, sep = _tower_kernel(kv)
+    out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
+    _lib.check(_lib.load().croc_wtowers_degrid(
+        _lib.ctx(), _lib.ptr(gt), N, _lib.ptr(uv), uv.shape[0], uv.shape[1], float(theta),
+        _lib.ptr(out), float(wstep), int(subgrid_size), int(margin), _lib.ptr(kern), Q, S, sep))
+    return _out(out, _torch_in(uvw, guv))

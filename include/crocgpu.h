@@ -134,6 +134,31 @@ int croc_kernel_oversample(croc_ctx *ctx, const void *ff, int n, int ndim,
 int croc_dft2d_batched(croc_ctx *ctx, const void *in, void *out, int n,
                        int64_t batch, int inverse);
 
+/* W-towers imaging: replaces the w-stacking-by-sub-grid drivers of
+ * examples/notebooks/wtowers.ipynb (imaging cell) and grid.c:482-710
+ * (grid_wtowers).  Visibilities are binned into (chunk, w-plane) jobs, chunk
+ * = subgrid - margin cells, w-plane = rint(w / wstep); each job is gridded
+ * on a subgrid x subgrid sub-grid (kern == NULL: nearest cell as
+ * simple_imaging; else the Q-oversampled kernel, separable [Q][S] or 2D
+ * [Q][Q][S][S]), inverse-transformed, multiplied by
+ * exp(-2 pi i wp wstep (1 - n)), summed per chunk, transformed back and
+ * added into grid (cf32 [N][N], in place).  subgrid must be 128, margin
+ * even, S <= margin <= subgrid/2.  uvw in wavelengths, stride >= 3. */
+int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double *uvw,
+                      int64_t nv, int64_t stride, double theta, const void *vis,
+                      double wstep, int subgrid, int margin, const void *kern,
+                      int Q, int S, int separable);
+
+/* W-towers predict: replaces examples/notebooks/wtowers-predict.ipynb
+ * (predict cell): per chunk, inverse transform of the grid window; per
+ * w-plane, multiply by exp(+2 pi i wp wstep (1 - n)), transform, degrid.
+ * vis_out cf32 [nv]. */
+int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N,
+                        const double *uvw, int64_t nv, int64_t stride,
+                        double theta, void *vis_out, double wstep, int subgrid,
+                        int margin, const void *kern, int Q, int S,
+                        int separable);
+
 /* make_grid_hermitian: replaces crocodile/synthesis.py:855-880 (in place,
  * [N][N], cf64 when is_double else cf32).
  * Native counterpart: examples/grid/grid.c:794-817. */
