@@ -43,6 +43,7 @@ SIGNATURES = {
     'croc_kernel_oversample': [_P, _P, _I, _I, _I, _I, _P],
     'croc_make_hermitian': [_P, _P, _I, _I],
     'croc_dft2d_batched': [_P, _P, _P, _I, _I64, _I],
+    'croc_convolve_same': [_P, _P, _I, _I, _P, _I, _I, _I64, _P],
     'croc_wtowers_grid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I],
     'croc_wtowers_degrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I],
 }
@@ -123,5 +124,12 @@ def set_tuning(tile=0, chunk=1024, batch=256):
 
 
 def ptr(t):
-    """Device address of a tensor (None stays NULL)."""
-    return None if t is None else _P(t.data_ptr())
+    """Device address of a tensor (None stays NULL).  Tensors carrying torch's
+    lazy conjugate / negative bits are refused: their memory does not hold
+    the values they represent."""
+    if t is None:
+        return None
+    if t.is_conj() or t.is_neg():
+        raise ValueError('pass a materialised tensor (resolve_conj / resolve_neg), '
+                         'not a lazy conjugate or negative view')
+    return _P(t.data_ptr())

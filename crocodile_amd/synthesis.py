@@ -38,7 +38,7 @@ def _dev(x, dtype):
     """Contiguous device tensor of ``dtype`` holding ``x``."""
     dev = _lib.device()
     if _is_torch(x):
-        return x.to(device=dev, dtype=dtype).contiguous()
+        return x.to(device=dev, dtype=dtype).resolve_conj().resolve_neg().contiguous()
     arr = numpy.asarray(x)
     if arr.dtype == object:
         arr = arr.astype(numpy.complex128 if dtype.is_complex else numpy.float64)
@@ -438,6 +438,9 @@ def kernel_bank(gcf, src, nv, for_grid):
 
 
 def _grid_call(fn, grid_t, N, uvw_t, lam, data_t, bank, mode, vplane, wstep=1.0, wbin0=0):
+    for t in (grid_t, uvw_t, data_t):
+        if not t.is_contiguous():
+            raise ValueError('device arrays must be contiguous')
     _lib.check(fn(_lib.ctx(), _lib.ptr(grid_t), int(N), _lib.ptr(uvw_t), uvw_t.shape[0],
                   uvw_t.shape[1], float(lam), _lib.ptr(data_t), _lib.ptr(bank.kern),
                   bank.Q, int(bank.separable), bank.nplanes, _lib.ptr(bank.plane_off),

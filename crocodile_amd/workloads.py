@@ -122,3 +122,40 @@ def antenna_layout(rng, nant, sigma_m=1500.0, arms_to_m=None):
         xy = numpy.concatenate([core, arms])
     z = rng.normal(0.0, 5.0, (nant, 1))
     return numpy.concatenate([xy, z], axis=1)
+
+
+def gaussian_blobs(rng, N, nblob=50, sigma=(1.0, 4.0)):
+    """float32 [N, N] image of Gaussian blobs with lognormal amplitudes,
+    centres in the inner 80% of the field."""
+    img = numpy.zeros((N, N), dtype=numpy.float64)
+    yy, xx = numpy.mgrid[0:N, 0:N]
+    for _ in range(nblob):
+        cx, cy = rng.uniform(0.1 * N, 0.9 * N, 2)
+        s = rng.uniform(*sigma)
+        a = rng.lognormal(0.0, 1.0)
+        r = int(5 * s) + 1
+        x0, x1 = max(int(cx) - r, 0), min(int(cx) + r + 1, N)
+        y0, y1 = max(int(cy) - r, 0), min(int(cy) + r + 1, N)
+        img[y0:y1, x0:x1] += a * numpy.exp(-((xx[y0:y1, x0:x1] - cx) ** 2 +
+                                            (yy[y0:y1, x0:x1] - cy) ** 2) / (2 * s * s))
+    return img.astype(numpy.float32)
+
+
+def diff_uvw(rng, nv, lam, wmax):
+    """uvw with surface density ~ r^-1.5 out to 0.45 lam, w uniform."""
+    R = 0.45 * lam
+    r = R * rng.uniform(0, 1, nv) ** 2
+    phi = rng.uniform(0, 2 * numpy.pi, nv)
+    return numpy.stack([r * numpy.cos(phi), r * numpy.sin(phi),
+                        rng.uniform(-wmax, wmax, nv)], axis=1)
+
+
+def diff1024(nv=None, N=None, seed=None):
+    """(image, uvw) of the differentiable-degridding workload."""
+    c = SPEC['diff1024']
+    rng = numpy.random.default_rng(c['seed'] if seed is None else seed)
+    N = c['N'] if N is None else N
+    lam = N / c['theta']
+    image = gaussian_blobs(rng, N, c['nblob'])
+    uvw = diff_uvw(rng, c['nv'] if nv is None else nv, lam, c['wmax'])
+    return image, uvw

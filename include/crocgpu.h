@@ -159,6 +159,13 @@ int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N,
                         int margin, const void *kern, int Q, int S,
                         int separable);
 
+/* 2D convolution, scipy.signal.convolve2d(a, b[n], mode='same') for nb
+ * kernels b sharing one a (cf64, [ah][aw] and [nb][bh][bw]; out
+ * [nb][ah][aw]), FP64 direct sums.  The building block of aw_kernel_fn,
+ * crocodile/synthesis.py:701-734 (:724 A1*A2, :729-731 per w sub-kernel). */
+int croc_convolve_same(croc_ctx *ctx, const void *a, int ah, int aw,
+                       const void *b, int bh, int bw, int64_t nb, void *out);
+
 /* make_grid_hermitian: replaces crocodile/synthesis.py:855-880 (in place,
  * [N][N], cf64 when is_double else cf32).
  * Native counterpart: examples/grid/grid.c:794-817. */

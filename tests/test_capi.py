@@ -60,3 +60,11 @@ def test_check_maps_reference_exceptions():
         _lib.check(_lib.EBOUNDS)
     with pytest.raises(_lib.CrocError):
         _lib.check(_lib.EHIP)
+
+
+def test_lazy_conjugate_views_are_refused():
+    import torch
+    t = torch.zeros(4, dtype=torch.complex64).conj()
+    with pytest.raises(ValueError):
+        _lib.ptr(t)
+    assert _lib.ptr(torch.conj_physical(torch.zeros(4, dtype=torch.complex64))) is not None
