@@ -32,11 +32,11 @@ def parse():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='wproj2048', choices=['wproj2048', 'plain512'])
-    ap.add_argument('--cpu-sample', type=int, default=4000,
-                    help='visibilities in the CPU-baseline sample (0 disables it)')
+    ap.add_argument('--cpu-sample', type=int, default=300_000,
+                    help='visibilities in the CPU-baseline sample, about 12 s of one '
+                         'core (0 disables it)')
     ap.add_argument('--tile', type=int, default=0)
     ap.add_argument('--chunk', type=int, default=1024)
-    ap.add_argument('--batch', type=int, default=256)
     return ap.parse_args()
 
 
@@ -143,7 +143,7 @@ def main():
     from crocodile_amd import _lib, workloads as W, synthesis as G
 
     dev = _lib.device()
-    _lib.set_tuning(args.tile, args.chunk, args.batch)
+    _lib.set_tuning(args.tile, args.chunk)
     print('[bench] building workload %s' % args.workload, file=sys.stderr, flush=True)
     if args.workload == 'wproj2048':
         wl = build_wproj(G, W, torch, dev)
@@ -240,7 +240,7 @@ def main():
         'metric': 'visibilities/s grid+degrid', 'value': round(value, 1),
         'unit': 'visibilities/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': round(ms_per_step, 4), 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'c64',
+        'vs_baseline': None, 'dtype': 'f32',
         'data': 'synthetic', 'config': config, 'roofline': roofline,
         'cpu_baseline': cpu, 'parity_vs_fp64_oracle': parity,
     }

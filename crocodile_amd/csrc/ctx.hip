@@ -114,16 +114,13 @@ extern "C" int croc_ctx_set_stream(croc_ctx *ctx, void *stream) {
   return CROC_OK;
 }
 
-extern "C" int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk,
-                                   int batch) {
+extern "C" int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk) {
   CROC_REQUIRE(ctx != nullptr, CROC_EINVAL, "null context");
   CROC_REQUIRE(tile == 0 || (tile >= 16 && tile <= 80 && tile % 16 == 0),
                CROC_EINVAL, "tile must be 0 or a multiple of 16 in [16, 80]");
   CROC_REQUIRE(chunk >= 64, CROC_EINVAL, "chunk must be >= 64");
-  CROC_REQUIRE(batch >= 8, CROC_EINVAL, "batch must be >= 8");
   ctx->tile = tile;
   ctx->chunk = chunk;
-  ctx->batch = batch;
   return CROC_OK;
 }
 

@@ -72,7 +72,6 @@ struct croc_ctx {
   int num_cu = 256;
   int tile = 0;     // 0: automatic
   int chunk = 1024; // visibilities per work item
-  int batch = 256;  // visibilities per compensated flush
 };
 
 // Grow-only workspace; returns nullptr on failure (error already set).

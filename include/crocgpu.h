@@ -58,9 +58,10 @@ const char *croc_last_error(void);
 int croc_ctx_create(int device, croc_ctx **out);
 int croc_ctx_destroy(croc_ctx *ctx);
 int croc_ctx_set_stream(croc_ctx *ctx, void *hip_stream);
-/* Tuning knobs: tile edge (0 = automatic), visibilities per work item,
- * visibilities per compensated flush batch. */
-int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk, int batch);
+/* Tuning knobs of convgrid / convdegrid: tile edge in grid cells (0 =
+ * automatic, else a multiple of 16 in [16, 80]) and the most visibilities
+ * one work item of a tile holds (>= 64).  Results do not depend on them. */
+int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk);
 /* Device time in milliseconds of the last gridding / degridding kernel
  * (HIP events around its launch on the context stream). */
 int croc_last_kernel_ms(croc_ctx *ctx, float *grid_ms, float *degrid_ms);

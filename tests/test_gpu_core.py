@@ -318,15 +318,15 @@ def test_tiling_knobs_do_not_change_results(gpu):
     ref = numpy.zeros((256, 256), dtype=complex)
     R.convgrid(gcf, ref, p, None, v)
     try:
-        for tile, chunk, batch in [(16, 64, 8), (48, 1000, 64), (80, 100000, 100000)]:
-            _lib.set_tuning(tile, chunk, batch)
+        for tile, chunk in [(16, 64), (48, 1000), (80, 100000)]:
+            _lib.set_tuning(tile, chunk)
             a = numpy.zeros((256, 256), dtype=complex)
             G.convgrid(gcf, a, p, None, v)
-            assert_close(a, ref, what='tile=%d chunk=%d batch=%d' % (tile, chunk, batch))
+            assert_close(a, ref, what='tile=%d chunk=%d' % (tile, chunk))
             assert_close(G.convdegrid(gcf, ref, p), R.convdegrid(gcf, ref, p),
                          what='degrid tile=%d' % tile)
     finally:
-        _lib.set_tuning(0, 1024, 256)
+        _lib.set_tuning(0, 1024)
 
 
 def test_gridding_is_bit_reproducible(gpu):
@@ -337,12 +337,12 @@ def test_gridding_is_bit_reproducible(gpu):
     grids = []
     try:
         for tile, chunk in [(0, 4096), (16, 64), (48, 999), (0, 4096)]:
-            _lib.set_tuning(tile, chunk, 256)
+            _lib.set_tuning(tile, chunk)
             a = numpy.zeros((512, 512), dtype=complex)
             G.convgrid(gcf, a, p, None, v)
             grids.append(a)
     finally:
-        _lib.set_tuning(0, 1024, 256)
+        _lib.set_tuning(0, 1024)
     for g in grids[1:]:
         assert (g == grids[0]).all()
 
