@@ -35,8 +35,12 @@ def parse():
     ap.add_argument('--cpu-sample', type=int, default=300_000,
                     help='visibilities in the CPU-baseline sample, about 12 s of one '
                          'core (0 disables it)')
+    ap.add_argument('--copy-probe-mib', type=int, default=0,
+                    help='before warm-up, copy a buffer of this many MiB once (a plain copy '
+                         'of known size, to scale HBM counters of a profiled run)')
     ap.add_argument('--tile', type=int, default=0)
     ap.add_argument('--chunk', type=int, default=1024)
+    ap.add_argument('--group-kib', type=int, default=0)
     return ap.parse_args()
 
 
@@ -143,7 +147,7 @@ def main():
     from crocodile_amd import _lib, workloads as W, synthesis as G
 
     dev = _lib.device()
-    _lib.set_tuning(args.tile, args.chunk)
+    _lib.set_tuning(args.tile, args.chunk, args.group_kib)
     print('[bench] building workload %s' % args.workload, file=sys.stderr, flush=True)
     if args.workload == 'wproj2048':
         wl = build_wproj(G, W, torch, dev)
@@ -174,6 +178,13 @@ def main():
     vis_d = torch.as_tensor(vis, device=dev).to(torch.complex64).contiguous()
     guv = torch.zeros((N, N), dtype=torch.complex64, device=dev)
     out = torch.empty(nv, dtype=torch.complex64, device=dev)
+
+    if args.copy_probe_mib > 0:
+        src = torch.full((args.copy_probe_mib << 18,), 1.0, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def step():
         guv.zero_()

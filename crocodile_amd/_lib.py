@@ -30,7 +30,7 @@ SIGNATURES = {
     'croc_ctx_create': [_I, ctypes.POINTER(_P)],
     'croc_ctx_destroy': [_P],
     'croc_ctx_set_stream': [_P, _P],
-    'croc_ctx_set_tuning': [_P, _I, _I],
+    'croc_ctx_set_tuning': [_P, _I, _I, _I],
     'croc_last_kernel_ms': [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     'croc_frac_coords': [_P, _P, _I64, _I64, _D, _I, _I, _I, _P, _P, _P, _P],
     'croc_convgrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _P, _I, _I, _I, _P, _P,
@@ -119,8 +119,8 @@ def last_kernel_ms():
     return a.value, b.value
 
 
-def set_tuning(tile=0, chunk=1024):
-    check(load().croc_ctx_set_tuning(ctx(), tile, chunk))
+def set_tuning(tile=0, chunk=1024, group_kib=0):
+    check(load().croc_ctx_set_tuning(ctx(), tile, chunk, group_kib))
 
 
 def ptr(t):

@@ -2,13 +2,15 @@
 //
 // Each visibility is located in FP64 exactly as crocodile's frac_coords does
 // (synthesis.py:419-449), its kernel plane is selected (make_gcf :451-461,
-// w-bin of w_cache_imaging :793), and it is assigned to the grid tile that
-// holds the first cell of its S x S window.  A counting sort then lays the
-// visibilities out tile by tile and cuts every tile into work items of at
-// most `chunk` visibilities, which the persistent tile kernels pull from a
-// queue.  Block-local LDS histograms keep global atomics to one per
-// (block, tile) pair, which matters for the strongly centre-peaked uv
-// densities of real arrays.
+// w-bin of w_cache_imaging :793) and resolved to the offset of its kernel
+// table, and it is assigned to the bin (tile, plane-group): the tile holds
+// the first cell of its S x S window, the plane-group is the L2-sized slice
+// of the kernel bank its table lies in.  A counting sort lays the records out
+// bin by bin (tile-major), and the scan cuts them into work items of at most
+// `chunk` visibilities for the persistent tile kernels, either per tile
+// (gridding) or per bin in group-major order (degridding).  Block-local LDS
+// histograms keep global atomics to one per (block, bin) pair, which matters
+// for the strongly centre-peaked uv densities of real arrays.
 #include "internal.h"
 
 namespace {
@@ -16,10 +18,15 @@ namespace {
 constexpr int BIN_NT = 1024;
 constexpr int BIN_PER = 4;
 constexpr int BIN_VPB = BIN_NT * BIN_PER;
-constexpr int HIST_MAX = 16384;
+constexpr int HIST_MAX = 32768;  // 128 KiB of LDS
 constexpr uint32_t INVALID = 0xffffffffu;
 
-__device__ __forceinline__ uint4 locate(const BinArgs &a, int T, int ntx,
+struct BinGeom {
+  int T, ntx, ngroups;
+  int64_t group_elems;
+};
+
+__device__ __forceinline__ uint4 locate(const BinArgs &a, const BinGeom &g,
                                         int64_t i, uint32_t *st) {
   const double *pu = a.uvw + i * a.stride;
   const double px = grid_fraction(pu[0], a.lam);
@@ -51,19 +58,29 @@ __device__ __forceinline__ uint4 locate(const BinArgs &a, int T, int ntx,
     *st |= ST_BOUNDS;
     return make_uint4(0, 0, 0, INVALID);
   }
-  const uint32_t tile = (uint32_t)((y0 / T) * ntx + x0 / T);
+  const int64_t poff = a.plane_off[plane];
+  const int64_t toff =
+      a.separable ? poff : poff + ((int64_t)yf * a.Q + xf) * S * S;
+  const int64_t tend = toff + (a.separable ? (int64_t)a.Q * S : (int64_t)S * S);
+  if (poff < 0 || tend > (int64_t)UINT32_MAX) {
+    *st |= ST_BANK;
+    return make_uint4(0, 0, 0, INVALID);
+  }
+  const int grp = (int)min((int64_t)g.ngroups - 1, toff / g.group_elems);
+  const uint32_t tile = (uint32_t)((y0 / g.T) * g.ntx + x0 / g.T);
   return make_uint4((uint32_t)x0 | ((uint32_t)y0 << 16),
-                    (uint32_t)xf | ((uint32_t)yf << 16), (uint32_t)plane, tile);
+                    (uint32_t)S | ((uint32_t)xf << 8) | ((uint32_t)yf << 20),
+                    (uint32_t)toff, tile * (uint32_t)g.ngroups + (uint32_t)grp);
 }
 
 __global__ __launch_bounds__(BIN_NT) void k_bin_count(
-    BinArgs a, int T, int ntx, int ntiles, uint4 *__restrict__ loc,
+    BinArgs a, BinGeom g, int nbins, uint4 *__restrict__ loc,
     uint32_t *__restrict__ count, uint32_t *__restrict__ status) {
   __shared__ uint32_t hist[HIST_MAX];
-  const bool lds = ntiles <= HIST_MAX;
+  const bool lds = nbins <= HIST_MAX;
   const int tid = threadIdx.x;
   if (lds)
-    for (int t = tid; t < ntiles; t += BIN_NT) hist[t] = 0;
+    for (int t = tid; t < nbins; t += BIN_NT) hist[t] = 0;
   __syncthreads();
   uint32_t st = 0;
   const int64_t base = (int64_t)blockIdx.x * BIN_VPB;
@@ -71,7 +88,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_count(
   for (int k = 0; k < BIN_PER; k++) {
     const int64_t i = base + k * BIN_NT + tid;
     if (i < a.nv) {
-      const uint4 L = locate(a, T, ntx, i, &st);
+      const uint4 L = locate(a, g, i, &st);
       loc[i] = L;
       if (L.w != INVALID) {
         if (lds)
@@ -84,60 +101,94 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_count(
   if (st) atomicOr(status, st);
   barrier_after_lds_atomics();
   if (lds)
-    for (int t = tid; t < ntiles; t += BIN_NT)
+    for (int t = tid; t < nbins; t += BIN_NT)
       if (hist[t]) atomicAdd(&count[t], hist[t]);
 }
 
-// One block: exclusive scans of visibility counts and of work-item counts,
-// then the item list.  An error status empties the item list so that no
-// output is modified.
-__global__ __launch_bounds__(BIN_NT) void k_bin_scan(
-    int ntiles, int chunk, const uint32_t *__restrict__ count,
-    uint32_t *__restrict__ offset, uint32_t *__restrict__ cursor,
-    int4 *__restrict__ items, uint32_t *__restrict__ nitems,
-    const uint32_t *__restrict__ status, uint32_t *__restrict__ qheads) {
-  __shared__ uint32_t sc[BIN_NT], si[BIN_NT];
+// Block-wide exclusive scan of one value per thread (Hillis-Steele in LDS);
+// returns the thread's exclusive prefix.
+__device__ uint32_t block_exclusive_scan(uint32_t x, uint32_t *sc) {
   const int tid = threadIdx.x;
-  const int per = (ntiles + BIN_NT - 1) / BIN_NT;
-  const int t0 = min(ntiles, tid * per), t1 = min(ntiles, t0 + per);
-  uint32_t cs = 0, is = 0;
-  for (int t = t0; t < t1; t++) {
-    const uint32_t c = count[t];
-    cs += c;
-    is += (c + chunk - 1) / chunk;
-  }
-  sc[tid] = cs;
-  si[tid] = is;
+  sc[tid] = x;
   __syncthreads();
   for (int off = 1; off < BIN_NT; off <<= 1) {
-    uint32_t a = 0, b = 0;
-    if (tid >= off) {
-      a = sc[tid - off];
-      b = si[tid - off];
-    }
+    const uint32_t a = (tid >= off) ? sc[tid - off] : 0u;
     __syncthreads();
     sc[tid] += a;
-    si[tid] += b;
     __syncthreads();
   }
-  uint32_t co = sc[tid] - cs, io = si[tid] - is;
-  for (int t = t0; t < t1; t++) {
-    const uint32_t c = count[t];
-    offset[t] = co;
-    cursor[t] = co;
-    for (uint32_t s = 0; s < c; s += chunk)
-      items[io++] = make_int4(t, (int)(co + s), (int)(co + min(c, s + chunk)), 0);
-    co += c;
+  const uint32_t r = sc[tid] - x;
+  __syncthreads();
+  return r;
+}
+
+// One block: exclusive scan of the bin counts (record offsets, tile-major),
+// then the work-item list in the requested order.  An error status empties
+// the item list so that no output is modified.
+__global__ __launch_bounds__(BIN_NT) void k_bin_scan(
+    int ntiles, int ngroups, int chunk, int order,
+    const uint32_t *__restrict__ count, uint32_t *__restrict__ offset,
+    uint32_t *__restrict__ cursor, int4 *__restrict__ items,
+    uint32_t *__restrict__ nitems, const uint32_t *__restrict__ status,
+    uint32_t *__restrict__ qheads) {
+  __shared__ uint32_t sc[BIN_NT];
+  const int tid = threadIdx.x;
+  const int nbins = ntiles * ngroups;
+
+  // record offsets, bins in tile-major order
+  const int per = (nbins + BIN_NT - 1) / BIN_NT;
+  const int b0 = min(nbins, tid * per), b1 = min(nbins, b0 + per);
+  uint32_t cs = 0;
+  for (int b = b0; b < b1; b++) cs += count[b];
+  uint32_t co = block_exclusive_scan(cs, sc);
+  for (int b = b0; b < b1; b++) {
+    offset[b] = co;
+    cursor[b] = co;
+    co += count[b];
+  }
+  if (tid == BIN_NT - 1) offset[nbins] = co;
+  __syncthreads();  // offsets visible to the whole block
+
+  uint32_t io;
+  if (order == ITEMS_BY_TILE) {
+    const int tper = (ntiles + BIN_NT - 1) / BIN_NT;
+    const int t0 = min(ntiles, tid * tper), t1 = min(ntiles, t0 + tper);
+    uint32_t is = 0;
+    for (int t = t0; t < t1; t++) {
+      const uint32_t c = offset[(t + 1) * ngroups] - offset[t * ngroups];
+      is += (c + chunk - 1) / chunk;
+    }
+    io = block_exclusive_scan(is, sc);
+    for (int t = t0; t < t1; t++) {
+      const uint32_t s0 = offset[t * ngroups], s1 = offset[(t + 1) * ngroups];
+      for (uint32_t s = s0; s < s1; s += chunk)
+        items[io++] = make_int4(t, (int)s, (int)min(s1, s + chunk), 0);
+    }
+  } else {
+    // group-major: index q = g * ntiles + t
+    const int q0 = min(nbins, tid * per), q1 = min(nbins, q0 + per);
+    uint32_t is = 0;
+    for (int q = q0; q < q1; q++) {
+      const int g = q / ntiles, t = q - g * ntiles;
+      is += (count[t * ngroups + g] + chunk - 1) / chunk;
+    }
+    io = block_exclusive_scan(is, sc);
+    for (int q = q0; q < q1; q++) {
+      const int g = q / ntiles, t = q - g * ntiles;
+      const int b = t * ngroups + g;
+      const uint32_t s0 = offset[b], s1 = s0 + count[b];
+      for (uint32_t s = s0; s < s1; s += chunk)
+        items[io++] = make_int4(t, (int)s, (int)min(s1, s + chunk), 0);
+    }
   }
   if (tid == BIN_NT - 1) {
-    offset[ntiles] = co;
     *nitems = (*status != 0) ? 0u : io;
     for (int q = 0; q < 8; q++) qheads[q] = 0;
   }
 }
 
 __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
-    int64_t nv, int ntiles, const uint4 *__restrict__ loc,
+    int64_t nv, int nbins, const uint4 *__restrict__ loc,
     uint32_t *__restrict__ cursor, uint4 *__restrict__ rec,
     const float2 *__restrict__ vis, float2 *__restrict__ svis) {
   __shared__ uint32_t hist[HIST_MAX];
@@ -145,8 +196,8 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
   const int64_t base = (int64_t)blockIdx.x * BIN_VPB;
   uint4 L[BIN_PER];
   uint32_t rank[BIN_PER];
-  if (ntiles <= HIST_MAX) {
-    for (int t = tid; t < ntiles; t += BIN_NT) hist[t] = 0;
+  if (nbins <= HIST_MAX) {
+    for (int t = tid; t < nbins; t += BIN_NT) hist[t] = 0;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < BIN_PER; k++) {
@@ -155,7 +206,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
       rank[k] = (L[k].w != INVALID) ? atomicAdd(&hist[L[k].w], 1u) : 0u;
     }
     __syncthreads();
-    for (int t = tid; t < ntiles; t += BIN_NT)
+    for (int t = tid; t < nbins; t += BIN_NT)
       if (hist[t]) hist[t] = atomicAdd(&cursor[t], hist[t]);
     __syncthreads();
 #pragma unroll
@@ -189,13 +240,13 @@ int choose_tile(croc_ctx *ctx, int max_support) {
 }
 
 int bin_visibilities(croc_ctx *ctx, const BinArgs &a, int max_support,
-                     BinPlan *plan) {
+                     int order, BinPlan *plan) {
   CROC_REQUIRE(max_support >= 1 && max_support <= 64, CROC_EINVAL,
                "kernel support %d outside [1, 64]", max_support);
   CROC_REQUIRE(a.N >= 1 && a.N <= 65535, CROC_EINVAL, "grid size %d outside "
                "[1, 65535]", a.N);
-  CROC_REQUIRE(a.Q >= 1 && a.Q <= 65535, CROC_EINVAL, "oversampling %d outside"
-               " [1, 65535]", a.Q);
+  CROC_REQUIRE(a.Q >= 1 && a.Q <= 4096, CROC_EINVAL, "oversampling %d outside"
+               " [1, 4096]", a.Q);
   CROC_REQUIRE(a.nv >= 0 && a.nv < (int64_t)INT32_MAX, CROC_EINVAL,
                "visibility count %lld outside [0, 2^31)", (long long)a.nv);
   CROC_REQUIRE(a.nplanes >= 1, CROC_EINVAL, "empty kernel bank");
@@ -210,38 +261,54 @@ int bin_visibilities(croc_ctx *ctx, const BinArgs &a, int max_support,
   p.nty = p.ntx;
   p.ntiles = p.ntx * p.nty;
   p.chunk = ctx->chunk;
-  p.max_items = a.nv / p.chunk + p.ntiles + 1;
+
+  // Plane-groups: slices of group_kib KiB of an upper bound of the bank's
+  // extent, as many as keep the bin histogram in LDS (at most 32).
+  const int64_t S = max_support;
+  const int64_t bound = (int64_t)a.nplanes * (a.separable ? a.Q * S : (int64_t)a.Q * a.Q * S * S);
+  int64_t ng = 1;
+  if (ctx->group_kib > 0) {
+    const int64_t gel = (int64_t)ctx->group_kib * 1024 / 8;
+    ng = (bound + gel - 1) / gel;
+    ng = std::min<int64_t>(ng, 32);
+    ng = std::min<int64_t>(ng, std::max<int64_t>(1, HIST_MAX / p.ntiles));
+    ng = std::max<int64_t>(ng, 1);
+  }
+  p.ngroups = (int)ng;
+  p.group_elems = std::max<int64_t>(1, (bound + ng - 1) / ng);
+  p.nbins = p.ntiles * p.ngroups;
+  p.max_items = a.nv / p.chunk + p.nbins + 1;
 
   p.loc = (uint4 *)ws_get(ctx, WS_LOC, sizeof(uint4) * a.nv);
   p.rec = (uint4 *)ws_get(ctx, WS_REC, sizeof(uint4) * a.nv);
   if (a.vis) p.svis = (float2 *)ws_get(ctx, WS_SVIS, sizeof(float2) * a.nv);
   uint32_t *tiles = (uint32_t *)ws_get(ctx, WS_TILES,
-                                       sizeof(uint32_t) * (3 * p.ntiles + 1));
+                                       sizeof(uint32_t) * (3 * (size_t)p.nbins + 1));
   p.items = (int4 *)ws_get(ctx, WS_ITEMS, sizeof(int4) * p.max_items);
   uint32_t *small = (uint32_t *)ws_get(ctx, WS_SMALL, 256);
   if (!p.loc || !p.rec || (a.vis && !p.svis) || !tiles || !p.items || !small)
     return CROC_ENOMEM;
   p.count = tiles;
-  p.offset = tiles + p.ntiles;
-  p.cursor = tiles + 2 * p.ntiles + 1;
+  p.offset = tiles + p.nbins;
+  p.cursor = tiles + 2 * p.nbins + 1;
   p.status = small;
   p.nitems = small + 1;
   p.qhead = small + 8;  // 8 queue heads, one per XCD
 
   hipStream_t s = ctx->stream;
-  int rc = zero_u32_async(ctx, p.count, p.ntiles);
+  int rc = zero_u32_async(ctx, p.count, p.nbins);
   if (rc) return rc;
   rc = zero_u32_async(ctx, small, 16);
   if (rc) return rc;
+  const BinGeom g{T, p.ntx, p.ngroups, p.group_elems};
   const int nblk = (int)((a.nv + BIN_VPB - 1) / BIN_VPB);
   if (nblk > 0)
-    k_bin_count<<<nblk, BIN_NT, 0, s>>>(a, T, p.ntx, p.ntiles, p.loc, p.count,
-                                        p.status);
-  k_bin_scan<<<1, BIN_NT, 0, s>>>(p.ntiles, p.chunk, p.count, p.offset,
-                                  p.cursor, p.items, p.nitems, p.status,
-                                  p.qhead);
+    k_bin_count<<<nblk, BIN_NT, 0, s>>>(a, g, p.nbins, p.loc, p.count, p.status);
+  k_bin_scan<<<1, BIN_NT, 0, s>>>(p.ntiles, p.ngroups, p.chunk, order, p.count,
+                                  p.offset, p.cursor, p.items, p.nitems,
+                                  p.status, p.qhead);
   if (nblk > 0)
-    k_bin_scatter<<<nblk, BIN_NT, 0, s>>>(a.nv, p.ntiles, p.loc, p.cursor,
+    k_bin_scatter<<<nblk, BIN_NT, 0, s>>>(a.nv, p.nbins, p.loc, p.cursor,
                                           p.rec, a.vis, p.svis);
   CROC_HIP(hipGetLastError());
   return CROC_OK;

@@ -26,6 +26,10 @@ int croc_status_to_code(uint32_t st) {
     croc_set_error("w-bin / plane index outside the kernel bank");
     return CROC_EPLANE;
   }
+  if (st & ST_BANK) {
+    croc_set_error("kernel bank larger than 2^32 complex elements");
+    return CROC_EINVAL;
+  }
   if (st & ST_SUPPORT) {
     croc_set_error("a kernel plane's support exceeds max_support");
     return CROC_EINVAL;
@@ -114,13 +118,16 @@ extern "C" int croc_ctx_set_stream(croc_ctx *ctx, void *stream) {
   return CROC_OK;
 }
 
-extern "C" int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk) {
+extern "C" int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk,
+                                   int group_kib) {
   CROC_REQUIRE(ctx != nullptr, CROC_EINVAL, "null context");
   CROC_REQUIRE(tile == 0 || (tile >= 16 && tile <= 80 && tile % 16 == 0),
                CROC_EINVAL, "tile must be 0 or a multiple of 16 in [16, 80]");
   CROC_REQUIRE(chunk >= 64, CROC_EINVAL, "chunk must be >= 64");
+  CROC_REQUIRE(group_kib >= 0, CROC_EINVAL, "group_kib must be >= 0");
   ctx->tile = tile;
   ctx->chunk = chunk;
+  ctx->group_kib = group_kib;
   return CROC_OK;
 }
 

@@ -1,12 +1,14 @@
 // Convolutional degridding: crocodile/synthesis.py:484-503 (convdegrid) on
 // CDNA4.
 //
-// Same tile work items as gridding.  A workgroup stages the 95 x 95 cell
-// grid region of its tile in LDS once (coalesced row reads from HBM), then
+// Work item = up to `chunk` visibilities of one (tile, plane-group) bin;
+// items are queued group-major, so at any time the whole chip reads kernel
+// taps from one L2-sized slice of the kernel bank.  A workgroup stages the
+// 95 x 95 cell grid region of its tile in LDS (coalesced row reads), then
 // each wavefront takes one visibility at a time: lanes walk the S*S taps,
-// gather grid cells from LDS and kernel taps from the (cache-resident)
-// kernel bank, accumulate complex products in FP32 and the wavefront
-// reduces the 64 partial sums with cross-lane shuffles.
+// gather grid cells from LDS and kernel taps from the kernel bank,
+// accumulate complex products in FP32 and the wavefront reduces the 64
+// partial sums with cross-lane shuffles.
 #include "internal.h"
 
 namespace {
@@ -19,13 +21,13 @@ __device__ __forceinline__ float wave_sum(float x) {
   return x;
 }
 
-// Partial sums of one visibility over a block of up to 64*NQ taps starting
-// at t0; loads unpredicated (clamped index) and issued before first use.
+// Partial sums of one visibility over a block of up to 64*NQ taps t0 .. of
+// its S x S window (tap t = row i, column t - i*S); loads unpredicated
+// (clamped index) and issued before first use.
 template <int NQ, bool SEPARABLE>
 __device__ __forceinline__ void gather_taps(
-    const float2 *sg, int base, int R, int S, int SS, float invS, int t0,
-    int lane, const float2 *k2, const float2 *ky, const float2 *kx, float &ar,
-    float &ai) {
+    const float2 *sg, int base, int S, int SS, float invS, int t0, int lane,
+    const float2 *k2, const float2 *ky, const float2 *kx, float &ar, float &ai) {
   float2 kv[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
@@ -34,8 +36,7 @@ __device__ __forceinline__ void gather_taps(
       kv[q] = k2[t];
     } else {
       const int i = (int)(((float)t + 0.5f) * invS);
-      const float2 a = ky[i], b = kx[t - i * S];
-      kv[q] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+      kv[q] = cmul(ky[i], kx[t - i * S]);
     }
   }
 #pragma unroll
@@ -43,7 +44,7 @@ __device__ __forceinline__ void gather_taps(
     const int t = t0 + q * 64 + lane;
     if (t < SS) {
       const int i = (int)(((float)t + 0.5f) * invS);
-      const float2 g = sg[base + i * (R - S) + t];
+      const float2 g = sg[base + i * (TILE_R - S) + t];
       ar += g.x * kv[q].x - g.y * kv[q].y;
       ai += g.x * kv[q].y + g.y * kv[q].x;
     }
@@ -54,10 +55,8 @@ template <int NT, bool SEPARABLE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_degrid_tiles(
     const float2 *__restrict__ grid, int N, int T, int ntx,
     const uint4 *__restrict__ rec, float2 *__restrict__ vis_out,
-    const float2 *__restrict__ kern, int Q,
-    const int64_t *__restrict__ plane_off, const int32_t *__restrict__ plane_s,
-    const int4 *__restrict__ items, const uint32_t *__restrict__ nitems,
-    uint32_t *__restrict__ qhead) {
+    const float2 *__restrict__ kern, const int4 *__restrict__ items,
+    const uint32_t *__restrict__ nitems, uint32_t *__restrict__ qhead) {
   constexpr int R = TILE_R, CELLS = R * R;
   constexpr int NW = NT / 64;
   __shared__ float2 sg[CELLS];
@@ -72,7 +71,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int it = s_item;
     if (it >= nit) break;
     const int4 item = items[it];
-    const int tx0 = (item.x % ntx) * T, ty0 = (item.x / ntx) * T;
+    const int tile = __builtin_amdgcn_readfirstlane(item.x);
+    const int tx0 = (tile % ntx) * T, ty0 = (tile / ntx) * T;
     for (int c = tid; c < CELLS; c += NT) {
       const int cy = c / R, cx = c - cy * R;
       const int gy = ty0 + cy, gx = tx0 + cx;
@@ -80,26 +80,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     __syncthreads();
 
-    for (int j = item.y + wave; j < item.z; j += NW) {
-      const uint4 r = rec[j];
-      const int S = plane_s[r.z];
-      const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+    // Each wave walks its visibilities (j = first + wave, + NW, ...) with
+    // the next record already in flight.
+    const int jend = __builtin_amdgcn_readfirstlane(item.z);
+    int j = __builtin_amdgcn_readfirstlane(item.y) + wave;
+    uint4 rn = make_uint4(0, 0, 0, 0);
+    if (j < jend) rn = rec[j];
+    for (; j < jend; j += NW) {
+      const uint4 r = uniform_u4(rn);
+      if (j + NW < jend) rn = rec[j + NW];
+      const int S = (int)(r.y & 0xffu);
+      const int xf = (int)((r.y >> 8) & 0xfffu), yf = (int)(r.y >> 20);
       const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
       const int SS = S * S;
       const float invS = 1.0f / (float)S;
+      const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
+      const float2 *ky = kp + yf * S, *kx = kp + xf * S;
       float ar = 0.f, ai = 0.f;
-      const float2 *kp = kern + plane_off[r.z];
-      const float2 *k2 = kp + (int64_t)(yf * Q + xf) * SS;  // 2D table
-      const float2 *ky = kp + (int64_t)yf * S;               // separable rows
-      const float2 *kx = kp + (int64_t)xf * S;
       const int nq = (SS + 63) >> 6;
       if (nq <= 4) {
-        gather_taps<4, SEPARABLE>(sg, base, R, S, SS, invS, 0, lane, k2, ky, kx, ar, ai);
+        gather_taps<4, SEPARABLE>(sg, base, S, SS, invS, 0, lane, kp, ky, kx, ar, ai);
       } else if (nq <= 8) {
-        gather_taps<8, SEPARABLE>(sg, base, R, S, SS, invS, 0, lane, k2, ky, kx, ar, ai);
+        gather_taps<8, SEPARABLE>(sg, base, S, SS, invS, 0, lane, kp, ky, kx, ar, ai);
       } else {
         for (int t0 = 0; t0 < SS; t0 += 64 * 16)
-          gather_taps<16, SEPARABLE>(sg, base, R, S, SS, invS, t0, lane, k2, ky, kx, ar, ai);
+          gather_taps<16, SEPARABLE>(sg, base, S, SS, invS, t0, lane, kp, ky, kx, ar, ai);
       }
       ar = wave_sum(ar);
       ai = wave_sum(ai);
@@ -132,23 +137,21 @@ extern "C" int croc_convdegrid(croc_ctx *ctx, const void *grid, int N,
   CROC_REQUIRE(lam > 0, CROC_EINVAL, "lam must be positive");
   if (nv == 0) return CROC_OK;
 
-  BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_mode, vplane,
-            wstep, wbin0, nullptr, max_support};
+  BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_off, separable,
+            plane_mode, vplane, wstep, wbin0, nullptr, max_support};
   BinPlan plan;
-  rc = bin_visibilities(ctx, a, max_support, &plan);
+  rc = bin_visibilities(ctx, a, max_support, ITEMS_BY_GROUP, &plan);
   if (rc) return rc;
   const int G = persistent_blocks(ctx, 2);
   CROC_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
   if (separable)
     k_degrid_tiles<DEGRID_NT, true><<<G, DEGRID_NT, 0, ctx->stream>>>(
         (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
-        (const float2 *)kern, Q, plane_off, plane_s, plan.items, plan.nitems,
-        plan.qhead);
+        (const float2 *)kern, plan.items, plan.nitems, plan.qhead);
   else
     k_degrid_tiles<DEGRID_NT, false><<<G, DEGRID_NT, 0, ctx->stream>>>(
         (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
-        (const float2 *)kern, Q, plane_off, plane_s, plan.items, plan.nitems,
-        plan.qhead);
+        (const float2 *)kern, plan.items, plan.nitems, plan.qhead);
   CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
   rc = read_status(ctx, plan);

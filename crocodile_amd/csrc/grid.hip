@@ -10,8 +10,9 @@
 // by 2^e and rounded to a 64-bit integer, then added with integer atomics
 // (ds_add_u64 in LDS, global_atomic_add_x2 into an int64 workspace grid).
 // e is chosen on the device from max|v| * max|k| and the visibility count so
-// that no partial sum can overflow; with 2M visibilities it leaves ~40
-// fractional bits relative to the largest contribution.  Integer addition is
+// that no partial sum can overflow and every single contribution stays below
+// 2^50; with 2M visibilities it leaves ~40 fractional bits relative to the
+// largest contribution.  Integer addition is
 // associative, so the result does not depend on scheduling: gridding is
 // bit-reproducible.  (LDS float atomics, ds_add_f32, run about ten times
 // slower than the integer ones on gfx950, which is what rules them out.)
@@ -42,12 +43,20 @@ __global__ void k_scale_maxima(const float2 *__restrict__ vis, int64_t nv,
                                const int64_t *__restrict__ plane_off,
                                const int32_t *__restrict__ plane_s, int nplanes,
                                int Q, int separable, ScaleState *st) {
-  int64_t nk = 0;
-  for (int p = 0; p < nplanes; p++) {
-    const int64_t S = plane_s[p];
-    const int64_t n = separable ? Q * S : (int64_t)Q * Q * S * S;
-    nk = max(nk, plane_off[p] + n);
+  __shared__ unsigned long long s_nk;
+  if (threadIdx.x == 0) s_nk = 0ull;
+  __syncthreads();
+  {
+    int64_t e = 0;
+    for (int p = threadIdx.x; p < nplanes; p += blockDim.x) {
+      const int64_t S = plane_s[p];
+      const int64_t n = separable ? Q * S : (int64_t)Q * Q * S * S;
+      e = max(e, plane_off[p] + n);
+    }
+    if (e > 0) atomicMax(&s_nk, (unsigned long long)e);
   }
+  barrier_after_lds_atomics();
+  const int64_t nk = (int64_t)s_nk;
   float mv = 0.f, mk = 0.f;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,36 +73,47 @@ __global__ void k_scale_maxima(const float2 *__restrict__ vis, int64_t nv,
   }
 }
 
-// 2^e with (nv + 1) * 2 * max|v| * max|k|^(1 or 2) * 2^e < 2^62.
+// 2^e with (nv + 1) * c * 2^e < 2^62 (no sum overflows) and c * 2^e < 2^50
+// (the conversion in fixed() below is exact), c = 2 * max|v| * max|k|^(1 or
+// 2) bounding one contribution.
 __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
   const double mv = (double)__uint_as_float(st->max_vis_bits);
   double mk = (double)__uint_as_float(st->max_k_bits);
   if (separable) mk = mk * mk * 2.0;  // product of two table entries
-  const double bound = 2.0 * mv * mk * (double)(nv + 1);
+  const double one = 2.0 * mv * mk;
+  const double bound = one * (double)(nv + 1);
   int e = 40;
   if (bound > 0.0 && isfinite(bound)) {
-    int ex;
+    int ex, ex1;
     frexp(bound, &ex);  // bound < 2^ex
-    e = 61 - ex;
+    frexp(one, &ex1);   // one < 2^ex1
+    e = min(61 - ex, 50 - ex1);
   }
   e = max(-1000, min(e, 1000));
   st->scale = ldexp(1.0, e);
   st->inv_scale = ldexp(1.0, -e);
 }
 
+// round-half-even(x * 2^e) as a 64-bit two's complement integer, for
+// |x * 2^e| < 2^51.  x * 2^e is exact, and adding 1.5 * 2^52 in the same fma
+// rounds it once to an integer held in the low mantissa bits of a double in
+// [2^52, 2^53): the bit pattern is 0x4338000000000000 + round(x * 2^e).
+// Three instructions instead of the emulated double -> int64 conversion.
 __device__ __forceinline__ unsigned long long fixed(float x, double scale) {
-  return (unsigned long long)__double2ll_rn((double)x * scale);
+  const double d = __builtin_fma((double)x, scale, 6755399441055744.0);  // 1.5 * 2^52
+  return (unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull;
 }
 
-// Spread one visibility over a block of up to 64*NQ taps starting at t0.
-// All NQ loads are issued before the first use; indices past the table end
-// are clamped to its last tap (a valid address, value unused), so the loads
-// need no predication and the compiler issues them back to back.
+// Spread one visibility over a block of up to 64*NQ taps t0 .. of its
+// S x S window (tap t = row i, column t - i*S).  All NQ loads are issued
+// before the first use; indices past the table end are clamped to its last
+// tap (a valid address, value unused), so the loads need no predication and
+// the compiler issues them back to back.  The region interleaves re / im:
+// cell c is acc[2c], acc[2c + 1].
 template <int NQ, bool SEPARABLE>
 __device__ __forceinline__ void spread_taps(
-    unsigned long long *sre, unsigned long long *sim, int base, int R, int S,
-    int SS, float invS, int t0, int lane, const float2 *k2, const float2 *ky,
-    const float2 *kx, float2 v, double scale) {
+    unsigned long long *acc, int base, int S, int SS, float invS, int t0, int lane,
+    const float2 *k2, const float2 *ky, const float2 *kx, float2 v, double scale) {
   float2 kv[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
@@ -102,8 +122,7 @@ __device__ __forceinline__ void spread_taps(
       kv[q] = k2[t];
     } else {
       const int i = (int)(((float)t + 0.5f) * invS);
-      const float2 a = ky[i], b = kx[t - i * S];
-      kv[q] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+      kv[q] = cmul(ky[i], kx[t - i * S]);
     }
   }
 #pragma unroll
@@ -111,9 +130,9 @@ __device__ __forceinline__ void spread_taps(
     const int t = t0 + q * 64 + lane;
     if (t < SS) {
       const int i = (int)(((float)t + 0.5f) * invS);
-      const int c = base + i * (R - S) + t;  // base + i*R + (t - i*S)
-      atomicAdd(&sre[c], fixed(v.x * kv[q].x - v.y * kv[q].y, scale));
-      atomicAdd(&sim[c], fixed(v.x * kv[q].y + v.y * kv[q].x, scale));
+      unsigned long long *cell = acc + 2 * (base + i * (TILE_R - S) + t);
+      atomicAdd(cell, fixed(v.x * kv[q].x - v.y * kv[q].y, scale));
+      atomicAdd(cell + 1, fixed(v.x * kv[q].y + v.y * kv[q].x, scale));
     }
   }
 }
@@ -122,56 +141,63 @@ template <int NT, bool SEPARABLE>
 __global__ __launch_bounds__(NT) void k_grid_tiles(
     unsigned long long *__restrict__ acc, int N, int T, int ntx,
     const uint4 *__restrict__ rec, const float2 *__restrict__ svis,
-    const float2 *__restrict__ kern, int Q,
-    const int64_t *__restrict__ plane_off, const int32_t *__restrict__ plane_s,
-    const int4 *__restrict__ items, const uint32_t *__restrict__ nitems,
-    uint32_t *__restrict__ qhead, const ScaleState *__restrict__ st) {
+    const float2 *__restrict__ kern, const int4 *__restrict__ items,
+    const uint32_t *__restrict__ nitems, uint32_t *__restrict__ qhead,
+    const ScaleState *__restrict__ st) {
   constexpr int R = TILE_R, CELLS = R * R;
   constexpr int NW = NT / 64;
-  __shared__ unsigned long long sre[CELLS];
-  __shared__ unsigned long long sim[CELLS];
+  __shared__ unsigned long long sacc[2 * CELLS];  // re, im interleaved
   __shared__ int s_item;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nit = (int)*nitems;
   const double scale = st->scale;
 
-  for (int c = tid; c < CELLS; c += NT) {
-    sre[c] = 0ull;
-    sim[c] = 0ull;
-  }
+  for (int c = tid; c < 2 * CELLS; c += NT) sacc[c] = 0ull;
   for (;;) {
     if (tid == 0) s_item = (int)atomicAdd(qhead, 1u);
     __syncthreads();
     const int it = s_item;
     if (it >= nit) break;
     const int4 item = items[it];
-    const int tx0 = (item.x % ntx) * T, ty0 = (item.x / ntx) * T;
+    const int tile = __builtin_amdgcn_readfirstlane(item.x);
+    const int tx0 = (tile % ntx) * T, ty0 = (tile / ntx) * T;
 
-    for (int j = item.y + wave; j < item.z; j += NW) {
-      const uint4 r = rec[j];
-      const float2 v = svis[j];
-      const int S = plane_s[r.z];
-      const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+    // Each wave walks its visibilities (j = first + wave, + NW, ...) with
+    // the next record and value already in flight.
+    const int jend = __builtin_amdgcn_readfirstlane(item.z);
+    int j = __builtin_amdgcn_readfirstlane(item.y) + wave;
+    uint4 rn = make_uint4(0, 0, 0, 0);
+    float2 vn = make_float2(0.f, 0.f);
+    if (j < jend) {
+      rn = rec[j];
+      vn = svis[j];
+    }
+    for (; j < jend; j += NW) {
+      const uint4 r = uniform_u4(rn);
+      const float2 v = uniform_f2(vn);
+      if (j + NW < jend) {
+        rn = rec[j + NW];
+        vn = svis[j + NW];
+      }
+      const int S = (int)(r.y & 0xffu);
+      const int xf = (int)((r.y >> 8) & 0xfffu), yf = (int)(r.y >> 20);
       const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
       const int SS = S * S;
       const float invS = 1.0f / (float)S;
-      const float2 *kp = kern + plane_off[r.z];
-      const float2 *k2 = kp + (int64_t)(yf * Q + xf) * SS;  // 2D table
-      const float2 *ky = kp + (int64_t)yf * S;               // separable rows
-      const float2 *kx = kp + (int64_t)xf * S;
+      const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
+      const float2 *ky = kp + yf * S, *kx = kp + xf * S;
       if (SS <= 256) {
-        spread_taps<4, SEPARABLE>(sre, sim, base, R, S, SS, invS, 0, lane, k2, ky, kx, v, scale);
+        spread_taps<4, SEPARABLE>(sacc, base, S, SS, invS, 0, lane, kp, ky, kx, v, scale);
       } else {
         for (int t0 = 0; t0 < SS; t0 += 512)
-          spread_taps<8, SEPARABLE>(sre, sim, base, R, S, SS, invS, t0, lane, k2, ky, kx, v,
-                                    scale);
+          spread_taps<8, SEPARABLE>(sacc, base, S, SS, invS, t0, lane, kp, ky, kx, v, scale);
       }
     }
     barrier_after_lds_atomics();
     // Flush the region (touched cells only) and clear it for the next item.
     for (int c = tid; c < CELLS; c += NT) {
-      const unsigned long long re = sre[c], im = sim[c];
+      const unsigned long long re = sacc[2 * c], im = sacc[2 * c + 1];
       if (re | im) {
         const int cy = c / R, cx = c - cy * R;
         const int gy = ty0 + cy, gx = tx0 + cx;
@@ -180,8 +206,8 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(
           if (re) atomicAdd(g, re);
           if (im) atomicAdd(g + 1, im);
         }
-        sre[c] = 0ull;
-        sim[c] = 0ull;
+        sacc[2 * c] = 0ull;
+        sacc[2 * c + 1] = 0ull;
       }
     }
     __syncthreads();
@@ -226,10 +252,10 @@ extern "C" int croc_convgrid(croc_ctx *ctx, void *grid, int N,
   CROC_REQUIRE(lam > 0, CROC_EINVAL, "lam must be positive");
   if (nv == 0) return CROC_OK;
 
-  BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_mode, vplane,
-            wstep, wbin0, (const float2 *)vis, max_support};
+  BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_off, separable,
+            plane_mode, vplane, wstep, wbin0, (const float2 *)vis, max_support};
   BinPlan plan;
-  rc = bin_visibilities(ctx, a, max_support, &plan);
+  rc = bin_visibilities(ctx, a, max_support, ITEMS_BY_TILE, &plan);
   if (rc) return rc;
 
   const int64_t cells = (int64_t)N * N;
@@ -251,12 +277,12 @@ extern "C" int croc_convgrid(croc_ctx *ctx, void *grid, int N,
   CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   if (separable)
     k_grid_tiles<GRID_NT, true><<<G, GRID_NT, 0, ctx->stream>>>(
-        acc, N, plan.T, plan.ntx, plan.rec, plan.svis, (const float2 *)kern, Q,
-        plane_off, plane_s, plan.items, plan.nitems, plan.qhead, st);
+        acc, N, plan.T, plan.ntx, plan.rec, plan.svis, (const float2 *)kern,
+        plan.items, plan.nitems, plan.qhead, st);
   else
     k_grid_tiles<GRID_NT, false><<<G, GRID_NT, 0, ctx->stream>>>(
-        acc, N, plan.T, plan.ntx, plan.rec, plan.svis, (const float2 *)kern, Q,
-        plane_off, plane_s, plan.items, plan.nitems, plan.qhead, st);
+        acc, N, plan.T, plan.ntx, plan.rec, plan.svis, (const float2 *)kern,
+        plan.items, plan.nitems, plan.qhead, st);
   CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
   k_grid_finish<<<(unsigned)((cells + 255) / 256), 256, 0, ctx->stream>>>(

@@ -36,6 +36,7 @@ enum : uint32_t {
   ST_BOUNDS = 2u,  // kernel window outside the grid
   ST_PLANE = 4u,   // plane index outside the bank
   ST_SUPPORT = 8u, // a plane's support exceeds the declared maximum
+  ST_BANK = 16u,   // a kernel table lies beyond element 2^32 of the bank
 };
 
 int croc_status_to_code(uint32_t st);
@@ -72,6 +73,7 @@ struct croc_ctx {
   int num_cu = 256;
   int tile = 0;     // 0: automatic
   int chunk = 1024; // visibilities per work item
+  int group_kib = 0;  // kernel-bank KiB per plane-group (0: one group)
 };
 
 // Grow-only workspace; returns nullptr on failure (error already set).
@@ -139,6 +141,8 @@ struct BinArgs {
   int Q;
   int nplanes;
   const int32_t *plane_s;
+  const int64_t *plane_off;
+  int separable;
   int plane_mode;
   const int32_t *vplane;
   double wstep;
@@ -147,8 +151,19 @@ struct BinArgs {
   int max_support;    // every plane_s must lie in [1, max_support]
 };
 
+// Tile-kernel record of one visibility (16 bytes):
+//   x  window origin x0 | y0 << 16
+//   y  support S | xf << 8 | yf << 20      (S <= 64, Q <= 4096)
+//   z  kernel-bank element offset of its table: the 2D table [S][S] of
+//      (yf, xf) in plane p, or plane p's separable table [Q][S]
+//   w  visibility index (before the scatter: its bin)
+// Bins are (tile, plane-group) pairs, bin = tile * ngroups + group, where a
+// plane-group is a slice of group_elems consecutive kernel-bank elements, so
+// that the tables one group uses fit in an XCD's L2.
 struct BinPlan {
   int T = 64, R = 95, ntx = 0, nty = 0, ntiles = 0, chunk = 4096;
+  int ngroups = 1, nbins = 0;
+  int64_t group_elems = 0;
   int64_t max_items = 0;
   uint4 *loc = nullptr, *rec = nullptr;
   float2 *svis = nullptr;
@@ -160,9 +175,31 @@ struct BinPlan {
 // Region edge held in LDS by the tile kernels.
 constexpr int TILE_R = 95;
 
+// Values every lane of the wavefront loaded from the same address (the
+// record of the visibility the wave works on): kept in scalar registers, so
+// the arithmetic derived from them runs on the scalar unit.
+__device__ __forceinline__ uint4 uniform_u4(uint4 r) {
+  return make_uint4(__builtin_amdgcn_readfirstlane(r.x), __builtin_amdgcn_readfirstlane(r.y),
+                    __builtin_amdgcn_readfirstlane(r.z), __builtin_amdgcn_readfirstlane(r.w));
+}
+
+__device__ __forceinline__ float2 uniform_f2(float2 v) {
+  return make_float2(__int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v.x))),
+                     __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v.y))));
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
 int choose_tile(croc_ctx *ctx, int max_support);
+// Work-item order: ITEMS_BY_TILE cuts each tile's records (all groups) into
+// items (gridding: one LDS flush per item); ITEMS_BY_GROUP cuts each (tile,
+// group) bin and queues them group-major (degridding: all CUs read the same
+// slice of the kernel bank at a time).
+enum { ITEMS_BY_TILE = 0, ITEMS_BY_GROUP = 1 };
 int bin_visibilities(croc_ctx *ctx, const BinArgs &a, int max_support,
-                     BinPlan *plan);
+                     int order, BinPlan *plan);
 int read_status(croc_ctx *ctx, const BinPlan &plan);
 int persistent_blocks(croc_ctx *ctx, int per_cu);
 int zero_u32_async(croc_ctx *ctx, uint32_t *p, int64_t n);

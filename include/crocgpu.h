@@ -59,9 +59,12 @@ int croc_ctx_create(int device, croc_ctx **out);
 int croc_ctx_destroy(croc_ctx *ctx);
 int croc_ctx_set_stream(croc_ctx *ctx, void *hip_stream);
 /* Tuning knobs of convgrid / convdegrid: tile edge in grid cells (0 =
- * automatic, else a multiple of 16 in [16, 80]) and the most visibilities
- * one work item of a tile holds (>= 64).  Results do not depend on them. */
-int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk);
+ * automatic, else a multiple of 16 in [16, 80]), the most visibilities one
+ * work item of a tile holds (>= 64), and the kernel-bank KiB per plane-group
+ * (degridding then processes one slice of the bank at a time, for banks far
+ * larger than the caches; 0 = no grouping, the default).  Results do not
+ * depend on them. */
+int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk, int group_kib);
 /* Device time in milliseconds of the last gridding / degridding kernel
  * (HIP events around its launch on the context stream). */
 int croc_last_kernel_ms(croc_ctx *ctx, float *grid_ms, float *degrid_ms);

@@ -47,7 +47,7 @@ def test_version_and_error_path_without_device():
 def test_null_context_rejected():
     L = _lib.load()
     assert L.croc_ctx_set_stream(None, None) == _lib.EINVAL
-    assert L.croc_ctx_set_tuning(None, 0, 4096) == _lib.EINVAL
+    assert L.croc_ctx_set_tuning(None, 0, 4096, 2048) == _lib.EINVAL
     with pytest.raises(ValueError):
         _lib.check(L.croc_convgrid(None, None, 16, None, 0, 3, 1.0, None, None, 1, 0, 1,
                                    None, None, 1, 0, None, 1.0, 0))

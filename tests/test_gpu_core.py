@@ -329,6 +329,36 @@ def test_tiling_knobs_do_not_change_results(gpu):
         _lib.set_tuning(0, 1024)
 
 
+def test_plane_groups_do_not_change_results(gpu):
+    """Binning by (tile, kernel-bank slice): every group size, from one
+    group to a group per fraction of a plane, gives the oracle's degridded
+    visibilities and bit-identical grids."""
+    rng = numpy.random.default_rng(14)
+    N, S, Q, nv, planes = 256, 12, 4, 30000, 12
+    tables, p, v = random_case(rng, N, S, Q, nv, planes)
+    vplane = rng.integers(0, planes, nv).astype(numpy.int32)
+    bank = cgrid.Bank(tables, False)
+    ref = numpy.zeros((N, N), dtype=complex)
+    cgrid.convgrid(bank, ref, p, v, vplane)
+    vref = cgrid.convdegrid(bank, ref, p, vplane)
+    gcf, src = numpy.array(tables), vplane[:, None]
+    grids = []
+    try:
+        # one table is 144 complex64 (1.1 KiB); a plane is 18 KiB
+        for group_kib in (0, 1, 2, 8, 64, 2048):
+            _lib.set_tuning(0, 1024, group_kib)
+            a = numpy.zeros((N, N), dtype=complex)
+            G.convgrid(gcf, a, p, src, v)
+            assert_close(a, ref, what='grid group_kib=%d' % group_kib)
+            grids.append(a)
+            assert_close(G.convdegrid(gcf, ref, p, src), vref,
+                         what='degrid group_kib=%d' % group_kib)
+    finally:
+        _lib.set_tuning(0, 1024)
+    for g in grids[1:]:
+        assert (g == grids[0]).all()
+
+
 def test_gridding_is_bit_reproducible(gpu):
     """Fixed-point accumulation is order independent: different tilings,
     chunkings and repeated runs give bit-identical grids."""
