@@ -31,7 +31,10 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='wproj2048', choices=['wproj2048', 'plain512'])
+    ap.add_argument('--workload', default='wproj2048',
+                    choices=['wproj2048', 'plain512', 'towers4096', 'towers8192'])
+    ap.add_argument('--nv', type=int, default=0,
+                    help='towers workloads: visibilities (0 = the workload\'s own count)')
     ap.add_argument('--cpu-sample', type=int, default=300_000,
                     help='visibilities in the CPU-baseline sample, about 12 s of one '
                          'core (0 disables it)')
@@ -137,6 +140,109 @@ def cpu_baseline(G, W, torch, dev, n):
     return base, parity
 
 
+def run_towers(args, torch, G, W, _lib, dev):
+    """W-towers round trip (croc_wtowers_grid + croc_wtowers_degrid, one
+    AW-style kernel S=16 for every visibility, w by stacking over sub-grid
+    planes) against the plain per-visibility path (convgrid + convdegrid
+    with one w-kernel plane per w-bin, the same 256 w-bins, S=16) on the
+    same antenna-track visibilities."""
+    import numpy
+    name = args.workload
+    c = W.SPEC[name]
+    N, theta = c['N'], c['theta']
+    lam = N / theta
+    nv = args.nv or c['nv']
+    print('[bench] generating %s (%d visibilities)' % (name, nv), file=sys.stderr, flush=True)
+    uvw_d, vis64 = W.towers(name, nv=nv, device=dev)
+    vis_d = vis64.to(torch.complex64).contiguous()
+    del vis64
+    w = uvw_d[:, 2]
+    wmin, wmax = float(w.min()), float(w.max())
+    nplanes = 256
+    wstep = (wmax - wmin) / (nplanes - 1)
+    Q, S, npix_ff = c['Q'], c['S'], 64
+    ff = W.aw_far_field(npix_ff, theta)
+    kv = G.kernel_oversample(torch.as_tensor(ff, device=dev), Q, S)
+    kv = (kv / kv[0, 0].sum().real).to(torch.complex64).contiguous()
+    wb = torch.round(w / wstep)
+    lo, hi = int(wb.min()), int(wb.max())
+    ws = [wstep * b for b in range(lo, hi + 1)]
+    t0 = time.perf_counter()
+    bank = G.w_kernel_bank(theta, ws, npix_ff, S, Q, True, 1.0 / npix_ff ** 2)
+    torch.cuda.synchronize()
+    kgen_ms = 1e3 * (time.perf_counter() - t0)
+    guv = torch.zeros((N, N), dtype=torch.complex64, device=dev)
+    out = torch.empty(nv, dtype=torch.complex64, device=dev)
+    L = _lib.load()
+    margin = 32
+
+    def towers_step():
+        guv.zero_()
+        _lib.check(L.croc_wtowers_grid(
+            _lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uvw_d), nv, 3, theta, _lib.ptr(vis_d),
+            wstep, 128, margin, _lib.ptr(kv), Q, S, 0))
+        g_ms, _ = _lib.last_kernel_ms()
+        _lib.check(L.croc_wtowers_degrid(
+            _lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uvw_d), nv, 3, theta, _lib.ptr(out),
+            wstep, 128, margin, _lib.ptr(kv), Q, S, 0))
+        _, d_ms = _lib.last_kernel_ms()
+        return g_ms, d_ms
+
+    def plain_step():
+        guv.zero_()
+        G.grid_device(bank, guv, uvw_d, lam, vis_d, _lib.PLANE_WBIN, None, wstep, lo)
+        g_ms, _ = _lib.last_kernel_ms()
+        G.degrid_device(bank, guv, uvw_d, lam, out, _lib.PLANE_WBIN, None, wstep, lo)
+        _, d_ms = _lib.last_kernel_ms()
+        return g_ms, d_ms
+
+    def timed(step, label):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        print('[bench] timing %d %s steps' % (args.steps, label), file=sys.stderr, flush=True)
+        gs, ds = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g, d = step()
+            gs.append(g)
+            ds.append(d)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return nv * args.steps / dt, 1e3 * dt / args.steps, sum(gs) / len(gs), sum(ds) / len(ds)
+
+    tv, tms, tg, td = timed(towers_step, 'w-towers')
+    pv, pms, pg, pd = timed(plain_step, 'plain')
+    # algorithmic work: taps (8 flop each way) and sub-grid DFTs (16 n^3 flop
+    # per 128^2 transform, three fp16 products each on the matrix cores)
+    taps = float(nv) * S * S
+    key = (torch.floor(uvw_d[:, 0] * theta + 0.5).to(torch.int64) + N // 2) // (128 - margin)
+    key = key * 100000 + (torch.floor(uvw_d[:, 1] * theta + 0.5).to(torch.int64) + N // 2) \
+        // (128 - margin)
+    key = key * 4096 + (wb.to(torch.int64) - lo)
+    njobs = int(torch.unique(key).numel())
+    nchunks = int(torch.unique(key // 4096).numel())
+    dft_flops = 16.0 * 128 ** 3 * (2 * njobs + 2 * nchunks)
+    line = {
+        'metric': 'visibilities/s grid+degrid', 'value': round(tv, 1),
+        'unit': 'visibilities/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(tms, 3), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': name, 'N': N, 'theta': theta, 'nv': nv, 'w_planes': nplanes,
+                   'wstep': round(wstep, 3), 'support': S, 'oversampling': Q,
+                   'subgrid': 128, 'margin': margin, 'jobs': njobs, 'chunks': nchunks},
+        'towers': {'ms_per_step': round(tms, 3), 'grid_ms': round(tg, 3),
+                   'degrid_ms': round(td, 3), 'tap_tflops': round(2 * 8 * taps / tms / 1e9, 3),
+                   'dft_tflops': round(dft_flops / tms / 1e9, 3)},
+        'plain': {'value': round(pv, 1), 'ms_per_step': round(pms, 3),
+                  'grid_ms': round(pg, 3), 'degrid_ms': round(pd, 3),
+                  'kernel_bank_mib': round(8 * bank.kern.numel() / 2 ** 20, 1),
+                  'kernel_gen_ms': round(kgen_ms, 1)},
+        'speedup_vs_plain': round(tv / pv, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -148,6 +254,8 @@ def main():
 
     dev = _lib.device()
     _lib.set_tuning(args.tile, args.chunk, args.group_kib)
+    if args.workload.startswith('towers'):
+        return run_towers(args, torch, G, W, _lib, dev)
     print('[bench] building workload %s' % args.workload, file=sys.stderr, flush=True)
     if args.workload == 'wproj2048':
         wl = build_wproj(G, W, torch, dev)

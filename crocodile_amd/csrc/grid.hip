@@ -94,15 +94,6 @@ __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
   st->inv_scale = ldexp(1.0, -e);
 }
 
-// round-half-even(x * 2^e) as a 64-bit two's complement integer, for
-// |x * 2^e| < 2^51.  x * 2^e is exact, and adding 1.5 * 2^52 in the same fma
-// rounds it once to an integer held in the low mantissa bits of a double in
-// [2^52, 2^53): the bit pattern is 0x4338000000000000 + round(x * 2^e).
-// Three instructions instead of the emulated double -> int64 conversion.
-__device__ __forceinline__ unsigned long long fixed(float x, double scale) {
-  const double d = __builtin_fma((double)x, scale, 6755399441055744.0);  // 1.5 * 2^52
-  return (unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull;
-}
 
 // Spread one visibility over a block of up to 64*NQ taps t0 .. of its
 // S x S window (tap t = row i, column t - i*S).  All NQ loads are issued

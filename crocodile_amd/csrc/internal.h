@@ -59,6 +59,7 @@ enum WsSlot {
   WS_DFT_INV,   // fp16-split centred DFT matrix, inverse sign
   WS_SUB,       // w-towers sub-grid stack
   WS_TOWER,     // w-towers job / chunk tables
+  WS_SORT,      // w-towers radix sort keys, indices, run tables, temporaries
   WS_COUNT
 };
 
@@ -174,6 +175,16 @@ struct BinPlan {
 
 // Region edge held in LDS by the tile kernels.
 constexpr int TILE_R = 95;
+
+// round-half-even(x * 2^e) as a 64-bit two's complement integer, for
+// |x * 2^e| < 2^51.  x * 2^e is exact, and adding 1.5 * 2^52 in the same fma
+// rounds it once to an integer held in the low mantissa bits of a double in
+// [2^52, 2^53): the bit pattern is 0x4338000000000000 + round(x * 2^e).
+// Three instructions instead of the emulated double -> int64 conversion.
+__device__ __forceinline__ unsigned long long fixed(float x, double scale) {
+  const double d = __builtin_fma((double)x, scale, 6755399441055744.0);  // 1.5 * 2^52
+  return (unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull;
+}
 
 // Values every lane of the wavefront loaded from the same address (the
 // record of the visibility the wave works on): kept in scalar registers, so

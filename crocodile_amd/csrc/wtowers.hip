@@ -7,11 +7,12 @@
 // u_mid = (x0 + sg/2 - N/2) / theta; the w-plane is rint(w / wstep).
 //
 // Imaging, per call:
-//   1. bin visibilities by job = (chunk, w-plane) with a counting sort;
-//   2. grid each job on its own sg x sg sub-grid: the workgroup holds the
-//      sub-grid in LDS as int32 fixed point for batches of at most 32
-//      visibilities (integer LDS atomics), and folds every batch into
-//      Kahan-compensated FP32 registers (each thread owns sg*sg/1024 cells);
+//   1. bin visibilities by job = (chunk, w-plane): stable radix sort of
+//      (job key, visibility index), run-length encoding of the sorted keys;
+//   2. grid each job on its own sg x sg sub-grid, one workgroup per job, in
+//      two passes of sg/2 rows: the pass's rows are held in LDS as exact
+//      64-bit fixed point (the int64 arithmetic of the tile gridder), every
+//      visibility adds the taps of its window rows that fall in the pass;
 //   3. inverse DFT of every job sub-grid on the matrix cores, with the
 //      Fresnel factor exp(-2 pi i wp wstep (1 - n)) applied to the output;
 //   4. sum the job images of each chunk (one workgroup per chunk and batch,
@@ -25,6 +26,8 @@
 #include <climits>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "internal.h"
 
 namespace {
@@ -32,8 +35,8 @@ namespace {
 constexpr int SG = 128;
 constexpr int SGN = SG * SG;
 constexpr int TW_NT = 1024;
-constexpr int TW_BATCH_VIS = 32;   // visibilities per int32 accumulation batch
-constexpr int JOB_BATCH = 2048;    // sub-grids transformed per launch
+constexpr int JOB_BATCH = 2048;    // sub-grid slots per batch
+constexpr int PART_VIS = 4096;     // visibilities per work part of a job
 constexpr uint32_t INVALID = 0xffffffffu;
 
 struct TowerArgs {
@@ -45,17 +48,39 @@ struct TowerArgs {
   int64_t nv, stride;
 };
 
+// Range of w-planes rint(w / wstep): grid-stride, reduced per wavefront
+// before one global atomic per wavefront.
 __global__ void k_tw_wrange(const double *__restrict__ uvw, int64_t nv, int64_t stride,
                             double wstep, int *mm, uint32_t *status) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  const double q = rint(uvw[i * stride + 2] / wstep);
-  if (!(fabs(q) < 1e9)) {  // checked before any integer conversion
-    atomicOr(status, (uint32_t)ST_PLANE);
-    return;
+  int lo = INT_MAX, hi = INT_MIN;
+  bool bad = false;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += step) {
+    const double q = rint(uvw[i * stride + 2] / wstep);
+    if (!(fabs(q) < 1e9)) {  // checked before any integer conversion
+      bad = true;
+      continue;
+    }
+    lo = min(lo, (int)q);
+    hi = max(hi, (int)q);
   }
-  atomicMin(&mm[0], (int)q);
-  atomicMax(&mm[1], (int)q);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off, 64));
+    hi = max(hi, __shfl_xor(hi, off, 64));
+  }
+  if (bad) atomicOr(status, (uint32_t)ST_PLANE);
+  if ((threadIdx.x & 63) == 0 && lo <= hi) {
+    atomicMin(&mm[0], lo);
+    atomicMax(&mm[1], hi);
+  }
+}
+
+// Zero the listed chunk images (blockIdx.y = list entry).
+__global__ void k_tw_zero_chunks(const int32_t *__restrict__ chunks, float2 *__restrict__ cimg) {
+  float4 *d = (float4 *)(cimg + (int64_t)chunks[blockIdx.y] * SGN);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < SGN / 2; i += gridDim.x * blockDim.x)
+    d[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Sub-grid window of visibility i; returns the job key or INVALID.
@@ -107,132 +132,51 @@ __device__ uint4 tw_locate(const TowerArgs &a, int64_t i, uint32_t *st) {
                     0u, key);
 }
 
-__global__ void k_tw_locate(TowerArgs a, uint4 *__restrict__ loc, uint32_t *__restrict__ count,
+// Located records, and (job key, visibility index) pairs for the sort; an
+// invalid visibility gets key `bad`, above every job key (the call then
+// fails on the status before any job is used).
+__global__ void k_tw_locate(TowerArgs a, uint32_t bad, uint4 *__restrict__ loc,
+                            uint32_t *__restrict__ keys, uint32_t *__restrict__ idx,
                             uint32_t *__restrict__ status) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nv) return;
   uint32_t st = 0;
   const uint4 L = tw_locate(a, i, &st);
   loc[i] = L;
-  if (L.w != INVALID) atomicAdd(&count[L.w], 1u);
+  keys[i] = L.w == INVALID ? bad : L.w;
+  idx[i] = (uint32_t)i;
   if (st) atomicOr(status, st);
 }
 
-// --- three-kernel exclusive scan of key counts, emitting non-empty jobs ---
-constexpr int SCAN_NT = 1024, SCAN_PER = 16, SCAN_SPAN = SCAN_NT * SCAN_PER;
-
-__device__ __forceinline__ void block_excl_scan2(uint32_t &a, uint32_t &b, uint32_t *sa,
-                                                 uint32_t *sb, uint32_t *ta, uint32_t *tb) {
-  const int tid = threadIdx.x;
-  sa[tid] = a;
-  sb[tid] = b;
-  __syncthreads();
-  for (int off = 1; off < SCAN_NT; off <<= 1) {
-    uint32_t x = 0, y = 0;
-    if (tid >= off) {
-      x = sa[tid - off];
-      y = sb[tid - off];
-    }
-    __syncthreads();
-    sa[tid] += x;
-    sb[tid] += y;
-    __syncthreads();
-  }
-  *ta = sa[SCAN_NT - 1];
-  *tb = sb[SCAN_NT - 1];
-  a = sa[tid] - a;
-  b = sb[tid] - b;
-  __syncthreads();
-}
-
-__global__ __launch_bounds__(SCAN_NT) void k_scan_blocks(const uint32_t *__restrict__ count,
-                                                         int64_t n, uint32_t *__restrict__ bsum) {
-  __shared__ uint32_t sa[SCAN_NT], sb[SCAN_NT];
-  const int64_t base = (int64_t)blockIdx.x * SCAN_SPAN + (int64_t)threadIdx.x * SCAN_PER;
-  uint32_t c = 0, nz = 0;
-  for (int k = 0; k < SCAN_PER; k++) {
-    const int64_t i = base + k;
-    if (i < n) {
-      c += count[i];
-      nz += count[i] ? 1u : 0u;
-    }
-  }
-  uint32_t tc, tn;
-  block_excl_scan2(c, nz, sa, sb, &tc, &tn);
-  if (threadIdx.x == 0) {
-    bsum[2 * blockIdx.x] = tc;
-    bsum[2 * blockIdx.x + 1] = tn;
-  }
-}
-
-__global__ __launch_bounds__(SCAN_NT) void k_scan_top(uint32_t *__restrict__ bsum, int nb,
-                                                      uint32_t *__restrict__ totals) {
-  __shared__ uint32_t sa[SCAN_NT], sb[SCAN_NT];
-  uint32_t carry_c = 0, carry_n = 0;
-  for (int b0 = 0; b0 < nb; b0 += SCAN_NT) {
-    const int b = b0 + threadIdx.x;
-    uint32_t c = b < nb ? bsum[2 * b] : 0u, z = b < nb ? bsum[2 * b + 1] : 0u;
-    uint32_t tc, tn;
-    block_excl_scan2(c, z, sa, sb, &tc, &tn);
-    if (b < nb) {
-      bsum[2 * b] = c + carry_c;
-      bsum[2 * b + 1] = z + carry_n;
-    }
-    carry_c += tc;
-    carry_n += tn;
-  }
-  if (threadIdx.x == 0) {
-    totals[0] = carry_c;
-    totals[1] = carry_n;
-  }
-}
-
-__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(
-    const uint32_t *__restrict__ count, int64_t n, const uint32_t *__restrict__ bsum,
-    uint32_t *__restrict__ cursor, int4 *__restrict__ jobs) {
-  __shared__ uint32_t sa[SCAN_NT], sb[SCAN_NT];
-  const int64_t base = (int64_t)blockIdx.x * SCAN_SPAN + (int64_t)threadIdx.x * SCAN_PER;
-  uint32_t c = 0, nz = 0;
-  for (int k = 0; k < SCAN_PER; k++) {
-    const int64_t i = base + k;
-    if (i < n) {
-      c += count[i];
-      nz += count[i] ? 1u : 0u;
-    }
-  }
-  uint32_t tc, tn;
-  block_excl_scan2(c, nz, sa, sb, &tc, &tn);
-  uint32_t off = bsum[2 * blockIdx.x] + c, job = bsum[2 * blockIdx.x + 1] + nz;
-  for (int k = 0; k < SCAN_PER; k++) {
-    const int64_t i = base + k;
-    if (i < n) {
-      const uint32_t ci = count[i];
-      cursor[i] = off;
-      if (ci) jobs[job++] = make_int4((int)i, (int)off, (int)(off + ci), 0);
-      off += ci;
-    }
-  }
-}
-
-__global__ void k_tw_scatter(int64_t nv, const uint4 *__restrict__ loc,
-                             uint32_t *__restrict__ cursor, uint4 *__restrict__ rec,
-                             const float2 *__restrict__ vis, float2 *__restrict__ svis) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
+// Records (and visibilities) in job order: rec[pos] = (window, offsets, 0,
+// visibility index) of the pos-th visibility of the sorted order.
+__global__ void k_tw_gather(int64_t nv, const uint32_t *__restrict__ sidx,
+                            const uint4 *__restrict__ loc, const float2 *__restrict__ vis,
+                            uint4 *__restrict__ rec, float2 *__restrict__ svis) {
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= nv) return;
+  const uint32_t i = sidx[pos];
   const uint4 L = loc[i];
-  if (L.w == INVALID) return;
-  const uint32_t pos = atomicAdd(&cursor[L.w], 1u);
-  rec[pos] = make_uint4(L.x, L.y, 0u, (uint32_t)i);
+  rec[pos] = make_uint4(L.x, L.y, 0u, i);
   if (vis) svis[pos] = vis[i];
 }
 
-struct FixScale {
-  unsigned int max_vis_bits, max_k_bits;
-  float scale, inv_scale;
+// jobs[r] = (key, first record, end record) of run r of the sorted keys.
+__global__ void k_tw_jobs(const uint32_t *__restrict__ ukeys, const uint32_t *__restrict__ counts,
+                          const uint32_t *__restrict__ starts, int nruns,
+                          int4 *__restrict__ jobs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  jobs[r] = make_int4((int)ukeys[r], (int)starts[r], (int)(starts[r] + counts[r]), 0);
+}
+
+struct TwScale {
+  unsigned int max_vis_bits, max_k_bits;  // float bits of max |re|, |im|
+  double scale, inv_scale;                // 2^e, 2^-e
 };
 
 __global__ void k_tw_maxima(const float2 *__restrict__ vis, int64_t nv,
-                            const float2 *__restrict__ kern, int64_t nk, FixScale *fs) {
+                            const float2 *__restrict__ kern, int64_t nk, TwScale *fs) {
   float mv = 0.f, mk = 0.f;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride)
@@ -250,118 +194,169 @@ __global__ void k_tw_maxima(const float2 *__restrict__ vis, int64_t nv,
   }
 }
 
-// 2^e with 2 * max|v| * max|k| * TW_BATCH_VIS * 2^e < 2^31.
-__global__ void k_tw_pick(FixScale *fs, int nearest, int separable) {
+// 2^e with (nv + 1) * c * 2^e < 2^62 (no sum overflows) and c * 2^e < 2^50
+// (fixed() is exact), c = 2 * max|v| * max|k| bounding one contribution.
+__global__ void k_tw_pick(TwScale *fs, int64_t nv, int nearest, int separable) {
   const double mv = (double)__uint_as_float(fs->max_vis_bits);
   double mk = nearest ? 1.0 : (double)__uint_as_float(fs->max_k_bits);
   if (separable && !nearest) mk = 2.0 * mk * mk;
-  const double bound = 2.0 * mv * mk * (double)TW_BATCH_VIS;
-  int e = 20;
+  const double one = 2.0 * mv * mk;
+  const double bound = one * (double)(nv + 1);
+  int e = 40;
   if (bound > 0.0 && isfinite(bound)) {
-    int ex;
+    int ex, ex1;
     frexp(bound, &ex);
-    e = 30 - ex;
+    frexp(one, &ex1);
+    e = min(61 - ex, 50 - ex1);
   }
-  e = max(-120, min(e, 120));
-  fs->scale = ldexpf(1.f, e);
-  fs->inv_scale = ldexpf(1.f, -e);
+  e = max(-1000, min(e, 1000));
+  fs->scale = ldexp(1.0, e);
+  fs->inv_scale = ldexp(1.0, -e);
 }
 
-__device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
-  const float y = x - c;
-  const float t = s + y;
-  c = (t - s) - y;
-  s = t;
+// Taps [tlo, thi) of one visibility's S x S window (tap t = row i, column
+// t - i*S) in blocks of 64*NQ: loads first (clamped to the range, so they
+// need no predication), then f(cell offset from the window origin, k).
+template <int NQ, bool SEPARABLE, class F>
+__device__ __forceinline__ void tap_block(int S, float invS, int t0, int thi, int lane,
+                                          const float2 *k2, const float2 *ky, const float2 *kx,
+                                          F f) {
+  float2 kv[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int t = min(t0 + q * 64 + lane, thi - 1);
+    if (!SEPARABLE) {
+      kv[q] = k2[t];
+    } else {
+      const int i = (int)(((float)t + 0.5f) * invS);
+      kv[q] = cmul(ky[i], kx[t - i * S]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int t = t0 + q * 64 + lane;
+    if (t < thi) {
+      const int i = (int)(((float)t + 0.5f) * invS);
+      f(i * (SG - S) + t, kv[q]);
+    }
+  }
 }
 
-// Grid jobs [j0, j0 + nb) into sub-grids sub[b] (sg x sg complex64).
+template <bool SEPARABLE, class F>
+__device__ __forceinline__ void tap_range(int S, float invS, int tlo, int thi, int lane,
+                                          const float2 *k2, const float2 *ky, const float2 *kx,
+                                          F f) {
+  for (int t0 = tlo; t0 < thi; t0 += 256)
+    tap_block<4, SEPARABLE>(S, invS, t0, thi, lane, k2, ky, kx, f);
+}
+
+// Grid job parts into sub-grid slots: parts[b] = (slot, first record, end
+// record); one workgroup per part, rows [0, sg/2) then [sg/2, sg): each pass
+// zeroes its int64 region, each wavefront adds the taps of one visibility
+// at a time that lie in the pass's rows (a contiguous tap range), and the
+// pass ends by converting the region to complex64 in sub[slot].
 template <bool SEPARABLE>
 __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
-    const int4 *__restrict__ jobs, int j0, int nb, const uint4 *__restrict__ rec,
+    const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
     const float2 *__restrict__ svis, const float2 *__restrict__ kern, int Q, int S,
-    int nearest, const FixScale *__restrict__ fs, float2 *__restrict__ sub) {
-  constexpr int PER = SGN / TW_NT;
-  __shared__ int sre[SGN], sim[SGN];
+    int nearest, const TwScale *__restrict__ fs, float2 *__restrict__ sub) {
+  constexpr int HALF = SG / 2, HC = HALF * SG;
+  __shared__ unsigned long long acc[2 * HC];  // re, im interleaved (128 KiB)
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int NW = TW_NT / 64;
-  const float scale = fs->scale, inv = fs->inv_scale;
+  const double scale = fs->scale, inv = fs->inv_scale;
   const int SS = S * S;
   const float invS = 1.0f / (float)S;
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-    const int4 job = jobs[j0 + b];
-    float ar[PER], ai[PER], cr[PER], ci[PER];
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-      ar[k] = ai[k] = cr[k] = ci[k] = 0.f;
-      sre[tid + k * TW_NT] = 0;
-      sim[tid + k * TW_NT] = 0;
-    }
-    __syncthreads();
-    for (int v0 = job.y; v0 < job.z; v0 += TW_BATCH_VIS) {
-      const int v1 = min(job.z, v0 + TW_BATCH_VIS);
+    const int4 job = parts[b];
+    const int jbeg = __builtin_amdgcn_readfirstlane(job.y);
+    const int jend = __builtin_amdgcn_readfirstlane(job.z);
+    float2 *out = sub + (int64_t)__builtin_amdgcn_readfirstlane(job.x) * SGN;
+    for (int r0 = 0; r0 < SG; r0 += HALF) {
+      for (int c = tid; c < 2 * HC; c += TW_NT) acc[c] = 0ull;
+      __syncthreads();
       if (nearest) {
-        for (int j = v0 + tid; j < v1; j += TW_NT) {
+        for (int j = jbeg + tid; j < jend; j += TW_NT) {
           const uint4 r = rec[j];
-          const float2 v = svis[j];
-          const int c = (int)(r.x >> 16) * SG + (int)(r.x & 0xffffu);
-          atomicAdd(&sre[c], __float2int_rn(v.x * scale));
-          atomicAdd(&sim[c], __float2int_rn(v.y * scale));
+          const int wy = (int)(r.x >> 16) - r0;
+          if (wy >= 0 && wy < HALF) {
+            const float2 v = svis[j];
+            unsigned long long *cell = acc + 2 * (wy * SG + (int)(r.x & 0xffffu));
+            atomicAdd(cell, fixed(v.x, scale));
+            atomicAdd(cell + 1, fixed(v.y, scale));
+          }
         }
       } else {
-        for (int j = v0 + wave; j < v1; j += NW) {
-          const uint4 r = rec[j];
-          const float2 v = svis[j];
+        for (int j = jbeg + wave; j < jend; j += NW) {
+          const uint4 r = uniform_u4(rec[j]);
+          const int wy = (int)(r.x >> 16), wx = (int)(r.x & 0xffffu);
+          // window rows wy + i in the pass: i in [ilo, ihi)
+          const int ilo = max(0, r0 - wy), ihi = min(S, r0 + HALF - wy);
+          if (ilo >= ihi) continue;
+          const float2 v = uniform_f2(svis[j]);
           const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
-          const int base = (int)(r.x >> 16) * SG + (int)(r.x & 0xffffu);
           const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
           const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
-          for (int t = lane; t < SS; t += 64) {
-            const int i = (int)(((float)t + 0.5f) * invS);
-            float2 kv;
-            if (!SEPARABLE) {
-              kv = k2[t];
-            } else {
-              const float2 p = ky[i], q = kx[t - i * S];
-              kv = make_float2(p.x * q.x - p.y * q.y, p.x * q.y + p.y * q.x);
-            }
-            const int c = base + i * (SG - S) + t;
-            atomicAdd(&sre[c], __float2int_rn((v.x * kv.x - v.y * kv.y) * scale));
-            atomicAdd(&sim[c], __float2int_rn((v.x * kv.y + v.y * kv.x) * scale));
-          }
+          // cell of tap c: (wy - r0) * SG + wx + c, in [0, HC) for the pass's rows
+          const int cbase = (wy - r0) * SG + wx;
+          tap_range<SEPARABLE>(S, invS, ilo * S, ihi * S, lane, k2, ky, kx,
+                               [&](int c, float2 k) {
+                                 unsigned long long *cell = acc + 2 * (cbase + c);
+                                 atomicAdd(cell, fixed(v.x * k.x - v.y * k.y, scale));
+                                 atomicAdd(cell + 1, fixed(v.x * k.y + v.y * k.x, scale));
+                               });
         }
       }
       barrier_after_lds_atomics();
-#pragma unroll
-      for (int k = 0; k < PER; k++) {
-        const int c = tid + k * TW_NT;
-        kahan_add(ar[k], cr[k], (float)sre[c] * inv);
-        kahan_add(ai[k], ci[k], (float)sim[c] * inv);
-        sre[c] = 0;
-        sim[c] = 0;
-      }
+      for (int c = tid; c < HC; c += TW_NT)
+        out[r0 * SG + c] = make_float2((float)((double)(long long)acc[2 * c] * inv),
+                                       (float)((double)(long long)acc[2 * c + 1] * inv));
       __syncthreads();
     }
-    float2 *out = sub + (int64_t)b * SGN;
-#pragma unroll
-    for (int k = 0; k < PER; k++)
-      out[tid + k * TW_NT] = make_float2(ar[k] - cr[k], ai[k] - ci[k]);
-    __syncthreads();
   }
 }
 
-// cimg[chunk] += sum of sub[b] over the batch jobs of that chunk.
-// groups[g] = (chunk id, first local job, end local job).
-__global__ void k_tw_chunk_accum(const int4 *__restrict__ groups, const float2 *__restrict__ sub,
-                                 float2 *__restrict__ cimg) {
-  const int4 g = groups[blockIdx.x];
-  float2 *dst = cimg + (int64_t)g.x * SGN;
-  for (int i = threadIdx.x; i < SGN; i += blockDim.x) {
-    float2 acc = dst[i];
+// sub[first] += sub[first + 1 .. first + n - 1], parts in order: multi[m] =
+// (first slot, n) of a job gridded in n parts.  blockIdx.y = job,
+// blockIdx.x = slice of the pixels (two per thread).
+constexpr int ACC_SLICES = 16;
+__global__ void k_tw_sum_parts(const int2 *__restrict__ multi, float2 *__restrict__ sub) {
+  const int2 m = multi[blockIdx.y];
+  float4 *dst = (float4 *)(sub + (int64_t)m.x * SGN);
+  constexpr int PER = SGN / 2 / ACC_SLICES;
+  for (int i = blockIdx.x * PER + threadIdx.x; i < (blockIdx.x + 1) * PER; i += blockDim.x) {
+    float4 acc = dst[i];
+    for (int k = 1; k < m.y; k++) {
+      const float4 t = ((const float4 *)(sub + (int64_t)(m.x + k) * SGN))[i];
+      acc.x += t.x;
+      acc.y += t.y;
+      acc.z += t.z;
+      acc.w += t.w;
+    }
+    dst[i] = acc;
+  }
+}
+
+// cimg[chunk] += sum of the batch's job images of that chunk, jobs in order
+// (deterministic).  groups[g] = (chunk id, first job, end job) of the batch,
+// jslot[job] = its sub-grid slot; blockIdx.y = group, blockIdx.x = slice of
+// the pixels (two per thread), so every group spreads over ACC_SLICES
+// workgroups.
+__global__ void k_tw_chunk_accum(const int4 *__restrict__ groups, const int32_t *__restrict__ jslot,
+                                 const float2 *__restrict__ sub, float2 *__restrict__ cimg) {
+  const int4 g = groups[blockIdx.y];
+  float4 *dst = (float4 *)(cimg + (int64_t)g.x * SGN);
+  const float4 *src = (const float4 *)sub;
+  constexpr int PER = SGN / 2 / ACC_SLICES;  // float4 (pixel pairs) per slice
+  for (int i = blockIdx.x * PER + threadIdx.x; i < (blockIdx.x + 1) * PER; i += blockDim.x) {
+    float4 acc = dst[i];
     for (int b = g.y; b < g.z; b++) {
-      const float2 s = sub[(int64_t)b * SGN + i];
+      const float4 s = src[(int64_t)jslot[b] * (SGN / 2) + i];
       acc.x += s.x;
       acc.y += s.y;
+      acc.z += s.z;
+      acc.w += s.w;
     }
     dst[i] = acc;
   }
@@ -398,10 +393,11 @@ __global__ void k_tw_cut(const int32_t *__restrict__ chunks, const float2 *__res
   }
 }
 
-// Degrid the visibilities of jobs [j0, j0 + nb) from their sub-grids.
+// Degrid job parts from their sub-grids: parts[b] = (slot, first record,
+// end record), one workgroup per part staging sub[slot] in LDS.
 template <bool SEPARABLE>
 __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
-    const int4 *__restrict__ jobs, int j0, int nb, const uint4 *__restrict__ rec,
+    const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
     const float2 *__restrict__ sub, const float2 *__restrict__ kern, int Q, int S, int nearest,
     float2 *__restrict__ vis_out) {
   __shared__ float2 sg[SGN];
@@ -411,8 +407,8 @@ __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
   const int SS = S * S;
   const float invS = 1.0f / (float)S;
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-    const int4 job = jobs[j0 + b];
-    const float2 *src = sub + (int64_t)b * SGN;
+    const int4 job = parts[b];
+    const float2 *src = sub + (int64_t)job.x * SGN;
     for (int i = tid; i < SGN; i += TW_NT) sg[i] = src[i];
     __syncthreads();
     if (nearest) {
@@ -421,26 +417,20 @@ __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
         vis_out[r.w] = sg[(int)(r.x >> 16) * SG + (int)(r.x & 0xffffu)];
       }
     } else {
-      for (int j = job.y + wave; j < job.z; j += NW) {
-        const uint4 r = rec[j];
+      const int jbeg = __builtin_amdgcn_readfirstlane(job.y);
+      const int jend = __builtin_amdgcn_readfirstlane(job.z);
+      for (int j = jbeg + wave; j < jend; j += NW) {
+        const uint4 r = uniform_u4(rec[j]);
         const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
         const int base = (int)(r.x >> 16) * SG + (int)(r.x & 0xffffu);
         const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
         const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
         float sr = 0.f, si = 0.f;
-        for (int t = lane; t < SS; t += 64) {
-          const int i = (int)(((float)t + 0.5f) * invS);
-          float2 kv;
-          if (!SEPARABLE) {
-            kv = k2[t];
-          } else {
-            const float2 p = ky[i], q = kx[t - i * S];
-            kv = make_float2(p.x * q.x - p.y * q.y, p.x * q.y + p.y * q.x);
-          }
-          const float2 g = sg[base + i * (SG - S) + t];
-          sr += g.x * kv.x - g.y * kv.y;
-          si += g.x * kv.y + g.y * kv.x;
-        }
+        tap_range<SEPARABLE>(S, invS, 0, SS, lane, k2, ky, kx, [&](int c, float2 k) {
+          const float2 g = sg[base + c];
+          sr += g.x * k.x - g.y * k.y;
+          si += g.x * k.y + g.y * k.x;
+        });
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
           sr += __shfl_xor(sr, off, 64);
@@ -470,7 +460,7 @@ struct Plan {
   TowerArgs a;
   uint4 *loc = nullptr, *rec = nullptr;
   float2 *svis = nullptr;
-  uint32_t *count = nullptr, *cursor = nullptr, *bsum = nullptr, *small = nullptr;
+  uint32_t *small = nullptr;
   int4 *jobs = nullptr;
   int njobs = 0;
   std::vector<int4> hjobs;
@@ -495,7 +485,8 @@ int tw_bin(croc_ctx *ctx, Plan &p, const float2 *vis) {
   const int init[2] = {INT_MAX, INT_MIN};
   CROC_HIP(hipMemcpyAsync(mm, init, sizeof init, hipMemcpyHostToDevice, s));
   const unsigned nb = (unsigned)((a.nv + 255) / 256);
-  k_tw_wrange<<<nb, 256, 0, s>>>(a.uvw, a.nv, a.stride, a.wstep, mm, p.small);
+  k_tw_wrange<<<(unsigned)std::min<int64_t>(nb, 8 * ctx->num_cu), 256, 0, s>>>(
+      a.uvw, a.nv, a.stride, a.wstep, mm, p.small);
   CROC_HIP(hipGetLastError());
   int hmm[2];
   CROC_HIP(hipMemcpyAsync(hmm, mm, sizeof hmm, hipMemcpyDeviceToHost, s));
@@ -506,29 +497,53 @@ int tw_bin(croc_ctx *ctx, Plan &p, const float2 *vis) {
   CROC_REQUIRE(nkeys < ((int64_t)1 << 31), CROC_EINVAL,
                "too many (chunk, w-plane) pairs (%lld); increase wstep", (long long)nkeys);
 
-  const int64_t nblk = (nkeys + SCAN_SPAN - 1) / SCAN_SPAN;
-  char *t = (char *)ws_get(ctx, WS_TOWER,
-                           sizeof(uint32_t) * (2 * nkeys + 2 * nblk + 64) + 256);
+  const uint32_t bad = (uint32_t)nkeys;
+  int end_bit = 1;
+  while (end_bit < 32 && ((uint64_t)1 << end_bit) <= bad) end_bit++;
+  const size_t n = (size_t)a.nv;
+  uint32_t *keys = nullptr, *skeys = nullptr, *idx = nullptr, *sidx = nullptr;
+  uint32_t *ukeys = nullptr, *counts = nullptr, *starts = nullptr, *nruns_d = nullptr;
+  size_t t_sort = 0, t_rle = 0, t_scan = 0;
+  CROC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t_sort, keys, skeys, idx, sidx, (int)n,
+                                              0, end_bit, s));
+  CROC_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, t_rle, skeys, ukeys, counts, nruns_d,
+                                                 (int)n, s));
+  CROC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, counts, starts, (int)n, s));
+  const size_t t_bytes = std::max(t_sort, std::max(t_rle, t_scan));
+  char *w = (char *)ws_get(ctx, WS_SORT, sizeof(uint32_t) * (7 * n + 64) + t_bytes + 256);
   p.loc = (uint4 *)ws_get(ctx, WS_LOC, sizeof(uint4) * a.nv);
   p.rec = (uint4 *)ws_get(ctx, WS_REC, sizeof(uint4) * a.nv);
   if (vis) p.svis = (float2 *)ws_get(ctx, WS_SVIS, sizeof(float2) * a.nv);
   p.jobs = (int4 *)ws_get(ctx, WS_ITEMS, sizeof(int4) * (a.nv + 1));
-  if (!t || !p.loc || !p.rec || (vis && !p.svis) || !p.jobs) return CROC_ENOMEM;
-  p.count = (uint32_t *)t;
-  p.cursor = p.count + nkeys;
-  p.bsum = p.cursor + nkeys;
-  uint32_t *totals = p.bsum + 2 * nblk;
-  if ((rc = zero_u32_async(ctx, p.count, nkeys))) return rc;
-  k_tw_locate<<<nb, 256, 0, s>>>(a, p.loc, p.count, p.small);
-  k_scan_blocks<<<(unsigned)nblk, SCAN_NT, 0, s>>>(p.count, nkeys, p.bsum);
-  k_scan_top<<<1, SCAN_NT, 0, s>>>(p.bsum, (int)nblk, totals);
-  k_scan_apply<<<(unsigned)nblk, SCAN_NT, 0, s>>>(p.count, nkeys, p.bsum, p.cursor, p.jobs);
-  k_tw_scatter<<<nb, 256, 0, s>>>(a.nv, p.loc, p.cursor, p.rec, vis, p.svis);
+  if (!w || !p.loc || !p.rec || (vis && !p.svis) || !p.jobs) return CROC_ENOMEM;
+  keys = (uint32_t *)w;
+  skeys = keys + n;
+  idx = skeys + n;
+  sidx = idx + n;
+  ukeys = sidx + n;
+  counts = ukeys + n;
+  starts = counts + n;
+  nruns_d = starts + n;
+  void *temp = (void *)(((uintptr_t)(nruns_d + 16) + 255) & ~(uintptr_t)255);
+  k_tw_locate<<<nb, 256, 0, s>>>(a, bad, p.loc, keys, idx, p.small);
   CROC_HIP(hipGetLastError());
-  uint32_t htot[2];
-  CROC_HIP(hipMemcpyAsync(htot, totals, sizeof htot, hipMemcpyDeviceToHost, s));
   if ((rc = tw_sync_status(ctx, p.small))) return rc;
-  p.njobs = (int)htot[1];
+  size_t tb = t_bytes;
+  CROC_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, keys, skeys, idx, sidx, (int)n, 0,
+                                              end_bit, s));
+  tb = t_bytes;
+  CROC_HIP(hipcub::DeviceRunLengthEncode::Encode(temp, tb, skeys, ukeys, counts, nruns_d, (int)n,
+                                                 s));
+  k_tw_gather<<<nb, 256, 0, s>>>(a.nv, sidx, p.loc, vis, p.rec, p.svis);
+  CROC_HIP(hipGetLastError());
+  int nruns = 0;
+  CROC_HIP(hipMemcpyAsync(&nruns, nruns_d, sizeof(int), hipMemcpyDeviceToHost, s));
+  CROC_HIP(hipStreamSynchronize(s));
+  tb = t_bytes;
+  CROC_HIP(hipcub::DeviceScan::ExclusiveSum(temp, tb, counts, starts, nruns, s));
+  k_tw_jobs<<<(unsigned)((nruns + 255) / 256), 256, 0, s>>>(ukeys, counts, starts, nruns, p.jobs);
+  CROC_HIP(hipGetLastError());
+  p.njobs = nruns;
   p.hjobs.resize(p.njobs);
   if (p.njobs) {
     CROC_HIP(hipMemcpyAsync(p.hjobs.data(), p.jobs, sizeof(int4) * p.njobs,
@@ -596,81 +611,131 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
   const TowerArgs &a = p.a;
   hipStream_t s = ctx->stream;
 
+  // Work plan on the host: jobs are cut into parts of at most PART_VIS
+  // visibilities (at most JOB_BATCH / 4 parts; longer parts beyond that);
+  // batches hold whole jobs and at most JOB_BATCH part slots.  Per batch:
+  // parts (slot, first, end), the first slot of each job, the multi-part
+  // jobs (first slot, parts) and the chunk groups (chunk, first job, end job).
+  struct Batch {
+    int j0, nj;
+    size_t p0, np, m0, nm, g0, ng;
+  };
+  std::vector<Batch> batches;
+  std::vector<int4> parts, groups;
+  std::vector<int2> multi;
+  std::vector<int32_t> jslot(p.njobs);
+  int slots = 0, max_slots = 0;
+  for (int j = 0; j < p.njobs; j++) {
+    const int4 job = p.hjobs[j];
+    const int n = job.z - job.y;
+    const int np = std::min(std::max(1, (n + PART_VIS - 1) / PART_VIS), JOB_BATCH / 4);
+    if (batches.empty() || slots + np > JOB_BATCH) {
+      batches.push_back(Batch{j, 0, parts.size(), 0, multi.size(), 0, groups.size(), 0});
+      slots = 0;
+    }
+    Batch &bt = batches.back();
+    jslot[j] = slots;
+    const int len = (n + np - 1) / np;
+    for (int k = 0; k < np; k++)
+      parts.push_back(make_int4(slots + k, job.y + k * len, std::min(job.z, job.y + (k + 1) * len), 0));
+    if (np > 1) multi.push_back(make_int2(slots, np));
+    const int c = job.x / a.nwp, lj = j - bt.j0;
+    if (bt.ng == 0 || groups.back().x != c) {
+      groups.push_back(make_int4(c, lj, lj + 1, 0));
+      bt.ng++;
+    } else {
+      groups.back().z = lj + 1;
+    }
+    slots += np;
+    max_slots = std::max(max_slots, slots);
+    bt.nj++;
+    bt.np = parts.size() - bt.p0;
+    bt.nm = multi.size() - bt.m0;
+  }
+
   // fixed-point scale, Fresnel phase table, per-job plane multipliers
   const int64_t nk = kern ? (separable ? (int64_t)Q * S : (int64_t)Q * Q * S * S) : 0;
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
-                                                sizeof(int32_t) * (3 * (size_t)p.njobs + 64));
+                                                sizeof(int32_t) * (5 * (size_t)p.njobs + 64));
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
-  const int jb = std::min(p.njobs, JOB_BATCH);
-  float2 *stack = (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)jb);
-  if (!aux || !stack) return CROC_ENOMEM;
-  FixScale *fs = (FixScale *)aux;
+  float2 *stack =
+      (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)max_slots);
+  char *plan = (char *)ws_get(ctx, WS_TILES, sizeof(int4) * (parts.size() + groups.size()) +
+                                                 sizeof(int2) * multi.size() + 256);
+  if (!aux || !stack || !plan) return CROC_ENOMEM;
+  TwScale *fs = (TwScale *)aux;
   double *phi = (double *)(aux + 256);
   int32_t *kw = (int32_t *)(phi + SGN);
   int32_t *dchunks = kw + p.njobs;
+  int32_t *djslot = dchunks + p.njobs;
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
+  int4 *dparts = (int4 *)plan;
+  int4 *dgroups = dparts + parts.size();
+  int2 *dmulti = (int2 *)(dgroups + groups.size());
   std::vector<int32_t> hk(p.njobs);
   for (int j = 0; j < p.njobs; j++) hk[j] = p.hjobs[j].x % a.nwp + a.wp_min;
   if ((rc = upload_i32(ctx, kw, hk))) return rc;
   if ((rc = upload_i32(ctx, dchunks, p.chunks))) return rc;
-  if ((rc = zero_u32_async(ctx, (uint32_t *)fs, 4))) return rc;
+  if ((rc = upload_i32(ctx, djslot, jslot))) return rc;
+  CROC_HIP(hipMemcpyAsync(dparts, parts.data(), sizeof(int4) * parts.size(),
+                          hipMemcpyHostToDevice, s));
+  CROC_HIP(hipMemcpyAsync(dgroups, groups.data(), sizeof(int4) * groups.size(),
+                          hipMemcpyHostToDevice, s));
+  if (!multi.empty())
+    CROC_HIP(hipMemcpyAsync(dmulti, multi.data(), sizeof(int2) * multi.size(),
+                            hipMemcpyHostToDevice, s));
+  if ((rc = zero_u32_async(ctx, (uint32_t *)fs, sizeof(TwScale) / 4))) return rc;
   k_tw_maxima<<<4 * ctx->num_cu, 256, 0, s>>>((const float2 *)vis, nv, (const float2 *)kern,
                                               nk, fs);
-  k_tw_pick<<<1, 1, 0, s>>>(fs, a.nearest, separable);
+  k_tw_pick<<<1, 1, 0, s>>>(fs, nv, a.nearest, separable);
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
   CROC_HIP(hipGetLastError());
   // zero the chunk images that will receive data
-  for (int c : p.chunks)
-    if ((rc = zero_u32_async(ctx, (uint32_t *)(cimg + (size_t)c * SGN), 2 * SGN))) return rc;
+  k_tw_zero_chunks<<<dim3(16, (unsigned)p.chunks.size()), 256, 0, s>>>(dchunks, cimg);
+  CROC_HIP(hipGetLastError());
 
-  std::vector<int4> groups;
-  int4 *dgroups = (int4 *)ws_get(ctx, WS_TILES, sizeof(int4) * (size_t)(p.chunks.size() + 64));
-  if (!dgroups) return CROC_ENOMEM;
   CROC_HIP(hipEventRecord(ctx->ev[0], s));
-  for (int j0 = 0; j0 < p.njobs; j0 += jb) {
-    const int nb = std::min(jb, p.njobs - j0);
-    const int G = std::min(nb, 2 * ctx->num_cu);
+  for (const Batch &bt : batches) {
+    const int G = (int)std::min<size_t>(bt.np, 2 * (size_t)ctx->num_cu);
     if (a.nearest || !separable)
-      k_tw_grid_jobs<false><<<G, TW_NT, 0, s>>>(p.jobs, j0, nb, p.rec, p.svis,
+      k_tw_grid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
                                                 (const float2 *)kern, a.Q, a.S, a.nearest, fs,
                                                 sub);
     else
-      k_tw_grid_jobs<true><<<G, TW_NT, 0, s>>>(p.jobs, j0, nb, p.rec, p.svis,
+      k_tw_grid_jobs<true><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
                                                (const float2 *)kern, a.Q, a.S, a.nearest, fs,
                                                sub);
     CROC_HIP(hipGetLastError());
-    // image = ifft(sub) * exp(-i wp phi)
-    rc = dft2d_launch(ctx, sub, sub, SG, nb, SGN, +1, 1.0f / (float)SGN, nullptr, nullptr, 0,
-                      phi, kw + j0, -1, nullptr, nullptr);
-    if (rc) return rc;
-    groups.clear();
-    for (int b = 0; b < nb; b++) {
-      const int c = p.hjobs[j0 + b].x / a.nwp;
-      if (groups.empty() || groups.back().x != c)
-        groups.push_back(make_int4(c, b, b + 1, 0));
-      else
-        groups.back().z = b + 1;
+    if (bt.nm) {
+      k_tw_sum_parts<<<dim3(ACC_SLICES, (unsigned)bt.nm), 256, 0, s>>>(dmulti + bt.m0, sub);
+      CROC_HIP(hipGetLastError());
     }
-    CROC_HIP(hipMemcpyAsync(dgroups, groups.data(), sizeof(int4) * groups.size(),
-                            hipMemcpyHostToDevice, s));
-    k_tw_chunk_accum<<<(unsigned)groups.size(), 256, 0, s>>>(dgroups, sub, cimg);
+    // image = ifft(sub[slot]) * exp(-i wp phi), in place in each job's slot
+    rc = dft2d_launch(ctx, sub, sub, SG, bt.nj, SGN, +1, 1.0f / (float)SGN, nullptr, nullptr, 0,
+                      phi, kw + bt.j0, -1, djslot + bt.j0, djslot + bt.j0);
+    if (rc) return rc;
+    k_tw_chunk_accum<<<dim3(ACC_SLICES, (unsigned)bt.ng), 256, 0, s>>>(
+        dgroups + bt.g0, djslot + bt.j0, sub, cimg);
     CROC_HIP(hipGetLastError());
-    CROC_HIP(hipStreamSynchronize(s));  // groups buffer is reused
   }
   // chunk sub-grids = fft(chunk image), then parity-ordered adds
   rc = dft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, -1, 1.0f, nullptr,
                     nullptr, 0, nullptr, nullptr, 0, dchunks, dchunks);
   if (rc) return rc;
-  std::vector<int32_t> par[4];
+  // parity classes, uploaded once, one launch each (stream order keeps
+  // overlapping margins of different classes apart)
+  std::vector<int32_t> par[4], allpar;
   for (int c : p.chunks) par[(c % a.ncx) % 2 + 2 * ((c / a.ncx) % 2)].push_back(c);
-  int32_t *dpar = dchunks + p.chunks.size();  // room: 3 * njobs >= njobs + chunks
+  for (int q = 0; q < 4; q++) allpar.insert(allpar.end(), par[q].begin(), par[q].end());
+  int32_t *dpar = djslot + p.njobs;  // chunks <= njobs entries
+  if ((rc = upload_i32(ctx, dpar, allpar))) return rc;
+  size_t po = 0;
   for (int q = 0; q < 4; q++) {
     if (par[q].empty()) continue;
-    if ((rc = upload_i32(ctx, dpar, par[q]))) return rc;
     dim3 g(16, (unsigned)par[q].size());
-    k_tw_add<<<g, 256, 0, s>>>(dpar, cimg, (float2 *)grid, a.N, a.chunk, a.margin, a.ncx);
+    k_tw_add<<<g, 256, 0, s>>>(dpar + po, cimg, (float2 *)grid, a.N, a.chunk, a.margin, a.ncx);
     CROC_HIP(hipGetLastError());
-    CROC_HIP(hipStreamSynchronize(s));  // dpar is reused
+    po += par[q].size();
   }
   CROC_HIP(hipEventRecord(ctx->ev[1], s));
   CROC_HIP(hipStreamSynchronize(s));
@@ -694,12 +759,26 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   if (p.njobs == 0) return CROC_OK;
   const TowerArgs &a = p.a;
   hipStream_t s = ctx->stream;
+  // batches of at most JOB_BATCH jobs (job b of a batch in slot b); every
+  // job cut into parts of at most PART_VIS visibilities sharing its slot
+  const int jb = std::min(p.njobs, JOB_BATCH);
+  std::vector<int4> parts;
+  std::vector<size_t> poff;
+  for (int j0 = 0; j0 < p.njobs; j0 += jb) {
+    poff.push_back(parts.size());
+    for (int b = 0; b < std::min(jb, p.njobs - j0); b++) {
+      const int4 job = p.hjobs[j0 + b];
+      for (int v = job.y; v < job.z; v += PART_VIS)
+        parts.push_back(make_int4(b, v, std::min(job.z, v + PART_VIS), 0));
+    }
+  }
+  poff.push_back(parts.size());
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
                                                 sizeof(int32_t) * (3 * (size_t)p.njobs + 64));
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
-  const int jb = std::min(p.njobs, JOB_BATCH);
   float2 *stack = (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)jb);
-  if (!aux || !stack) return CROC_ENOMEM;
+  int4 *dparts = (int4 *)ws_get(ctx, WS_TILES, sizeof(int4) * (parts.size() + 16));
+  if (!aux || !stack || !dparts) return CROC_ENOMEM;
   double *phi = (double *)(aux + 256);
   int32_t *kw = (int32_t *)(phi + SGN);
   int32_t *jchunk = kw + p.njobs;
@@ -713,6 +792,8 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   if ((rc = upload_i32(ctx, kw, hk))) return rc;
   if ((rc = upload_i32(ctx, jchunk, hc))) return rc;
   if ((rc = upload_i32(ctx, dchunks, p.chunks))) return rc;
+  CROC_HIP(hipMemcpyAsync(dparts, parts.data(), sizeof(int4) * parts.size(),
+                          hipMemcpyHostToDevice, s));
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
   dim3 g(16, (unsigned)p.chunks.size());
   CROC_HIP(hipEventRecord(ctx->ev[2], s));
@@ -721,19 +802,20 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   rc = dft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, +1, 1.0f / (float)SGN,
                     nullptr, nullptr, 0, nullptr, nullptr, 0, dchunks, dchunks);
   if (rc) return rc;
-  for (int j0 = 0; j0 < p.njobs; j0 += jb) {
+  for (int j0 = 0, bi = 0; j0 < p.njobs; j0 += jb, bi++) {
     const int nb = std::min(jb, p.njobs - j0);
     // plane sub-grid = fft(chunk image * exp(+i wp phi))
     rc = dft2d_launch(ctx, cimg, sub, SG, nb, SGN, -1, 1.0f, phi, kw + j0, +1, nullptr, nullptr,
                       0, jchunk + j0, nullptr);
     if (rc) return rc;
-    const int G = std::min(nb, 2 * ctx->num_cu);
+    const int npart = (int)(poff[bi + 1] - poff[bi]);
+    const int G = std::min(npart, 2 * ctx->num_cu);
     if (a.nearest || !separable)
-      k_tw_degrid_jobs<false><<<G, TW_NT, 0, s>>>(p.jobs, j0, nb, p.rec, sub,
+      k_tw_degrid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
                                                   (const float2 *)kern, a.Q, a.S, a.nearest,
                                                   (float2 *)vis_out);
     else
-      k_tw_degrid_jobs<true><<<G, TW_NT, 0, s>>>(p.jobs, j0, nb, p.rec, sub,
+      k_tw_degrid_jobs<true><<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
                                                  (const float2 *)kern, a.Q, a.S, a.nearest,
                                                  (float2 *)vis_out);
     CROC_HIP(hipGetLastError());

@@ -14,9 +14,14 @@ Workload names:
 * ``wproj2048``  N=2048, theta=0.1, lam=20480, 2e6 vis with surface density
                  r^-1.5, w uniform +-1500, w-kernels every 20 wavelengths with
                  support 16..32 growing with |w|, Q=8.
-* ``towers4096`` N=4096, theta=0.08, lam=51200, 1e7 vis from 128 antennas.
+* ``towers4096`` N=4096, theta=0.08, lam=51200, 1e7 vis from 128 antennas
+                 (Gaussian, sigma 1.5 km) over 6 h at 1.4 GHz, dec -45 deg,
+                 AW-style kernel S=16 Q=16.
 * ``diff1024``   1024^2 image of Gaussian blobs, 1e6 vis, w +-500.
-* ``towers8192`` N=8192, theta=0.05, lam=163840, 5e7 vis from 256 antennas.
+* ``towers8192`` N=8192, theta=0.05, lam=163840, 5e7 vis from 256 antennas
+                 (Gaussian core plus log-spiral arms to 40 km) over 8 h at
+                 150 MHz, dec -45 deg, AW-style kernel S=16 Q=32, 256
+                 w-planes.
 """
 
 import math
@@ -106,6 +111,26 @@ def wproj2048(nv=None, seed=None):
     return uvw, vis
 
 
+def xyz_at_latitude(local_xyz, lat):
+    """Local (east, north, up) antenna coordinates rotated to celestial
+    (x east, z towards the pole) at latitude ``lat`` (simulate.py:48-67)."""
+    x, y, z = local_xyz[:, 0], local_xyz[:, 1], local_xyz[:, 2]
+    c = numpy.pi / 2 - lat
+    return numpy.stack([x, -z * numpy.sin(c) + y * numpy.cos(c),
+                        z * numpy.cos(c) + y * numpy.sin(c)], axis=1)
+
+
+def xyz_to_uvw(xyz, ha, dec):
+    """Celestial (x, y, z) rotated to (u, v, w) for hour angle ``ha`` and
+    declination ``dec`` (simulate.py:74-100); xyz [n, 3], ha scalar or [n]."""
+    x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+    u = x * numpy.cos(ha) - y * numpy.sin(ha)
+    v0 = x * numpy.sin(ha) + y * numpy.cos(ha)
+    w = z * numpy.sin(dec) - v0 * numpy.cos(dec)
+    v = z * numpy.cos(dec) + v0 * numpy.sin(dec)
+    return numpy.stack([u, v, w], axis=1)
+
+
 def antenna_layout(rng, nant, sigma_m=1500.0, arms_to_m=None):
     """Antenna (x, y, z) in metres: a 2D Gaussian core (z small), optionally
     plus three log-spiral arms reaching ``arms_to_m``."""
@@ -122,6 +147,81 @@ def antenna_layout(rng, nant, sigma_m=1500.0, arms_to_m=None):
         xy = numpy.concatenate([core, arms])
     z = rng.normal(0.0, 5.0, (nant, 1))
     return numpy.concatenate([xy, z], axis=1)
+
+
+LATITUDE = -26.82 * numpy.pi / 180   # SKA-Low site
+C_LIGHT = 299792458.0
+
+
+def baseline_tracks(xyz, hours, freq, dec_deg, ndump):
+    """uvw in wavelengths [ndump * nbl, 3] of every antenna pair over
+    ``hours`` of hour angle centred on transit, time-major."""
+    nant = xyz.shape[0]
+    a1, a2 = numpy.triu_indices(nant, 1)
+    bl = xyz_at_latitude(xyz, LATITUDE)
+    bl = bl[a2] - bl[a1]
+    dec = dec_deg * numpy.pi / 180
+    has = (numpy.arange(ndump) + 0.5) / ndump - 0.5
+    has = has * hours / 24 * 2 * numpy.pi
+    lam_m = C_LIGHT / freq
+    out = numpy.empty((ndump, bl.shape[0], 3))
+    for t, ha in enumerate(has):
+        out[t] = xyz_to_uvw(bl, ha, dec) / lam_m
+    return out.reshape(-1, 3)
+
+
+def towers(name, nv=None, seed=None, device=None, margin_cells=48):
+    """(uvw [nv, 3] float64, vis [nv] complex) of the ``towers4096`` /
+    ``towers8192`` workloads: antenna tracks, samples whose (u, v) would lie
+    within ``margin_cells`` cells of the grid edge dropped, the first ``nv``
+    kept; visibilities of 20 point sources plus noise.  With ``device`` the
+    arrays are torch tensors there (the direct Fourier sum then runs on
+    the device, FP64)."""
+    c = SPEC[name]
+    rng = numpy.random.default_rng(c['seed'] if seed is None else seed)
+    nv = c['nv'] if nv is None else nv
+    if name == 'towers8192':
+        xyz = antenna_layout(rng, c['nant'], sigma_m=1500.0, arms_to_m=40000.0)
+    else:
+        xyz = antenna_layout(rng, c['nant'], sigma_m=1500.0)
+    nbl = c['nant'] * (c['nant'] - 1) // 2
+    lim = (c['N'] // 2 - margin_cells) / c['theta']
+    ndump = max(1, -(-nv // nbl))
+    while True:
+        uvw = baseline_tracks(xyz, c['hours'], c['freq'], c['dec'], ndump)
+        keep = (numpy.abs(uvw[:, 0]) < lim) & (numpy.abs(uvw[:, 1]) < lim)
+        if keep.sum() >= nv:
+            break
+        ndump = int(ndump * 1.25 * nv / max(1, keep.sum())) + 1
+    uvw = uvw[keep][:nv]
+    l, m, f = point_sources(rng, c['nsrc'], c['theta'])
+    if device is None:
+        return uvw, add_noise(rng, dft_vis(uvw, l, m, f))
+    import torch
+    ut = torch.as_tensor(uvw, device=device)
+    n1 = numpy.sqrt(1.0 - l ** 2 - m ** 2) - 1.0
+    vis = torch.zeros(nv, dtype=torch.complex128, device=device)
+    for k in range(len(l)):
+        ph = ut[:, 0] * l[k] + ut[:, 1] * m[k] + ut[:, 2] * n1[k]
+        vis += f[k] * torch.exp(-2j * numpy.pi * ph)
+    g = torch.Generator(device=device).manual_seed(int(c['seed']))
+    vis += 0.01 * torch.complex(torch.randn(nv, generator=g, device=device, dtype=torch.float64),
+                                torch.randn(nv, generator=g, device=device, dtype=torch.float64))
+    return ut, vis
+
+
+def aw_far_field(n, theta, c_pswf=None):
+    """Far field [n, n] of an AW-style kernel without its w part: the PSWF
+    anti-aliasing function times a Gaussian primary-beam A-term (FWHM =
+    the field of view).  Input to kernel_oversample."""
+    import scipy.special
+    x = numpy.arange(-n // 2, n - n // 2) / n
+    c = n // 2 if c_pswf is None else c_pswf
+    aa = scipy.special.pro_ang1(0, 0, c, numpy.clip(2 * numpy.abs(x), 0, 0.999999))[0]
+    ll = x * theta
+    beam = numpy.exp(-4 * numpy.log(2) * (ll / theta) ** 2)
+    a1 = aa * beam
+    return numpy.outer(a1, a1)
 
 
 def gaussian_blobs(rng, N, nblob=50, sigma=(1.0, 4.0)):
