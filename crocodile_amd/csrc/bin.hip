@@ -239,7 +239,7 @@ int choose_tile(croc_ctx *ctx, int max_support) {
   return TILE_R + 1 - s16;  // keeps T + S - 1 <= TILE_R
 }
 
-int bin_visibilities(croc_ctx *ctx, const BinArgs &a, int max_support,
+int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_support,
                      int order, BinPlan *plan) {
   CROC_REQUIRE(max_support >= 1 && max_support <= 64, CROC_EINVAL,
                "kernel support %d outside [1, 64]", max_support);
@@ -279,13 +279,13 @@ int bin_visibilities(croc_ctx *ctx, const BinArgs &a, int max_support,
   p.nbins = p.ntiles * p.ngroups;
   p.max_items = a.nv / p.chunk + p.nbins + 1;
 
-  p.loc = (uint4 *)ws_get(ctx, WS_LOC, sizeof(uint4) * a.nv);
-  p.rec = (uint4 *)ws_get(ctx, WS_REC, sizeof(uint4) * a.nv);
-  if (a.vis) p.svis = (float2 *)ws_get(ctx, WS_SVIS, sizeof(float2) * a.nv);
-  uint32_t *tiles = (uint32_t *)ws_get(ctx, WS_TILES,
+  p.loc = (uint4 *)ws_get_in(ctx, set, WS_LOC, sizeof(uint4) * a.nv);
+  p.rec = (uint4 *)ws_get_in(ctx, set, WS_REC, sizeof(uint4) * a.nv);
+  if (a.vis) p.svis = (float2 *)ws_get_in(ctx, set, WS_SVIS, sizeof(float2) * a.nv);
+  uint32_t *tiles = (uint32_t *)ws_get_in(ctx, set, WS_TILES,
                                        sizeof(uint32_t) * (3 * (size_t)p.nbins + 1));
-  p.items = (int4 *)ws_get(ctx, WS_ITEMS, sizeof(int4) * p.max_items);
-  uint32_t *small = (uint32_t *)ws_get(ctx, WS_SMALL, 256);
+  p.items = (int4 *)ws_get_in(ctx, set, WS_ITEMS, sizeof(int4) * p.max_items);
+  uint32_t *small = (uint32_t *)ws_get_in(ctx, set, WS_SMALL, 256);
   if (!p.loc || !p.rec || (a.vis && !p.svis) || !tiles || !p.items || !small)
     return CROC_ENOMEM;
   p.count = tiles;

@@ -47,18 +47,18 @@ int ctx_enter(croc_ctx *ctx) {
   return CROC_OK;
 }
 
-void *ws_get(croc_ctx *ctx, WsSlot slot, size_t bytes) {
+void *ws_get_in(croc_ctx *ctx, WsSet &set, WsSlot slot, size_t bytes) {
   if (bytes == 0) bytes = 256;
-  if (ctx->ws_size[slot] >= bytes) return ctx->ws[slot];
-  if (ctx->ws[slot]) {
+  if (set.ws_size[slot] >= bytes) return set.ws[slot];
+  if (set.ws[slot]) {
     // The stream may still use the old buffer.
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
       croc_set_error("stream synchronisation failed while growing workspace");
       return nullptr;
     }
-    (void)hipFree(ctx->ws[slot]);
-    ctx->ws[slot] = nullptr;
-    ctx->ws_size[slot] = 0;
+    (void)hipFree(set.ws[slot]);
+    set.ws[slot] = nullptr;
+    set.ws_size[slot] = 0;
   }
   size_t want = bytes + bytes / 4;  // headroom against repeated growth
   void *p = nullptr;
@@ -66,9 +66,21 @@ void *ws_get(croc_ctx *ctx, WsSlot slot, size_t bytes) {
     croc_set_error("hipMalloc of %zu bytes failed", want);
     return nullptr;
   }
-  ctx->ws[slot] = p;
-  ctx->ws_size[slot] = want;
+  set.ws[slot] = p;
+  set.ws_size[slot] = want;
   return p;
+}
+
+void *ws_get(croc_ctx *ctx, WsSlot slot, size_t bytes) {
+  return ws_get_in(ctx, ctx->ws, slot, bytes);
+}
+
+void ws_free(WsSet &set) {
+  for (int i = 0; i < WS_COUNT; i++) {
+    if (set.ws[i]) (void)hipFree(set.ws[i]);
+    set.ws[i] = nullptr;
+    set.ws_size[i] = 0;
+  }
 }
 
 extern "C" int croc_ctx_create(int device, croc_ctx **out) {
@@ -103,8 +115,7 @@ extern "C" int croc_ctx_destroy(croc_ctx *ctx) {
   if (!ctx) return CROC_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  for (int i = 0; i < WS_COUNT; i++)
-    if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
+  ws_free(ctx->ws);
   for (int i = 0; i < 4; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->host_status) (void)hipHostFree(ctx->host_status);
@@ -134,8 +145,18 @@ extern "C" int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk,
 extern "C" int croc_last_kernel_ms(croc_ctx *ctx, float *grid_ms,
                                    float *degrid_ms) {
   CROC_REQUIRE(ctx != nullptr, CROC_EINVAL, "null context");
-  if (grid_ms) *grid_ms = ctx->grid_ms;
-  if (degrid_ms) *degrid_ms = ctx->degrid_ms;
+  CROC_HIP(hipSetDevice(ctx->device));
+  float g = 0.f, d = 0.f;
+  if (ctx->ev_grid) {
+    CROC_HIP(hipEventSynchronize(ctx->ev[1]));
+    CROC_HIP(hipEventElapsedTime(&g, ctx->ev[0], ctx->ev[1]));
+  }
+  if (ctx->ev_degrid) {
+    CROC_HIP(hipEventSynchronize(ctx->ev[3]));
+    CROC_HIP(hipEventElapsedTime(&d, ctx->ev[2], ctx->ev[3]));
+  }
+  if (grid_ms) *grid_ms = g;
+  if (degrid_ms) *degrid_ms = d;
   return CROC_OK;
 }
 

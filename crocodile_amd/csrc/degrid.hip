@@ -116,6 +116,25 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
 }  // namespace
 
+// Degridding of binned visibilities from grid into vis_out.  No status read.
+int degrid_binned(croc_ctx *ctx, const BinPlan &plan, const void *grid, int N, void *vis_out,
+                  const void *kern, int separable) {
+  const int G = persistent_blocks(ctx, 2);
+  CROC_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+  if (separable)
+    k_degrid_tiles<DEGRID_NT, true><<<G, DEGRID_NT, 0, ctx->stream>>>(
+        (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
+        (const float2 *)kern, plan.items, plan.nitems, plan.qhead);
+  else
+    k_degrid_tiles<DEGRID_NT, false><<<G, DEGRID_NT, 0, ctx->stream>>>(
+        (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
+        (const float2 *)kern, plan.items, plan.nitems, plan.qhead);
+  CROC_HIP(hipGetLastError());
+  CROC_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+  ctx->ev_degrid = true;
+  return CROC_OK;
+}
+
 extern "C" int croc_convdegrid(croc_ctx *ctx, const void *grid, int N,
                                const double *uvw, int64_t nv, int64_t stride,
                                double lam, void *vis_out, const void *kern,
@@ -140,21 +159,10 @@ extern "C" int croc_convdegrid(croc_ctx *ctx, const void *grid, int N,
   BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_off, separable,
             plane_mode, vplane, wstep, wbin0, nullptr, max_support};
   BinPlan plan;
-  rc = bin_visibilities(ctx, a, max_support, ITEMS_BY_GROUP, &plan);
+  rc = bin_visibilities(ctx, ctx->ws, a, max_support, ITEMS_BY_GROUP, &plan);
   if (rc) return rc;
-  const int G = persistent_blocks(ctx, 2);
-  CROC_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-  if (separable)
-    k_degrid_tiles<DEGRID_NT, true><<<G, DEGRID_NT, 0, ctx->stream>>>(
-        (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
-        (const float2 *)kern, plan.items, plan.nitems, plan.qhead);
-  else
-    k_degrid_tiles<DEGRID_NT, false><<<G, DEGRID_NT, 0, ctx->stream>>>(
-        (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
-        (const float2 *)kern, plan.items, plan.nitems, plan.qhead);
-  CROC_HIP(hipGetLastError());
-  CROC_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+  rc = degrid_binned(ctx, plan, grid, N, vis_out, kern, separable);
+  if (rc) return rc;
   rc = read_status(ctx, plan);
-  (void)hipEventElapsedTime(&ctx->degrid_ms, ctx->ev[2], ctx->ev[3]);
   return rc;
 }

@@ -234,7 +234,7 @@ int dft_tables(croc_ctx *ctx, int n, int sign, const _Float16 **out) {
   CROC_REQUIRE(n == DFT_N, CROC_EINVAL, "sub-grid size %d unsupported (only %d)", n, DFT_N);
   const WsSlot slot = sign < 0 ? WS_DFT_FWD : WS_DFT_INV;
   const size_t bytes = sizeof(_Float16) * 4 * n * n;
-  const bool fresh = ctx->ws_size[slot] < bytes;
+  const bool fresh = ctx->ws.ws_size[slot] < bytes;
   _Float16 *d = (_Float16 *)ws_get(ctx, slot, bytes);
   if (!d) return CROC_ENOMEM;
   if (fresh) {

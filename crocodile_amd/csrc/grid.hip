@@ -223,6 +223,46 @@ __global__ void k_grid_finish(float2 *__restrict__ grid,
 
 }  // namespace
 
+// Gridding of binned visibilities (svis in record order) into grid; vis is
+// the caller's array, read for the fixed-point scale only.  No status read.
+int grid_binned(croc_ctx *ctx, const BinPlan &plan, const float2 *svis, void *grid, int N,
+                int64_t nv, const void *vis, const void *kern, int Q, int separable,
+                int nplanes, const int64_t *plane_off, const int32_t *plane_s) {
+  int rc;
+  const int64_t cells = (int64_t)N * N;
+  unsigned long long *acc =
+      (unsigned long long *)ws_get(ctx, WS_ACC, sizeof(unsigned long long) * 2 * cells);
+  ScaleState *st = (ScaleState *)ws_get(ctx, WS_SCALE, sizeof(ScaleState));
+  if (!acc || !st) return CROC_ENOMEM;
+  rc = zero_u32_async(ctx, (uint32_t *)acc, 4 * cells);
+  if (rc) return rc;
+  rc = zero_u32_async(ctx, (uint32_t *)st, sizeof(ScaleState) / 4);
+  if (rc) return rc;
+  k_scale_maxima<<<4 * ctx->num_cu, 256, 0, ctx->stream>>>(
+      (const float2 *)vis, nv, (const float2 *)kern, plane_off, plane_s, nplanes, Q,
+      separable, st);
+  k_scale_pick<<<1, 1, 0, ctx->stream>>>(st, nv, separable);
+  CROC_HIP(hipGetLastError());
+
+  const int G = persistent_blocks(ctx, 1);
+  CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  if (separable)
+    k_grid_tiles<GRID_NT, true><<<G, GRID_NT, 0, ctx->stream>>>(
+        acc, N, plan.T, plan.ntx, plan.rec, svis, (const float2 *)kern, plan.items,
+        plan.nitems, plan.qhead, st);
+  else
+    k_grid_tiles<GRID_NT, false><<<G, GRID_NT, 0, ctx->stream>>>(
+        acc, N, plan.T, plan.ntx, plan.rec, svis, (const float2 *)kern, plan.items,
+        plan.nitems, plan.qhead, st);
+  CROC_HIP(hipGetLastError());
+  CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  ctx->ev_grid = true;
+  k_grid_finish<<<(unsigned)((cells + 255) / 256), 256, 0, ctx->stream>>>(
+      (float2 *)grid, (const long long *)acc, cells, st);
+  CROC_HIP(hipGetLastError());
+  return CROC_OK;
+}
+
 extern "C" int croc_convgrid(croc_ctx *ctx, void *grid, int N,
                              const double *uvw, int64_t nv, int64_t stride,
                              double lam, const void *vis, const void *kern,
@@ -246,40 +286,11 @@ extern "C" int croc_convgrid(croc_ctx *ctx, void *grid, int N,
   BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_off, separable,
             plane_mode, vplane, wstep, wbin0, (const float2 *)vis, max_support};
   BinPlan plan;
-  rc = bin_visibilities(ctx, a, max_support, ITEMS_BY_TILE, &plan);
+  rc = bin_visibilities(ctx, ctx->ws, a, max_support, ITEMS_BY_TILE, &plan);
   if (rc) return rc;
-
-  const int64_t cells = (int64_t)N * N;
-  unsigned long long *acc =
-      (unsigned long long *)ws_get(ctx, WS_ACC, sizeof(unsigned long long) * 2 * cells);
-  ScaleState *st = (ScaleState *)ws_get(ctx, WS_SCALE, sizeof(ScaleState));
-  if (!acc || !st) return CROC_ENOMEM;
-  rc = zero_u32_async(ctx, (uint32_t *)acc, 4 * cells);
+  rc = grid_binned(ctx, plan, plan.svis, grid, N, nv, vis, kern, Q, separable, nplanes,
+                   plane_off, plane_s);
   if (rc) return rc;
-  rc = zero_u32_async(ctx, (uint32_t *)st, sizeof(ScaleState) / 4);
-  if (rc) return rc;
-  k_scale_maxima<<<4 * ctx->num_cu, 256, 0, ctx->stream>>>(
-      (const float2 *)vis, nv, (const float2 *)kern, plane_off, plane_s, nplanes, Q,
-      separable, st);
-  k_scale_pick<<<1, 1, 0, ctx->stream>>>(st, nv, separable);
-  CROC_HIP(hipGetLastError());
-
-  const int G = persistent_blocks(ctx, 1);
-  CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-  if (separable)
-    k_grid_tiles<GRID_NT, true><<<G, GRID_NT, 0, ctx->stream>>>(
-        acc, N, plan.T, plan.ntx, plan.rec, plan.svis, (const float2 *)kern,
-        plan.items, plan.nitems, plan.qhead, st);
-  else
-    k_grid_tiles<GRID_NT, false><<<G, GRID_NT, 0, ctx->stream>>>(
-        acc, N, plan.T, plan.ntx, plan.rec, plan.svis, (const float2 *)kern,
-        plan.items, plan.nitems, plan.qhead, st);
-  CROC_HIP(hipGetLastError());
-  CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-  k_grid_finish<<<(unsigned)((cells + 255) / 256), 256, 0, ctx->stream>>>(
-      (float2 *)grid, (const long long *)acc, cells, st);
-  CROC_HIP(hipGetLastError());
   rc = read_status(ctx, plan);
-  (void)hipEventElapsedTime(&ctx->grid_ms, ctx->ev[0], ctx->ev[1]);
   return rc;
 }

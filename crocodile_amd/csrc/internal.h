@@ -63,14 +63,21 @@ enum WsSlot {
   WS_COUNT
 };
 
+// A set of grow-only device buffers, one per slot.
+struct WsSet {
+  void *ws[WS_COUNT] = {};
+  size_t ws_size[WS_COUNT] = {};
+};
+
 struct croc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  void *ws[WS_COUNT] = {};
-  size_t ws_size[WS_COUNT] = {};
+  WsSet ws;                         // the context's own workspace
   uint32_t *host_status = nullptr;  // pinned
   hipEvent_t ev[4] = {};
-  float grid_ms = 0.f, degrid_ms = 0.f;
+  // ev[0], ev[1] bracket the last gridding kernel(s), ev[2], ev[3] the last
+  // degridding; the durations are read on demand (croc_last_kernel_ms)
+  bool ev_grid = false, ev_degrid = false;
   int num_cu = 256;
   int tile = 0;     // 0: automatic
   int chunk = 1024; // visibilities per work item
@@ -78,7 +85,10 @@ struct croc_ctx {
 };
 
 // Grow-only workspace; returns nullptr on failure (error already set).
+// ws_get uses the context's set; ws_get_in a given one (a plan's).
 void *ws_get(croc_ctx *ctx, WsSlot slot, size_t bytes);
+void *ws_get_in(croc_ctx *ctx, WsSet &set, WsSlot slot, size_t bytes);
+void ws_free(WsSet &set);
 
 // Make the context's device current.
 int ctx_enter(croc_ctx *ctx);
@@ -204,12 +214,19 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 }
 
 int choose_tile(croc_ctx *ctx, int max_support);
+// Tile-kernel launches on binned visibilities (grid.hip, degrid.hip); they
+// read no status.  svis: the visibilities in record order.
+int grid_binned(croc_ctx *ctx, const BinPlan &plan, const float2 *svis, void *grid, int N,
+                int64_t nv, const void *vis, const void *kern, int Q, int separable,
+                int nplanes, const int64_t *plane_off, const int32_t *plane_s);
+int degrid_binned(croc_ctx *ctx, const BinPlan &plan, const void *grid, int N, void *vis_out,
+                  const void *kern, int separable);
 // Work-item order: ITEMS_BY_TILE cuts each tile's records (all groups) into
 // items (gridding: one LDS flush per item); ITEMS_BY_GROUP cuts each (tile,
 // group) bin and queues them group-major (degridding: all CUs read the same
 // slice of the kernel bank at a time).
 enum { ITEMS_BY_TILE = 0, ITEMS_BY_GROUP = 1 };
-int bin_visibilities(croc_ctx *ctx, const BinArgs &a, int max_support,
+int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_support,
                      int order, BinPlan *plan);
 int read_status(croc_ctx *ctx, const BinPlan &plan);
 int persistent_blocks(croc_ctx *ctx, int per_cu);

@@ -738,8 +738,8 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
     po += par[q].size();
   }
   CROC_HIP(hipEventRecord(ctx->ev[1], s));
+  ctx->ev_grid = true;
   CROC_HIP(hipStreamSynchronize(s));
-  (void)hipEventElapsedTime(&ctx->grid_ms, ctx->ev[0], ctx->ev[1]);
   return CROC_OK;
 }
 
@@ -821,7 +821,7 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
     CROC_HIP(hipGetLastError());
   }
   CROC_HIP(hipEventRecord(ctx->ev[3], s));
+  ctx->ev_degrid = true;
   CROC_HIP(hipStreamSynchronize(s));
-  (void)hipEventElapsedTime(&ctx->degrid_ms, ctx->ev[2], ctx->ev[3]);
   return CROC_OK;
 }

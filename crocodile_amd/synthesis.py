@@ -16,6 +16,7 @@ FP32 with compensated accumulation.  Nothing here falls back to the CPU; a
 missing library or GPU raises.
 """
 
+import ctypes
 import math
 import numbers
 
@@ -462,6 +463,76 @@ def degrid_device(bank, guv, uvw, lam, out, mode=_lib.PLANE_SINGLE, vplane=None,
     """Degrid complex64 device grid ``guv`` into ``out`` (complex64 [nv])."""
     _grid_call(_lib.load().croc_convdegrid, guv, guv.shape[0], uvw, lam, out, bank,
                mode, vplane, wstep, wbin0)
+
+
+class GridPlan:
+    """Binning of one set of coordinates, reused by repeated gridding and
+    degridding of the same visibilities with one kernel bank layout (the
+    C ABI's croc_plan).  bin() validates every coordinate exactly as
+    convgrid does (same exceptions); grid() / degrid() then only enqueue
+    device work.  Constructed with ``uvw`` it bins at once.
+
+    ``uvw`` is a float64 device tensor [nv, 2 or 3]; ``bank`` a
+    :class:`KernelBank` whose layout (plane offsets, supports) the plan
+    keeps; its table values may be replaced by a bank of the same layout.
+    The plan's buffers are kept and reused by later bin() calls.
+    """
+
+    def __init__(self, bank, N, uvw=None, lam=None, mode=_lib.PLANE_SINGLE, vplane=None,
+                 wstep=1.0, wbin0=0):
+        self.bank, self.N, self.nv = bank, int(N), 0
+        h = _lib._P()
+        _lib.check(_lib.load().croc_plan_create(_lib.ctx(), ctypes.byref(h)))
+        self._h = h
+        if uvw is not None:
+            self.bin(uvw, lam, mode, vplane, wstep, wbin0)
+
+    def bin(self, uvw, lam, mode=_lib.PLANE_SINGLE, vplane=None, wstep=1.0, wbin0=0):
+        """Bin coordinates ``uvw / lam`` (plane selection as convgrid)."""
+        if not uvw.is_contiguous():
+            raise ValueError('device arrays must be contiguous')
+        bank = self.bank
+        self.nv = 0
+        _lib.check(_lib.load().croc_plan_bin(
+            _lib.ctx(), self._h, self.N, _lib.ptr(uvw), uvw.shape[0], uvw.shape[1],
+            float(lam), bank.Q, int(bank.separable), bank.nplanes, _lib.ptr(bank.plane_off),
+            _lib.ptr(bank.plane_s), bank.max_support, int(mode), _lib.ptr(vplane),
+            float(wstep), int(wbin0)))
+        self.nv = int(uvw.shape[0])
+
+    def _same_layout(self, bank):
+        if bank is not self.bank and not (
+                bank.Q == self.bank.Q and bank.separable == self.bank.separable
+                and numpy.array_equal(bank.plane_off_h, self.bank.plane_off_h)
+                and numpy.array_equal(bank.plane_s_h, self.bank.plane_s_h)):
+            raise ValueError('kernel bank layout differs from the planned one')
+
+    def grid(self, guv, vis, bank=None):
+        """guv (complex64 device [N, N]) += convolution of vis (complex64 [nv])."""
+        bank = self.bank if bank is None else bank
+        self._same_layout(bank)
+        if guv.shape != (self.N, self.N) or vis.shape[0] != self.nv:
+            raise ValueError('grid or visibility shape differs from the plan')
+        _lib.check(_lib.load().croc_convgrid_plan(_lib.ctx(), self._h, _lib.ptr(guv),
+                                                  _lib.ptr(vis), _lib.ptr(bank.kern)))
+
+    def degrid(self, guv, out, bank=None):
+        """out (complex64 device [nv]) = degridded visibilities of guv."""
+        bank = self.bank if bank is None else bank
+        self._same_layout(bank)
+        if guv.shape != (self.N, self.N) or out.shape[0] != self.nv:
+            raise ValueError('grid or visibility shape differs from the plan')
+        _lib.check(_lib.load().croc_convdegrid_plan(_lib.ctx(), self._h, _lib.ptr(guv),
+                                                    _lib.ptr(out), _lib.ptr(bank.kern)))
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and _lib._lib is not None:
+            try:
+                _lib.load().croc_plan_destroy(_lib.ctx(), h)
+            except Exception:
+                pass
+            self._h = None
 
 
 def _square(a):

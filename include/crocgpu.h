@@ -102,6 +102,32 @@ int croc_convdegrid(croc_ctx *ctx, const void *grid, int N, const double *uvw,
                     int max_support, int plane_mode, const int32_t *vplane,
                     double wstep, int wbin0);
 
+/* Plans (no reference counterpart: crocodile bins nothing, each convgrid
+ * call walks its visibilities one by one).  A plan holds the binning of one
+ * set of coordinates for one kernel-bank layout, for gridding and
+ * degridding the same visibilities repeatedly, as imaging followed by
+ * prediction in an iterative reconstruction does.  croc_plan_bin takes the
+ * arguments of croc_convgrid without grid / vis / kern, validates every
+ * coordinate with the same error codes and synchronises; after a
+ * successful binning croc_convgrid_plan and croc_convdegrid_plan only
+ * enqueue work on the context stream and report no per-visibility errors.
+ * uvw may change after binning; plane_off and plane_s (device arrays) must
+ * stay valid and unchanged while the plan is used, and kern must keep the
+ * bank layout they describe (its values may change).  Calls with one plan
+ * must not overlap on two streams. */
+typedef struct croc_plan croc_plan;
+int croc_plan_create(croc_ctx *ctx, croc_plan **out);
+int croc_plan_destroy(croc_ctx *ctx, croc_plan *plan);
+int croc_plan_bin(croc_ctx *ctx, croc_plan *plan, int N, const double *uvw,
+                  int64_t nv, int64_t stride, double lam, int Q, int separable,
+                  int nplanes, const int64_t *plane_off, const int32_t *plane_s,
+                  int max_support, int plane_mode, const int32_t *vplane,
+                  double wstep, int wbin0);
+int croc_convgrid_plan(croc_ctx *ctx, croc_plan *plan, void *grid,
+                       const void *vis, const void *kern);
+int croc_convdegrid_plan(croc_ctx *ctx, croc_plan *plan, const void *grid,
+                         void *vis_out, const void *kern);
+
 /* grid / degrid (nearest cell): replace crocodile/synthesis.py:389-401 and
  * :404-416, cell N//2 + floor(0.5 + N*p) (round-half-up). */
 int croc_grid_nearest(croc_ctx *ctx, void *grid, int N, const double *uvw,

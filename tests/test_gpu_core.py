@@ -388,3 +388,54 @@ def test_gridding_dynamic_range(gpu):
     a = numpy.zeros((256, 256), dtype=complex)
     G.convgrid(gcf, a, p, None, v)
     assert_close(a, ref, what='dynamic range')
+
+
+def test_plan_matches_one_shot_calls(gpu):
+    """A plan bins once; its gridding is bit-identical to convgrid's and its
+    degridding identical to convdegrid's, on repeated calls with new
+    visibility values and new kernel values of the same layout."""
+    rng = numpy.random.default_rng(21)
+    N, S, Q, nv, planes = 512, 16, 8, 40000, 5
+    tables, p, v = random_case(rng, N, S, Q, nv, planes)
+    bank = G.KernelBank.from_tables(tables)
+    bank2 = G.KernelBank.from_tables([t * (1 + 0.5j) for t in tables])
+    uvw = torch.as_tensor(p, device=gpu).contiguous()
+    vp = torch.as_tensor(rng.integers(0, planes, nv).astype(numpy.int32), device=gpu)
+    plan = G.GridPlan(bank, N, uvw, 1.0, _lib.PLANE_INDEX, vp)
+    for it, bk in enumerate((bank, bank2, bank)):
+        vis = torch.as_tensor(rng.standard_normal(nv) + 1j * rng.standard_normal(nv),
+                              device=gpu).to(torch.complex64)
+        a = torch.zeros((N, N), dtype=torch.complex64, device=gpu)
+        b = torch.zeros((N, N), dtype=torch.complex64, device=gpu)
+        G.grid_device(bk, a, uvw, 1.0, vis, _lib.PLANE_INDEX, vp)
+        plan.grid(b, vis, bk)
+        assert torch.equal(a, b), 'grid differs on call %d' % it
+        o1 = torch.empty(nv, dtype=torch.complex64, device=gpu)
+        o2 = torch.empty(nv, dtype=torch.complex64, device=gpu)
+        G.degrid_device(bk, a, uvw, 1.0, o1, _lib.PLANE_INDEX, vp)
+        plan.degrid(a, o2, bk)
+        assert torch.equal(o1, o2), 'degrid differs on call %d' % it
+
+
+def test_plan_errors(gpu):
+    rng = numpy.random.default_rng(22)
+    (gcf,), p, v = random_case(rng, 128, 8, 4, 1000)
+    bank = G.KernelBank.from_tables([gcf])
+    bad = p.copy()
+    bad[7, 0] = 0.5
+    plan = G.GridPlan(bank, 128)
+    with pytest.raises(AssertionError):
+        plan.bin(torch.as_tensor(bad, device=gpu).contiguous(), 1.0)
+    g = torch.zeros((128, 128), dtype=torch.complex64, device=gpu)
+    vis = torch.zeros(1000, dtype=torch.complex64, device=gpu)
+    with pytest.raises(ValueError):
+        plan.grid(g, vis)  # no successful binning
+    edge = p.copy()
+    edge[3, 1] = 0.5 - 1.0 / 128  # window leaves the grid
+    with pytest.raises(ValueError):
+        plan.bin(torch.as_tensor(edge, device=gpu).contiguous(), 1.0)
+    plan.bin(torch.as_tensor(p, device=gpu).contiguous(), 1.0)
+    plan.grid(g, vis)
+    other = G.KernelBank.from_tables([numpy.zeros((4, 4, 6, 6), complex)])
+    with pytest.raises(ValueError):
+        plan.grid(g, vis, other)
