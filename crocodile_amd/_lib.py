@@ -48,6 +48,7 @@ SIGNATURES = {
     'croc_kernel_oversample': [_P, _P, _I, _I, _I, _I, _P],
     'croc_make_hermitian': [_P, _P, _I, _I],
     'croc_dft2d_batched': [_P, _P, _P, _I, _I64, _I],
+    'croc_fft2d_batched': [_P, _P, _P, _I, _I64, _I],
     'croc_convolve_same': [_P, _P, _I, _I, _P, _I, _I, _I64, _P],
     'croc_wtowers_grid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I],
     'croc_wtowers_degrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I],

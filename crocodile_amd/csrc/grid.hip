@@ -30,9 +30,7 @@ struct ScaleState {
   double inv_scale;           // 2^-e
 };
 
-__device__ __forceinline__ float cabs_max(float2 a) {
-  return fmaxf(fabsf(a.x), fabsf(a.y));
-}
+
 
 // Grid-stride maxima of the visibility and kernel component magnitudes.
 // The bank is scanned as one extent [0, max_k(plane_off[k] + table size)),
@@ -57,24 +55,14 @@ __global__ void k_scale_maxima(const float2 *__restrict__ vis, int64_t nv,
   }
   barrier_after_lds_atomics();
   const int64_t nk = (int64_t)s_nk;
-  float mv = 0.f, mk = 0.f;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int64_t i = i0; i < nv; i += stride) mv = fmaxf(mv, cabs_max(vis[i]));
-  for (int64_t i = i0; i < nk; i += stride) mk = fmaxf(mk, cabs_max(kern[i]));
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    mv = fmaxf(mv, __shfl_xor(mv, off, 64));
-    mk = fmaxf(mk, __shfl_xor(mk, off, 64));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(&st->max_vis_bits, __float_as_uint(mv));
-    atomicMax(&st->max_k_bits, __float_as_uint(mk));
-  }
+  block_max_to(array_max(vis, nv, i0, stride), &st->max_vis_bits);
+  block_max_to(array_max(kern, nk, i0, stride), &st->max_k_bits);
 }
 
 // 2^e with (nv + 1) * c * 2^e < 2^62 (no sum overflows) and c * 2^e < 2^50
-// (the conversion in fixed() below is exact), c = 2 * max|v| * max|k|^(1 or
+// (the conversion in fixed() is exact), c = 2 * max|v| * max|k|^(1 or
 // 2) bounding one contribution.
 __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
   const double mv = (double)__uint_as_float(st->max_vis_bits);
@@ -238,7 +226,7 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, const float2 *svis, void *gr
   if (rc) return rc;
   rc = zero_u32_async(ctx, (uint32_t *)st, sizeof(ScaleState) / 4);
   if (rc) return rc;
-  k_scale_maxima<<<4 * ctx->num_cu, 256, 0, ctx->stream>>>(
+  k_scale_maxima<<<ctx->num_cu, 1024, 0, ctx->stream>>>(
       (const float2 *)vis, nv, (const float2 *)kern, plane_off, plane_s, nplanes, Q,
       separable, st);
   k_scale_pick<<<1, 1, 0, ctx->stream>>>(st, nv, separable);

@@ -196,6 +196,45 @@ __device__ __forceinline__ unsigned long long fixed(float x, double scale) {
   return (unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull;
 }
 
+__device__ __forceinline__ float cabs_max(float2 a) {
+  return fmaxf(fabsf(a.x), fabsf(a.y));
+}
+
+// max |re|, |im| of a[i0], a[i0 + stride], ...: eight independent loads in
+// flight per thread per round (the scan is latency-bound otherwise).
+__device__ __forceinline__ float array_max(const float2 *__restrict__ a, int64_t n, int64_t i0,
+                                           int64_t stride) {
+  float m[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int64_t i = i0;
+  for (; i + 7 * stride < n; i += 8 * stride) {
+    float2 x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = a[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < 8; k++) m[k] = fmaxf(m[k], cabs_max(x[k]));
+  }
+  for (; i < n; i += stride) m[0] = fmaxf(m[0], cabs_max(a[i]));
+#pragma unroll
+  for (int k = 1; k < 8; k++) m[0] = fmaxf(m[0], m[k]);
+  return m[0];
+}
+
+// Block-wide maximum of non-negative floats, combined into *dst (float
+// bits; non-negative floats order like their bit patterns) with one global
+// atomic per block: thousands of wavefronts hitting one address serialise.
+// blockDim.x must be a multiple of 64, at most 1024.
+__device__ __forceinline__ void block_max_to(float v, unsigned int *dst) {
+  __shared__ unsigned int s_bm;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  if (threadIdx.x == 0) s_bm = 0u;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) atomicMax(&s_bm, __float_as_uint(v));
+  barrier_after_lds_atomics();
+  if (threadIdx.x == 0 && s_bm) atomicMax(dst, s_bm);
+  __syncthreads();
+}
+
 // Values every lane of the wavefront loaded from the same address (the
 // record of the visibility the wave works on): kept in scalar registers, so
 // the arithmetic derived from them runs on the scalar unit.
@@ -235,6 +274,13 @@ int zero_u32_async(croc_ctx *ctx, uint32_t *p, int64_t n);
 // Batched centred 2D DFT (dft.hip): out[oi(b)] = out_scale * D in[ii(b)] D
 // with optional per-pixel modulation exp(i sign k[b] phi[pixel]) of the
 // input and / or output; ii / oi are in_idx / out_idx or the identity.
+// Same contract on the vector ALUs (fft.hip): radix 16 x 8 FFT in LDS, FP32,
+// n = 128 only.
+int fft2d_launch(croc_ctx *ctx, const void *in, void *out, int n, int64_t batch,
+                 int64_t batch_stride, int sign, float out_scale, const double *phi_in,
+                 const int32_t *k_in, int sign_in, const double *phi_out,
+                 const int32_t *k_out, int sign_out, const int32_t *in_idx,
+                 const int32_t *out_idx);
 int dft2d_launch(croc_ctx *ctx, const void *in, void *out, int n, int64_t batch,
                  int64_t batch_stride, int sign, float out_scale, const double *phi_in,
                  const int32_t *k_in, int sign_in, const double *phi_out,

@@ -13,14 +13,14 @@
 //      two passes of sg/2 rows: the pass's rows are held in LDS as exact
 //      64-bit fixed point (the int64 arithmetic of the tile gridder), every
 //      visibility adds the taps of its window rows that fall in the pass;
-//   3. inverse DFT of every job sub-grid on the matrix cores, with the
-//      Fresnel factor exp(-2 pi i wp wstep (1 - n)) applied to the output;
+//   3. inverse FFT of every job sub-grid (fft.hip), with the Fresnel factor
+//      exp(-2 pi i wp wstep (1 - n)) applied to the output;
 //   4. sum the job images of each chunk (one workgroup per chunk and batch,
 //      in job order: deterministic);
-//   5. forward DFT of every chunk image;
+//   5. forward FFT of every chunk image;
 //   6. add the chunk sub-grids into the grid in four passes by chunk parity,
 //      so overlapping margins are never written concurrently (deterministic).
-// Predict reverses it: cut + inverse DFT per chunk, forward DFT with
+// Predict reverses it: cut + inverse FFT per chunk, forward FFT with
 // exp(+2 pi i wp wstep (1 - n)) applied to the input per job, degrid.
 #include <algorithm>
 #include <climits>
@@ -48,8 +48,8 @@ struct TowerArgs {
   int64_t nv, stride;
 };
 
-// Range of w-planes rint(w / wstep): grid-stride, reduced per wavefront
-// before one global atomic per wavefront.
+// Range of w-planes rint(w / wstep): grid-stride, reduced per block before
+// one global atomic pair per block.
 __global__ void k_tw_wrange(const double *__restrict__ uvw, int64_t nv, int64_t stride,
                             double wstep, int *mm, uint32_t *status) {
   int lo = INT_MAX, hi = INT_MIN;
@@ -69,10 +69,21 @@ __global__ void k_tw_wrange(const double *__restrict__ uvw, int64_t nv, int64_t 
     lo = min(lo, __shfl_xor(lo, off, 64));
     hi = max(hi, __shfl_xor(hi, off, 64));
   }
-  if (bad) atomicOr(status, (uint32_t)ST_PLANE);
+  __shared__ int s_lo, s_hi;
+  if (threadIdx.x == 0) {
+    s_lo = INT_MAX;
+    s_hi = INT_MIN;
+  }
+  __syncthreads();
   if ((threadIdx.x & 63) == 0 && lo <= hi) {
-    atomicMin(&mm[0], lo);
-    atomicMax(&mm[1], hi);
+    atomicMin(&s_lo, lo);
+    atomicMax(&s_hi, hi);
+  }
+  barrier_after_lds_atomics();
+  if (bad) atomicOr(status, (uint32_t)ST_PLANE);
+  if (threadIdx.x == 0 && s_lo <= s_hi) {
+    atomicMin(&mm[0], s_lo);
+    atomicMax(&mm[1], s_hi);
   }
 }
 
@@ -177,21 +188,10 @@ struct TwScale {
 
 __global__ void k_tw_maxima(const float2 *__restrict__ vis, int64_t nv,
                             const float2 *__restrict__ kern, int64_t nk, TwScale *fs) {
-  float mv = 0.f, mk = 0.f;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride)
-    mv = fmaxf(mv, fmaxf(fabsf(vis[i].x), fabsf(vis[i].y)));
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += stride)
-    mk = fmaxf(mk, fmaxf(fabsf(kern[i].x), fabsf(kern[i].y)));
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    mv = fmaxf(mv, __shfl_xor(mv, off, 64));
-    mk = fmaxf(mk, __shfl_xor(mk, off, 64));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(&fs->max_vis_bits, __float_as_uint(mv));
-    atomicMax(&fs->max_k_bits, __float_as_uint(mk));
-  }
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  block_max_to(array_max(vis, nv, i0, stride), &fs->max_vis_bits);
+  block_max_to(array_max(kern, nk, i0, stride), &fs->max_k_bits);
 }
 
 // 2^e with (nv + 1) * c * 2^e < 2^62 (no sum overflows) and c * 2^e < 2^50
@@ -485,7 +485,7 @@ int tw_bin(croc_ctx *ctx, Plan &p, const float2 *vis) {
   const int init[2] = {INT_MAX, INT_MIN};
   CROC_HIP(hipMemcpyAsync(mm, init, sizeof init, hipMemcpyHostToDevice, s));
   const unsigned nb = (unsigned)((a.nv + 255) / 256);
-  k_tw_wrange<<<(unsigned)std::min<int64_t>(nb, 8 * ctx->num_cu), 256, 0, s>>>(
+  k_tw_wrange<<<(unsigned)std::min<int64_t>(nb, 2 * ctx->num_cu), 256, 0, s>>>(
       a.uvw, a.nv, a.stride, a.wstep, mm, p.small);
   CROC_HIP(hipGetLastError());
   int hmm[2];
@@ -685,7 +685,7 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
     CROC_HIP(hipMemcpyAsync(dmulti, multi.data(), sizeof(int2) * multi.size(),
                             hipMemcpyHostToDevice, s));
   if ((rc = zero_u32_async(ctx, (uint32_t *)fs, sizeof(TwScale) / 4))) return rc;
-  k_tw_maxima<<<4 * ctx->num_cu, 256, 0, s>>>((const float2 *)vis, nv, (const float2 *)kern,
+  k_tw_maxima<<<ctx->num_cu, 1024, 0, s>>>((const float2 *)vis, nv, (const float2 *)kern,
                                               nk, fs);
   k_tw_pick<<<1, 1, 0, s>>>(fs, nv, a.nearest, separable);
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
@@ -711,7 +711,7 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
       CROC_HIP(hipGetLastError());
     }
     // image = ifft(sub[slot]) * exp(-i wp phi), in place in each job's slot
-    rc = dft2d_launch(ctx, sub, sub, SG, bt.nj, SGN, +1, 1.0f / (float)SGN, nullptr, nullptr, 0,
+    rc = fft2d_launch(ctx, sub, sub, SG, bt.nj, SGN, +1, 1.0f / (float)SGN, nullptr, nullptr, 0,
                       phi, kw + bt.j0, -1, djslot + bt.j0, djslot + bt.j0);
     if (rc) return rc;
     k_tw_chunk_accum<<<dim3(ACC_SLICES, (unsigned)bt.ng), 256, 0, s>>>(
@@ -719,7 +719,7 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
     CROC_HIP(hipGetLastError());
   }
   // chunk sub-grids = fft(chunk image), then parity-ordered adds
-  rc = dft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, -1, 1.0f, nullptr,
+  rc = fft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, -1, 1.0f, nullptr,
                     nullptr, 0, nullptr, nullptr, 0, dchunks, dchunks);
   if (rc) return rc;
   // parity classes, uploaded once, one launch each (stream order keeps
@@ -799,13 +799,13 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   CROC_HIP(hipEventRecord(ctx->ev[2], s));
   k_tw_cut<<<g, 256, 0, s>>>(dchunks, (const float2 *)grid, cimg, a.N, a.chunk, a.margin, a.ncx);
   CROC_HIP(hipGetLastError());
-  rc = dft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, +1, 1.0f / (float)SGN,
+  rc = fft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, +1, 1.0f / (float)SGN,
                     nullptr, nullptr, 0, nullptr, nullptr, 0, dchunks, dchunks);
   if (rc) return rc;
   for (int j0 = 0, bi = 0; j0 < p.njobs; j0 += jb, bi++) {
     const int nb = std::min(jb, p.njobs - j0);
     // plane sub-grid = fft(chunk image * exp(+i wp phi))
-    rc = dft2d_launch(ctx, cimg, sub, SG, nb, SGN, -1, 1.0f, phi, kw + j0, +1, nullptr, nullptr,
+    rc = fft2d_launch(ctx, cimg, sub, SG, nb, SGN, -1, 1.0f, phi, kw + j0, +1, nullptr, nullptr,
                       0, jchunk + j0, nullptr);
     if (rc) return rc;
     const int npart = (int)(poff[bi + 1] - poff[bi]);

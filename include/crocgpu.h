@@ -164,6 +164,13 @@ int croc_kernel_oversample(croc_ctx *ctx, const void *ff, int n, int ndim,
 int croc_dft2d_batched(croc_ctx *ctx, const void *in, void *out, int n,
                        int64_t batch, int inverse);
 
+/* The same centred transforms as croc_dft2d_batched (n = 128 only), as a
+ * radix 16 x 8 FFT in FP32 on the vector ALUs with the sub-grid held in LDS:
+ * ~30x fewer operations than the dense products, the transform the w-towers
+ * drivers use. */
+int croc_fft2d_batched(croc_ctx *ctx, const void *in, void *out, int n,
+                       int64_t batch, int inverse);
+
 /* W-towers imaging: replaces the w-stacking-by-sub-grid drivers of
  * examples/notebooks/wtowers.ipynb (imaging cell) and grid.c:482-710
  * (grid_wtowers).  Visibilities are binned into (chunk, w-plane) jobs, chunk
