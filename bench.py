@@ -35,7 +35,7 @@ def parse():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='wproj2048',
-                    choices=['wproj2048', 'plain512', 'towers4096', 'towers8192'])
+                    choices=['wproj2048', 'plain512', 'towers4096', 'towers8192', 'diff1024'])
     ap.add_argument('--nv', type=int, default=0,
                     help='towers workloads: visibilities (0 = the workload\'s own count)')
     ap.add_argument('--cpu-sample', type=int, default=300_000,
@@ -246,6 +246,61 @@ def run_towers(args, torch, G, W, _lib, dev):
     print(json.dumps(line), flush=True)
 
 
+def run_diff(args, torch, G, W, _lib, dev):
+    """Differentiable degridding on the 1024^2 Gaussian-blob image with 1e6
+    visibilities (w +-500, w-kernels per 20-wavelength bin, S 16..32, Q 8):
+    one step = forward V = degrid(fft(I)) (croc_convdegrid_plan), the
+    least-squares loss sum |V - V_obs|^2 and its gradient by the adjoint
+    gridding (croc_convgrid_plan) and ifft; the coordinates are binned once
+    when the operator is built."""
+    from crocodile_amd.autograd import WProjectionOperator, degrid_image
+    c = W.SPEC['diff1024']
+    image, uvw = W.diff1024()
+    N = c['N']
+    lam = N / c['theta']
+    sup = lambda w: W.wproj_support(w, c['wmax'], 16, 32)
+    t0 = time.perf_counter()
+    op = WProjectionOperator(c['theta'], lam, uvw, 20.0, 64, sup, c['Q'])
+    torch.cuda.synchronize()
+    setup_ms = 1e3 * (time.perf_counter() - t0)
+    img_true = torch.as_tensor(image, device=dev)
+    v_obs = op.apply(img_true)
+    x = torch.zeros((N, N), dtype=torch.float32, device=dev, requires_grad=True)
+
+    def step():
+        x.grad = None
+        v = degrid_image(x, op)
+        loss = (v - v_obs).abs().pow(2).sum()
+        loss.backward()
+        return _lib.last_kernel_ms()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    gms, dms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g, d = step()
+        gms.append(g)
+        dms.append(d)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nv = op.nv
+    line = {
+        'metric': 'visibilities/s grid+degrid', 'value': round(nv * args.steps / dt, 1),
+        'unit': 'visibilities/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(1e3 * dt / args.steps, 4), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'diff1024', 'N': N, 'nv': nv, 'w_planes': op.bank.nplanes,
+                   'support': [16, 32], 'oversampling': c['Q'],
+                   'step': 'loss and gradient: degrid forward, adjoint grid backward',
+                   'operator_setup_ms': round(setup_ms, 1)},
+        'kernel_ms': {'k_grid_tiles': round(sum(gms) / len(gms), 4),
+                      'k_degrid_tiles': round(sum(dms) / len(dms), 4)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -259,6 +314,8 @@ def main():
     _lib.set_tuning(args.tile, args.chunk, args.group_kib)
     if args.workload.startswith('towers'):
         return run_towers(args, torch, G, W, _lib, dev)
+    if args.workload == 'diff1024':
+        return run_diff(args, torch, G, W, _lib, dev)
     print('[bench] building workload %s' % args.workload, file=sys.stderr, flush=True)
     if args.workload == 'wproj2048':
         wl = build_wproj(G, W, torch, dev)

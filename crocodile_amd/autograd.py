@@ -10,8 +10,10 @@ gradient of a real loss is the adjoint applied to the incoming gradient:
 
 The backward pass is therefore the adjoint gridding kernel of libcrocgpu run
 on conj(g) with the forward's own kernel bank: taps are recomputed from the
-bank, nothing per visibility is stored between the passes.  Both passes use
-the FFT of torch (rocFFT) for the full-image transform.
+bank, nothing per visibility is stored between the passes.  The coordinates
+are binned once, when the operator is built (a plan), since they stay fixed
+while the image is fitted.  Both passes use the FFT of torch (rocFFT) for
+the full-image transform.
 """
 
 import numpy
@@ -43,6 +45,8 @@ class WProjectionOperator:
         S = [support(w) if callable(support) else int(support) for w in ws]
         self.bank = G.w_kernel_bank(theta, ws, npix_ff, S, Qpx, invert=False,
                                     scale=1.0 / npix_ff ** 2)
+        self.plan = G.GridPlan(self.bank, self.N, self.uvw, lam, _lib.PLANE_WBIN, None,
+                               self.wstep, self.wbin0)
 
     @property
     def nv(self):
@@ -52,15 +56,13 @@ class WProjectionOperator:
         """V = degrid(fft(image))."""
         grid = G.fft(image.to(torch.complex64)).contiguous()
         out = torch.empty(self.nv, dtype=torch.complex64, device=grid.device)
-        G.degrid_device(self.bank, grid, self.uvw, self.lam, out, _lib.PLANE_WBIN, None,
-                        self.wstep, self.wbin0)
+        self.plan.degrid(grid, out)
         return out
 
     def adjoint(self, g):
         """fft^H degrid^H g (complex image)."""
         acc = torch.zeros((self.N, self.N), dtype=torch.complex64, device=g.device)
-        G.grid_device(self.bank, acc, self.uvw, self.lam, torch.conj_physical(g).contiguous(),
-                      _lib.PLANE_WBIN, None, self.wstep, self.wbin0)
+        self.plan.grid(acc, torch.conj_physical(g).to(torch.complex64).contiguous())
         return G.ifft(torch.conj_physical(acc)) * (self.N * self.N)
 
 
