@@ -11,9 +11,10 @@ place) or torch tensors, and returns NumPy for NumPy input and torch device
 tensors for torch input.
 
 All arithmetic on the path runs in libcrocgpu (hand-written HIP):
-coordinates and w-kernel phases in FP64, gridding / degridding in complex
-FP32 with compensated accumulation.  Nothing here falls back to the CPU; a
-missing library or GPU raises.
+coordinates and w-kernel phases in FP64, tap products in complex FP32,
+gridding accumulated exactly in 64-bit fixed point (bit-reproducible),
+degridding in FP32.  Nothing here falls back to the CPU; a missing library
+or GPU raises.
 """
 
 import ctypes
