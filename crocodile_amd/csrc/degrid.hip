@@ -24,7 +24,7 @@ __device__ __forceinline__ float wave_sum(float x) {
 // Partial sums of one visibility over a block of up to 64*NQ taps t0 .. of
 // its S x S window (tap t = row i, column t - i*S); loads unpredicated
 // (clamped index) and issued before first use.
-template <int NQ, bool SEPARABLE>
+template <int NQ, bool SEPARABLE, int R>
 __device__ __forceinline__ void gather_taps(
     const float2 *sg, int base, int S, int SS, float invS, int t0, int lane,
     const float2 *k2, const float2 *ky, const float2 *kx, float &ar, float &ai) {
@@ -44,20 +44,20 @@ __device__ __forceinline__ void gather_taps(
     const int t = t0 + q * 64 + lane;
     if (t < SS) {
       const int i = (int)(((float)t + 0.5f) * invS);
-      const float2 g = sg[base + i * (TILE_R - S) + t];
+      const float2 g = sg[base + i * (R - S) + t];
       ar += g.x * kv[q].x - g.y * kv[q].y;
       ai += g.x * kv[q].y + g.y * kv[q].x;
     }
   }
 }
 
-template <int NT, bool SEPARABLE>
+template <int NT, bool SEPARABLE, int R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_degrid_tiles(
     const float2 *__restrict__ grid, int N, int T, int ntx,
     const uint4 *__restrict__ rec, float2 *__restrict__ vis_out,
     const float2 *__restrict__ kern, const int4 *__restrict__ items,
     const uint32_t *__restrict__ nitems, uint32_t *__restrict__ qhead) {
-  constexpr int R = TILE_R, CELLS = R * R;
+  constexpr int CELLS = R * R;
   constexpr int NW = NT / 64;
   __shared__ float2 sg[CELLS];
   __shared__ int s_item;
@@ -99,12 +99,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       float ar = 0.f, ai = 0.f;
       const int nq = (SS + 63) >> 6;
       if (nq <= 4) {
-        gather_taps<4, SEPARABLE>(sg, base, S, SS, invS, 0, lane, kp, ky, kx, ar, ai);
+        gather_taps<4, SEPARABLE, R>(sg, base, S, SS, invS, 0, lane, kp, ky, kx, ar, ai);
       } else if (nq <= 8) {
-        gather_taps<8, SEPARABLE>(sg, base, S, SS, invS, 0, lane, kp, ky, kx, ar, ai);
+        gather_taps<8, SEPARABLE, R>(sg, base, S, SS, invS, 0, lane, kp, ky, kx, ar, ai);
       } else {
         for (int t0 = 0; t0 < SS; t0 += 64 * 16)
-          gather_taps<16, SEPARABLE>(sg, base, S, SS, invS, t0, lane, kp, ky, kx, ar, ai);
+          gather_taps<16, SEPARABLE, R>(sg, base, S, SS, invS, t0, lane, kp, ky, kx, ar, ai);
       }
       ar = wave_sum(ar);
       ai = wave_sum(ai);
@@ -119,16 +119,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // Degridding of binned visibilities from grid into vis_out.  No status read.
 int degrid_binned(croc_ctx *ctx, const BinPlan &plan, const void *grid, int N, void *vis_out,
                   const void *kern, int separable) {
-  const int G = persistent_blocks(ctx, 2);
   CROC_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-  if (separable)
-    k_degrid_tiles<DEGRID_NT, true><<<G, DEGRID_NT, 0, ctx->stream>>>(
-        (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
-        (const float2 *)kern, plan.items, plan.nitems, plan.qhead);
-  else
-    k_degrid_tiles<DEGRID_NT, false><<<G, DEGRID_NT, 0, ctx->stream>>>(
-        (const float2 *)grid, N, plan.T, plan.ntx, plan.rec, (float2 *)vis_out,
-        (const float2 *)kern, plan.items, plan.nitems, plan.qhead);
+  const float2 *g = (const float2 *)grid;
+  const float2 *kt = (const float2 *)kern;
+  float2 *vo = (float2 *)vis_out;
+  if (plan.R == TILE_R_SMALL) {
+    const int G = persistent_blocks(ctx, 2);  // two fit by registers (16 waves)
+    if (separable)
+      k_degrid_tiles<DEGRID_NT, true, TILE_R_SMALL><<<G, DEGRID_NT, 0, ctx->stream>>>(
+          g, N, plan.T, plan.ntx, plan.rec, vo, kt, plan.items, plan.nitems, plan.qhead);
+    else
+      k_degrid_tiles<DEGRID_NT, false, TILE_R_SMALL><<<G, DEGRID_NT, 0, ctx->stream>>>(
+          g, N, plan.T, plan.ntx, plan.rec, vo, kt, plan.items, plan.nitems, plan.qhead);
+  } else {
+    const int G = persistent_blocks(ctx, 2);
+    if (separable)
+      k_degrid_tiles<DEGRID_NT, true, TILE_R><<<G, DEGRID_NT, 0, ctx->stream>>>(
+          g, N, plan.T, plan.ntx, plan.rec, vo, kt, plan.items, plan.nitems, plan.qhead);
+    else
+      k_degrid_tiles<DEGRID_NT, false, TILE_R><<<G, DEGRID_NT, 0, ctx->stream>>>(
+          g, N, plan.T, plan.ntx, plan.rec, vo, kt, plan.items, plan.nitems, plan.qhead);
+  }
   CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
   ctx->ev_degrid = true;

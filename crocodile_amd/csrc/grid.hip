@@ -89,7 +89,7 @@ __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
 // tap (a valid address, value unused), so the loads need no predication and
 // the compiler issues them back to back.  The region interleaves re / im:
 // cell c is acc[2c], acc[2c + 1].
-template <int NQ, bool SEPARABLE>
+template <int NQ, bool SEPARABLE, int R>
 __device__ __forceinline__ void spread_taps(
     unsigned long long *acc, int base, int S, int SS, float invS, int t0, int lane,
     const float2 *k2, const float2 *ky, const float2 *kx, float2 v, double scale) {
@@ -109,21 +109,21 @@ __device__ __forceinline__ void spread_taps(
     const int t = t0 + q * 64 + lane;
     if (t < SS) {
       const int i = (int)(((float)t + 0.5f) * invS);
-      unsigned long long *cell = acc + 2 * (base + i * (TILE_R - S) + t);
+      unsigned long long *cell = acc + 2 * (base + i * (R - S) + t);
       atomicAdd(cell, fixed(v.x * kv[q].x - v.y * kv[q].y, scale));
       atomicAdd(cell + 1, fixed(v.x * kv[q].y + v.y * kv[q].x, scale));
     }
   }
 }
 
-template <int NT, bool SEPARABLE>
+template <int NT, bool SEPARABLE, int R>
 __global__ __launch_bounds__(NT) void k_grid_tiles(
     unsigned long long *__restrict__ acc, int N, int T, int ntx,
     const uint4 *__restrict__ rec, const float2 *__restrict__ svis,
     const float2 *__restrict__ kern, const int4 *__restrict__ items,
     const uint32_t *__restrict__ nitems, uint32_t *__restrict__ qhead,
     const ScaleState *__restrict__ st) {
-  constexpr int R = TILE_R, CELLS = R * R;
+  constexpr int CELLS = R * R;
   constexpr int NW = NT / 64;
   __shared__ unsigned long long sacc[2 * CELLS];  // re, im interleaved
   __shared__ int s_item;
@@ -167,10 +167,10 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(
       const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
       const float2 *ky = kp + yf * S, *kx = kp + xf * S;
       if (SS <= 256) {
-        spread_taps<4, SEPARABLE>(sacc, base, S, SS, invS, 0, lane, kp, ky, kx, v, scale);
+        spread_taps<4, SEPARABLE, R>(sacc, base, S, SS, invS, 0, lane, kp, ky, kx, v, scale);
       } else {
         for (int t0 = 0; t0 < SS; t0 += 512)
-          spread_taps<8, SEPARABLE>(sacc, base, S, SS, invS, t0, lane, kp, ky, kx, v, scale);
+          spread_taps<8, SEPARABLE, R>(sacc, base, S, SS, invS, t0, lane, kp, ky, kx, v, scale);
       }
     }
     barrier_after_lds_atomics();
@@ -232,16 +232,25 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, const float2 *svis, void *gr
   k_scale_pick<<<1, 1, 0, ctx->stream>>>(st, nv, separable);
   CROC_HIP(hipGetLastError());
 
-  const int G = persistent_blocks(ctx, 1);
   CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-  if (separable)
-    k_grid_tiles<GRID_NT, true><<<G, GRID_NT, 0, ctx->stream>>>(
-        acc, N, plan.T, plan.ntx, plan.rec, svis, (const float2 *)kern, plan.items,
-        plan.nitems, plan.qhead, st);
-  else
-    k_grid_tiles<GRID_NT, false><<<G, GRID_NT, 0, ctx->stream>>>(
-        acc, N, plan.T, plan.ntx, plan.rec, svis, (const float2 *)kern, plan.items,
-        plan.nitems, plan.qhead, st);
+  const float2 *kt = (const float2 *)kern;
+  if (plan.R == TILE_R_SMALL) {
+    const int G = persistent_blocks(ctx, 2);  // 63.5 KiB of LDS each
+    if (separable)
+      k_grid_tiles<GRID_NT, true, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
+          acc, N, plan.T, plan.ntx, plan.rec, svis, kt, plan.items, plan.nitems, plan.qhead, st);
+    else
+      k_grid_tiles<GRID_NT, false, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
+          acc, N, plan.T, plan.ntx, plan.rec, svis, kt, plan.items, plan.nitems, plan.qhead, st);
+  } else {
+    const int G = persistent_blocks(ctx, 1);
+    if (separable)
+      k_grid_tiles<GRID_NT, true, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
+          acc, N, plan.T, plan.ntx, plan.rec, svis, kt, plan.items, plan.nitems, plan.qhead, st);
+    else
+      k_grid_tiles<GRID_NT, false, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
+          acc, N, plan.T, plan.ntx, plan.rec, svis, kt, plan.items, plan.nitems, plan.qhead, st);
+  }
   CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
   ctx->ev_grid = true;

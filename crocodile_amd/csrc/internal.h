@@ -183,8 +183,11 @@ struct BinPlan {
   uint32_t *status = nullptr, *nitems = nullptr, *qhead = nullptr;
 };
 
-// Region edge held in LDS by the tile kernels.
+// Region edges the tile kernels hold in LDS: a tile of T cells with
+// windows of support up to S needs T + S - 1 <= R.  The smaller region lets
+// more workgroups share a compute unit (grid: 63.5 vs 141 KiB).
 constexpr int TILE_R = 95;
+constexpr int TILE_R_SMALL = 63;
 
 // round-half-even(x * 2^e) as a 64-bit two's complement integer, for
 // |x * 2^e| < 2^51.  x * 2^e is exact, and adding 1.5 * 2^52 in the same fma
@@ -253,6 +256,7 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 }
 
 int choose_tile(croc_ctx *ctx, int max_support);
+int choose_region(int T, int max_support);
 // Tile-kernel launches on binned visibilities (grid.hip, degrid.hip); they
 // read no status.  svis: the visibilities in record order.
 int grid_binned(croc_ctx *ctx, const BinPlan &plan, const float2 *svis, void *grid, int N,
