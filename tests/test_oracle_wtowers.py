@@ -106,3 +106,26 @@ def test_towers_linearity():
     gb = WT.w_towers_imaging(THETA, LAM, uvw, None, b, 25.0, SG, MARGIN, kv)
     gab = WT.w_towers_imaging(THETA, LAM, uvw, None, 2 * a - 3j * b, 25.0, SG, MARGIN, kv)
     assert_allclose(gab, 2 * ga - 3j * gb, atol=1e-10)
+
+
+@pytest.mark.parametrize('with_kernel', [False, True])
+def test_imaging_and_predict_are_adjoint(with_kernel):
+    """<A v, G> = <v, B' G>: w-towers imaging A and predict B' with the
+    conjugated kernel tables are adjoint (F^H = n^2 iF, the Fresnel factors
+    conjugate, cutting is the transpose of adding).  Holds for any data."""
+    rng = numpy.random.default_rng(3)
+    N, theta = 256, 0.1
+    lam = N / theta
+    nv = 3000
+    uvw = numpy.stack([rng.uniform(-0.45, 0.45, nv) * lam, rng.uniform(-0.45, 0.45, nv) * lam,
+                       rng.uniform(-300, 300, nv)], 1)
+    v = rng.standard_normal(nv) + 1j * rng.standard_normal(nv)
+    G = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    kv = None
+    if with_kernel:
+        kv = rng.standard_normal((4, 4, 8, 8)) + 1j * rng.standard_normal((4, 4, 8, 8))
+    A = WT.w_towers_imaging(theta, lam, uvw, None, v, 20.0, 64, 16, kv)
+    B = WT.w_towers_predict(theta, lam, uvw, None, G, 20.0, 64, 16,
+                           None if kv is None else numpy.conj(kv))
+    lhs, rhs = numpy.vdot(A, G), numpy.vdot(v, B)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
