@@ -27,6 +27,7 @@ import time
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+LDS_PEAK_TBS = 78.6  # 256 CUs x 128 B/clk x 2.4 GHz
 
 
 def parse():
@@ -44,6 +45,8 @@ def parse():
     ap.add_argument('--copy-probe-mib', type=int, default=0,
                     help='before warm-up, copy a buffer of this many MiB once (a plain copy '
                          'of known size, to scale HBM counters of a profiled run)')
+    ap.add_argument('--no-autotune', action='store_true',
+                    help='skip the measured choice of tile / chunk during warm-up')
     ap.add_argument('--tile', type=int, default=0)
     ap.add_argument('--chunk', type=int, default=1024)
     ap.add_argument('--group-kib', type=int, default=0)
@@ -354,6 +357,14 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    # Warm-up includes the measured choice of tile and chunk (results do not
+    # depend on them) on this workload.
+    if not args.no_autotune:
+        from crocodile_amd import tuning
+        choice, _ = tuning.autotune(bank_img, N, uvw_d, lam, vis_d, mode, None, wstep, wbin0,
+                                    repeats=3, tiles=(0, 48), chunks=(512, 1024, 2048))
+        config['tuning'] = {'tile': choice[0], 'chunk': choice[1], 'group_kib': choice[2]}
+
     # One step: bin the coordinates once (croc_plan_bin, which validates
     # them), grid into a cleared grid, degrid it (croc_convgrid_plan /
     # croc_convdegrid_plan).  The one-shot calls, which bin on every call,
@@ -419,9 +430,14 @@ def main():
                        '%s_%s_pmc.json' % (name, config['workload']))
     if os.path.exists(pmc):
         traffic = json.load(open(pmc)).get('hbm_bytes_per_launch')
+    # LDS side (supplementary): gridding adds 16 B per tap (two 64-bit
+    # integer atomics), degridding reads 8 B per tap; peak 128 B/clk/CU.
+    lds_bytes = (16.0 if name == 'k_grid_tiles' else 8.0) * taps
     roofline.update({'kernel': name, 'traffic': traffic, 'algorithmic_bytes': alg_bytes,
                      'algorithmic_flops': flops,
                      'hbm_frac': round(alg_bytes / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                     'lds_bytes': lds_bytes,
+                     'lds_frac': round(lds_bytes / (t_ms * 1e-3) / (LDS_PEAK_TBS * 1e12), 5),
                      'kernel_ms': {'k_grid_tiles': round(g_avg, 4),
                                    'k_degrid_tiles': round(d_avg, 4)}})
 
