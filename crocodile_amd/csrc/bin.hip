@@ -123,17 +123,23 @@ __device__ uint32_t block_exclusive_scan(uint32_t x, uint32_t *sc) {
 }
 
 // One block: exclusive scan of the bin counts (record offsets, tile-major),
-// then the work-item list in the requested order.  An error status empties
-// the item list so that no output is modified.
+// then the requested work-item lists: tile-major `items` and / or the
+// per-XCD queues `items_g`.  The groups are cut into nq = min(8, ngroups)
+// contiguous ranges; queue x lists, tile by tile, the records of range x
+// (group-sorted within each item), and gstart[first group of range x] is
+// its first item.  An error status empties the item lists so that no
+// output is modified.
 __global__ __launch_bounds__(BIN_NT) void k_bin_scan(
     int ntiles, int ngroups, int chunk, int order,
     const uint32_t *__restrict__ count, uint32_t *__restrict__ offset,
-    uint32_t *__restrict__ cursor, int4 *__restrict__ items,
-    uint32_t *__restrict__ nitems, const uint32_t *__restrict__ status,
+    uint32_t *__restrict__ cursor, int4 *__restrict__ items, int4 *__restrict__ items_g,
+    uint32_t *__restrict__ gstart, uint32_t *__restrict__ nitems,
+    uint32_t *__restrict__ nitems_g, const uint32_t *__restrict__ status,
     uint32_t *__restrict__ qheads) {
   __shared__ uint32_t sc[BIN_NT];
   const int tid = threadIdx.x;
   const int nbins = ntiles * ngroups;
+  const bool failed = *status != 0;
 
   // record offsets, bins in tile-major order
   const int per = (nbins + BIN_NT - 1) / BIN_NT;
@@ -149,8 +155,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_scan(
   if (tid == BIN_NT - 1) offset[nbins] = co;
   __syncthreads();  // offsets visible to the whole block
 
-  uint32_t io;
-  if (order == ITEMS_BY_TILE) {
+  if (order & ITEMS_BY_TILE) {
     const int tper = (ntiles + BIN_NT - 1) / BIN_NT;
     const int t0 = min(ntiles, tid * tper), t1 = min(ntiles, t0 + tper);
     uint32_t is = 0;
@@ -158,33 +163,42 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_scan(
       const uint32_t c = offset[(t + 1) * ngroups] - offset[t * ngroups];
       is += (c + chunk - 1) / chunk;
     }
-    io = block_exclusive_scan(is, sc);
+    uint32_t io = block_exclusive_scan(is, sc);
     for (int t = t0; t < t1; t++) {
       const uint32_t s0 = offset[t * ngroups], s1 = offset[(t + 1) * ngroups];
       for (uint32_t s = s0; s < s1; s += chunk)
         items[io++] = make_int4(t, (int)s, (int)min(s1, s + chunk), 0);
     }
-  } else {
-    // group-major: index q = g * ntiles + t
-    const int q0 = min(nbins, tid * per), q1 = min(nbins, q0 + per);
+    if (tid == BIN_NT - 1) *nitems = failed ? 0u : io;
+  }
+  if (order & ITEMS_BY_GROUP) {
+    // queue-major: index q = x * ntiles + t covers tile t's groups of queue x
+    const int nq = min(XCD_QUEUES, ngroups);
+    const int nqs = nq * ntiles;
+    const int qper = (nqs + BIN_NT - 1) / BIN_NT;
+    const int q0 = min(nqs, tid * qper), q1 = min(nqs, q0 + qper);
     uint32_t is = 0;
     for (int q = q0; q < q1; q++) {
-      const int g = q / ntiles, t = q - g * ntiles;
-      is += (count[t * ngroups + g] + chunk - 1) / chunk;
+      const int x = q / ntiles, t = q - x * ntiles;
+      const int ga = (int)((int64_t)x * ngroups / nq), gb = (int)((int64_t)(x + 1) * ngroups / nq);
+      is += (offset[t * ngroups + gb] - offset[t * ngroups + ga] + chunk - 1) / chunk;
     }
-    io = block_exclusive_scan(is, sc);
+    uint32_t io = block_exclusive_scan(is, sc);
     for (int q = q0; q < q1; q++) {
-      const int g = q / ntiles, t = q - g * ntiles;
-      const int b = t * ngroups + g;
-      const uint32_t s0 = offset[b], s1 = s0 + count[b];
+      const int x = q / ntiles, t = q - x * ntiles;
+      const int ga = (int)((int64_t)x * ngroups / nq), gb = (int)((int64_t)(x + 1) * ngroups / nq);
+      if (t == 0) gstart[ga] = failed ? 0u : io;
+      const uint32_t s0 = offset[t * ngroups + ga], s1 = offset[t * ngroups + gb];
       for (uint32_t s = s0; s < s1; s += chunk)
-        items[io++] = make_int4(t, (int)s, (int)min(s1, s + chunk), 0);
+        items_g[io++] = make_int4(t, (int)s, (int)min(s1, s + chunk), 0);
+    }
+    if (tid == BIN_NT - 1) {
+      *nitems_g = failed ? 0u : io;
+      gstart[ngroups] = failed ? 0u : io;
     }
   }
-  if (tid == BIN_NT - 1) {
-    *nitems = (*status != 0) ? 0u : io;
-    for (int q = 0; q < 8; q++) qheads[q] = 0;
-  }
+  if (tid == 0)
+    for (int q = 0; q < XCD_QUEUES; q++) qheads[q] = 0;
 }
 
 __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
@@ -270,15 +284,15 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   p.chunk = ctx->chunk;
 
   // Plane-groups: slices of group_kib KiB of an upper bound of the bank's
-  // extent, as many as keep the bin histogram in LDS (at most 32).
+  // extent, at most 64 (bin counts beyond the LDS histogram use global
+  // atomics).
   const int64_t S = max_support;
   const int64_t bound = (int64_t)a.nplanes * (a.separable ? a.Q * S : (int64_t)a.Q * a.Q * S * S);
   int64_t ng = 1;
   if (ctx->group_kib > 0) {
     const int64_t gel = (int64_t)ctx->group_kib * 1024 / 8;
     ng = (bound + gel - 1) / gel;
-    ng = std::min<int64_t>(ng, 32);
-    ng = std::min<int64_t>(ng, std::max<int64_t>(1, HIST_MAX / p.ntiles));
+    ng = std::min<int64_t>(ng, 64);
     ng = std::max<int64_t>(ng, 1);
   }
   p.ngroups = (int)ng;
@@ -289,18 +303,26 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   p.loc = (uint4 *)ws_get_in(ctx, set, WS_LOC, sizeof(uint4) * a.nv);
   p.rec = (uint4 *)ws_get_in(ctx, set, WS_REC, sizeof(uint4) * a.nv);
   if (a.vis) p.svis = (float2 *)ws_get_in(ctx, set, WS_SVIS, sizeof(float2) * a.nv);
-  uint32_t *tiles = (uint32_t *)ws_get_in(ctx, set, WS_TILES,
-                                       sizeof(uint32_t) * (3 * (size_t)p.nbins + 1));
-  p.items = (int4 *)ws_get_in(ctx, set, WS_ITEMS, sizeof(int4) * p.max_items);
+  uint32_t *tiles = (uint32_t *)ws_get_in(
+      ctx, set, WS_TILES, sizeof(uint32_t) * (3 * (size_t)p.nbins + p.ngroups + 2));
+  p.items = (order & ITEMS_BY_TILE)
+                ? (int4 *)ws_get_in(ctx, set, WS_ITEMS, sizeof(int4) * p.max_items)
+                : nullptr;
+  p.items_g = (order & ITEMS_BY_GROUP)
+                  ? (int4 *)ws_get_in(ctx, set, WS_ITEMS_G, sizeof(int4) * p.max_items)
+                  : nullptr;
   uint32_t *small = (uint32_t *)ws_get_in(ctx, set, WS_SMALL, 256);
-  if (!p.loc || !p.rec || (a.vis && !p.svis) || !tiles || !p.items || !small)
+  if (!p.loc || !p.rec || (a.vis && !p.svis) || !tiles || !small ||
+      (!p.items && (order & ITEMS_BY_TILE)) || (!p.items_g && (order & ITEMS_BY_GROUP)))
     return CROC_ENOMEM;
   p.count = tiles;
   p.offset = tiles + p.nbins;
   p.cursor = tiles + 2 * p.nbins + 1;
+  p.gstart = tiles + 3 * p.nbins + 1;
   p.status = small;
   p.nitems = small + 1;
-  p.qhead = small + 8;  // 8 queue heads, one per XCD
+  p.nitems_g = small + 2;
+  p.qhead = small + 8;  // XCD_QUEUES queue heads
 
   hipStream_t s = ctx->stream;
   int rc = zero_u32_async(ctx, p.count, p.nbins);
@@ -312,8 +334,8 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   if (nblk > 0)
     k_bin_count<<<nblk, BIN_NT, 0, s>>>(a, g, p.nbins, p.loc, p.count, p.status);
   k_bin_scan<<<1, BIN_NT, 0, s>>>(p.ntiles, p.ngroups, p.chunk, order, p.count,
-                                  p.offset, p.cursor, p.items, p.nitems,
-                                  p.status, p.qhead);
+                                  p.offset, p.cursor, p.items, p.items_g, p.gstart,
+                                  p.nitems, p.nitems_g, p.status, p.qhead);
   if (nblk > 0)
     k_bin_scatter<<<nblk, BIN_NT, 0, s>>>(a.nv, p.nbins, p.loc, p.cursor,
                                           p.rec, a.vis, p.svis);

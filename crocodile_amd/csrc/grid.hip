@@ -6,8 +6,10 @@
 // covering every such window in LDS and lets each wavefront spread one
 // visibility at a time over its S*S taps.
 //
-// Accumulation is exact fixed point: every FP32 contribution k*v is scaled
-// by 2^e and rounded to a 64-bit integer, then added with integer atomics
+// Accumulation is exact fixed point: every contribution k*v (the FP64
+// product of the FP32 tap and visibility; for separable kernels the tap is
+// the FP32 product of its two 1D factors) is scaled by 2^e and rounded to a
+// 64-bit integer (fixed2, within 1 unit), then added with integer atomics
 // (ds_add_u64 in LDS, global_atomic_add_x2 into an int64 workspace grid).
 // e is chosen on the device from max|v| * max|k| and the visibility count so
 // that no partial sum can overflow and every single contribution stays below
@@ -62,8 +64,8 @@ __global__ void k_scale_maxima(const float2 *__restrict__ vis, int64_t nv,
 }
 
 // 2^e with (nv + 1) * c * 2^e < 2^62 (no sum overflows) and c * 2^e < 2^50
-// (the conversion in fixed() is exact), c = 2 * max|v| * max|k|^(1 or
-// 2) bounding one contribution.
+// (the range fixed2 needs), c = 2 * max|v| * max|k|^(1 or 2) bounding one
+// contribution.
 __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
   const double mv = (double)__uint_as_float(st->max_vis_bits);
   double mk = (double)__uint_as_float(st->max_k_bits);
@@ -83,41 +85,70 @@ __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
 }
 
 
-// Spread one visibility over a block of up to 64*NQ taps t0 .. of its
-// S x S window (tap t = row i, column t - i*S).  All NQ loads are issued
-// before the first use; indices past the table end are clamped to its last
-// tap (a valid address, value unused), so the loads need no predication and
-// the compiler issues them back to back.  The region interleaves re / im:
-// cell c is acc[2c], acc[2c + 1].
-template <int NQ, bool SEPARABLE, int R>
-__device__ __forceinline__ void spread_taps(
-    unsigned long long *acc, int base, int S, int SS, float invS, int t0, int lane,
-    const float2 *k2, const float2 *ky, const float2 *kx, float2 v, double scale) {
+// Spread one visibility over the block of 64*NQ taps t0 .. of its S x S
+// window (tap t = row i = t / S, column t - i*S; cell base + t + i*(R - S)).
+// All NQ kernel loads are issued before the first use.  FULL: every tap of
+// the block lies in the window, so there are no clamps and no masks; the
+// last, partial block of a window clamps its load indices to the last tap
+// (a valid address, value unused) and masks the adds.  cb = 16 * (base + t0
+// + lane) is the lane's LDS byte address in the interleaved region (cell c
+// is acc[2c], acc[2c + 1]); rs16 = 16 * (R - S); kt = 8 * (t0 + lane).
+template <int NQ, bool FULL, bool SEPARABLE>
+__device__ __forceinline__ void spread_block(
+    unsigned long long *acc, uint32_t cb, uint32_t rs16, uint32_t kt, int S, int SS,
+    uint32_t m, int t0, int lane, const float2 *k2, const float2 *ky, const float2 *kx,
+    double vr, double vi) {
   float2 kv[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    const int t = min(t0 + q * 64 + lane, SS - 1);
+    const int t = FULL ? t0 + q * 64 + lane : min(t0 + q * 64 + lane, SS - 1);
     if (!SEPARABLE) {
-      kv[q] = k2[t];
+      kv[q] = FULL ? ldk(k2 + q * 64, kt) : ldk(k2, (uint32_t)t * 8u);
     } else {
-      const int i = (int)(((float)t + 0.5f) * invS);
-      kv[q] = cmul(ky[i], kx[t - i * S]);
+      const int i = tap_row(t, m);
+      kv[q] = cmul(ldk(ky, (uint32_t)i * 8u), ldk(kx, (uint32_t)(t - i * S) * 8u));
     }
   }
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
+    if (!FULL && t0 + q * 64 >= SS) break;  // wave-uniform
     const int t = t0 + q * 64 + lane;
-    if (t < SS) {
-      const int i = (int)(((float)t + 0.5f) * invS);
-      unsigned long long *cell = acc + 2 * (base + i * (R - S) + t);
-      atomicAdd(cell, fixed(v.x * kv[q].x - v.y * kv[q].y, scale));
-      atomicAdd(cell + 1, fixed(v.x * kv[q].y + v.y * kv[q].x, scale));
+    if (FULL || t < SS) {
+      const uint32_t a = cb + __umul24((uint32_t)tap_row(t, m), rs16) + 1024u * q;
+      unsigned long long *cell = (unsigned long long *)((char *)acc + a);
+      const double kr = kv[q].x, ki = kv[q].y;
+      atomicAdd(cell, fixed2(vr, kr, -vi, ki));
+      atomicAdd(cell + 1, fixed2(vr, ki, vi, kr));
     }
   }
 }
 
+// All taps of one visibility: blocks of 512 and 256 taps without masks,
+// then at most one masked block of the remaining < 256 taps.
+template <bool SEPARABLE, int R>
+__device__ __forceinline__ void spread_window(
+    unsigned long long *acc, int base, int S, int lane, const float2 *k2, const float2 *ky,
+    const float2 *kx, double vr, double vi) {
+  const int SS = S * S;
+  const uint32_t m = __builtin_amdgcn_readfirstlane(tap_div(S));
+  const uint32_t rs16 = 16u * (uint32_t)(R - S);
+  const uint32_t cb0 = 16u * (uint32_t)(base + lane), kt0 = 8u * (uint32_t)lane;
+  int t0 = 0;
+  for (; t0 + 512 <= SS; t0 += 512)
+    spread_block<8, true, SEPARABLE>(acc, cb0 + 16u * t0, rs16, kt0 + 8u * t0, S, SS, m, t0,
+                                     lane, k2, ky, kx, vr, vi);
+  if (t0 + 256 <= SS) {
+    spread_block<4, true, SEPARABLE>(acc, cb0 + 16u * t0, rs16, kt0 + 8u * t0, S, SS, m, t0,
+                                     lane, k2, ky, kx, vr, vi);
+    t0 += 256;
+  }
+  if (t0 < SS)
+    spread_block<4, false, SEPARABLE>(acc, cb0 + 16u * t0, rs16, kt0 + 8u * t0, S, SS, m, t0,
+                                      lane, k2, ky, kx, vr, vi);
+}
+
 template <int NT, bool SEPARABLE, int R>
-__global__ __launch_bounds__(NT) void k_grid_tiles(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_grid_tiles(
     unsigned long long *__restrict__ acc, int N, int T, int ntx,
     const uint4 *__restrict__ rec, const float2 *__restrict__ svis,
     const float2 *__restrict__ kern, const int4 *__restrict__ items,
@@ -162,16 +193,10 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(
       const int S = (int)(r.y & 0xffu);
       const int xf = (int)((r.y >> 8) & 0xfffu), yf = (int)(r.y >> 20);
       const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
-      const int SS = S * S;
-      const float invS = 1.0f / (float)S;
       const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
       const float2 *ky = kp + yf * S, *kx = kp + xf * S;
-      if (SS <= 256) {
-        spread_taps<4, SEPARABLE, R>(sacc, base, S, SS, invS, 0, lane, kp, ky, kx, v, scale);
-      } else {
-        for (int t0 = 0; t0 < SS; t0 += 512)
-          spread_taps<8, SEPARABLE, R>(sacc, base, S, SS, invS, t0, lane, kp, ky, kx, v, scale);
-      }
+      spread_window<SEPARABLE, R>(sacc, base, S, lane, kp, ky, kx, (double)v.x * scale,
+                                  (double)v.y * scale);
     }
     barrier_after_lds_atomics();
     // Flush the region (touched cells only) and clear it for the next item.

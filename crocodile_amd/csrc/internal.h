@@ -49,7 +49,8 @@ enum WsSlot {
   WS_REC,       // tile-sorted record (uint4)
   WS_SVIS,      // tile-sorted visibilities (float2)
   WS_TILES,     // count / offset / cursor per tile
-  WS_ITEMS,     // work items (int4)
+  WS_ITEMS,     // work items (int4), tile-major
+  WS_ITEMS_G,   // work items (int4), plane-group-major
   WS_SMALL,     // status, item count, queue heads
   WS_FF,        // w-kernel far fields (double2)
   WS_TMP,       // oversampling intermediate (double2)
@@ -81,7 +82,7 @@ struct croc_ctx {
   int num_cu = 256;
   int tile = 0;     // 0: automatic
   int chunk = 1024; // visibilities per work item
-  int group_kib = 0;  // kernel-bank KiB per plane-group (0: one group)
+  int group_kib = 4096;  // kernel-bank KiB per plane-group (0: one group)
 };
 
 // Grow-only workspace; returns nullptr on failure (error already set).
@@ -179,9 +180,16 @@ struct BinPlan {
   uint4 *loc = nullptr, *rec = nullptr;
   float2 *svis = nullptr;
   uint32_t *count = nullptr, *offset = nullptr, *cursor = nullptr;
-  int4 *items = nullptr;
-  uint32_t *status = nullptr, *nitems = nullptr, *qhead = nullptr;
+  int4 *items = nullptr;    // tile-major (gridding)
+  int4 *items_g = nullptr;  // per-XCD queues (degridding)
+  uint32_t *gstart = nullptr;  // [ngroups + 1]: first item of each queue's first group
+  uint32_t *status = nullptr, *nitems = nullptr, *nitems_g = nullptr, *qhead = nullptr;
 };
+
+// Work queues of the degridding kernel: the plane groups are split into
+// XCD_QUEUES contiguous ranges, one per XCD (each XCD has its own L2), so
+// that every XCD reads kernel taps from its own slice of the bank.
+constexpr int XCD_QUEUES = 8;
 
 // Region edges the tile kernels hold in LDS: a tile of T cells with
 // windows of support up to S needs T + S - 1 <= R.  The smaller region lets
@@ -197,6 +205,48 @@ constexpr int TILE_R_SMALL = 63;
 __device__ __forceinline__ unsigned long long fixed(float x, double scale) {
   const double d = __builtin_fma((double)x, scale, 6755399441055744.0);  // 1.5 * 2^52
   return (unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull;
+}
+
+// round(a * b + c * d) as a 64-bit integer, for |a * b| + |c * d| < 2^50
+// (a and c carry the fixed-point scale 2^e): the inner fma rounds c * d +
+// 1.5 * 2^52 once to an integer, the outer adds a * b and rounds once more,
+// so the result is within 1 of the exact value.  Products of FP32 inputs
+// are exact in FP64; four fma and two integer adds per complex tap.
+__device__ __forceinline__ unsigned long long fixed2(double a, double b, double c, double d) {
+  const double s = __builtin_fma(a, b, __builtin_fma(c, d, 6755399441055744.0));
+  return (unsigned long long)__double_as_longlong(s) - 0x4338000000000000ull;
+}
+
+// Tap t of an S x S window (row-major, S <= 64, t < S * S) lies in row
+// t / S.  tap_div(S) returns m with 2^20 / S + 0.875 <= m <= 2^20 / S +
+// 2.125 (v_rcp_f32 is within 1 ulp); then (t * m) >> 20 == t / S, because
+// the excess t * (m - 2^20 / S) / 2^20 < 4096 * 2.125 / 2^20 stays below
+// 1 / S.  t * m < 2^32 and both factors fit the 24-bit multiplier.
+__device__ __forceinline__ uint32_t tap_div(int S) {
+  return (uint32_t)(1048576.0f * __builtin_amdgcn_rcpf((float)S)) + 2u;
+}
+__device__ __forceinline__ int tap_row(int t, uint32_t m) {
+  return (int)(__umul24((uint32_t)t, m) >> 20);
+}
+
+// p[off / 8] for a wave-uniform table pointer and a 32-bit byte offset
+// (scalar base + vector offset addressing, no 64-bit address arithmetic).
+__device__ __forceinline__ float2 ldk(const float2 *p, uint32_t off) {
+  return *(const float2 *)((const char *)p + off);
+}
+
+// Load / store of a complex64 with the non-temporal hint: data read or
+// written once stays out of the way of the kernel tables in L2.
+typedef float croc_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 load_stream(const float2 *p) {
+  const croc_v2f v = __builtin_nontemporal_load((const croc_v2f *)p);
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void store_stream(float2 *p, float2 v) {
+  croc_v2f w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, (croc_v2f *)p);
 }
 
 __device__ __forceinline__ float cabs_max(float2 a) {
@@ -264,11 +314,12 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, const float2 *svis, void *gr
                 int nplanes, const int64_t *plane_off, const int32_t *plane_s);
 int degrid_binned(croc_ctx *ctx, const BinPlan &plan, const void *grid, int N, void *vis_out,
                   const void *kern, int separable);
-// Work-item order: ITEMS_BY_TILE cuts each tile's records (all groups) into
-// items (gridding: one LDS flush per item); ITEMS_BY_GROUP cuts each (tile,
-// group) bin and queues them group-major (degridding: all CUs read the same
-// slice of the kernel bank at a time).
-enum { ITEMS_BY_TILE = 0, ITEMS_BY_GROUP = 1 };
+// Work-item lists (bit mask): ITEMS_BY_TILE cuts each tile's records (all
+// groups) into items (gridding: one LDS flush per item); ITEMS_BY_GROUP
+// splits the plane groups into one contiguous range per XCD queue and cuts
+// each tile's records of a range into items (degridding: each XCD reads
+// kernel taps from its own slice of the bank).
+enum { ITEMS_BY_TILE = 1, ITEMS_BY_GROUP = 2 };
 int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_support,
                      int order, BinPlan *plan);
 int read_status(croc_ctx *ctx, const BinPlan &plan);

@@ -86,7 +86,8 @@ extern "C" int croc_plan_bin(croc_ctx *ctx, croc_plan *plan, int N, const double
   if (nv > 0) {
     BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_off, separable,
               plane_mode, vplane, wstep, wbin0, nullptr, max_support};
-    rc = bin_visibilities(ctx, plan->ws, a, max_support, ITEMS_BY_TILE, &plan->bp);
+    rc = bin_visibilities(ctx, plan->ws, a, max_support, ITEMS_BY_TILE | ITEMS_BY_GROUP,
+                         &plan->bp);
     if (rc) return rc;
     rc = read_status(ctx, plan->bp);
     if (rc) return rc;
@@ -107,7 +108,7 @@ extern "C" int croc_convgrid_plan(croc_ctx *ctx, croc_plan *plan, void *grid,
   k_gather_vis<<<(unsigned)((plan->nv + 255) / 256), 256, 0, ctx->stream>>>(
       plan->nv, plan->bp.rec, (const float2 *)vis, svis);
   CROC_HIP(hipGetLastError());
-  rc = zero_u32_async(ctx, plan->bp.qhead, 8);  // work queue restarts
+  rc = zero_u32_async(ctx, plan->bp.qhead, XCD_QUEUES);  // work queue restarts
   if (rc) return rc;
   return grid_binned(ctx, plan->bp, svis, grid, plan->N, plan->nv, vis, kern, plan->Q,
                      plan->separable, plan->nplanes, plan->plane_off, plan->plane_s);
@@ -120,7 +121,7 @@ extern "C" int croc_convdegrid_plan(croc_ctx *ctx, croc_plan *plan, const void *
   CROC_REQUIRE(grid && kern, CROC_EINVAL, "null array argument");
   if (plan->nv == 0) return CROC_OK;
   CROC_REQUIRE(vis_out != nullptr, CROC_EINVAL, "null vis_out");
-  rc = zero_u32_async(ctx, plan->bp.qhead, 8);  // work queue restarts
+  rc = zero_u32_async(ctx, plan->bp.qhead, XCD_QUEUES);  // work queue restarts
   if (rc) return rc;
   return degrid_binned(ctx, plan->bp, grid, plan->N, vis_out, kern, plan->separable);
 }

@@ -179,6 +179,26 @@ def test_convgrid_convdegrid_parity(gpu, N, S, Q, nv, planes, separable):
     assert_close(vd, vref, what='degrid')
 
 
+def test_every_support_size(gpu):
+    """Supports 1..64: the window walk splits S*S taps into unmasked blocks
+    of 1024 / 512 / 256 taps and one masked remainder, and finds each tap's
+    row with a 24-bit reciprocal multiply; every split and every S is hit."""
+    for S in range(1, 65):
+        for separable in (False, True):
+            rng = numpy.random.default_rng(1000 + 2 * S + separable)
+            N, Q, nv = 2 * S + 40, 3, 150
+            tables, p, v = random_case(rng, N, S, Q, nv, 1, separable)
+            bank = cgrid.Bank(tables, separable)
+            ref = numpy.zeros((N, N), dtype=complex)
+            cgrid.convgrid(bank, ref, p, v, None)
+            a = numpy.zeros((N, N), dtype=complex)
+            G.convgrid(tables[0], a, p, None, v)
+            assert_close(a, ref, what='grid S=%d sep=%d' % (S, separable))
+            vd = G.convdegrid(tables[0], ref, p, None)
+            assert_close(vd, cgrid.convdegrid(bank, ref, p, None),
+                         what='degrid S=%d sep=%d' % (S, separable))
+
+
 def test_convgrid_accumulates_in_place(gpu):
     rng = numpy.random.default_rng(2)
     (gcf,), p, v = random_case(rng, 64, 6, 4, 500)
