@@ -49,7 +49,7 @@ def parse():
                     help='skip the measured choice of tile / chunk during warm-up')
     ap.add_argument('--tile', type=int, default=0)
     ap.add_argument('--chunk', type=int, default=1024)
-    ap.add_argument('--group-kib', type=int, default=4096)
+    ap.add_argument('--group-kib', type=int, default=0)
     return ap.parse_args()
 
 

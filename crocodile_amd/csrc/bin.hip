@@ -5,13 +5,16 @@
 // w-bin of w_cache_imaging :793) and resolved to the offset of its kernel
 // table, and it is assigned to the bin (tile, plane-group): the tile holds
 // the first cell of its S x S window, the plane-group is the L2-sized slice
-// of the kernel bank its table lies in.  A counting sort lays the records out
-// bin by bin (tile-major), and the scan cuts them into work items of at most
-// `chunk` visibilities for the persistent tile kernels, either per tile
-// (gridding) or per bin in group-major order (degridding).  Block-local LDS
-// histograms keep global atomics to one per (block, bin) pair, which matters
+// of the kernel bank its table lies in.  A counting sort lays the records
+// out bin by bin (tile-major), and two scans cut them into work items of at
+// most `chunk` visibilities for the persistent tile kernels: per tile
+// (gridding), and per (XCD queue, tile) where each queue covers a contiguous
+// range of plane groups (degridding).  Block-local LDS histograms keep
+// global atomics to one per (block, bin) pair, which matters
 // for the strongly centre-peaked uv densities of real arrays.
 #include "internal.h"
+
+#include <hipcub/hipcub.hpp>
 
 namespace {
 
@@ -105,100 +108,74 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_count(
       if (hist[t]) atomicAdd(&count[t], hist[t]);
 }
 
-// Block-wide exclusive scan of one value per thread (Hillis-Steele in LDS);
-// returns the thread's exclusive prefix.
-__device__ uint32_t block_exclusive_scan(uint32_t x, uint32_t *sc) {
-  const int tid = threadIdx.x;
-  sc[tid] = x;
-  __syncthreads();
-  for (int off = 1; off < BIN_NT; off <<= 1) {
-    const uint32_t a = (tid >= off) ? sc[tid - off] : 0u;
-    __syncthreads();
-    sc[tid] += a;
-    __syncthreads();
+// Item slots: slot t < ntiles is tile t of the tile-major list; slot ntiles
+// + x * ntiles + t is tile t's records of queue x (groups [x * ngroups / nq,
+// (x + 1) * ngroups / nq)) in the per-XCD list.  scount[slot] = items the
+// slot needs; the record offsets are copied to the scatter cursors.
+__global__ void k_item_counts(int ntiles, int ngroups, int nq, int chunk, int order, int nbins,
+                              const uint32_t *__restrict__ offset,
+                              uint32_t *__restrict__ cursor, uint32_t *__restrict__ scount) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbins) cursor[i] = offset[i];
+  const int nslots = ntiles * (1 + nq);
+  if (i >= nslots) return;
+  uint32_t c = 0;
+  if (i < ntiles) {
+    if (order & ITEMS_BY_TILE) c = offset[(i + 1) * ngroups] - offset[i * ngroups];
+  } else if (order & ITEMS_BY_GROUP) {
+    const int q = i - ntiles, x = q / ntiles, t = q - x * ntiles;
+    const int ga = (int)((int64_t)x * ngroups / nq), gb = (int)((int64_t)(x + 1) * ngroups / nq);
+    c = offset[t * ngroups + gb] - offset[t * ngroups + ga];
   }
-  const uint32_t r = sc[tid] - x;
-  __syncthreads();
-  return r;
+  scount[i] = (c + chunk - 1) / chunk;
 }
 
-// One block: exclusive scan of the bin counts (record offsets, tile-major),
-// then the requested work-item lists: tile-major `items` and / or the
-// per-XCD queues `items_g`.  The groups are cut into nq = min(8, ngroups)
-// contiguous ranges; queue x lists, tile by tile, the records of range x
-// (group-sorted within each item), and gstart[first group of range x] is
-// its first item.  An error status empties the item lists so that no
-// output is modified.
-__global__ __launch_bounds__(BIN_NT) void k_bin_scan(
-    int ntiles, int ngroups, int chunk, int order,
-    const uint32_t *__restrict__ count, uint32_t *__restrict__ offset,
-    uint32_t *__restrict__ cursor, int4 *__restrict__ items, int4 *__restrict__ items_g,
-    uint32_t *__restrict__ gstart, uint32_t *__restrict__ nitems,
-    uint32_t *__restrict__ nitems_g, const uint32_t *__restrict__ status,
-    uint32_t *__restrict__ qheads) {
-  __shared__ uint32_t sc[BIN_NT];
-  const int tid = threadIdx.x;
-  const int nbins = ntiles * ngroups;
+// Writes every slot's items at its scanned offset soff[slot] (the per-XCD
+// list renumbered from 0), gstart for the first group of each queue, the
+// item counts, and resets the queue heads.  An error status leaves both
+// lists empty, so that no output is modified.
+__global__ void k_item_write(int ntiles, int ngroups, int nq, int chunk,
+                             const uint32_t *__restrict__ offset,
+                             const uint32_t *__restrict__ scount,
+                             const uint32_t *__restrict__ soff, int4 *__restrict__ items,
+                             int4 *__restrict__ items_g, uint32_t *__restrict__ gstart,
+                             uint32_t *__restrict__ nitems, uint32_t *__restrict__ nitems_g,
+                             const uint32_t *__restrict__ status,
+                             uint32_t *__restrict__ qheads) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nslots = ntiles * (1 + nq);
   const bool failed = *status != 0;
-
-  // record offsets, bins in tile-major order
-  const int per = (nbins + BIN_NT - 1) / BIN_NT;
-  const int b0 = min(nbins, tid * per), b1 = min(nbins, b0 + per);
-  uint32_t cs = 0;
-  for (int b = b0; b < b1; b++) cs += count[b];
-  uint32_t co = block_exclusive_scan(cs, sc);
-  for (int b = b0; b < b1; b++) {
-    offset[b] = co;
-    cursor[b] = co;
-    co += count[b];
+  if (i < XCD_QUEUES) qheads[i] = 0;
+  if (i >= nslots) return;
+  const uint32_t base_g = soff[ntiles];  // items of the tile-major list
+  if (i == nslots - 1) {
+    const uint32_t total = soff[i] + scount[i];
+    *nitems = failed ? 0u : base_g;
+    *nitems_g = failed ? 0u : total - base_g;
+    gstart[ngroups] = failed ? 0u : total - base_g;
   }
-  if (tid == BIN_NT - 1) offset[nbins] = co;
-  __syncthreads();  // offsets visible to the whole block
-
-  if (order & ITEMS_BY_TILE) {
-    const int tper = (ntiles + BIN_NT - 1) / BIN_NT;
-    const int t0 = min(ntiles, tid * tper), t1 = min(ntiles, t0 + tper);
-    uint32_t is = 0;
-    for (int t = t0; t < t1; t++) {
-      const uint32_t c = offset[(t + 1) * ngroups] - offset[t * ngroups];
-      is += (c + chunk - 1) / chunk;
-    }
-    uint32_t io = block_exclusive_scan(is, sc);
-    for (int t = t0; t < t1; t++) {
-      const uint32_t s0 = offset[t * ngroups], s1 = offset[(t + 1) * ngroups];
-      for (uint32_t s = s0; s < s1; s += chunk)
-        items[io++] = make_int4(t, (int)s, (int)min(s1, s + chunk), 0);
-    }
-    if (tid == BIN_NT - 1) *nitems = failed ? 0u : io;
+  int t;
+  uint32_t s0, s1, io;
+  int4 *dst;
+  if (i < ntiles) {
+    t = i;
+    s0 = offset[t * ngroups];
+    s1 = offset[(t + 1) * ngroups];
+    io = soff[i];
+    dst = items;
+  } else {
+    const int q = i - ntiles, x = q / ntiles;
+    t = q - x * ntiles;
+    const int ga = (int)((int64_t)x * ngroups / nq), gb = (int)((int64_t)(x + 1) * ngroups / nq);
+    s0 = offset[t * ngroups + ga];
+    s1 = offset[t * ngroups + gb];
+    io = soff[i] - base_g;
+    dst = items_g;
+    if (t == 0) gstart[ga] = failed ? 0u : io;
   }
-  if (order & ITEMS_BY_GROUP) {
-    // queue-major: index q = x * ntiles + t covers tile t's groups of queue x
-    const int nq = min(XCD_QUEUES, ngroups);
-    const int nqs = nq * ntiles;
-    const int qper = (nqs + BIN_NT - 1) / BIN_NT;
-    const int q0 = min(nqs, tid * qper), q1 = min(nqs, q0 + qper);
-    uint32_t is = 0;
-    for (int q = q0; q < q1; q++) {
-      const int x = q / ntiles, t = q - x * ntiles;
-      const int ga = (int)((int64_t)x * ngroups / nq), gb = (int)((int64_t)(x + 1) * ngroups / nq);
-      is += (offset[t * ngroups + gb] - offset[t * ngroups + ga] + chunk - 1) / chunk;
-    }
-    uint32_t io = block_exclusive_scan(is, sc);
-    for (int q = q0; q < q1; q++) {
-      const int x = q / ntiles, t = q - x * ntiles;
-      const int ga = (int)((int64_t)x * ngroups / nq), gb = (int)((int64_t)(x + 1) * ngroups / nq);
-      if (t == 0) gstart[ga] = failed ? 0u : io;
-      const uint32_t s0 = offset[t * ngroups + ga], s1 = offset[t * ngroups + gb];
-      for (uint32_t s = s0; s < s1; s += chunk)
-        items_g[io++] = make_int4(t, (int)s, (int)min(s1, s + chunk), 0);
-    }
-    if (tid == BIN_NT - 1) {
-      *nitems_g = failed ? 0u : io;
-      gstart[ngroups] = failed ? 0u : io;
-    }
-  }
-  if (tid == 0)
-    for (int q = 0; q < XCD_QUEUES; q++) qheads[q] = 0;
+  if (failed || scount[i] == 0) return;
+  for (uint32_t s = s0; s < s1; s += chunk)
+    dst[io++] = make_int4(t, (int)s, (int)min(s1, s + chunk), 0);
 }
 
 __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
@@ -284,17 +261,21 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   p.chunk = ctx->chunk;
 
   // Plane-groups: slices of group_kib KiB of an upper bound of the bank's
-  // extent, at most 64 (bin counts beyond the LDS histogram use global
-  // atomics).
+  // extent; automatically (group_kib 0) one group per XCD queue when that
+  // bound exceeds one XCD's L2 (4 MiB), else one group.  At most 64, and as
+  // many as keep the bin histogram in LDS.
   const int64_t S = max_support;
   const int64_t bound = (int64_t)a.nplanes * (a.separable ? a.Q * S : (int64_t)a.Q * a.Q * S * S);
-  int64_t ng = 1;
+  int64_t ng;
   if (ctx->group_kib > 0) {
     const int64_t gel = (int64_t)ctx->group_kib * 1024 / 8;
     ng = (bound + gel - 1) / gel;
-    ng = std::min<int64_t>(ng, 64);
-    ng = std::max<int64_t>(ng, 1);
+  } else {
+    ng = 8 * bound > ((int64_t)4 << 20) ? XCD_QUEUES : 1;
   }
+  ng = std::min<int64_t>(ng, 64);
+  ng = std::min<int64_t>(ng, std::max<int64_t>(1, HIST_MAX / p.ntiles));
+  ng = std::max<int64_t>(ng, 1);
   p.ngroups = (int)ng;
   p.group_elems = std::max<int64_t>(1, (bound + ng - 1) / ng);
   p.nbins = p.ntiles * p.ngroups;
@@ -303,8 +284,21 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   p.loc = (uint4 *)ws_get_in(ctx, set, WS_LOC, sizeof(uint4) * a.nv);
   p.rec = (uint4 *)ws_get_in(ctx, set, WS_REC, sizeof(uint4) * a.nv);
   if (a.vis) p.svis = (float2 *)ws_get_in(ctx, set, WS_SVIS, sizeof(float2) * a.nv);
+  const int nq = std::min(XCD_QUEUES, p.ngroups);
+  const int nslots = p.ntiles * (1 + nq);
+  size_t scan_bytes = 0, t_bytes = 0;
+  CROC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t_bytes, (const uint32_t *)nullptr,
+                                            (uint32_t *)nullptr, p.nbins + 1, ctx->stream));
+  scan_bytes = t_bytes;
+  CROC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t_bytes, (const uint32_t *)nullptr,
+                                            (uint32_t *)nullptr, nslots, ctx->stream));
+  scan_bytes = std::max(scan_bytes, t_bytes);
+  // count, offset [nbins + 1]; cursor [nbins]; gstart [ngroups + 1];
+  // scount, soff [nslots]
   uint32_t *tiles = (uint32_t *)ws_get_in(
-      ctx, set, WS_TILES, sizeof(uint32_t) * (3 * (size_t)p.nbins + p.ngroups + 2));
+      ctx, set, WS_TILES,
+      sizeof(uint32_t) * (3 * (size_t)p.nbins + p.ngroups + 3 + 2 * (size_t)nslots));
+  void *scan_tmp = ws_get_in(ctx, set, WS_SCAN, scan_bytes + 16);
   p.items = (order & ITEMS_BY_TILE)
                 ? (int4 *)ws_get_in(ctx, set, WS_ITEMS, sizeof(int4) * p.max_items)
                 : nullptr;
@@ -312,20 +306,22 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
                   ? (int4 *)ws_get_in(ctx, set, WS_ITEMS_G, sizeof(int4) * p.max_items)
                   : nullptr;
   uint32_t *small = (uint32_t *)ws_get_in(ctx, set, WS_SMALL, 256);
-  if (!p.loc || !p.rec || (a.vis && !p.svis) || !tiles || !small ||
+  if (!p.loc || !p.rec || (a.vis && !p.svis) || !tiles || !scan_tmp || !small ||
       (!p.items && (order & ITEMS_BY_TILE)) || (!p.items_g && (order & ITEMS_BY_GROUP)))
     return CROC_ENOMEM;
   p.count = tiles;
-  p.offset = tiles + p.nbins;
-  p.cursor = tiles + 2 * p.nbins + 1;
-  p.gstart = tiles + 3 * p.nbins + 1;
+  p.offset = tiles + p.nbins + 1;
+  p.cursor = tiles + 2 * p.nbins + 2;
+  p.gstart = tiles + 3 * p.nbins + 2;
+  uint32_t *scount = p.gstart + p.ngroups + 1;
+  uint32_t *soff = scount + nslots;
   p.status = small;
   p.nitems = small + 1;
   p.nitems_g = small + 2;
   p.qhead = small + 8;  // XCD_QUEUES queue heads
 
   hipStream_t s = ctx->stream;
-  int rc = zero_u32_async(ctx, p.count, p.nbins);
+  int rc = zero_u32_async(ctx, p.count, p.nbins + 1);
   if (rc) return rc;
   rc = zero_u32_async(ctx, small, 16);
   if (rc) return rc;
@@ -333,9 +329,18 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   const int nblk = (int)((a.nv + BIN_VPB - 1) / BIN_VPB);
   if (nblk > 0)
     k_bin_count<<<nblk, BIN_NT, 0, s>>>(a, g, p.nbins, p.loc, p.count, p.status);
-  k_bin_scan<<<1, BIN_NT, 0, s>>>(p.ntiles, p.ngroups, p.chunk, order, p.count,
-                                  p.offset, p.cursor, p.items, p.items_g, p.gstart,
-                                  p.nitems, p.nitems_g, p.status, p.qhead);
+  // record offsets (bins tile-major, offset[nbins] = total), then the items
+  t_bytes = scan_bytes;
+  CROC_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, t_bytes, p.count, p.offset,
+                                            p.nbins + 1, s));
+  const int nth = std::max(p.nbins, nslots);
+  k_item_counts<<<(nth + 255) / 256, 256, 0, s>>>(p.ntiles, p.ngroups, nq, p.chunk, order,
+                                                  p.nbins, p.offset, p.cursor, scount);
+  t_bytes = scan_bytes;
+  CROC_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, t_bytes, scount, soff, nslots, s));
+  k_item_write<<<(nslots + 255) / 256, 256, 0, s>>>(
+      p.ntiles, p.ngroups, nq, p.chunk, p.offset, scount, soff, p.items, p.items_g, p.gstart,
+      p.nitems, p.nitems_g, p.status, p.qhead);
   if (nblk > 0)
     k_bin_scatter<<<nblk, BIN_NT, 0, s>>>(a.nv, p.nbins, p.loc, p.cursor,
                                           p.rec, a.vis, p.svis);

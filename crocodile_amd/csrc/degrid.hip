@@ -102,11 +102,6 @@ __device__ __forceinline__ void gather_window(
                                       lane, k2, ky, kx, ar, ai);
 }
 
-// XCD this workgroup runs on (hardware register XCC_ID, gfx940 and later).
-__device__ __forceinline__ int xcd_id() {
-  return (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & (XCD_QUEUES - 1));
-}
-
 // Items are listed group-major; the groups are cut into nq contiguous
 // ranges, queue x holding the items of range x.  A workgroup drains the
 // queue of its own XCD first, then helps with the others in turn, so every

@@ -52,6 +52,7 @@ enum WsSlot {
   WS_ITEMS,     // work items (int4), tile-major
   WS_ITEMS_G,   // work items (int4), plane-group-major
   WS_SMALL,     // status, item count, queue heads
+  WS_SCAN,      // binning scan temporaries
   WS_FF,        // w-kernel far fields (double2)
   WS_TMP,       // oversampling intermediate (double2)
   WS_ACC,       // int64 fixed-point gridding accumulator [N][N][2]
@@ -82,7 +83,7 @@ struct croc_ctx {
   int num_cu = 256;
   int tile = 0;     // 0: automatic
   int chunk = 1024; // visibilities per work item
-  int group_kib = 4096;  // kernel-bank KiB per plane-group (0: one group)
+  int group_kib = 0;  // kernel-bank KiB per plane-group (0: automatic)
 };
 
 // Grow-only workspace; returns nullptr on failure (error already set).
@@ -190,6 +191,11 @@ struct BinPlan {
 // XCD_QUEUES contiguous ranges, one per XCD (each XCD has its own L2), so
 // that every XCD reads kernel taps from its own slice of the bank.
 constexpr int XCD_QUEUES = 8;
+
+// XCD this workgroup runs on (hardware register XCC_ID, gfx940 and later).
+__device__ __forceinline__ int xcd_id() {
+  return (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & (XCD_QUEUES - 1));
+}
 
 // Region edges the tile kernels hold in LDS: a tile of T cells with
 // windows of support up to S needs T + S - 1 <= R.  The smaller region lets

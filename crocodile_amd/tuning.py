@@ -17,7 +17,7 @@ from . import synthesis as G
 
 TILES = (0, 48, 32)          # 0: the largest tile the support allows
 CHUNKS = (512, 1024, 2048)   # visibilities per work item
-GROUPS = (4096,)             # KiB of kernel bank per plane group (per-XCD degridding queues)
+GROUPS = (0,)                # plane groups: automatic (one per XCD for large banks)
 
 
 def autotune(bank, N, uvw, lam, vis, mode=_lib.PLANE_SINGLE, vplane=None, wstep=1.0, wbin0=0,

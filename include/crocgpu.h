@@ -61,10 +61,11 @@ int croc_ctx_set_stream(croc_ctx *ctx, void *hip_stream);
 /* Tuning knobs of convgrid / convdegrid: tile edge in grid cells (0 =
  * automatic, else a multiple of 16 in [16, 80]), the most visibilities one
  * work item of a tile holds (>= 64), and the kernel-bank KiB per plane-group
- * (default 4096; 0 = one group).  Degridding splits the groups into one
- * contiguous range per XCD, so that each XCD's L2 serves its own slice of a
- * bank far larger than one L2; gridding visits each tile's visibilities
- * group by group.  Results do not depend on them. */
+ * (0 = automatic, the default: eight groups when the bank can exceed 4 MiB,
+ * else one).  Degridding splits the groups into one contiguous range per
+ * XCD, so that each XCD's L2 serves its own slice of a bank far larger than
+ * one L2; gridding visits each tile's visibilities group by group.  Results
+ * do not depend on them. */
 int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk, int group_kib);
 /* Device time in milliseconds of the last gridding / degridding kernel
  * (HIP events around its launch on the context stream). */
