@@ -27,6 +27,7 @@ import time
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP16_MFMA_PEAK_TFLOPS = 2500.0  # dense
 LDS_PEAK_TBS = 78.6  # 256 CUs x 128 B/clk x 2.4 GHz
 
 
@@ -36,7 +37,8 @@ def parse():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='wproj2048',
-                    choices=['wproj2048', 'plain512', 'towers4096', 'towers8192', 'diff1024'])
+                    choices=['wproj2048', 'plain512', 'towers4096', 'towers8192', 'diff1024',
+                             'dft128'])
     ap.add_argument('--nv', type=int, default=0,
                     help='towers workloads: visibilities (0 = the workload\'s own count)')
     ap.add_argument('--cpu-sample', type=int, default=300_000,
@@ -45,10 +47,12 @@ def parse():
     ap.add_argument('--copy-probe-mib', type=int, default=0,
                     help='before warm-up, copy a buffer of this many MiB once (a plain copy '
                          'of known size, to scale HBM counters of a profiled run)')
-    ap.add_argument('--no-autotune', action='store_true',
-                    help='skip the measured choice of tile / chunk during warm-up')
+    ap.add_argument('--autotune', action='store_true',
+                    help='measure a few tile / chunk settings during warm-up and keep the '
+                         'fastest (the default is the library default: automatic tile, '
+                         'chunk 512)')
     ap.add_argument('--tile', type=int, default=0)
-    ap.add_argument('--chunk', type=int, default=1024)
+    ap.add_argument('--chunk', type=int, default=512)
     ap.add_argument('--group-kib', type=int, default=0)
     return ap.parse_args()
 
@@ -249,6 +253,59 @@ def run_towers(args, torch, G, W, _lib, dev):
     print(json.dumps(line), flush=True)
 
 
+def run_dft(args, torch, _lib, dev):
+    """The w-towers sub-grid transform stage on its own: a batch of 2048
+    centred 2D transforms of 128 x 128 complex64 sub-grids (crocodile's fft,
+    synthesis.py:83-93, per sub-grid), as the dense DFT Y = D X D on the
+    matrix cores (croc_dft2d_batched, fp16x3 split products with FP32
+    accumulation) and as the radix 16 x 8 FFT on the vector ALUs
+    (croc_fft2d_batched).  The roofline is the matrix-core one: algorithmic
+    16 n^3 flop per transform (two complex GEMMs) against the fp16 dense
+    peak; the fp16x3 split executes three times that on the matrix cores."""
+    n, batch = 128, 2048
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1005)
+    x = torch.randn((batch, n, n), dtype=torch.complex64, device=dev, generator=gen)
+    y = torch.empty_like(x)
+    L = _lib.load()
+
+    def run(entry):
+        fn = getattr(L, entry)
+        for _ in range(args.warmup):
+            _lib.check(fn(_lib.ctx(), _lib.ptr(x), _lib.ptr(y), n, batch, 0))
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            _lib.check(fn(_lib.ctx(), _lib.ptr(x), _lib.ptr(y), n, batch, 0))
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    print('[bench] timing %d dft128 steps' % args.steps, file=sys.stderr, flush=True)
+    dft_ms = run('croc_dft2d_batched')
+    fft_ms = run('croc_fft2d_batched')
+    flops = 16.0 * n ** 3 * batch
+    achieved = flops / (dft_ms * 1e-3) / 1e12
+    line = {
+        'metric': 'sub-grid 2D transforms/s (128x128 complex64, centred DFT)',
+        'value': round(batch / (dft_ms * 1e-3), 1), 'unit': 'transforms/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(dft_ms, 4),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'fp16x3 (fp32 accumulate)', 'data': 'synthetic',
+        'config': {'workload': 'dft128', 'n': n, 'batch': batch},
+        'roofline': {'bound': 'mfma', 'achieved': round(achieved, 2),
+                     'peak': FP16_MFMA_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': round(achieved / FP16_MFMA_PEAK_TFLOPS, 5),
+                     'executed_tflops': round(3 * achieved, 2), 'kernel': 'k_dft2d',
+                     'traffic': None},
+        'fft': {'entry': 'croc_fft2d_batched', 'ms_per_step': round(fft_ms, 4),
+                'transforms_per_s': round(batch / (fft_ms * 1e-3), 1),
+                'note': 'radix 16 x 8 FFT on the vector ALUs, the w-towers default'},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_diff(args, torch, G, W, _lib, dev):
     """Differentiable degridding on the 1024^2 Gaussian-blob image with 1e6
     visibilities (w +-500, w-kernels per 20-wavelength bin, S 16..32, Q 8):
@@ -317,6 +374,8 @@ def main():
     _lib.set_tuning(args.tile, args.chunk, args.group_kib)
     if args.workload.startswith('towers'):
         return run_towers(args, torch, G, W, _lib, dev)
+    if args.workload == 'dft128':
+        return run_dft(args, torch, _lib, dev)
     if args.workload == 'diff1024':
         return run_diff(args, torch, G, W, _lib, dev)
     print('[bench] building workload %s' % args.workload, file=sys.stderr, flush=True)
@@ -357,13 +416,15 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # Warm-up includes the measured choice of tile and chunk (results do not
-    # depend on them) on this workload.
-    if not args.no_autotune:
+    # With --autotune the warm-up also measures a few tile / chunk settings on
+    # this workload and keeps the fastest (results do not depend on them).
+    if args.autotune:
         from crocodile_amd import tuning
         choice, _ = tuning.autotune(bank_img, N, uvw_d, lam, vis_d, mode, None, wstep, wbin0,
                                     repeats=3, tiles=(0, 48), chunks=(512, 1024, 2048))
         config['tuning'] = {'tile': choice[0], 'chunk': choice[1], 'group_kib': choice[2]}
+    else:
+        config['tuning'] = {'tile': args.tile, 'chunk': args.chunk, 'group_kib': args.group_kib}
 
     # One step: bin the coordinates once (croc_plan_bin, which validates
     # them), grid into a cleared grid, degrid it (croc_convgrid_plan /

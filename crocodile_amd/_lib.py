@@ -125,7 +125,7 @@ def last_kernel_ms():
     return a.value, b.value
 
 
-def set_tuning(tile=0, chunk=1024, group_kib=0):
+def set_tuning(tile=0, chunk=512, group_kib=0):
     check(load().croc_ctx_set_tuning(ctx(), tile, chunk, group_kib))
 
 

@@ -109,32 +109,33 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_count(
 }
 
 // Item slots: slot t < ntiles is tile t of the tile-major list; slot ntiles
-// + x * ntiles + t is tile t's records of queue x (groups [x * ngroups / nq,
-// (x + 1) * ngroups / nq)) in the per-XCD list.  scount[slot] = items the
-// slot needs; the record offsets are copied to the scatter cursors.
-__global__ void k_item_counts(int ntiles, int ngroups, int nq, int chunk, int order, int nbins,
+// + r * ntiles + t is tile t's records of range r (groups [r * ngroups / nr,
+// (r + 1) * ngroups / nr)) in the range-major per-XCD list.  scount[slot] =
+// items the slot needs; the record offsets are copied to the scatter
+// cursors.
+__global__ void k_item_counts(int ntiles, int ngroups, int nr, int chunk, int order, int nbins,
                               const uint32_t *__restrict__ offset,
                               uint32_t *__restrict__ cursor, uint32_t *__restrict__ scount) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nbins) cursor[i] = offset[i];
-  const int nslots = ntiles * (1 + nq);
+  const int nslots = ntiles * (1 + nr);
   if (i >= nslots) return;
   uint32_t c = 0;
   if (i < ntiles) {
     if (order & ITEMS_BY_TILE) c = offset[(i + 1) * ngroups] - offset[i * ngroups];
   } else if (order & ITEMS_BY_GROUP) {
-    const int q = i - ntiles, x = q / ntiles, t = q - x * ntiles;
-    const int ga = (int)((int64_t)x * ngroups / nq), gb = (int)((int64_t)(x + 1) * ngroups / nq);
+    const int q = i - ntiles, r = q / ntiles, t = q - r * ntiles;
+    const int ga = (int)((int64_t)r * ngroups / nr), gb = (int)((int64_t)(r + 1) * ngroups / nr);
     c = offset[t * ngroups + gb] - offset[t * ngroups + ga];
   }
   scount[i] = (c + chunk - 1) / chunk;
 }
 
 // Writes every slot's items at its scanned offset soff[slot] (the per-XCD
-// list renumbered from 0), gstart for the first group of each queue, the
+// list renumbered from 0), gstart for the first group of each range, the
 // item counts, and resets the queue heads.  An error status leaves both
 // lists empty, so that no output is modified.
-__global__ void k_item_write(int ntiles, int ngroups, int nq, int chunk,
+__global__ void k_item_write(int ntiles, int ngroups, int nr, int chunk,
                              const uint32_t *__restrict__ offset,
                              const uint32_t *__restrict__ scount,
                              const uint32_t *__restrict__ soff, int4 *__restrict__ items,
@@ -143,7 +144,7 @@ __global__ void k_item_write(int ntiles, int ngroups, int nq, int chunk,
                              const uint32_t *__restrict__ status,
                              uint32_t *__restrict__ qheads) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nslots = ntiles * (1 + nq);
+  const int nslots = ntiles * (1 + nr);
   const bool failed = *status != 0;
   if (i < XCD_QUEUES) qheads[i] = 0;
   if (i >= nslots) return;
@@ -164,9 +165,9 @@ __global__ void k_item_write(int ntiles, int ngroups, int nq, int chunk,
     io = soff[i];
     dst = items;
   } else {
-    const int q = i - ntiles, x = q / ntiles;
-    t = q - x * ntiles;
-    const int ga = (int)((int64_t)x * ngroups / nq), gb = (int)((int64_t)(x + 1) * ngroups / nq);
+    const int q = i - ntiles, r = q / ntiles;
+    t = q - r * ntiles;
+    const int ga = (int)((int64_t)r * ngroups / nr), gb = (int)((int64_t)(r + 1) * ngroups / nr);
     s0 = offset[t * ngroups + ga];
     s1 = offset[t * ngroups + gb];
     io = soff[i] - base_g;
@@ -284,8 +285,9 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   p.loc = (uint4 *)ws_get_in(ctx, set, WS_LOC, sizeof(uint4) * a.nv);
   p.rec = (uint4 *)ws_get_in(ctx, set, WS_REC, sizeof(uint4) * a.nv);
   if (a.vis) p.svis = (float2 *)ws_get_in(ctx, set, WS_SVIS, sizeof(float2) * a.nv);
-  const int nq = std::min(XCD_QUEUES, p.ngroups);
-  const int nslots = p.ntiles * (1 + nq);
+  // Ranges of the per-XCD list: one per group, at least one per queue.
+  const int nr = p.ngroups;
+  const int nslots = p.ntiles * (1 + nr);
   size_t scan_bytes = 0, t_bytes = 0;
   CROC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t_bytes, (const uint32_t *)nullptr,
                                             (uint32_t *)nullptr, p.nbins + 1, ctx->stream));
@@ -334,12 +336,12 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   CROC_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, t_bytes, p.count, p.offset,
                                             p.nbins + 1, s));
   const int nth = std::max(p.nbins, nslots);
-  k_item_counts<<<(nth + 255) / 256, 256, 0, s>>>(p.ntiles, p.ngroups, nq, p.chunk, order,
+  k_item_counts<<<(nth + 255) / 256, 256, 0, s>>>(p.ntiles, p.ngroups, nr, p.chunk, order,
                                                   p.nbins, p.offset, p.cursor, scount);
   t_bytes = scan_bytes;
   CROC_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, t_bytes, scount, soff, nslots, s));
   k_item_write<<<(nslots + 255) / 256, 256, 0, s>>>(
-      p.ntiles, p.ngroups, nq, p.chunk, p.offset, scount, soff, p.items, p.items_g, p.gstart,
+      p.ntiles, p.ngroups, nr, p.chunk, p.offset, scount, soff, p.items, p.items_g, p.gstart,
       p.nitems, p.nitems_g, p.status, p.qhead);
   if (nblk > 0)
     k_bin_scatter<<<nblk, BIN_NT, 0, s>>>(a.nv, p.nbins, p.loc, p.cursor,

@@ -82,7 +82,7 @@ struct croc_ctx {
   bool ev_grid = false, ev_degrid = false;
   int num_cu = 256;
   int tile = 0;     // 0: automatic
-  int chunk = 1024; // visibilities per work item
+  int chunk = 512;  // visibilities per work item
   int group_kib = 0;  // kernel-bank KiB per plane-group (0: automatic)
 };
 

@@ -346,7 +346,7 @@ def test_tiling_knobs_do_not_change_results(gpu):
             assert_close(G.convdegrid(gcf, ref, p), R.convdegrid(gcf, ref, p),
                          what='degrid tile=%d' % tile)
     finally:
-        _lib.set_tuning(0, 1024)
+        _lib.set_tuning()
 
 
 def test_plane_groups_do_not_change_results(gpu):
@@ -374,7 +374,7 @@ def test_plane_groups_do_not_change_results(gpu):
             assert_close(G.convdegrid(gcf, ref, p, src), vref,
                          what='degrid group_kib=%d' % group_kib)
     finally:
-        _lib.set_tuning(0, 1024)
+        _lib.set_tuning()
     for g in grids[1:]:
         assert (g == grids[0]).all()
 
@@ -392,7 +392,7 @@ def test_gridding_is_bit_reproducible(gpu):
             G.convgrid(gcf, a, p, None, v)
             grids.append(a)
     finally:
-        _lib.set_tuning(0, 1024)
+        _lib.set_tuning()
     for g in grids[1:]:
         assert (g == grids[0]).all()
 
