@@ -1,14 +1,15 @@
 // Convolutional degridding: crocodile/synthesis.py:484-503 (convdegrid) on
 // CDNA4.
 //
-// Work item = up to `chunk` visibilities of one (tile, plane-group) bin;
-// items are queued group-major, so at any time the whole chip reads kernel
-// taps from one L2-sized slice of the kernel bank.  A workgroup stages the
-// 95 x 95 cell grid region of its tile in LDS (coalesced row reads), then
-// each wavefront takes one visibility at a time: lanes walk the S*S taps,
-// gather grid cells from LDS and kernel taps from the kernel bank,
-// accumulate complex products in FP32 and the wavefront reduces the 64
-// partial sums with cross-lane shuffles.
+// Work item = up to `chunk` visibilities of one tile whose kernel tables lie
+// in one range of plane groups.  There is one queue per XCD, each holding a
+// contiguous range of the kernel bank, so that an XCD's L2 serves its own
+// slice of a bank far larger than one L2.  A workgroup stages the R x R cell
+// grid region of its tile in LDS (R = 63 or 95; coalesced, streaming row
+// reads), then each wavefront takes one visibility at a time: lanes walk the
+// S*S taps, gather grid cells from LDS and kernel taps from the kernel bank,
+// accumulate complex products in FP32, and the wavefront sums the 64
+// partial sums with DPP row operations.
 #include "internal.h"
 
 namespace {

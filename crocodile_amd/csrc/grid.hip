@@ -2,9 +2,9 @@
 // CDNA4.
 //
 // Work item = up to `chunk` visibilities whose kernel windows start inside
-// one T x T grid tile.  A workgroup privatises the 95 x 95 cell region
-// covering every such window in LDS and lets each wavefront spread one
-// visibility at a time over its S*S taps.
+// one T x T grid tile.  A workgroup privatises the R x R cell region (R =
+// 63 or 95, T + S - 1 <= R) covering every such window in LDS and lets each
+// wavefront spread one visibility at a time over its S*S taps.
 //
 // Accumulation is exact fixed point: every contribution k*v (the FP64
 // product of the FP32 tap and visibility; for separable kernels the tap is
