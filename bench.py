@@ -494,8 +494,13 @@ def main():
     # LDS side (supplementary): gridding adds 16 B per tap (two 64-bit
     # integer atomics), degridding reads 8 B per tap; peak 128 B/clk/CU.
     lds_bytes = (16.0 if name == 'k_grid_tiles' else 8.0) * taps
+    # traffic_frac: the profiled bytes beyond L2 (kernel-table taps missing an
+    # XCD's L2) per launch at this launch duration, against the HBM peak --
+    # the bound the kernel actually runs into.
     roofline.update({'kernel': name, 'traffic': traffic, 'algorithmic_bytes': alg_bytes,
                      'algorithmic_flops': flops,
+                     'traffic_frac': (round(traffic / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                      if traffic else None),
                      'hbm_frac': round(alg_bytes / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                      'lds_bytes': lds_bytes,
                      'lds_frac': round(lds_bytes / (t_ms * 1e-3) / (LDS_PEAK_TBS * 1e12), 5),
