@@ -10,103 +10,12 @@
 // S*S taps, gather grid cells from LDS and kernel taps from the kernel bank,
 // accumulate complex products in FP32, and the wavefront sums the 64
 // partial sums with DPP row operations.
-#include "internal.h"
+#include "taps.h"
 
 namespace {
 
 constexpr int DEGRID_NT = 512;
 
-// x + (x moved across lanes by the DPP control `ctrl`); rows outside
-// row_mask add 0.
-template <int CTRL, int ROW_MASK = 0xf>
-__device__ __forceinline__ float dpp_add(float x) {
-  const int y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, false);
-  return x + __int_as_float(y);
-}
-
-// Sum over the 64 lanes of a wavefront, left in lane 63: butterflies within
-// rows of 16 lanes (quad permutes, half-row and row mirrors), then the row
-// broadcasts of lanes 15 and 31.  VALU data movement only (DPP); no LDS
-// round trips on the critical path.
-__device__ __forceinline__ float wave_sum_to_63(float x) {
-  x = dpp_add<0xb1>(x);        // quad_perm [1, 0, 3, 2]
-  x = dpp_add<0x4e>(x);        // quad_perm [2, 3, 0, 1]
-  x = dpp_add<0x141>(x);       // row_half_mirror
-  x = dpp_add<0x140>(x);       // row_mirror
-  x = dpp_add<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
-  x = dpp_add<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
-  return x;
-}
-
-// Partial sums of one visibility over the block of 64*NQ taps t0 .. of its
-// S x S window (tap t = row i = t / S, column t - i*S; cell base + t +
-// i*(R - S)).  Kernel loads are issued before first use.  FULL: every tap
-// of the block lies in the window (no clamps, no masks); otherwise load
-// indices are clamped to the last tap and the products masked.  cb = 8 *
-// (base + t0 + lane) is the lane's LDS byte address, rs8 = 8 * (R - S),
-// kt = 8 * (t0 + lane).
-template <int NQ, bool FULL, bool SEPARABLE>
-__device__ __forceinline__ void gather_block(
-    const float2 *sg, uint32_t cb, uint32_t rs8, uint32_t kt, int S, int SS, uint32_t m,
-    int t0, int lane, const float2 *k2, const float2 *ky, const float2 *kx, float &ar,
-    float &ai) {
-  float2 kv[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    const int t = FULL ? t0 + q * 64 + lane : min(t0 + q * 64 + lane, SS - 1);
-    if (!SEPARABLE) {
-      kv[q] = FULL ? ldk(k2 + q * 64, kt) : ldk(k2, (uint32_t)t * 8u);
-    } else {
-      const int i = tap_row(t, m);
-      kv[q] = cmul(ldk(ky, (uint32_t)i * 8u), ldk(kx, (uint32_t)(t - i * S) * 8u));
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    if (!FULL && t0 + q * 64 >= SS) break;  // wave-uniform
-    const int t = t0 + q * 64 + lane;
-    if (FULL || t < SS) {
-      const uint32_t a = cb + __umul24((uint32_t)tap_row(t, m), rs8) + 512u * q;
-      const float2 g = *(const float2 *)((const char *)sg + a);
-      ar = fmaf(g.x, kv[q].x, fmaf(-g.y, kv[q].y, ar));
-      ai = fmaf(g.x, kv[q].y, fmaf(g.y, kv[q].x, ai));
-    }
-  }
-}
-
-// All taps of one visibility: unmasked blocks of 1024, 512 and 256 taps,
-// then at most one masked block of the remaining < 256 taps.
-template <bool SEPARABLE, int R>
-__device__ __forceinline__ void gather_window(
-    const float2 *sg, int base, int S, int lane, const float2 *k2, const float2 *ky,
-    const float2 *kx, float &ar, float &ai) {
-  const int SS = S * S;
-  const uint32_t m = __builtin_amdgcn_readfirstlane(tap_div(S));
-  const uint32_t rs8 = 8u * (uint32_t)(R - S);
-  const uint32_t cb0 = 8u * (uint32_t)(base + lane), kt0 = 8u * (uint32_t)lane;
-  int t0 = 0;
-  for (; t0 + 1024 <= SS; t0 += 1024)
-    gather_block<16, true, SEPARABLE>(sg, cb0 + 8u * t0, rs8, kt0 + 8u * t0, S, SS, m, t0,
-                                      lane, k2, ky, kx, ar, ai);
-  if (t0 + 512 <= SS) {
-    gather_block<8, true, SEPARABLE>(sg, cb0 + 8u * t0, rs8, kt0 + 8u * t0, S, SS, m, t0,
-                                     lane, k2, ky, kx, ar, ai);
-    t0 += 512;
-  }
-  if (t0 + 256 <= SS) {
-    gather_block<4, true, SEPARABLE>(sg, cb0 + 8u * t0, rs8, kt0 + 8u * t0, S, SS, m, t0,
-                                     lane, k2, ky, kx, ar, ai);
-    t0 += 256;
-  }
-  if (t0 < SS)
-    gather_block<4, false, SEPARABLE>(sg, cb0 + 8u * t0, rs8, kt0 + 8u * t0, S, SS, m, t0,
-                                      lane, k2, ky, kx, ar, ai);
-}
-
-// Items are listed group-major; the groups are cut into nq contiguous
-// ranges, queue x holding the items of range x.  A workgroup drains the
-// queue of its own XCD first, then helps with the others in turn, so every
-// XCD's L2 mostly serves one slice of the kernel bank.
 template <int NT, bool SEPARABLE, int R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_degrid_tiles(
     const float2 *__restrict__ grid, int N, int T, int ntx,
@@ -169,7 +78,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
         const float2 *ky = kp + yf * S, *kx = kp + xf * S;
         float ar = 0.f, ai = 0.f;
-        gather_window<SEPARABLE, R>(sg, base, S, lane, kp, ky, kx, ar, ai);
+        gather_range<SEPARABLE>(sg, base, R, S, 0, S * S, lane, kp, ky, kx, ar, ai);
         ar = wave_sum_to_63(ar);
         ai = wave_sum_to_63(ai);
         if (lane == 63) store_stream(vis_out + r.w, make_float2(ar, ai));

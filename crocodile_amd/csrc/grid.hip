@@ -19,7 +19,7 @@
 // bit-reproducible.  (LDS float atomics, ds_add_f32, run about ten times
 // slower than the integer ones on gfx950, which is what rules them out.)
 // A final pass adds workspace / 2^e into the caller's complex64 grid.
-#include "internal.h"
+#include "taps.h"
 
 namespace {
 
@@ -85,68 +85,6 @@ __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
 }
 
 
-// Spread one visibility over the block of 64*NQ taps t0 .. of its S x S
-// window (tap t = row i = t / S, column t - i*S; cell base + t + i*(R - S)).
-// All NQ kernel loads are issued before the first use.  FULL: every tap of
-// the block lies in the window, so there are no clamps and no masks; the
-// last, partial block of a window clamps its load indices to the last tap
-// (a valid address, value unused) and masks the adds.  cb = 16 * (base + t0
-// + lane) is the lane's LDS byte address in the interleaved region (cell c
-// is acc[2c], acc[2c + 1]); rs16 = 16 * (R - S); kt = 8 * (t0 + lane).
-template <int NQ, bool FULL, bool SEPARABLE>
-__device__ __forceinline__ void spread_block(
-    unsigned long long *acc, uint32_t cb, uint32_t rs16, uint32_t kt, int S, int SS,
-    uint32_t m, int t0, int lane, const float2 *k2, const float2 *ky, const float2 *kx,
-    double vr, double vi) {
-  float2 kv[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    const int t = FULL ? t0 + q * 64 + lane : min(t0 + q * 64 + lane, SS - 1);
-    if (!SEPARABLE) {
-      kv[q] = FULL ? ldk(k2 + q * 64, kt) : ldk(k2, (uint32_t)t * 8u);
-    } else {
-      const int i = tap_row(t, m);
-      kv[q] = cmul(ldk(ky, (uint32_t)i * 8u), ldk(kx, (uint32_t)(t - i * S) * 8u));
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    if (!FULL && t0 + q * 64 >= SS) break;  // wave-uniform
-    const int t = t0 + q * 64 + lane;
-    if (FULL || t < SS) {
-      const uint32_t a = cb + __umul24((uint32_t)tap_row(t, m), rs16) + 1024u * q;
-      unsigned long long *cell = (unsigned long long *)((char *)acc + a);
-      const double kr = kv[q].x, ki = kv[q].y;
-      atomicAdd(cell, fixed2(vr, kr, -vi, ki));
-      atomicAdd(cell + 1, fixed2(vr, ki, vi, kr));
-    }
-  }
-}
-
-// All taps of one visibility: blocks of 512 and 256 taps without masks,
-// then at most one masked block of the remaining < 256 taps.
-template <bool SEPARABLE, int R>
-__device__ __forceinline__ void spread_window(
-    unsigned long long *acc, int base, int S, int lane, const float2 *k2, const float2 *ky,
-    const float2 *kx, double vr, double vi) {
-  const int SS = S * S;
-  const uint32_t m = __builtin_amdgcn_readfirstlane(tap_div(S));
-  const uint32_t rs16 = 16u * (uint32_t)(R - S);
-  const uint32_t cb0 = 16u * (uint32_t)(base + lane), kt0 = 8u * (uint32_t)lane;
-  int t0 = 0;
-  for (; t0 + 512 <= SS; t0 += 512)
-    spread_block<8, true, SEPARABLE>(acc, cb0 + 16u * t0, rs16, kt0 + 8u * t0, S, SS, m, t0,
-                                     lane, k2, ky, kx, vr, vi);
-  if (t0 + 256 <= SS) {
-    spread_block<4, true, SEPARABLE>(acc, cb0 + 16u * t0, rs16, kt0 + 8u * t0, S, SS, m, t0,
-                                     lane, k2, ky, kx, vr, vi);
-    t0 += 256;
-  }
-  if (t0 < SS)
-    spread_block<4, false, SEPARABLE>(acc, cb0 + 16u * t0, rs16, kt0 + 8u * t0, S, SS, m, t0,
-                                      lane, k2, ky, kx, vr, vi);
-}
-
 template <int NT, bool SEPARABLE, int R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_grid_tiles(
     unsigned long long *__restrict__ acc, int N, int T, int ntx,
@@ -195,8 +133,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
       const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
       const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
       const float2 *ky = kp + yf * S, *kx = kp + xf * S;
-      spread_window<SEPARABLE, R>(sacc, base, S, lane, kp, ky, kx, (double)v.x * scale,
-                                  (double)v.y * scale);
+      spread_range<SEPARABLE>(sacc, base, R, S, 0, S * S, lane, kp, ky, kx,
+                              (double)v.x * scale, (double)v.y * scale);
     }
     barrier_after_lds_atomics();
     // Flush the region (touched cells only) and clear it for the next item.

@@ -28,7 +28,7 @@
 
 #include <hipcub/hipcub.hpp>
 
-#include "internal.h"
+#include "taps.h"
 
 namespace {
 
@@ -214,42 +214,6 @@ __global__ void k_tw_pick(TwScale *fs, int64_t nv, int nearest, int separable) {
   fs->inv_scale = ldexp(1.0, -e);
 }
 
-// Taps [tlo, thi) of one visibility's S x S window (tap t = row i, column
-// t - i*S) in blocks of 64*NQ: loads first (clamped to the range, so they
-// need no predication), then f(cell offset from the window origin, k).
-template <int NQ, bool SEPARABLE, class F>
-__device__ __forceinline__ void tap_block(int S, float invS, int t0, int thi, int lane,
-                                          const float2 *k2, const float2 *ky, const float2 *kx,
-                                          F f) {
-  float2 kv[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    const int t = min(t0 + q * 64 + lane, thi - 1);
-    if (!SEPARABLE) {
-      kv[q] = k2[t];
-    } else {
-      const int i = (int)(((float)t + 0.5f) * invS);
-      kv[q] = cmul(ky[i], kx[t - i * S]);
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    const int t = t0 + q * 64 + lane;
-    if (t < thi) {
-      const int i = (int)(((float)t + 0.5f) * invS);
-      f(i * (SG - S) + t, kv[q]);
-    }
-  }
-}
-
-template <bool SEPARABLE, class F>
-__device__ __forceinline__ void tap_range(int S, float invS, int tlo, int thi, int lane,
-                                          const float2 *k2, const float2 *ky, const float2 *kx,
-                                          F f) {
-  for (int t0 = tlo; t0 < thi; t0 += 256)
-    tap_block<4, SEPARABLE>(S, invS, t0, thi, lane, k2, ky, kx, f);
-}
-
 // Grid job parts into sub-grid slots: parts[b] = (slot, first record, end
 // record); one workgroup per part, rows [0, sg/2) then [sg/2, sg): each pass
 // zeroes its int64 region, each wavefront adds the taps of one visibility
@@ -267,7 +231,6 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
   constexpr int NW = TW_NT / 64;
   const double scale = fs->scale, inv = fs->inv_scale;
   const int SS = S * S;
-  const float invS = 1.0f / (float)S;
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     const int4 job = parts[b];
     const int jbeg = __builtin_amdgcn_readfirstlane(job.y);
@@ -298,14 +261,10 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
           const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
           const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
           const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
-          // cell of tap c: (wy - r0) * SG + wx + c, in [0, HC) for the pass's rows
-          const int cbase = (wy - r0) * SG + wx;
-          tap_range<SEPARABLE>(S, invS, ilo * S, ihi * S, lane, k2, ky, kx,
-                               [&](int c, float2 k) {
-                                 unsigned long long *cell = acc + 2 * (cbase + c);
-                                 atomicAdd(cell, fixed(v.x * k.x - v.y * k.y, scale));
-                                 atomicAdd(cell + 1, fixed(v.x * k.y + v.y * k.x, scale));
-                               });
+          // cell of tap t: (wy - r0) * SG + wx + t + i * (SG - S), in [0, HC)
+          // for the pass's rows
+          spread_range<SEPARABLE>(acc, (wy - r0) * SG + wx, SG, S, ilo * S, ihi * S, lane, k2,
+                                  ky, kx, (double)v.x * scale, (double)v.y * scale);
         }
       }
       barrier_after_lds_atomics();
@@ -405,11 +364,18 @@ __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int NW = TW_NT / 64;
   const int SS = S * S;
-  const float invS = 1.0f / (float)S;
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     const int4 job = parts[b];
     const float2 *src = sub + (int64_t)job.x * SGN;
-    for (int i = tid; i < SGN; i += TW_NT) sg[i] = src[i];
+    // stage the sub-grid, eight streaming loads in flight per thread
+    static_assert(SGN % (8 * TW_NT) == 0, "staging covers the sub-grid exactly");
+    for (int i0 = tid; i0 < SGN; i0 += 8 * TW_NT) {
+      float2 gv[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) gv[k] = load_stream(src + i0 + k * TW_NT);
+#pragma unroll
+      for (int k = 0; k < 8; k++) sg[i0 + k * TW_NT] = gv[k];
+    }
     __syncthreads();
     if (nearest) {
       for (int j = job.y + tid; j < job.z; j += TW_NT) {
@@ -426,17 +392,10 @@ __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
         const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
         const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
         float sr = 0.f, si = 0.f;
-        tap_range<SEPARABLE>(S, invS, 0, SS, lane, k2, ky, kx, [&](int c, float2 k) {
-          const float2 g = sg[base + c];
-          sr += g.x * k.x - g.y * k.y;
-          si += g.x * k.y + g.y * k.x;
-        });
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-          sr += __shfl_xor(sr, off, 64);
-          si += __shfl_xor(si, off, 64);
-        }
-        if (lane == 0) vis_out[r.w] = make_float2(sr, si);
+        gather_range<SEPARABLE>(sg, base, SG, S, 0, SS, lane, k2, ky, kx, sr, si);
+        sr = wave_sum_to_63(sr);
+        si = wave_sum_to_63(si);
+        if (lane == 63) store_stream(vis_out + r.w, make_float2(sr, si));
       }
     }
     __syncthreads();
