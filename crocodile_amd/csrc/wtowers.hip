@@ -226,6 +226,8 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
     int nearest, const TwScale *__restrict__ fs, float2 *__restrict__ sub) {
   constexpr int HALF = SG / 2, HC = HALF * SG;
   __shared__ unsigned long long acc[2 * HC];  // re, im interleaved (128 KiB)
+  __shared__ uint16_t s_list[PART_VIS];       // the pass's visibilities (offsets in the part)
+  __shared__ int s_n;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int NW = TW_NT / 64;
@@ -238,6 +240,7 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
     float2 *out = sub + (int64_t)__builtin_amdgcn_readfirstlane(job.x) * SGN;
     for (int r0 = 0; r0 < SG; r0 += HALF) {
       for (int c = tid; c < 2 * HC; c += TW_NT) acc[c] = 0ull;
+      if (tid == 0) s_n = 0;
       __syncthreads();
       if (nearest) {
         for (int j = jbeg + tid; j < jend; j += TW_NT) {
@@ -251,20 +254,46 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
           }
         }
       } else {
-        for (int j = jbeg + wave; j < jend; j += NW) {
-          const uint4 r = uniform_u4(rec[j]);
-          const int wy = (int)(r.x >> 16), wx = (int)(r.x & 0xffffu);
-          // window rows wy + i in the pass: i in [ilo, ihi)
-          const int ilo = max(0, r0 - wy), ihi = min(S, r0 + HALF - wy);
-          if (ilo >= ihi) continue;
-          const float2 v = uniform_f2(svis[j]);
-          const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
-          const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
-          const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
-          // cell of tap t: (wy - r0) * SG + wx + t + i * (SG - S), in [0, HC)
-          // for the pass's rows
-          spread_range<SEPARABLE>(acc, (wy - r0) * SG + wx, SG, S, ilo * S, ihi * S, lane, k2,
-                                  ky, kx, (double)v.x * scale, (double)v.y * scale);
+        // Walk the part in windows of PART_VIS records.  Each window first
+        // lists its visibilities whose kernel window meets the pass's rows
+        // (order is immaterial: integer sums), so that the walk pays its
+        // per-visibility set-up once per visibility and pass it touches.
+        for (int jb = jbeg; jb < jend; jb += PART_VIS) {
+          const int je = min(jend, jb + PART_VIS);
+          for (int j0 = jb; j0 < je; j0 += TW_NT) {
+            const int j = j0 + tid;
+            bool hit = false;
+            if (j < je) {
+              const int wy = (int)(rec[j].x >> 16);
+              hit = wy < r0 + HALF && wy + S > r0;
+            }
+            const unsigned long long bal = __ballot(hit);
+            int base = 0;
+            if (lane == 0 && bal) base = atomicAdd(&s_n, __popcll(bal));
+            base = __shfl(base, 0, 64);
+            if (hit)
+              s_list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)(j - jb);
+          }
+          barrier_after_lds_atomics();
+          const int nhit = s_n;
+          for (int k = wave; k < nhit; k += NW) {
+            const int j = jb + (int)s_list[k];
+            const uint4 r = uniform_u4(rec[j]);
+            const int wy = (int)(r.x >> 16), wx = (int)(r.x & 0xffffu);
+            // window rows wy + i in the pass: i in [ilo, ihi)
+            const int ilo = max(0, r0 - wy), ihi = min(S, r0 + HALF - wy);
+            const float2 v = uniform_f2(svis[j]);
+            const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+            const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
+            const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
+            // cell of tap t: (wy - r0) * SG + wx + t + i * (SG - S), in
+            // [0, HC) for the pass's rows
+            spread_range<SEPARABLE>(acc, (wy - r0) * SG + wx, SG, S, ilo * S, ihi * S, lane,
+                                    k2, ky, kx, (double)v.x * scale, (double)v.y * scale);
+          }
+          __syncthreads();  // the list is read; the next window rewrites it
+          if (tid == 0) s_n = 0;
+          __syncthreads();
         }
       }
       barrier_after_lds_atomics();
