@@ -111,30 +111,30 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int tile = __builtin_amdgcn_readfirstlane(item.x);
     const int tx0 = (tile % ntx) * T, ty0 = (tile / ntx) * T;
 
-    // Each wave walks its visibilities (j = first + wave, + NW, ...) with
-    // the next record and value already in flight.
+    // Each wave walks its visibilities j = first + wave, + NW, ..., records
+    // and values fetched 64 at a time (lane l: the l-th of the batch).
     const int jend = __builtin_amdgcn_readfirstlane(item.z);
-    int j = __builtin_amdgcn_readfirstlane(item.y) + wave;
-    uint4 rn = make_uint4(0, 0, 0, 0);
-    float2 vn = make_float2(0.f, 0.f);
-    if (j < jend) {
-      rn = rec[j];
-      vn = svis[j];
-    }
-    for (; j < jend; j += NW) {
-      const uint4 r = uniform_u4(rn);
-      const float2 v = uniform_f2(vn);
-      if (j + NW < jend) {
-        rn = rec[j + NW];
-        vn = svis[j + NW];
+    const int jfirst = __builtin_amdgcn_readfirstlane(item.y) + wave;
+    for (int jb = jfirst; jb < jend; jb += 64 * NW) {
+      const int jl = jb + lane * NW;
+      uint4 rb = make_uint4(0, 0, 0, 0);
+      float2 vb = make_float2(0.f, 0.f);
+      if (jl < jend) {
+        rb = rec[jl];
+        vb = svis[jl];
       }
-      const int S = (int)(r.y & 0xffu);
-      const int xf = (int)((r.y >> 8) & 0xfffu), yf = (int)(r.y >> 20);
-      const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
-      const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
-      const float2 *ky = kp + yf * S, *kx = kp + xf * S;
-      spread_range<SEPARABLE>(sacc, base, R, S, 0, S * S, lane, kp, ky, kx,
-                              (double)v.x * scale, (double)v.y * scale);
+      const int nb = min(64, (jend - jb + NW - 1) / NW);
+      for (int it = 0; it < nb; it++) {
+        const uint4 r = lane_u4(rb, it);
+        const float2 v = lane_f2(vb, it);
+        const int S = (int)(r.y & 0xffu);
+        const int xf = (int)((r.y >> 8) & 0xfffu), yf = (int)(r.y >> 20);
+        const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
+        const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
+        const float2 *ky = kp + yf * S, *kx = kp + xf * S;
+        spread_range<SEPARABLE>(sacc, base, R, S, 0, S * S, lane, kp, ky, kx,
+                                (double)v.x * scale, (double)v.y * scale);
+      }
     }
     barrier_after_lds_atomics();
     // Flush the region (touched cells only) and clear it for the next item.

@@ -150,3 +150,17 @@ __device__ __forceinline__ float wave_sum_to_63(float x) {
   x = dpp_add<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
   return x;
 }
+
+// Visibility records reach a wavefront 64 at a time: one vector load per
+// lane (lane l holds the wave's l-th next record), handed out with
+// v_readlane.  A scalar-cache load per visibility would instead make the
+// wave wait (lgkmcnt) for its own outstanding LDS operations before every
+// visibility.
+__device__ __forceinline__ uint4 lane_u4(uint4 x, int l) {
+  return make_uint4(__builtin_amdgcn_readlane(x.x, l), __builtin_amdgcn_readlane(x.y, l),
+                    __builtin_amdgcn_readlane(x.z, l), __builtin_amdgcn_readlane(x.w, l));
+}
+__device__ __forceinline__ float2 lane_f2(float2 x, int l) {
+  return make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.x), l)),
+                     __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.y), l)));
+}
