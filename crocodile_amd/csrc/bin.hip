@@ -19,7 +19,7 @@
 namespace {
 
 constexpr int BIN_NT = 1024;
-constexpr int BIN_PER = 4;
+constexpr int BIN_PER = 8;
 constexpr int BIN_VPB = BIN_NT * BIN_PER;
 constexpr int HIST_MAX = 32768;  // 128 KiB of LDS
 constexpr uint32_t INVALID = 0xffffffffu;
@@ -131,11 +131,30 @@ __global__ void k_item_counts(int ntiles, int ngroups, int nr, int chunk, int or
   scount[i] = (c + chunk - 1) / chunk;
 }
 
-// Writes every slot's items at its scanned offset soff[slot] (the per-XCD
-// list renumbered from 0), gstart for the first group of each range, the
-// item counts, and resets the queue heads.  An error status leaves both
-// lists empty, so that no output is modified.
-__global__ void k_item_write(int ntiles, int ngroups, int nr, int chunk,
+// Slot s's first record and end record.
+__device__ __forceinline__ void slot_range(int s, int ntiles, int ngroups, int nr,
+                                           const uint32_t *__restrict__ offset, int *t,
+                                           uint32_t *s0, uint32_t *s1) {
+  if (s < ntiles) {
+    *t = s;
+    *s0 = offset[s * ngroups];
+    *s1 = offset[(s + 1) * ngroups];
+  } else {
+    const int q = s - ntiles, r = q / ntiles;
+    *t = q - r * ntiles;
+    const int ga = (int)((int64_t)r * ngroups / nr), gb = (int)((int64_t)(r + 1) * ngroups / nr);
+    *s0 = offset[*t * ngroups + ga];
+    *s1 = offset[*t * ngroups + gb];
+  }
+}
+
+// One thread per item (items of all slots numbered by soff, the tile-major
+// list first): finds its slot by binary search over soff and writes {tile,
+// first, end}, the per-XCD list renumbered from 0.  Threads below nr + 1
+// also write gstart for the first group of each range and the item counts,
+// and the first XCD_QUEUES reset the queue heads.  An error status leaves
+// both lists empty, so that no output is modified.
+__global__ void k_item_write(int ntiles, int ngroups, int nr, int chunk, int max_total,
                              const uint32_t *__restrict__ offset,
                              const uint32_t *__restrict__ scount,
                              const uint32_t *__restrict__ soff, int4 *__restrict__ items,
@@ -146,37 +165,36 @@ __global__ void k_item_write(int ntiles, int ngroups, int nr, int chunk,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nslots = ntiles * (1 + nr);
   const bool failed = *status != 0;
-  if (i < XCD_QUEUES) qheads[i] = 0;
-  if (i >= nslots) return;
   const uint32_t base_g = soff[ntiles];  // items of the tile-major list
-  if (i == nslots - 1) {
-    const uint32_t total = soff[i] + scount[i];
-    *nitems = failed ? 0u : base_g;
-    *nitems_g = failed ? 0u : total - base_g;
-    gstart[ngroups] = failed ? 0u : total - base_g;
+  const uint32_t total = soff[nslots - 1] + scount[nslots - 1];
+  if (i < XCD_QUEUES) qheads[i] = 0;
+  if (i <= nr) {
+    if (i == nr) {
+      *nitems = failed ? 0u : base_g;
+      *nitems_g = failed ? 0u : total - base_g;
+      gstart[ngroups] = failed ? 0u : total - base_g;
+    } else {
+      gstart[(int64_t)i * ngroups / nr] = failed ? 0u : soff[ntiles + i * ntiles] - base_g;
+    }
+  }
+  if (failed || i >= max_total || (uint32_t)i >= total) return;
+  // slot s: soff[s] <= i < soff[s] + scount[s] (the last slot whose
+  // offset is <= i; empty slots share their successor's offset)
+  int lo = 0, hi = nslots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (soff[mid] <= (uint32_t)i) lo = mid; else hi = mid - 1;
   }
   int t;
-  uint32_t s0, s1, io;
-  int4 *dst;
-  if (i < ntiles) {
-    t = i;
-    s0 = offset[t * ngroups];
-    s1 = offset[(t + 1) * ngroups];
-    io = soff[i];
-    dst = items;
-  } else {
-    const int q = i - ntiles, r = q / ntiles;
-    t = q - r * ntiles;
-    const int ga = (int)((int64_t)r * ngroups / nr), gb = (int)((int64_t)(r + 1) * ngroups / nr);
-    s0 = offset[t * ngroups + ga];
-    s1 = offset[t * ngroups + gb];
-    io = soff[i] - base_g;
-    dst = items_g;
-    if (t == 0) gstart[ga] = failed ? 0u : io;
-  }
-  if (failed || scount[i] == 0) return;
-  for (uint32_t s = s0; s < s1; s += chunk)
-    dst[io++] = make_int4(t, (int)s, (int)min(s1, s + chunk), 0);
+  uint32_t s0, s1;
+  slot_range(lo, ntiles, ngroups, nr, offset, &t, &s0, &s1);
+  const uint32_t k = (uint32_t)i - soff[lo];
+  const uint32_t a = s0 + k * (uint32_t)chunk;
+  const int4 item = make_int4(t, (int)a, (int)min(s1, a + (uint32_t)chunk), 0);
+  if (lo < ntiles)
+    items[i] = item;
+  else
+    items_g[i - base_g] = item;
 }
 
 __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
@@ -340,9 +358,12 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
                                                   p.nbins, p.offset, p.cursor, scount);
   t_bytes = scan_bytes;
   CROC_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, t_bytes, scount, soff, nslots, s));
-  k_item_write<<<(nslots + 255) / 256, 256, 0, s>>>(
-      p.ntiles, p.ngroups, nr, p.chunk, p.offset, scount, soff, p.items, p.items_g, p.gstart,
-      p.nitems, p.nitems_g, p.status, p.qhead);
+  // at most max_items per list
+  const int max_total = (int)std::min<int64_t>(2 * p.max_items, INT32_MAX);
+  const int nthr = std::max(max_total, std::max(nr + 1, XCD_QUEUES));
+  k_item_write<<<(nthr + 255) / 256, 256, 0, s>>>(
+      p.ntiles, p.ngroups, nr, p.chunk, max_total, p.offset, scount, soff, p.items, p.items_g,
+      p.gstart, p.nitems, p.nitems_g, p.status, p.qhead);
   if (nblk > 0)
     k_bin_scatter<<<nblk, BIN_NT, 0, s>>>(a.nv, p.nbins, p.loc, p.cursor,
                                           p.rec, a.vis, p.svis);
