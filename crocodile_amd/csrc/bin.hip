@@ -199,8 +199,7 @@ __global__ void k_item_write(int ntiles, int ngroups, int nr, int chunk, int max
 
 __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
     int64_t nv, int nbins, const uint4 *__restrict__ loc,
-    uint32_t *__restrict__ cursor, uint4 *__restrict__ rec,
-    const float2 *__restrict__ vis, float2 *__restrict__ svis) {
+    uint32_t *__restrict__ cursor, uint4 *__restrict__ rec) {
   __shared__ uint32_t hist[HIST_MAX];
   const int tid = threadIdx.x;
   const int64_t base = (int64_t)blockIdx.x * BIN_VPB;
@@ -225,7 +224,6 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
       const int64_t i = base + k * BIN_NT + tid;
       const uint32_t pos = hist[L[k].w] + rank[k];
       rec[pos] = make_uint4(L[k].x, L[k].y, L[k].z, (uint32_t)i);
-      if (vis) svis[pos] = vis[i];
     }
   } else {
 #pragma unroll
@@ -236,7 +234,6 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
       if (l.w == INVALID) continue;
       const uint32_t pos = atomicAdd(&cursor[l.w], 1u);
       rec[pos] = make_uint4(l.x, l.y, l.z, (uint32_t)i);
-      if (vis) svis[pos] = vis[i];
     }
   }
 }
@@ -302,7 +299,6 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
 
   p.loc = (uint4 *)ws_get_in(ctx, set, WS_LOC, sizeof(uint4) * a.nv);
   p.rec = (uint4 *)ws_get_in(ctx, set, WS_REC, sizeof(uint4) * a.nv);
-  if (a.vis) p.svis = (float2 *)ws_get_in(ctx, set, WS_SVIS, sizeof(float2) * a.nv);
   // Ranges of the per-XCD list: one per group, at least one per queue.
   const int nr = p.ngroups;
   const int nslots = p.ntiles * (1 + nr);
@@ -326,7 +322,7 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
                   ? (int4 *)ws_get_in(ctx, set, WS_ITEMS_G, sizeof(int4) * p.max_items)
                   : nullptr;
   uint32_t *small = (uint32_t *)ws_get_in(ctx, set, WS_SMALL, 256);
-  if (!p.loc || !p.rec || (a.vis && !p.svis) || !tiles || !scan_tmp || !small ||
+  if (!p.loc || !p.rec || !tiles || !scan_tmp || !small ||
       (!p.items && (order & ITEMS_BY_TILE)) || (!p.items_g && (order & ITEMS_BY_GROUP)))
     return CROC_ENOMEM;
   p.count = tiles;
@@ -366,7 +362,7 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
       p.gstart, p.nitems, p.nitems_g, p.status, p.qhead);
   if (nblk > 0)
     k_bin_scatter<<<nblk, BIN_NT, 0, s>>>(a.nv, p.nbins, p.loc, p.cursor,
-                                          p.rec, a.vis, p.svis);
+                                          p.rec);
   CROC_HIP(hipGetLastError());
   return CROC_OK;
 }

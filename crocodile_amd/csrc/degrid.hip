@@ -143,7 +143,7 @@ extern "C" int croc_convdegrid(croc_ctx *ctx, const void *grid, int N,
   if (nv == 0) return CROC_OK;
 
   BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_off, separable,
-            plane_mode, vplane, wstep, wbin0, nullptr, max_support};
+            plane_mode, vplane, wstep, wbin0, max_support};
   BinPlan plan;
   rc = bin_visibilities(ctx, ctx->ws, a, max_support, ITEMS_BY_GROUP, &plan);
   if (rc) return rc;

@@ -88,7 +88,7 @@ __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
 template <int NT, bool SEPARABLE, int R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_grid_tiles(
     unsigned long long *__restrict__ acc, int N, int T, int ntx,
-    const uint4 *__restrict__ rec, const float2 *__restrict__ svis,
+    const uint4 *__restrict__ rec, const float2 *__restrict__ vis,
     const float2 *__restrict__ kern, const int4 *__restrict__ items,
     const uint32_t *__restrict__ nitems, uint32_t *__restrict__ qhead,
     const ScaleState *__restrict__ st) {
@@ -112,7 +112,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int tx0 = (tile % ntx) * T, ty0 = (tile / ntx) * T;
 
     // Each wave walks its visibilities j = first + wave, + NW, ..., records
-    // and values fetched 64 at a time (lane l: the l-th of the batch).
+    // and values fetched 64 at a time (lane l: the l-th of the batch; the
+    // value is gathered from the caller's array by the record's index).
     const int jend = __builtin_amdgcn_readfirstlane(item.z);
     const int jfirst = __builtin_amdgcn_readfirstlane(item.y) + wave;
     for (int jb = jfirst; jb < jend; jb += 64 * NW) {
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
       float2 vb = make_float2(0.f, 0.f);
       if (jl < jend) {
         rb = rec[jl];
-        vb = svis[jl];
+        vb = vis[rb.w];
       }
       const int nb = min(64, (jend - jb + NW - 1) / NW);
       for (int it = 0; it < nb; it++) {
@@ -174,9 +175,9 @@ __global__ void k_grid_finish(float2 *__restrict__ grid,
 
 }  // namespace
 
-// Gridding of binned visibilities (svis in record order) into grid; vis is
-// the caller's array, read for the fixed-point scale only.  No status read.
-int grid_binned(croc_ctx *ctx, const BinPlan &plan, const float2 *svis, void *grid, int N,
+// Gridding of binned visibilities into grid (record w: index into vis).  No
+// status read.
+int grid_binned(croc_ctx *ctx, const BinPlan &plan, void *grid, int N,
                 int64_t nv, const void *vis, const void *kern, int Q, int separable,
                 int nplanes, const int64_t *plane_off, const int32_t *plane_s) {
   int rc;
@@ -201,18 +202,18 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, const float2 *svis, void *gr
     const int G = persistent_blocks(ctx, 2);  // 63.5 KiB of LDS each
     if (separable)
       k_grid_tiles<GRID_NT, true, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, svis, kt, plan.items, plan.nitems, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.qhead, st);
     else
       k_grid_tiles<GRID_NT, false, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, svis, kt, plan.items, plan.nitems, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.qhead, st);
   } else {
     const int G = persistent_blocks(ctx, 1);
     if (separable)
       k_grid_tiles<GRID_NT, true, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, svis, kt, plan.items, plan.nitems, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.qhead, st);
     else
       k_grid_tiles<GRID_NT, false, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, svis, kt, plan.items, plan.nitems, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.qhead, st);
   }
   CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -244,11 +245,11 @@ extern "C" int croc_convgrid(croc_ctx *ctx, void *grid, int N,
   if (nv == 0) return CROC_OK;
 
   BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_off, separable,
-            plane_mode, vplane, wstep, wbin0, (const float2 *)vis, max_support};
+            plane_mode, vplane, wstep, wbin0, max_support};
   BinPlan plan;
   rc = bin_visibilities(ctx, ctx->ws, a, max_support, ITEMS_BY_TILE, &plan);
   if (rc) return rc;
-  rc = grid_binned(ctx, plan, plan.svis, grid, N, nv, vis, kern, Q, separable, nplanes,
+  rc = grid_binned(ctx, plan, grid, N, nv, vis, kern, Q, separable, nplanes,
                    plane_off, plane_s);
   if (rc) return rc;
   rc = read_status(ctx, plan);

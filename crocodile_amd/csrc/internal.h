@@ -160,7 +160,6 @@ struct BinArgs {
   const int32_t *vplane;
   double wstep;
   int wbin0;
-  const float2 *vis;  // non-null: gather visibilities into tile order
   int max_support;    // every plane_s must lie in [1, max_support]
 };
 
@@ -179,7 +178,6 @@ struct BinPlan {
   int64_t group_elems = 0;
   int64_t max_items = 0;
   uint4 *loc = nullptr, *rec = nullptr;
-  float2 *svis = nullptr;
   uint32_t *count = nullptr, *offset = nullptr, *cursor = nullptr;
   int4 *items = nullptr;    // tile-major (gridding)
   int4 *items_g = nullptr;  // per-XCD queues (degridding)
@@ -314,8 +312,8 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 int choose_tile(croc_ctx *ctx, int max_support);
 int choose_region(int T, int max_support);
 // Tile-kernel launches on binned visibilities (grid.hip, degrid.hip); they
-// read no status.  svis: the visibilities in record order.
-int grid_binned(croc_ctx *ctx, const BinPlan &plan, const float2 *svis, void *grid, int N,
+// read no status.  Record w holds the visibility's index into vis.
+int grid_binned(croc_ctx *ctx, const BinPlan &plan, void *grid, int N,
                 int64_t nv, const void *vis, const void *kern, int Q, int separable,
                 int nplanes, const int64_t *plane_off, const int32_t *plane_s);
 int degrid_binned(croc_ctx *ctx, const BinPlan &plan, const void *grid, int N, void *vis_out,

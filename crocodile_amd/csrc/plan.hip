@@ -9,7 +9,7 @@
 
 struct croc_plan {
   int device = 0;
-  WsSet ws;           // the plan's own buffers (records, items, gathered values)
+  WsSet ws;           // the plan's own buffers (records, work items, scan space)
   BinPlan bp;
   bool ready = false;
   int N = 0, Q = 0, separable = 0, nplanes = 0;
@@ -19,13 +19,6 @@ struct croc_plan {
 };
 
 namespace {
-
-// svis[pos] = vis[visibility index of record pos]
-__global__ void k_gather_vis(int64_t n, const uint4 *__restrict__ rec,
-                             const float2 *__restrict__ vis, float2 *__restrict__ svis) {
-  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos < n) svis[pos] = vis[rec[pos].w];
-}
 
 int plan_check(croc_ctx *ctx, const croc_plan *plan) {
   int rc = ctx_enter(ctx);
@@ -85,7 +78,7 @@ extern "C" int croc_plan_bin(croc_ctx *ctx, croc_plan *plan, int N, const double
   plan->plane_s = plane_s;
   if (nv > 0) {
     BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_off, separable,
-              plane_mode, vplane, wstep, wbin0, nullptr, max_support};
+              plane_mode, vplane, wstep, wbin0, max_support};
     rc = bin_visibilities(ctx, plan->ws, a, max_support, ITEMS_BY_TILE | ITEMS_BY_GROUP,
                          &plan->bp);
     if (rc) return rc;
@@ -103,14 +96,9 @@ extern "C" int croc_convgrid_plan(croc_ctx *ctx, croc_plan *plan, void *grid,
   CROC_REQUIRE(grid && kern, CROC_EINVAL, "null array argument");
   if (plan->nv == 0) return CROC_OK;
   CROC_REQUIRE(vis != nullptr, CROC_EINVAL, "null vis");
-  float2 *svis = (float2 *)ws_get_in(ctx, plan->ws, WS_SVIS, sizeof(float2) * plan->nv);
-  if (!svis) return CROC_ENOMEM;
-  k_gather_vis<<<(unsigned)((plan->nv + 255) / 256), 256, 0, ctx->stream>>>(
-      plan->nv, plan->bp.rec, (const float2 *)vis, svis);
-  CROC_HIP(hipGetLastError());
   rc = zero_u32_async(ctx, plan->bp.qhead, XCD_QUEUES);  // work queue restarts
   if (rc) return rc;
-  return grid_binned(ctx, plan->bp, svis, grid, plan->N, plan->nv, vis, kern, plan->Q,
+  return grid_binned(ctx, plan->bp, grid, plan->N, plan->nv, vis, kern, plan->Q,
                      plan->separable, plan->nplanes, plan->plane_off, plan->plane_s);
 }
 
