@@ -74,3 +74,21 @@ def test_wtowers_clips_last_chunk_like_oracle(gpu):
     g = G.w_towers_imaging(THETA, LAM, uvw, None, vis, 30.0, SG, MARGIN, None)
     ref = WT.w_towers_imaging(THETA, LAM, uvw, None, vis, 30.0, SG, MARGIN, None)
     assert_close(g, ref, what='clipped chunk')
+
+
+def test_wtowers_job_longer_than_a_part_window(gpu):
+    """One (chunk, w-plane) job of 2.2e6 visibilities.  Gridding cuts a job
+    into at most 512 parts, so each part here holds ~4300 visibilities, more
+    than the 4096-record window the job kernel lists per pass: the kernel
+    must walk a part in several windows.  With every w = 0 the towers
+    reproduce conv_imaging (the oracle equality of the notebook), here
+    against the device conv_imaging, whose parity is tested separately."""
+    rng = numpy.random.default_rng(7)
+    nv = 2_200_000
+    uvw = numpy.stack([rng.uniform(-300, 300, nv), rng.uniform(-300, 300, nv),
+                       numpy.zeros(nv)], 1)   # cells 226..286: one 96-cell chunk
+    vis = rng.standard_normal(nv) + 1j * rng.standard_normal(nv)
+    kv = kernels(rng)['separable']
+    g = G.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, kv)
+    ref = G.conv_imaging(THETA, LAM, uvw, None, vis, kv)
+    assert_close(g, ref, what='w-towers long job vs conv_imaging')
