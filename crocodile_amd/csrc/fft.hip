@@ -22,7 +22,10 @@ namespace {
 
 constexpr int FN = 128;
 constexpr int FP = FN + 1;  // padded LDS row
-constexpr int FFT_NT = 512;
+constexpr int FFT_NT = 1024;
+constexpr int PASS_A = FN * 8 / FFT_NT;   // pass-A items (line, j2) per thread
+constexpr int PASS_B = FN * 16 / FFT_NT;  // pass-B items (line, k1) per thread
+static_assert(PASS_A * FFT_NT == FN * 8 && PASS_B * FFT_NT == FN * 16, "item split");
 
 struct FModArgs {
   const double *phi;
@@ -85,9 +88,9 @@ template <bool LINE_FAST>
 __device__ __forceinline__ void fft128_lines(float2 *lds, int line_stride, int es,
                                              const float2 *tw, float sgn) {
   const int tid = threadIdx.x;
-  // pass A: 128 lines x 8 items, two per thread
+  // pass A: 128 lines x 8 items
 #pragma unroll
-  for (int r = 0; r < 2; r++) {
+  for (int r = 0; r < PASS_A; r++) {
     const int it = tid + r * FFT_NT;
     const int line = LINE_FAST ? (it & 127) : (it >> 3);
     const int j2 = LINE_FAST ? (it >> 7) : (it & 7);
@@ -103,11 +106,11 @@ __device__ __forceinline__ void fft128_lines(float2 *lds, int line_stride, int e
     }
   }
   __syncthreads();
-  // pass B: 128 lines x 16 items, four per thread; reads then writes
-  float2 v[4][8];
-  int base[4], k1s[4];
+  // pass B: 128 lines x 16 items; reads then writes
+  float2 v[PASS_B][8];
+  int base[PASS_B], k1s[PASS_B];
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
+  for (int r = 0; r < PASS_B; r++) {
     const int it = tid + r * FFT_NT;
     const int line = LINE_FAST ? (it & 127) : (it >> 4);
     const int k1 = LINE_FAST ? (it >> 7) : (it & 15);
@@ -119,7 +122,7 @@ __device__ __forceinline__ void fft128_lines(float2 *lds, int line_stride, int e
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int r = 0; r < PASS_B; r++)
 #pragma unroll
     for (int k2 = 0; k2 < 8; k2++) lds[base[r] + (k1s[r] + 16 * k2) * es] = v[r][k2];
   __syncthreads();
