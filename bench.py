@@ -206,6 +206,14 @@ def run_towers(args, torch, G, W, _lib, dev):
         _, d_ms = _lib.last_kernel_ms()
         return g_ms, d_ms
 
+    def pervis_step():
+        guv.zero_()
+        G.grid_device_pervis(bank, guv, uvw_d, lam, vis_d, _lib.PLANE_WBIN, None, wstep, lo)
+        g_ms, _ = _lib.last_kernel_ms()
+        G.degrid_device_pervis(bank, guv, uvw_d, lam, out, _lib.PLANE_WBIN, None, wstep, lo)
+        _, d_ms = _lib.last_kernel_ms()
+        return g_ms, d_ms
+
     def timed(step, label):
         for _ in range(args.warmup):
             step()
@@ -223,6 +231,7 @@ def run_towers(args, torch, G, W, _lib, dev):
 
     tv, tms, tg, td = timed(towers_step, 'w-towers')
     pv, pms, pg, pd = timed(plain_step, 'plain')
+    qv, qms, qg, qd = timed(pervis_step, 'per-visibility')
     # algorithmic work: taps (8 flop each way) and sub-grid DFTs (16 n^3 flop
     # per 128^2 transform, three fp16 products each on the matrix cores)
     taps = float(nv) * S * S
@@ -248,7 +257,13 @@ def run_towers(args, torch, G, W, _lib, dev):
                   'grid_ms': round(pg, 3), 'degrid_ms': round(pd, 3),
                   'kernel_bank_mib': round(8 * bank.kern.numel() / 2 ** 20, 1),
                   'kernel_gen_ms': round(kgen_ms, 1)},
+        'per_visibility': {'value': round(qv, 1), 'ms_per_step': round(qms, 3),
+                           'grid_ms': round(qg, 3), 'degrid_ms': round(qd, 3),
+                           'note': 'croc_convgrid_pervis + croc_convdegrid_pervis: one '
+                                   'wavefront per visibility, FP32 atomics, no binning, '
+                                   'same kernel bank as plain'},
         'speedup_vs_plain': round(tv / pv, 3),
+        'speedup_vs_per_visibility': round(tv / qv, 3),
     }
     print(json.dumps(line), flush=True)
 

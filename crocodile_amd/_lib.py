@@ -37,6 +37,10 @@ SIGNATURES = {
                       _I, _I, _P, _D, _I],
     'croc_convdegrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _P, _I, _I, _I, _P, _P,
                         _I, _I, _P, _D, _I],
+    'croc_convgrid_pervis': [_P, _P, _I, _P, _I64, _I64, _D, _P, _P, _I, _I, _I, _P, _P,
+                        _I, _I, _P, _D, _I],
+    'croc_convdegrid_pervis': [_P, _P, _I, _P, _I64, _I64, _D, _P, _P, _I, _I, _I, _P, _P,
+                        _I, _I, _P, _D, _I],
     'croc_plan_create': [_P, ctypes.POINTER(_P)],
     'croc_plan_destroy': [_P, _P],
     'croc_plan_bin': [_P, _P, _I, _P, _I64, _I64, _D, _I, _I, _I, _P, _P, _I, _I, _P, _D, _I],

@@ -12,7 +12,7 @@
 // range of plane groups (degridding).  Block-local LDS histograms keep
 // global atomics to one per (block, bin) pair, which matters
 // for the strongly centre-peaked uv densities of real arrays.
-#include "internal.h"
+#include "locate.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -22,60 +22,6 @@ constexpr int BIN_NT = 1024;
 constexpr int BIN_PER = 8;
 constexpr int BIN_VPB = BIN_NT * BIN_PER;
 constexpr int HIST_MAX = 32768;  // 128 KiB of LDS
-constexpr uint32_t INVALID = 0xffffffffu;
-
-struct BinGeom {
-  int T, ntx, ngroups;
-  int64_t group_elems;
-};
-
-__device__ __forceinline__ uint4 locate(const BinArgs &a, const BinGeom &g,
-                                        int64_t i, uint32_t *st) {
-  const double *pu = a.uvw + i * a.stride;
-  const double px = grid_fraction(pu[0], a.lam);
-  const double py = grid_fraction(pu[1], a.lam);
-  int ix, xf, iy, yf;
-  const bool okx = frac_coord_dev(a.N, a.Q, px, &ix, &xf);
-  const bool oky = frac_coord_dev(a.N, a.Q, py, &iy, &yf);
-  if (!(okx && oky)) {
-    *st |= ST_DOMAIN;
-    return make_uint4(0, 0, 0, INVALID);
-  }
-  int64_t plane = 0;
-  if (a.plane_mode == CROC_PLANE_INDEX) {
-    plane = a.vplane[i];
-  } else if (a.plane_mode == CROC_PLANE_WBIN) {
-    plane = wbin_dev(pu[2], a.wstep) - a.wbin0;
-  }
-  if (plane < 0 || plane >= a.nplanes) {
-    *st |= ST_PLANE;
-    return make_uint4(0, 0, 0, INVALID);
-  }
-  const int S = a.plane_s[plane];
-  if (S < 1 || S > a.max_support) {  // LDS regions are sized for max_support
-    *st |= ST_SUPPORT;
-    return make_uint4(0, 0, 0, INVALID);
-  }
-  const int x0 = ix - S / 2, y0 = iy - S / 2;
-  if (x0 < 0 || y0 < 0 || x0 + S > a.N || y0 + S > a.N) {
-    *st |= ST_BOUNDS;
-    return make_uint4(0, 0, 0, INVALID);
-  }
-  const int64_t poff = a.plane_off[plane];
-  const int64_t toff =
-      a.separable ? poff : poff + ((int64_t)yf * a.Q + xf) * S * S;
-  const int64_t tend = toff + (a.separable ? (int64_t)a.Q * S : (int64_t)S * S);
-  if (poff < 0 || tend > (int64_t)UINT32_MAX) {
-    *st |= ST_BANK;
-    return make_uint4(0, 0, 0, INVALID);
-  }
-  const int grp = (int)min((int64_t)g.ngroups - 1, toff / g.group_elems);
-  const uint32_t tile = (uint32_t)((y0 / g.T) * g.ntx + x0 / g.T);
-  return make_uint4((uint32_t)x0 | ((uint32_t)y0 << 16),
-                    (uint32_t)S | ((uint32_t)xf << 8) | ((uint32_t)yf << 20),
-                    (uint32_t)toff, tile * (uint32_t)g.ngroups + (uint32_t)grp);
-}
-
 __global__ __launch_bounds__(BIN_NT) void k_bin_count(
     BinArgs a, BinGeom g, int nbins, uint4 *__restrict__ loc,
     uint32_t *__restrict__ count, uint32_t *__restrict__ status) {
@@ -93,7 +39,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_count(
     if (i < a.nv) {
       const uint4 L = locate(a, g, i, &st);
       loc[i] = L;
-      if (L.w != INVALID) {
+      if (L.w != LOC_INVALID) {
         if (lds)
           atomicAdd(&hist[L.w], 1u);
         else
@@ -211,8 +157,8 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
 #pragma unroll
     for (int k = 0; k < BIN_PER; k++) {
       const int64_t i = base + k * BIN_NT + tid;
-      L[k] = (i < nv) ? loc[i] : make_uint4(0, 0, 0, INVALID);
-      rank[k] = (L[k].w != INVALID) ? atomicAdd(&hist[L[k].w], 1u) : 0u;
+      L[k] = (i < nv) ? loc[i] : make_uint4(0, 0, 0, LOC_INVALID);
+      rank[k] = (L[k].w != LOC_INVALID) ? atomicAdd(&hist[L[k].w], 1u) : 0u;
     }
     __syncthreads();
     for (int t = tid; t < nbins; t += BIN_NT)
@@ -220,7 +166,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < BIN_PER; k++) {
-      if (L[k].w == INVALID) continue;
+      if (L[k].w == LOC_INVALID) continue;
       const int64_t i = base + k * BIN_NT + tid;
       const uint32_t pos = hist[L[k].w] + rank[k];
       rec[pos] = make_uint4(L[k].x, L[k].y, L[k].z, (uint32_t)i);
@@ -231,7 +177,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_scatter(
       const int64_t i = base + k * BIN_NT + tid;
       if (i >= nv) continue;
       const uint4 l = loc[i];
-      if (l.w == INVALID) continue;
+      if (l.w == LOC_INVALID) continue;
       const uint32_t pos = atomicAdd(&cursor[l.w], 1u);
       rec[pos] = make_uint4(l.x, l.y, l.z, (uint32_t)i);
     }

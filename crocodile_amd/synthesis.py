@@ -466,6 +466,23 @@ def degrid_device(bank, guv, uvw, lam, out, mode=_lib.PLANE_SINGLE, vplane=None,
                mode, vplane, wstep, wbin0)
 
 
+def grid_device_pervis(bank, guv, uvw, lam, vis, mode=_lib.PLANE_SINGLE, vplane=None,
+                       wstep=1.0, wbin0=0):
+    """grid_device through the plain per-visibility kernels
+    (croc_convgrid_pervis: FP32 atomics into the grid, no binning), the
+    baseline of the w-towers comparison."""
+    _grid_call(_lib.load().croc_convgrid_pervis, guv, guv.shape[0], uvw, lam, vis, bank,
+               mode, vplane, wstep, wbin0)
+
+
+def degrid_device_pervis(bank, guv, uvw, lam, out, mode=_lib.PLANE_SINGLE, vplane=None,
+                         wstep=1.0, wbin0=0):
+    """degrid_device through the plain per-visibility kernels
+    (croc_convdegrid_pervis)."""
+    _grid_call(_lib.load().croc_convdegrid_pervis, guv, guv.shape[0], uvw, lam, out, bank,
+               mode, vplane, wstep, wbin0)
+
+
 class GridPlan:
     """Binning of one set of coordinates, reused by repeated gridding and
     degridding of the same visibilities with one kernel bank layout (the

@@ -104,6 +104,25 @@ int croc_convdegrid(croc_ctx *ctx, const void *grid, int N, const double *uvw,
                     int max_support, int plane_mode, const int32_t *vplane,
                     double wstep, int wbin0);
 
+/* The plain per-visibility GPU form of convgrid / convdegrid
+ * (crocodile/synthesis.py:464-503, same arguments and errors as
+ * croc_convgrid / croc_convdegrid): one wavefront per visibility, FP32
+ * atomic adds straight into the grid, no binning and no LDS.  It is the
+ * baseline the w-towers pipeline is compared with; its gridding rounding
+ * depends on the order of the atomics. */
+int croc_convgrid_pervis(croc_ctx *ctx, void *grid, int N, const double *uvw,
+                         int64_t nv, int64_t stride, double lam, const void *vis,
+                         const void *kern, int Q, int separable, int nplanes,
+                         const int64_t *plane_off, const int32_t *plane_s,
+                         int max_support, int plane_mode, const int32_t *vplane,
+                         double wstep, int wbin0);
+int croc_convdegrid_pervis(croc_ctx *ctx, const void *grid, int N, const double *uvw,
+                           int64_t nv, int64_t stride, double lam, void *vis_out,
+                           const void *kern, int Q, int separable, int nplanes,
+                           const int64_t *plane_off, const int32_t *plane_s,
+                           int max_support, int plane_mode, const int32_t *vplane,
+                           double wstep, int wbin0);
+
 /* Plans (no reference counterpart: crocodile bins nothing, each convgrid
  * call walks its visibilities one by one).  A plan holds the binning of one
  * set of coordinates for one kernel-bank layout, for gridding and

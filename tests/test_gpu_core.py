@@ -459,3 +459,35 @@ def test_plan_errors(gpu):
     other = G.KernelBank.from_tables([numpy.zeros((4, 4, 6, 6), complex)])
     with pytest.raises(ValueError):
         plan.grid(g, vis, other)
+
+
+@pytest.mark.parametrize('N,S,Q,nv,planes,separable', [CASES[2], CASES[5], CASES[7]])
+def test_pervis_kernels_parity(gpu, N, S, Q, nv, planes, separable):
+    """The plain per-visibility kernels (croc_convgrid_pervis /
+    croc_convdegrid_pervis, the w-towers comparison baseline) against the
+    FP64 oracle, with the same kernel selection as the binned path."""
+    rng = numpy.random.default_rng(N * 11 + S)
+    tables, p, v = random_case(rng, N, S, Q, nv, planes, separable)
+    vplane = rng.integers(0, planes, nv).astype(numpy.int32)
+    bank_ref = cgrid.Bank(tables, separable)
+    ref = numpy.zeros((N, N), dtype=complex)
+    cgrid.convgrid(bank_ref, ref, p, v, vplane)
+    bank = G.KernelBank.from_tables(tables, separable)
+    uvw = torch.as_tensor(p, device=gpu).contiguous()
+    vp = torch.as_tensor(vplane, device=gpu)
+    vis = torch.as_tensor(v, device=gpu).to(torch.complex64)
+    a = torch.zeros((N, N), dtype=torch.complex64, device=gpu)
+    G.grid_device_pervis(bank, a, uvw, 1.0, vis, _lib.PLANE_INDEX, vp)
+    assert_close(a.cpu().numpy(), ref, what='pervis grid')
+    g = torch.as_tensor(ref, device=gpu).to(torch.complex64)
+    out = torch.empty(nv, dtype=torch.complex64, device=gpu)
+    G.degrid_device_pervis(bank, g, uvw, 1.0, out, _lib.PLANE_INDEX, vp)
+    assert_close(out.cpu().numpy(), cgrid.convdegrid(bank_ref, ref, p, vplane),
+                 what='pervis degrid')
+    # errors as convgrid: a window outside the grid leaves the grid unmodified
+    bad = uvw.clone()
+    bad[3, 0] = 0.5 - 1.0 / N
+    before = a.clone()
+    with pytest.raises(ValueError):
+        G.grid_device_pervis(bank, a, bad, 1.0, vis, _lib.PLANE_INDEX, vp)
+    assert torch.equal(a, before)
