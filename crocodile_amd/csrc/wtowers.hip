@@ -223,7 +223,8 @@ template <bool SEPARABLE>
 __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
     const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
     const float2 *__restrict__ svis, const float2 *__restrict__ kern, int Q, int S,
-    int nearest, const TwScale *__restrict__ fs, float2 *__restrict__ sub) {
+    int nearest, const TwScale *__restrict__ fs, float2 *__restrict__ sub,
+    uint32_t *__restrict__ qc) {
   constexpr int HALF = SG / 2, HC = HALF * SG;
   __shared__ unsigned long long acc[2 * HC];  // re, im interleaved (128 KiB)
   __shared__ uint16_t s_list[PART_VIS];       // the pass's visibilities (offsets in the part)
@@ -233,7 +234,12 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
   constexpr int NW = TW_NT / 64;
   const double scale = fs->scale, inv = fs->inv_scale;
   const int SS = S * S;
-  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+  __shared__ int s_b;
+  for (;;) {  // parts taken from the launch's queue counter qc
+    if (tid == 0) s_b = (int)atomicAdd(qc, 1u);
+    __syncthreads();
+    const int b = s_b;
+    if (b >= nb) break;
     const int4 job = parts[b];
     const int jbeg = __builtin_amdgcn_readfirstlane(job.y);
     const int jend = __builtin_amdgcn_readfirstlane(job.z);
@@ -387,13 +393,18 @@ template <bool SEPARABLE>
 __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
     const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
     const float2 *__restrict__ sub, const float2 *__restrict__ kern, int Q, int S, int nearest,
-    float2 *__restrict__ vis_out) {
+    float2 *__restrict__ vis_out, uint32_t *__restrict__ qc) {
   __shared__ float2 sg[SGN];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int NW = TW_NT / 64;
   const int SS = S * S;
-  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+  __shared__ int s_b;
+  for (;;) {  // parts taken from the launch's queue counter qc
+    if (tid == 0) s_b = (int)atomicAdd(qc, 1u);
+    __syncthreads();
+    const int b = s_b;
+    if (b >= nb) break;
     const int4 job = parts[b];
     const float2 *src = sub + (int64_t)job.x * SGN;
     // stage the sub-grid, eight streaming loads in flight per thread
@@ -644,7 +655,8 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
   // fixed-point scale, Fresnel phase table, per-job plane multipliers
   const int64_t nk = kern ? (separable ? (int64_t)Q * S : (int64_t)Q * Q * S * S) : 0;
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
-                                                sizeof(int32_t) * (5 * (size_t)p.njobs + 64));
+                                                sizeof(int32_t) * (5 * (size_t)p.njobs + 64) +
+                                                sizeof(uint32_t) * (batches.size() + 1));
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
   float2 *stack =
       (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)max_slots);
@@ -656,6 +668,7 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
   int32_t *kw = (int32_t *)(phi + SGN);
   int32_t *dchunks = kw + p.njobs;
   int32_t *djslot = dchunks + p.njobs;
+  uint32_t *qc = (uint32_t *)(kw + 5 * (size_t)p.njobs + 64);  // a part queue per batch
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
   int4 *dparts = (int4 *)plan;
   int4 *dgroups = dparts + parts.size();
@@ -682,17 +695,19 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
   k_tw_zero_chunks<<<dim3(16, (unsigned)p.chunks.size()), 256, 0, s>>>(dchunks, cimg);
   CROC_HIP(hipGetLastError());
 
+  if ((rc = zero_u32_async(ctx, qc, (int64_t)batches.size()))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[0], s));
-  for (const Batch &bt : batches) {
-    const int G = (int)std::min<size_t>(bt.np, 2 * (size_t)ctx->num_cu);
+  for (size_t bi = 0; bi < batches.size(); bi++) {
+    const Batch &bt = batches[bi];
+    const int G = (int)std::min<size_t>(bt.np, (size_t)ctx->num_cu);  // one per CU (LDS)
     if (a.nearest || !separable)
       k_tw_grid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
                                                 (const float2 *)kern, a.Q, a.S, a.nearest, fs,
-                                                sub);
+                                                sub, qc + bi);
     else
       k_tw_grid_jobs<true><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
                                                (const float2 *)kern, a.Q, a.S, a.nearest, fs,
-                                               sub);
+                                               sub, qc + bi);
     CROC_HIP(hipGetLastError());
     if (bt.nm) {
       k_tw_sum_parts<<<dim3(ACC_SLICES, (unsigned)bt.nm), 256, 0, s>>>(dmulti + bt.m0, sub);
@@ -762,7 +777,8 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   }
   poff.push_back(parts.size());
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
-                                                sizeof(int32_t) * (3 * (size_t)p.njobs + 64));
+                                                sizeof(int32_t) * (3 * (size_t)p.njobs + 64) +
+                                                sizeof(uint32_t) * poff.size());
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
   float2 *stack = (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)jb);
   int4 *dparts = (int4 *)ws_get(ctx, WS_TILES, sizeof(int4) * (parts.size() + 16));
@@ -771,6 +787,7 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   int32_t *kw = (int32_t *)(phi + SGN);
   int32_t *jchunk = kw + p.njobs;
   int32_t *dchunks = jchunk + p.njobs;
+  uint32_t *qc = (uint32_t *)(kw + 3 * (size_t)p.njobs + 64);  // a part queue per batch
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
   std::vector<int32_t> hk(p.njobs), hc(p.njobs);
   for (int j = 0; j < p.njobs; j++) {
@@ -783,6 +800,7 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   CROC_HIP(hipMemcpyAsync(dparts, parts.data(), sizeof(int4) * parts.size(),
                           hipMemcpyHostToDevice, s));
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
+  if ((rc = zero_u32_async(ctx, qc, (int64_t)poff.size()))) return rc;
   dim3 g(16, (unsigned)p.chunks.size());
   CROC_HIP(hipEventRecord(ctx->ev[2], s));
   k_tw_cut<<<g, 256, 0, s>>>(dchunks, (const float2 *)grid, cimg, a.N, a.chunk, a.margin, a.ncx);
@@ -797,15 +815,15 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
                       0, jchunk + j0, nullptr);
     if (rc) return rc;
     const int npart = (int)(poff[bi + 1] - poff[bi]);
-    const int G = std::min(npart, 2 * ctx->num_cu);
+    const int G = std::min(npart, ctx->num_cu);  // one per CU (LDS)
     if (a.nearest || !separable)
       k_tw_degrid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
                                                   (const float2 *)kern, a.Q, a.S, a.nearest,
-                                                  (float2 *)vis_out);
+                                                  (float2 *)vis_out, qc + bi);
     else
       k_tw_degrid_jobs<true><<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
                                                  (const float2 *)kern, a.Q, a.S, a.nearest,
-                                                 (float2 *)vis_out);
+                                                 (float2 *)vis_out, qc + bi);
     CROC_HIP(hipGetLastError());
   }
   CROC_HIP(hipEventRecord(ctx->ev[3], s));
