@@ -215,10 +215,11 @@ __global__ void k_tw_pick(TwScale *fs, int64_t nv, int nearest, int separable) {
 }
 
 // Grid job parts into sub-grid slots: parts[b] = (slot, first record, end
-// record); one workgroup per part, rows [0, sg/2) then [sg/2, sg): each pass
-// zeroes its int64 region, each wavefront adds the taps of one visibility
-// at a time that lie in the pass's rows (a contiguous tap range), and the
-// pass ends by converting the region to complex64 in sub[slot].
+// record); one workgroup per part, rows [0, sg/2) then [sg/2, sg): each
+// wavefront adds the taps of one visibility at a time that lie in the pass's
+// rows (a contiguous tap range) to the int64 region, and the pass ends by
+// converting the region to complex64 in sub[slot] and zeroing it for the
+// next pass (it is zeroed once at the start of the launch).
 template <bool SEPARABLE>
 __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
     const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
@@ -235,6 +236,8 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
   const double scale = fs->scale, inv = fs->inv_scale;
   const int SS = S * S;
   __shared__ int s_b;
+  // The region starts zeroed; each pass's read-out zeroes it again.
+  for (int c = tid; c < 2 * HC; c += TW_NT) acc[c] = 0ull;
   for (;;) {  // parts taken from the launch's queue counter qc
     if (tid == 0) s_b = (int)atomicAdd(qc, 1u);
     __syncthreads();
@@ -245,7 +248,6 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
     const int jend = __builtin_amdgcn_readfirstlane(job.z);
     float2 *out = sub + (int64_t)__builtin_amdgcn_readfirstlane(job.x) * SGN;
     for (int r0 = 0; r0 < SG; r0 += HALF) {
-      for (int c = tid; c < 2 * HC; c += TW_NT) acc[c] = 0ull;
       if (tid == 0) s_n = 0;
       __syncthreads();
       if (nearest) {
@@ -303,9 +305,12 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
         }
       }
       barrier_after_lds_atomics();
-      for (int c = tid; c < HC; c += TW_NT)
-        out[r0 * SG + c] = make_float2((float)((double)(long long)acc[2 * c] * inv),
-                                       (float)((double)(long long)acc[2 * c + 1] * inv));
+      for (int c = tid; c < HC; c += TW_NT) {
+        const long long re = (long long)acc[2 * c], im = (long long)acc[2 * c + 1];
+        acc[2 * c] = 0ull;
+        acc[2 * c + 1] = 0ull;
+        out[r0 * SG + c] = make_float2((float)((double)re * inv), (float)((double)im * inv));
+      }
       __syncthreads();
     }
   }
