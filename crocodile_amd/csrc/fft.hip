@@ -33,9 +33,10 @@ struct FModArgs {
   int sign;
 };
 
-__device__ __forceinline__ float2 fmod_phase(float2 v, const FModArgs &md, int b, int pix) {
-  if (!md.phi) return v;
-  const double ang = (double)md.kplane[b] * md.phi[pix] * (double)md.sign;
+// v * exp(i kf phi), kf = s k (the plane multiplier times the sign), the
+// angle reduced to [-pi, pi] in FP64 before the FP32 sincos.
+__device__ __forceinline__ float2 rotate(float2 v, double kf, double phi) {
+  const double ang = kf * phi;
   const double red = ang - 6.283185307179586 * rint(ang * 0.15915494309189535);
   float s, c;
   sincosf((float)red, &s, &c);
@@ -79,16 +80,15 @@ __device__ __forceinline__ void fft_reg(float2 *v, float sgn) {
   }
 }
 
-// One 128-point transform of all 128 LDS lines: line l starts at
-// l * line_stride, elements at stride es.  Pass A on items (line, j2), pass
-// B on items (line, k1); tw[m] = W128^m.  LINE_FAST puts consecutive lanes
-// on consecutive lines (columns: consecutive addresses), otherwise on
-// consecutive items of one line (rows).
+// Pass A of the 128-point transforms of all 128 LDS lines (line l starts at
+// l * line_stride, elements at stride es) on items (line, j2), in place;
+// tw[m] = W128^m.  LINE_FAST puts consecutive lanes on consecutive lines
+// (columns: consecutive addresses), otherwise on consecutive items of one
+// line (rows).
 template <bool LINE_FAST>
-__device__ __forceinline__ void fft128_lines(float2 *lds, int line_stride, int es,
-                                             const float2 *tw, float sgn) {
+__device__ __forceinline__ void pass_a(float2 *lds, int line_stride, int es, const float2 *tw,
+                                       float sgn) {
   const int tid = threadIdx.x;
-  // pass A: 128 lines x 8 items
 #pragma unroll
   for (int r = 0; r < PASS_A; r++) {
     const int it = tid + r * FFT_NT;
@@ -105,29 +105,35 @@ __device__ __forceinline__ void fft128_lines(float2 *lds, int line_stride, int e
       p[8 * k1 * es] = cmul(v[k1], make_float2(w.x, sgn * w.y));
     }
   }
-  __syncthreads();
-  // pass B: 128 lines x 16 items; reads then writes
-  float2 v[PASS_B][8];
-  int base[PASS_B], k1s[PASS_B];
-#pragma unroll
-  for (int r = 0; r < PASS_B; r++) {
-    const int it = tid + r * FFT_NT;
-    const int line = LINE_FAST ? (it & 127) : (it >> 4);
-    const int k1 = LINE_FAST ? (it >> 7) : (it & 15);
-    base[r] = line * line_stride;
-    k1s[r] = k1;
-#pragma unroll
-    for (int j2 = 0; j2 < 8; j2++) v[r][j2] = lds[base[r] + (8 * k1 + j2) * es];
-    fft_reg<8>(v[r], sgn);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < PASS_B; r++)
-#pragma unroll
-    for (int k2 = 0; k2 < 8; k2++) lds[base[r] + (k1s[r] + 16 * k2) * es] = v[r][k2];
-  __syncthreads();
 }
 
+// Pass-B item r of the thread: (line, k1).
+template <bool LINE_FAST>
+__device__ __forceinline__ void pass_b_item(int r, int &line, int &k1) {
+  const int it = threadIdx.x + r * FFT_NT;
+  line = LINE_FAST ? (it & 127) : (it >> 4);
+  k1 = LINE_FAST ? (it >> 7) : (it & 15);
+}
+
+// Pass B on items (line, k1): v[r][k2] = output k1 + 16 k2 of the thread's
+// r-th line, read from LDS; the caller stores it.
+template <bool LINE_FAST>
+__device__ __forceinline__ void pass_b(const float2 *lds, int line_stride, int es, float sgn,
+                                       float2 (&v)[PASS_B][8]) {
+#pragma unroll
+  for (int r = 0; r < PASS_B; r++) {
+    int line, k1;
+    pass_b_item<LINE_FAST>(r, line, k1);
+    const float2 *p = lds + line * line_stride + 8 * k1 * es;
+#pragma unroll
+    for (int j2 = 0; j2 < 8; j2++) v[r][j2] = p[j2 * es];
+    fft_reg<8>(v[r], sgn);
+  }
+}
+
+// Rows are transformed first, columns second.  The rows' pass A takes its
+// input straight from memory and the columns' pass B writes straight to
+// memory, so the sub-grid crosses LDS six times (three passes in place).
 __global__ __launch_bounds__(FFT_NT) void k_fft2d_128(
     const float2 *in, float2 *out, int64_t batch_stride, const int32_t *__restrict__ in_idx,
     const int32_t *__restrict__ out_idx, int sign, float out_scale, FModArgs mod_in,
@@ -138,33 +144,89 @@ __global__ __launch_bounds__(FFT_NT) void k_fft2d_128(
   const float2 *X = in + (int64_t)(in_idx ? in_idx[b] : b) * batch_stride;
   float2 *Y = out + (int64_t)(out_idx ? out_idx[b] : b) * batch_stride;
   const float sgn = sign < 0 ? -1.0f : 1.0f;
-  if (tid < FN) {
-    double s, c;
-    sincospi((double)tid / 64.0, &s, &c);  // W128^m = exp(sgn 2 pi i m / 128)
-    tw[tid] = make_float2((float)c, (float)s);
-  }
-  // load: eight pixels in flight per thread per round
-  for (int i0 = tid; i0 < FN * FN; i0 += 8 * FFT_NT) {
-    float2 xv[8];
+  // The phase tables are read in batches ahead of use so that their loads
+  // overlap one another.
+  const bool mi = mod_in.phi != nullptr, mo = mod_out.phi != nullptr;
+  {
+    // Row pass A: the thread owns row y and its elements x = 8 j1 + j2,
+    // which share the parity of y + j2 (the (-1)^(x + y) centring sign).
+    const int y = tid >> 3, j2 = tid & 7;
+    const float2 *row = X + y * FN + j2;
+    float2 v[16];
+    double ph[16];
 #pragma unroll
-    for (int k = 0; k < 8; k++) xv[k] = X[i0 + k * FFT_NT];
+    for (int j1 = 0; j1 < 16; j1++) v[j1] = row[8 * j1];
+    if (mi) {
+      const double *prow = mod_in.phi + y * FN + j2;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const int i = i0 + k * FFT_NT, y = i >> 7, x = i & 127;
-      float2 v = fmod_phase(xv[k], mod_in, b, i);
-      if ((x + y) & 1) v = make_float2(-v.x, -v.y);
-      lds[y * FP + x] = v;
+      for (int j1 = 0; j1 < 16; j1++) ph[j1] = prow[8 * j1];
+    }
+    if (tid < FN) {
+      double s, c;
+      sincospi((double)tid / 64.0, &s, &c);  // W128^m = exp(sgn 2 pi i m / 128)
+      tw[tid] = make_float2((float)c, (float)s);
+    }
+    __syncthreads();
+    const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
+    const double kin = mi ? (double)mod_in.kplane[b] * (double)mod_in.sign : 0.0;
+#pragma unroll
+    for (int j1 = 0; j1 < 16; j1++) {
+      const float2 t = mi ? rotate(v[j1], kin, ph[j1]) : v[j1];
+      v[j1] = make_float2(neg * t.x, neg * t.y);
+    }
+    fft_reg<16>(v, sgn);
+    float2 *p = lds + y * FP + j2;
+#pragma unroll
+    for (int k1 = 0; k1 < 16; k1++) {
+      const float2 w = tw[(j2 * k1) & 127];
+      p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
     }
   }
   __syncthreads();
-  fft128_lines<false>(lds, FP, 1, tw, sgn);  // rows
-  fft128_lines<true>(lds, 1, FP, tw, sgn);   // columns
-  for (int i = tid; i < FN * FN; i += FFT_NT) {
-    const int y = i >> 7, x = i & 127;
-    float2 v = lds[y * FP + x];
-    const float s = ((x + y) & 1) ? -out_scale : out_scale;
-    v = make_float2(v.x * s, v.y * s);
-    Y[i] = fmod_phase(v, mod_out, b, i);
+  {
+    // Row pass B, written back to the natural positions once all are read.
+    float2 v[PASS_B][8];
+    pass_b<false>(lds, FP, 1, sgn, v);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PASS_B; r++) {
+      int line, k1;
+      pass_b_item<false>(r, line, k1);
+#pragma unroll
+      for (int k2 = 0; k2 < 8; k2++) lds[line * FP + k1 + 16 * k2] = v[r][k2];
+    }
+  }
+  __syncthreads();
+  pass_a<true>(lds, 1, FP, tw, sgn);  // columns
+  __syncthreads();
+  {
+    // Column pass B to memory: item (column x, k1) holds rows y = k1 + 16 k2,
+    // which share the parity of x + k1.
+    double ph[PASS_B][8];
+    if (mo) {
+#pragma unroll
+      for (int r = 0; r < PASS_B; r++) {
+        int x, k1;
+        pass_b_item<true>(r, x, k1);
+#pragma unroll
+        for (int k2 = 0; k2 < 8; k2++) ph[r][k2] = mod_out.phi[(k1 + 16 * k2) * FN + x];
+      }
+    }
+    float2 v[PASS_B][8];
+    pass_b<true>(lds, 1, FP, sgn, v);
+    const double kout = mo ? (double)mod_out.kplane[b] * (double)mod_out.sign : 0.0;
+#pragma unroll
+    for (int r = 0; r < PASS_B; r++) {
+      int x, k1;
+      pass_b_item<true>(r, x, k1);
+      const float sc = ((x + k1) & 1) ? -out_scale : out_scale;
+#pragma unroll
+      for (int k2 = 0; k2 < 8; k2++) {
+        const int pix = (k1 + 16 * k2) * FN + x;
+        const float2 t = make_float2(v[r][k2].x * sc, v[r][k2].y * sc);
+        Y[pix] = mo ? rotate(t, kout, ph[r][k2]) : t;
+      }
+    }
   }
 }
 
