@@ -26,8 +26,14 @@ __global__ __launch_bounds__(BIN_NT) void k_bin_count(
     BinArgs a, BinGeom g, int nbins, uint4 *__restrict__ loc,
     uint32_t *__restrict__ count, uint32_t *__restrict__ status) {
   __shared__ uint32_t hist[HIST_MAX];
+  __shared__ unsigned long long s_ext;
   const bool lds = nbins <= HIST_MAX;
   const int tid = threadIdx.x;
+  // Groups are equal slices of the bank's actual extent.
+  if (g.ngroups > 1) {
+    const int64_t ext = bank_extent(a.plane_off, a.plane_s, a.nplanes, a.Q, a.separable, &s_ext);
+    g.group_elems = max((int64_t)1, (ext + g.ngroups - 1) / g.ngroups);
+  }
   if (lds)
     for (int t = tid; t < nbins; t += BIN_NT) hist[t] = 0;
   __syncthreads();
@@ -222,10 +228,12 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   p.ntiles = p.ntx * p.nty;
   p.chunk = ctx->chunk;
 
-  // Plane-groups: slices of group_kib KiB of an upper bound of the bank's
-  // extent; automatically (group_kib 0) one group per XCD queue when that
-  // bound exceeds one XCD's L2 (4 MiB), else one group.  At most 64, and as
-  // many as keep the bin histogram in LDS.
+  // Plane-groups: their number is set from an upper bound of the bank's
+  // extent (every plane at max_support), group_kib KiB per group, or
+  // automatically (group_kib 0) one per XCD queue when that bound exceeds
+  // one XCD's L2 (4 MiB), else one; at most 64, and as many as keep the bin
+  // histogram in LDS.  The count kernel then cuts the bank's actual extent
+  // into that many equal groups.
   const int64_t S = max_support;
   const int64_t bound = (int64_t)a.nplanes * (a.separable ? a.Q * S : (int64_t)a.Q * a.Q * S * S);
   int64_t ng;

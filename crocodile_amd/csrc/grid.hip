@@ -44,19 +44,7 @@ __global__ void k_scale_maxima(const float2 *__restrict__ vis, int64_t nv,
                                const int32_t *__restrict__ plane_s, int nplanes,
                                int Q, int separable, ScaleState *st) {
   __shared__ unsigned long long s_nk;
-  if (threadIdx.x == 0) s_nk = 0ull;
-  __syncthreads();
-  {
-    int64_t e = 0;
-    for (int p = threadIdx.x; p < nplanes; p += blockDim.x) {
-      const int64_t S = plane_s[p];
-      const int64_t n = separable ? Q * S : (int64_t)Q * Q * S * S;
-      e = max(e, plane_off[p] + n);
-    }
-    if (e > 0) atomicMax(&s_nk, (unsigned long long)e);
-  }
-  barrier_after_lds_atomics();
-  const int64_t nk = (int64_t)s_nk;
+  const int64_t nk = bank_extent(plane_off, plane_s, nplanes, Q, separable, &s_nk);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   block_max_to(array_max(vis, nv, i0, stride), &st->max_vis_bits);

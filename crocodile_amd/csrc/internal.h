@@ -142,6 +142,25 @@ __device__ __forceinline__ void barrier_after_lds_atomics() {
   __syncthreads();
 }
 
+// Extent of the kernel bank in elements: the largest plane_off[p] + table
+// size of plane p (Q*Q*S*S, or Q*S separable), 0 for none.  Called by
+// every thread of a workgroup; s_ext is the workgroup's LDS cell for the
+// maximum.
+__device__ __forceinline__ int64_t bank_extent(const int64_t *plane_off, const int32_t *plane_s,
+                                               int nplanes, int Q, int separable,
+                                               unsigned long long *s_ext) {
+  if (threadIdx.x == 0) *s_ext = 0ull;
+  __syncthreads();
+  int64_t e = 0;
+  for (int p = threadIdx.x; p < nplanes; p += blockDim.x) {
+    const int64_t S = plane_s[p];
+    e = max(e, plane_off[p] + (separable ? Q * S : (int64_t)Q * Q * S * S));
+  }
+  if (e > 0) atomicMax(s_ext, (unsigned long long)e);
+  barrier_after_lds_atomics();
+  return (int64_t)*s_ext;
+}
+
 // ---------------------------------------------------------------------------
 // Binning of visibilities by the grid tile holding their kernel window's
 // first cell (shared by gridding and degridding).

@@ -10,7 +10,7 @@ constexpr uint32_t LOC_INVALID = 0xffffffffu;
 
 struct BinGeom {
   int T, ntx, ngroups;
-  int64_t group_elems;
+  int64_t group_elems;  // plane-group width in bank elements
 };
 
 __device__ __forceinline__ uint4 locate(const BinArgs &a, const BinGeom &g,
