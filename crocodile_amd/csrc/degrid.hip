@@ -17,7 +17,7 @@ namespace {
 constexpr int DEGRID_NT = 512;
 
 template <int NT, bool SEPARABLE, int R>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_degrid_tiles(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R == 63 ? 8 : 4))) void k_degrid_tiles(
     const float2 *__restrict__ grid, int N, int T, int ntx,
     const uint4 *__restrict__ rec, float2 *__restrict__ vis_out,
     const float2 *__restrict__ kern, const int4 *__restrict__ items,
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
         const float2 *ky = kp + yf * S, *kx = kp + xf * S;
         float ar = 0.f, ai = 0.f;
-        gather_range<SEPARABLE>(sg, base, R, S, 0, S * S, lane, kp, ky, kx, ar, ai);
+        gather_range<SEPARABLE, R != 63>(sg, base, R, S, 0, S * S, lane, kp, ky, kx, ar, ai);
         ar = wave_sum_to_63(ar);
         ai = wave_sum_to_63(ai);
         if (lane == 63) store_stream(vis_out + r.w, make_float2(ar, ai));
@@ -99,7 +99,7 @@ int degrid_binned(croc_ctx *ctx, const BinPlan &plan, const void *grid, int N, v
   const float2 *kt = (const float2 *)kern;
   float2 *vo = (float2 *)vis_out;
   if (plan.R == TILE_R_SMALL) {
-    const int G = persistent_blocks(ctx, 2);  // two fit by registers (16 waves)
+    const int G = persistent_blocks(ctx, 4);  // four fit by registers (32 waves)
     if (separable)
       k_degrid_tiles<DEGRID_NT, true, TILE_R_SMALL><<<G, DEGRID_NT, 0, ctx->stream>>>(
           g, N, plan.T, plan.ntx, plan.rec, vo, kt, plan.items_g, plan.gstart, plan.ngroups, plan.qhead);

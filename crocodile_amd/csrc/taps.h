@@ -101,9 +101,10 @@ __device__ __forceinline__ void gather_block(
   }
 }
 
-// Taps [tlo, thi) of one visibility: unmasked blocks of 1024, 512 and 256
-// taps, then at most one masked block of the remaining < 256.
-template <bool SEPARABLE>
+// Taps [tlo, thi) of one visibility: unmasked blocks of 1024 (when
+// BIG_BLOCKS), 512 and 256 taps, then at most one masked block of the
+// remaining < 256.
+template <bool SEPARABLE, bool BIG_BLOCKS = true>
 __device__ __forceinline__ void gather_range(
     const float2 *sg, int cbase, int RS, int S, int tlo, int thi, int lane, const float2 *k2,
     const float2 *ky, const float2 *kx, float &ar, float &ai) {
@@ -111,14 +112,13 @@ __device__ __forceinline__ void gather_range(
   const uint32_t rs8 = 8u * (uint32_t)(RS - S);
   const uint32_t cb0 = 8u * (uint32_t)(cbase + lane), kt0 = 8u * (uint32_t)lane;
   int t0 = tlo;
-  for (; t0 + 1024 <= thi; t0 += 1024)
-    gather_block<16, true, SEPARABLE>(sg, cb0 + 8u * t0, rs8, kt0 + 8u * t0, S, thi, m, t0,
-                                      lane, k2, ky, kx, ar, ai);
-  if (t0 + 512 <= thi) {
+  if (BIG_BLOCKS)
+    for (; t0 + 1024 <= thi; t0 += 1024)
+      gather_block<16, true, SEPARABLE>(sg, cb0 + 8u * t0, rs8, kt0 + 8u * t0, S, thi, m, t0,
+                                        lane, k2, ky, kx, ar, ai);
+  for (; t0 + 512 <= thi; t0 += 512)
     gather_block<8, true, SEPARABLE>(sg, cb0 + 8u * t0, rs8, kt0 + 8u * t0, S, thi, m, t0,
                                      lane, k2, ky, kx, ar, ai);
-    t0 += 512;
-  }
   if (t0 + 256 <= thi) {
     gather_block<4, true, SEPARABLE>(sg, cb0 + 8u * t0, rs8, kt0 + 8u * t0, S, thi, m, t0,
                                      lane, k2, ky, kx, ar, ai);
