@@ -4,6 +4,9 @@
 // A visibility's S x S window is walked in tap order t (row i = t / S,
 // column t - i*S) over a range [tlo, thi) of whole rows.  Its cells lie in a
 // row-major LDS region with row stride RS cells: cell cbase + t + i*(RS - S).
+// Rows may carry `pad` extra bytes (a row pitch that is not a multiple of the
+// LDS bank width keeps the rows a wavefront touches on different banks);
+// `extra` is then the padding before the window's first row.
 // Lanes take consecutive taps; blocks of 64*NQ taps issue all NQ kernel
 // loads before the first use.  Blocks that lie wholly inside the range run
 // with no clamps and no masks; the last, partial block clamps its load
@@ -52,10 +55,11 @@ __device__ __forceinline__ void spread_block(
 template <bool SEPARABLE>
 __device__ __forceinline__ void spread_range(
     unsigned long long *acc, int cbase, int RS, int S, int tlo, int thi, int lane,
-    const float2 *k2, const float2 *ky, const float2 *kx, double vr, double vi) {
+    const float2 *k2, const float2 *ky, const float2 *kx, double vr, double vi,
+    uint32_t pad = 0u, uint32_t extra = 0u) {
   const uint32_t m = __builtin_amdgcn_readfirstlane(tap_div(S));
-  const uint32_t rs16 = 16u * (uint32_t)(RS - S);
-  const uint32_t cb0 = 16u * (uint32_t)(cbase + lane), kt0 = 8u * (uint32_t)lane;
+  const uint32_t rs16 = 16u * (uint32_t)(RS - S) + pad;
+  const uint32_t cb0 = 16u * (uint32_t)(cbase + lane) + extra, kt0 = 8u * (uint32_t)lane;
   int t0 = tlo;
   for (; t0 + 512 <= thi; t0 += 512)
     spread_block<8, true, SEPARABLE>(acc, cb0 + 16u * t0, rs16, kt0 + 8u * t0, S, thi, m, t0,
@@ -107,10 +111,11 @@ __device__ __forceinline__ void gather_block(
 template <bool SEPARABLE, bool BIG_BLOCKS = true>
 __device__ __forceinline__ void gather_range(
     const float2 *sg, int cbase, int RS, int S, int tlo, int thi, int lane, const float2 *k2,
-    const float2 *ky, const float2 *kx, float &ar, float &ai) {
+    const float2 *ky, const float2 *kx, float &ar, float &ai, uint32_t pad = 0u,
+    uint32_t extra = 0u) {
   const uint32_t m = __builtin_amdgcn_readfirstlane(tap_div(S));
-  const uint32_t rs8 = 8u * (uint32_t)(RS - S);
-  const uint32_t cb0 = 8u * (uint32_t)(cbase + lane), kt0 = 8u * (uint32_t)lane;
+  const uint32_t rs8 = 8u * (uint32_t)(RS - S) + pad;
+  const uint32_t cb0 = 8u * (uint32_t)(cbase + lane) + extra, kt0 = 8u * (uint32_t)lane;
   int t0 = tlo;
   if (BIG_BLOCKS)
     for (; t0 + 1024 <= thi; t0 += 1024)

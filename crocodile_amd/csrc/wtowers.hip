@@ -215,19 +215,25 @@ __global__ void k_tw_pick(TwScale *fs, int64_t nv, int nearest, int separable) {
 }
 
 // Grid job parts into sub-grid slots: parts[b] = (slot, first record, end
-// record); one workgroup per part, rows [0, sg/2) then [sg/2, sg): each
+// record); one workgroup per part, in passes of GRID_ROWS rows: each
 // wavefront adds the taps of one visibility at a time that lie in the pass's
 // rows (a contiguous tap range) to the int64 region, and the pass ends by
 // converting the region to complex64 in sub[slot] and zeroing it for the
-// next pass (it is zeroed once at the start of the launch).
+// next pass (it is zeroed once at the start of the launch).  Region rows are
+// padded by 8 bytes (GRID_RU64 u64 per row): with the bare 2 KiB pitch the
+// rows of a 64-tap block would all fall on the same LDS banks.  (Passes of
+// 32 rows, two workgroups per compute unit, measured slower: every pass
+// lists the part's visibilities again.)
+constexpr int GRID_ROWS = SG / 2;
+constexpr int GRID_RU64 = 2 * SG + 1;
 template <bool SEPARABLE>
 __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
     const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
     const float2 *__restrict__ svis, const float2 *__restrict__ kern, int Q, int S,
     int nearest, const TwScale *__restrict__ fs, float2 *__restrict__ sub,
     uint32_t *__restrict__ qc) {
-  constexpr int HALF = SG / 2, HC = HALF * SG;
-  __shared__ unsigned long long acc[2 * HC];  // re, im interleaved (128 KiB)
+  constexpr int ROWS = GRID_ROWS, HC = ROWS * SG;
+  __shared__ unsigned long long acc[ROWS * GRID_RU64];  // re, im interleaved (128 KiB + pads)
   __shared__ uint16_t s_list[PART_VIS];       // the pass's visibilities (offsets in the part)
   __shared__ int s_n;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -237,7 +243,7 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
   const int SS = S * S;
   __shared__ int s_b;
   // The region starts zeroed; each pass's read-out zeroes it again.
-  for (int c = tid; c < 2 * HC; c += TW_NT) acc[c] = 0ull;
+  for (int c = tid; c < ROWS * GRID_RU64; c += TW_NT) acc[c] = 0ull;
   for (;;) {  // parts taken from the launch's queue counter qc
     if (tid == 0) s_b = (int)atomicAdd(qc, 1u);
     __syncthreads();
@@ -247,16 +253,16 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
     const int jbeg = __builtin_amdgcn_readfirstlane(job.y);
     const int jend = __builtin_amdgcn_readfirstlane(job.z);
     float2 *out = sub + (int64_t)__builtin_amdgcn_readfirstlane(job.x) * SGN;
-    for (int r0 = 0; r0 < SG; r0 += HALF) {
+    for (int r0 = 0; r0 < SG; r0 += ROWS) {
       if (tid == 0) s_n = 0;
       __syncthreads();
       if (nearest) {
         for (int j = jbeg + tid; j < jend; j += TW_NT) {
           const uint4 r = rec[j];
           const int wy = (int)(r.x >> 16) - r0;
-          if (wy >= 0 && wy < HALF) {
+          if (wy >= 0 && wy < ROWS) {
             const float2 v = svis[j];
-            unsigned long long *cell = acc + 2 * (wy * SG + (int)(r.x & 0xffffu));
+            unsigned long long *cell = acc + wy * GRID_RU64 + 2 * (int)(r.x & 0xffffu);
             atomicAdd(cell, fixed(v.x, scale));
             atomicAdd(cell + 1, fixed(v.y, scale));
           }
@@ -273,7 +279,7 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
             bool hit = false;
             if (j < je) {
               const int wy = (int)(rec[j].x >> 16);
-              hit = wy < r0 + HALF && wy + S > r0;
+              hit = wy < r0 + ROWS && wy + S > r0;
             }
             const unsigned long long bal = __ballot(hit);
             int base = 0;
@@ -283,21 +289,36 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
               s_list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)(j - jb);
           }
           barrier_after_lds_atomics();
-          const int nhit = s_n;
-          for (int k = wave; k < nhit; k += NW) {
-            const int j = jb + (int)s_list[k];
-            const uint4 r = uniform_u4(rec[j]);
-            const int wy = (int)(r.x >> 16), wx = (int)(r.x & 0xffffu);
-            // window rows wy + i in the pass: i in [ilo, ihi)
-            const int ilo = max(0, r0 - wy), ihi = min(S, r0 + HALF - wy);
-            const float2 v = uniform_f2(svis[j]);
-            const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
-            const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
-            const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
-            // cell of tap t: (wy - r0) * SG + wx + t + i * (SG - S), in
-            // [0, HC) for the pass's rows
-            spread_range<SEPARABLE>(acc, (wy - r0) * SG + wx, SG, S, ilo * S, ihi * S, lane,
-                                    k2, ky, kx, (double)v.x * scale, (double)v.y * scale);
+          const int nhit = __builtin_amdgcn_readfirstlane(s_n);
+          // The wave's listed visibilities k = wave, + NW, ..., records and
+          // values fetched 64 at a time (lane l: the l-th of the batch) and
+          // handed out with readlane: no scalar load, and so no wait for the
+          // wave's own LDS atomics, before each visibility.
+          for (int kb = wave; kb < nhit; kb += 64 * NW) {
+            const int kl = kb + lane * NW;
+            uint4 rb = make_uint4(0, 0, 0, 0);
+            float2 vb = make_float2(0.f, 0.f);
+            if (kl < nhit) {
+              const int j = jb + (int)s_list[kl];
+              rb = rec[j];
+              vb = svis[j];
+            }
+            const int nbat = min(64, (nhit - kb + NW - 1) / NW);
+            for (int it = 0; it < nbat; it++) {
+              const uint4 r = lane_u4(rb, it);
+              const float2 v = lane_f2(vb, it);
+              const int wy = (int)(r.x >> 16), wx = (int)(r.x & 0xffffu);
+              // window rows wy + i in the pass: i in [ilo, ihi)
+              const int ilo = max(0, r0 - wy), ihi = min(S, r0 + ROWS - wy);
+              const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+              const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
+              const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
+              // cell of tap t: (wy - r0) * SG + wx + t + i * (SG - S), in
+              // [0, HC) for the pass's rows, at byte 16 * cell + 8 * its row
+              spread_range<SEPARABLE>(acc, (wy - r0) * SG + wx, SG, S, ilo * S, ihi * S, lane,
+                                      k2, ky, kx, (double)v.x * scale, (double)v.y * scale, 8u,
+                                      8u * (uint32_t)(wy - r0));
+            }
           }
           __syncthreads();  // the list is read; the next window rewrites it
           if (tid == 0) s_n = 0;
@@ -306,10 +327,13 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
       }
       barrier_after_lds_atomics();
       for (int c = tid; c < HC; c += TW_NT) {
-        const long long re = (long long)acc[2 * c], im = (long long)acc[2 * c + 1];
-        acc[2 * c] = 0ull;
-        acc[2 * c + 1] = 0ull;
-        out[r0 * SG + c] = make_float2((float)((double)re * inv), (float)((double)im * inv));
+        const int ca = (c / SG) * GRID_RU64 + 2 * (c % SG);
+        const long long re = (long long)acc[ca], im = (long long)acc[ca + 1];
+        acc[ca] = 0ull;
+        acc[ca + 1] = 0ull;
+        // streaming store: the sub-grids must not evict the kernel table from L2
+        store_stream(out + r0 * SG + c,
+                     make_float2((float)((double)re * inv), (float)((double)im * inv)));
       }
       __syncthreads();
     }
@@ -399,7 +423,10 @@ __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
     const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
     const float2 *__restrict__ sub, const float2 *__restrict__ kern, int Q, int S, int nearest,
     float2 *__restrict__ vis_out, uint32_t *__restrict__ qc) {
-  __shared__ float2 sg[SGN];
+  // Rows padded by 128 bytes (DEG_PITCH cells): the bare 1 KiB pitch would
+  // put the rows of a 64-tap block on the same LDS banks.
+  constexpr int DEG_PITCH = SG + 16;
+  __shared__ float2 sg[SG * DEG_PITCH];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int NW = TW_NT / 64;
@@ -419,28 +446,40 @@ __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
 #pragma unroll
       for (int k = 0; k < 8; k++) gv[k] = load_stream(src + i0 + k * TW_NT);
 #pragma unroll
-      for (int k = 0; k < 8; k++) sg[i0 + k * TW_NT] = gv[k];
+      for (int k = 0; k < 8; k++) {
+        const int i = i0 + k * TW_NT;
+        sg[(i / SG) * DEG_PITCH + i % SG] = gv[k];
+      }
     }
     __syncthreads();
     if (nearest) {
       for (int j = job.y + tid; j < job.z; j += TW_NT) {
         const uint4 r = rec[j];
-        vis_out[r.w] = sg[(int)(r.x >> 16) * SG + (int)(r.x & 0xffffu)];
+        vis_out[r.w] = sg[(int)(r.x >> 16) * DEG_PITCH + (int)(r.x & 0xffffu)];
       }
     } else {
       const int jbeg = __builtin_amdgcn_readfirstlane(job.y);
       const int jend = __builtin_amdgcn_readfirstlane(job.z);
-      for (int j = jbeg + wave; j < jend; j += NW) {
-        const uint4 r = uniform_u4(rec[j]);
-        const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
-        const int base = (int)(r.x >> 16) * SG + (int)(r.x & 0xffffu);
-        const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
-        const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
-        float sr = 0.f, si = 0.f;
-        gather_range<SEPARABLE>(sg, base, SG, S, 0, SS, lane, k2, ky, kx, sr, si);
-        sr = wave_sum_to_63(sr);
-        si = wave_sum_to_63(si);
-        if (lane == 63) store_stream(vis_out + r.w, make_float2(sr, si));
+      // records of the wave's visibilities j = first + wave, + NW, ...
+      // fetched 64 at a time and handed out with readlane
+      for (int jb = jbeg + wave; jb < jend; jb += 64 * NW) {
+        const int jl = jb + lane * NW;
+        const uint4 rb = jl < jend ? rec[jl] : make_uint4(0, 0, 0, 0);
+        const int nbat = min(64, (jend - jb + NW - 1) / NW);
+        for (int it = 0; it < nbat; it++) {
+          const uint4 r = lane_u4(rb, it);
+          const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+          const int wy = (int)(r.x >> 16);
+          const int base = wy * SG + (int)(r.x & 0xffffu);
+          const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
+          const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
+          float sr = 0.f, si = 0.f;
+          gather_range<SEPARABLE>(sg, base, SG, S, 0, SS, lane, k2, ky, kx, sr, si,
+                                  8u * (DEG_PITCH - SG), 8u * (DEG_PITCH - SG) * (uint32_t)wy);
+          sr = wave_sum_to_63(sr);
+          si = wave_sum_to_63(si);
+          if (lane == 63) store_stream(vis_out + r.w, make_float2(sr, si));
+        }
       }
     }
     __syncthreads();
