@@ -102,3 +102,24 @@ def test_fft_agrees_with_dft_and_rejects_other_sizes(gpu):
     assert rel(fft(fft(x, False), True), x) < 1e-6
     with pytest.raises(ValueError):
         fft(numpy.zeros((1, 64, 64), complex), False)
+
+
+@pytest.mark.parametrize('in_place', [False, True])
+def test_fft_batch_larger_than_the_device(gpu, in_place):
+    """More sub-grids than compute units: each workgroup transforms several,
+    loading the next one's rows while it works on the current one."""
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    nb = 2 * ncu + 37
+    rng = numpy.random.default_rng(15 + in_place)
+    x = (rng.standard_normal((nb, 128, 128)) +
+         1j * rng.standard_normal((nb, 128, 128))).astype(numpy.complex64)
+    t = torch.as_tensor(x, device='cuda')
+    out = t if in_place else torch.empty_like(t)
+    _lib.check(_lib.load().croc_fft2d_batched(_lib.ctx(), _lib.ptr(t), _lib.ptr(out), 128, nb,
+                                              0))
+    got = out.cpu().numpy()
+    worst = 0.0
+    for b in range(nb):
+        worst = max(worst, rel(got[b], R.fft(x[b].astype(complex))))
+    print('batch %d worst rms_rel %.3e' % (nb, worst))
+    assert worst < 1e-6, worst
