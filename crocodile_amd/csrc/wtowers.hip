@@ -344,19 +344,25 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
 // (first slot, n) of a job gridded in n parts.  blockIdx.y = job,
 // blockIdx.x = slice of the pixels (two per thread).
 constexpr int ACC_SLICES = 16;
+__device__ __forceinline__ float4 add4(float4 a, float4 b) {
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
 __global__ void k_tw_sum_parts(const int2 *__restrict__ multi, float2 *__restrict__ sub) {
   const int2 m = multi[blockIdx.y];
   float4 *dst = (float4 *)(sub + (int64_t)m.x * SGN);
   constexpr int PER = SGN / 2 / ACC_SLICES;
   for (int i = blockIdx.x * PER + threadIdx.x; i < (blockIdx.x + 1) * PER; i += blockDim.x) {
     float4 acc = dst[i];
-    for (int k = 1; k < m.y; k++) {
-      const float4 t = ((const float4 *)(sub + (int64_t)(m.x + k) * SGN))[i];
-      acc.x += t.x;
-      acc.y += t.y;
-      acc.z += t.z;
-      acc.w += t.w;
+    int k = 1;
+    // four parts in flight, added in part order
+    for (; k + 4 <= m.y; k += 4) {
+      float4 t[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) t[q] = ((const float4 *)(sub + (int64_t)(m.x + k + q) * SGN))[i];
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc = add4(acc, t[q]);
     }
+    for (; k < m.y; k++) acc = add4(acc, ((const float4 *)(sub + (int64_t)(m.x + k) * SGN))[i]);
     dst[i] = acc;
   }
 }
@@ -374,13 +380,16 @@ __global__ void k_tw_chunk_accum(const int4 *__restrict__ groups, const int32_t 
   constexpr int PER = SGN / 2 / ACC_SLICES;  // float4 (pixel pairs) per slice
   for (int i = blockIdx.x * PER + threadIdx.x; i < (blockIdx.x + 1) * PER; i += blockDim.x) {
     float4 acc = dst[i];
-    for (int b = g.y; b < g.z; b++) {
-      const float4 s = src[(int64_t)jslot[b] * (SGN / 2) + i];
-      acc.x += s.x;
-      acc.y += s.y;
-      acc.z += s.z;
-      acc.w += s.w;
+    int b = g.y;
+    // eight job images in flight, added in job order
+    for (; b + 8 <= g.z; b += 8) {
+      float4 s[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) s[k] = src[(int64_t)jslot[b + k] * (SGN / 2) + i];
+#pragma unroll
+      for (int k = 0; k < 8; k++) acc = add4(acc, s[k]);
     }
+    for (; b < g.z; b++) acc = add4(acc, src[(int64_t)jslot[b] * (SGN / 2) + i]);
     dst[i] = acc;
   }
 }
