@@ -20,6 +20,7 @@ and the CPU baseline (the FP64 NumPy restatement of the same per-visibility
 loops, one core, on a bounded sample of the same workload).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -198,6 +199,33 @@ def run_towers(args, torch, G, W, _lib, dev):
         _, d_ms = _lib.last_kernel_ms()
         return g_ms, d_ms
 
+    # planned round trips: the coordinates binned once per step and used by
+    # both calls (croc_wtowers_plan_*, croc_plan_*)
+    twp = _lib._P()
+    _lib.check(L.croc_wtowers_plan_create(_lib.ctx(), ctypes.byref(twp)))
+    gplan = G.GridPlan(bank, N)
+
+    def towers_plan_step():
+        _lib.check(L.croc_wtowers_plan_bin(_lib.ctx(), twp, N, _lib.ptr(uvw_d), nv, 3, theta,
+                                           wstep, 128, margin, 1, Q, S))
+        guv.zero_()
+        _lib.check(L.croc_wtowers_grid_plan(_lib.ctx(), twp, _lib.ptr(guv), _lib.ptr(vis_d),
+                                            _lib.ptr(kv), 0))
+        g_ms, _ = _lib.last_kernel_ms()
+        _lib.check(L.croc_wtowers_degrid_plan(_lib.ctx(), twp, _lib.ptr(guv), _lib.ptr(out),
+                                              _lib.ptr(kv), 0))
+        _, d_ms = _lib.last_kernel_ms()
+        return g_ms, d_ms
+
+    def plain_plan_step():
+        gplan.bin(uvw_d, lam, _lib.PLANE_WBIN, None, wstep, lo)
+        guv.zero_()
+        gplan.grid(guv, vis_d)
+        g_ms, _ = _lib.last_kernel_ms()
+        gplan.degrid(guv, out)
+        _, d_ms = _lib.last_kernel_ms()
+        return g_ms, d_ms
+
     def plain_step():
         guv.zero_()
         G.grid_device(bank, guv, uvw_d, lam, vis_d, _lib.PLANE_WBIN, None, wstep, lo)
@@ -230,7 +258,11 @@ def run_towers(args, torch, G, W, _lib, dev):
         return nv * args.steps / dt, 1e3 * dt / args.steps, sum(gs) / len(gs), sum(ds) / len(ds)
 
     tv, tms, tg, td = timed(towers_step, 'w-towers')
+    tpv, tpms, _, _ = timed(towers_plan_step, 'planned w-towers')
+    _lib.check(L.croc_wtowers_plan_destroy(_lib.ctx(), twp))
     pv, pms, pg, pd = timed(plain_step, 'plain')
+    ppv, ppms, _, _ = timed(plain_plan_step, 'planned plain')
+    del gplan
     qv, qms, qg, qd = timed(pervis_step, 'per-visibility')
     # algorithmic work: taps (8 flop each way) and sub-grid DFTs (16 n^3 flop
     # per 128^2 transform, three fp16 products each on the matrix cores)
@@ -253,6 +285,11 @@ def run_towers(args, torch, G, W, _lib, dev):
         'towers': {'ms_per_step': round(tms, 3), 'grid_ms': round(tg, 3),
                    'degrid_ms': round(td, 3), 'tap_tflops': round(2 * 8 * taps / tms / 1e9, 3),
                    'dft_tflops': round(dft_flops / tms / 1e9, 3)},
+        'towers_planned': {'value': round(tpv, 1), 'ms_per_step': round(tpms, 3),
+                           'note': 'croc_wtowers_plan_bin + croc_wtowers_grid_plan + '
+                                   'croc_wtowers_degrid_plan: coordinates binned once per step'},
+        'plain_planned': {'value': round(ppv, 1), 'ms_per_step': round(ppms, 3),
+                          'note': 'croc_plan_bin + croc_convgrid_plan + croc_convdegrid_plan'},
         'plain': {'value': round(pv, 1), 'ms_per_step': round(pms, 3),
                   'grid_ms': round(pg, 3), 'degrid_ms': round(pd, 3),
                   'kernel_bank_mib': round(8 * bank.kern.numel() / 2 ** 20, 1),

@@ -56,6 +56,11 @@ SIGNATURES = {
     'croc_convolve_same': [_P, _P, _I, _I, _P, _I, _I, _I64, _P],
     'croc_wtowers_grid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I],
     'croc_wtowers_degrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I],
+    'croc_wtowers_plan_create': [_P, ctypes.POINTER(_P)],
+    'croc_wtowers_plan_destroy': [_P, _P],
+    'croc_wtowers_plan_bin': [_P, _P, _I, _P, _I64, _I64, _D, _D, _I, _I, _I, _I, _I],
+    'croc_wtowers_grid_plan': [_P, _P, _P, _P, _P, _I],
+    'croc_wtowers_degrid_plan': [_P, _P, _P, _P, _P, _I],
 }
 
 _lib = None

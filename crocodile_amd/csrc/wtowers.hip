@@ -510,6 +510,7 @@ __global__ void k_tw_phi(double *phi, double theta, double wstep) {
 
 struct Plan {
   TowerArgs a;
+  WsSet *keep = nullptr;  // where rec, svis and jobs live (null: the context's set)
   uint4 *loc = nullptr, *rec = nullptr;
   float2 *svis = nullptr;
   uint32_t *small = nullptr;
@@ -563,10 +564,11 @@ int tw_bin(croc_ctx *ctx, Plan &p, const float2 *vis) {
   CROC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, counts, starts, (int)n, s));
   const size_t t_bytes = std::max(t_sort, std::max(t_rle, t_scan));
   char *w = (char *)ws_get(ctx, WS_SORT, sizeof(uint32_t) * (7 * n + 64) + t_bytes + 256);
+  WsSet &keep = p.keep ? *p.keep : ctx->ws;
   p.loc = (uint4 *)ws_get(ctx, WS_LOC, sizeof(uint4) * a.nv);
-  p.rec = (uint4 *)ws_get(ctx, WS_REC, sizeof(uint4) * a.nv);
-  if (vis) p.svis = (float2 *)ws_get(ctx, WS_SVIS, sizeof(float2) * a.nv);
-  p.jobs = (int4 *)ws_get(ctx, WS_ITEMS, sizeof(int4) * (a.nv + 1));
+  p.rec = (uint4 *)ws_get_in(ctx, keep, WS_REC, sizeof(uint4) * a.nv);
+  if (vis) p.svis = (float2 *)ws_get_in(ctx, keep, WS_SVIS, sizeof(float2) * a.nv);
+  p.jobs = (int4 *)ws_get_in(ctx, keep, WS_ITEMS, sizeof(int4) * (a.nv + 1));
   if (!w || !p.loc || !p.rec || (vis && !p.svis) || !p.jobs) return CROC_ENOMEM;
   keys = (uint32_t *)w;
   skeys = keys + n;
@@ -611,14 +613,14 @@ int tw_bin(croc_ctx *ctx, Plan &p, const float2 *vis) {
 }
 
 int tw_setup(croc_ctx *ctx, Plan &p, int N, const double *uvw, int64_t nv, int64_t stride,
-             double theta, double wstep, int subgrid, int margin, const void *kern, int Q,
+             double theta, double wstep, int subgrid, int margin, bool has_kernel, int Q,
              int S) {
   CROC_REQUIRE(subgrid == SG, CROC_EINVAL, "subgrid size must be %d", SG);
   CROC_REQUIRE(margin >= 0 && margin % 2 == 0 && margin <= subgrid / 2, CROC_EINVAL,
                "margin must be even and in [0, subgrid/2]");
   CROC_REQUIRE(N >= 2 && theta > 0 && wstep > 0 && stride >= 3, CROC_EINVAL, "bad arguments");
   CROC_REQUIRE(nv < (int64_t)INT32_MAX, CROC_EINVAL, "too many visibilities");
-  CROC_REQUIRE(kern == nullptr || (Q >= 1 && Q <= 65535 && S >= 1 && S <= margin),
+  CROC_REQUIRE(!has_kernel || (Q >= 1 && Q <= 65535 && S >= 1 && S <= margin),
                CROC_EINVAL, "kernel support %d must not exceed the margin %d", S, margin);
   TowerArgs &a = p.a;
   a.N = N;
@@ -628,9 +630,9 @@ int tw_setup(croc_ctx *ctx, Plan &p, int N, const double *uvw, int64_t nv, int64
   a.ncx = N / a.chunk + 1;
   a.theta = theta;
   a.wstep = wstep;
-  a.Q = kern ? Q : 1;
-  a.S = kern ? S : 1;
-  a.nearest = kern == nullptr;
+  a.Q = has_kernel ? Q : 1;
+  a.S = has_kernel ? S : 1;
+  a.nearest = !has_kernel;
   a.uvw = uvw;
   a.nv = nv;
   a.stride = stride;
@@ -644,6 +646,10 @@ int upload_i32(croc_ctx *ctx, int32_t *dst, const std::vector<int32_t> &v) {
   return CROC_OK;
 }
 
+int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int separable);
+int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
+                  const void *kern, int separable);
+
 }  // namespace
 
 extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double *uvw,
@@ -656,9 +662,18 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
   if (nv == 0) return CROC_OK;
   CROC_REQUIRE(uvw && vis, CROC_EINVAL, "null uvw / vis");
   Plan p;
-  if ((rc = tw_setup(ctx, p, N, uvw, nv, stride, theta, wstep, subgrid, margin, kern, Q, S)))
+  if ((rc = tw_setup(ctx, p, N, uvw, nv, stride, theta, wstep, subgrid, margin,
+                     kern != nullptr, Q, S)))
     return rc;
   if ((rc = tw_bin(ctx, p, (const float2 *)vis))) return rc;
+  return tw_grid_run(ctx, p, grid, kern, separable);
+}
+
+namespace {
+
+// Imaging from binned jobs (p.svis holds the visibilities in record order).
+int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int separable) {
+  int rc;
   if (p.njobs == 0) return CROC_OK;
   const TowerArgs &a = p.a;
   hipStream_t s = ctx->stream;
@@ -706,7 +721,8 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
   }
 
   // fixed-point scale, Fresnel phase table, per-job plane multipliers
-  const int64_t nk = kern ? (separable ? (int64_t)Q * S : (int64_t)Q * Q * S * S) : 0;
+  const int64_t Q = a.Q, S = a.S;
+  const int64_t nk = kern ? (separable ? Q * S : Q * Q * S * S) : 0;
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
                                                 sizeof(int32_t) * (5 * (size_t)p.njobs + 64) +
                                                 sizeof(uint32_t) * (batches.size() + 1));
@@ -739,9 +755,8 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
     CROC_HIP(hipMemcpyAsync(dmulti, multi.data(), sizeof(int2) * multi.size(),
                             hipMemcpyHostToDevice, s));
   if ((rc = zero_u32_async(ctx, (uint32_t *)fs, sizeof(TwScale) / 4))) return rc;
-  k_tw_maxima<<<ctx->num_cu, 1024, 0, s>>>((const float2 *)vis, nv, (const float2 *)kern,
-                                              nk, fs);
-  k_tw_pick<<<1, 1, 0, s>>>(fs, nv, a.nearest, separable);
+  k_tw_maxima<<<ctx->num_cu, 1024, 0, s>>>(p.svis, a.nv, (const float2 *)kern, nk, fs);
+  k_tw_pick<<<1, 1, 0, s>>>(fs, a.nv, a.nearest, separable);
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
   CROC_HIP(hipGetLastError());
   // zero the chunk images that will receive data
@@ -799,6 +814,8 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
   return CROC_OK;
 }
 
+}  // namespace
+
 extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const double *uvw,
                                    int64_t nv, int64_t stride, double theta, void *vis_out,
                                    double wstep, int subgrid, int margin, const void *kern,
@@ -809,9 +826,19 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   if (nv == 0) return CROC_OK;
   CROC_REQUIRE(uvw && vis_out, CROC_EINVAL, "null uvw / vis_out");
   Plan p;
-  if ((rc = tw_setup(ctx, p, N, uvw, nv, stride, theta, wstep, subgrid, margin, kern, Q, S)))
+  if ((rc = tw_setup(ctx, p, N, uvw, nv, stride, theta, wstep, subgrid, margin,
+                     kern != nullptr, Q, S)))
     return rc;
   if ((rc = tw_bin(ctx, p, nullptr))) return rc;
+  return tw_degrid_run(ctx, p, grid, vis_out, kern, separable);
+}
+
+namespace {
+
+// Predict from binned jobs.
+int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
+                  const void *kern, int separable) {
+  int rc;
   if (p.njobs == 0) return CROC_OK;
   const TowerArgs &a = p.a;
   hipStream_t s = ctx->stream;
@@ -883,4 +910,108 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   ctx->ev_degrid = true;
   CROC_HIP(hipStreamSynchronize(s));
   return CROC_OK;
+}
+
+}  // namespace
+
+// W-towers plans: the (chunk, w-plane) binning of one set of coordinates,
+// reused by any number of imaging and predict calls on them (the one-shot
+// calls bin on every call).  croc_wtowers_plan_bin validates every
+// coordinate, as the one-shot calls do.
+struct croc_tw_plan {
+  int device = 0;
+  WsSet ws;  // records, visibilities in record order, jobs
+  Plan p;
+  bool ready = false;
+};
+
+namespace {
+
+// svis[pos] = vis[rec[pos].w]: the visibilities in the plan's record order.
+__global__ void k_tw_vis_order(int64_t nv, const uint4 *__restrict__ rec,
+                               const float2 *__restrict__ vis, float2 *__restrict__ svis) {
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos < nv) svis[pos] = vis[rec[pos].w];
+}
+
+int tw_plan_check(croc_ctx *ctx, const croc_tw_plan *plan, const void *kern) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  CROC_REQUIRE(plan != nullptr, CROC_EINVAL, "null plan");
+  CROC_REQUIRE(plan->ready, CROC_EINVAL, "plan holds no successful binning");
+  CROC_REQUIRE(plan->device == ctx->device, CROC_EINVAL, "plan belongs to another device");
+  CROC_REQUIRE((kern != nullptr) == !plan->p.a.nearest, CROC_EINVAL,
+               plan->p.a.nearest ? "plan was binned without a kernel"
+                                 : "plan was binned for a kernel, none given");
+  return CROC_OK;
+}
+
+}  // namespace
+
+extern "C" int croc_wtowers_plan_create(croc_ctx *ctx, croc_tw_plan **out) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  CROC_REQUIRE(out != nullptr, CROC_EINVAL, "null output pointer");
+  croc_tw_plan *p = new croc_tw_plan();
+  p->device = ctx->device;
+  *out = p;
+  return CROC_OK;
+}
+
+extern "C" int croc_wtowers_plan_destroy(croc_ctx *ctx, croc_tw_plan *plan) {
+  if (!plan) return CROC_OK;
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  (void)hipStreamSynchronize(ctx->stream);
+  ws_free(plan->ws);
+  delete plan;
+  return CROC_OK;
+}
+
+extern "C" int croc_wtowers_plan_bin(croc_ctx *ctx, croc_tw_plan *plan, int N,
+                                     const double *uvw, int64_t nv, int64_t stride,
+                                     double theta, double wstep, int subgrid, int margin,
+                                     int has_kernel, int Q, int S) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  CROC_REQUIRE(plan != nullptr, CROC_EINVAL, "null plan");
+  CROC_REQUIRE(plan->device == ctx->device, CROC_EINVAL, "plan belongs to another device");
+  CROC_REQUIRE(nv >= 0, CROC_EINVAL, "negative visibility count");
+  CROC_REQUIRE(nv == 0 || uvw, CROC_EINVAL, "null uvw");
+  plan->ready = false;
+  Plan &p = plan->p;
+  p = Plan();
+  p.keep = &plan->ws;
+  if ((rc = tw_setup(ctx, p, N, uvw, nv, stride, theta, wstep, subgrid, margin,
+                     has_kernel != 0, Q, S)))
+    return rc;
+  if (nv > 0 && (rc = tw_bin(ctx, p, nullptr))) return rc;
+  plan->ready = true;
+  return CROC_OK;
+}
+
+extern "C" int croc_wtowers_grid_plan(croc_ctx *ctx, croc_tw_plan *plan, void *grid,
+                                      const void *vis, const void *kern, int separable) {
+  int rc = tw_plan_check(ctx, plan, kern);
+  if (rc) return rc;
+  CROC_REQUIRE(grid, CROC_EINVAL, "null grid");
+  Plan &p = plan->p;
+  if (p.njobs == 0) return CROC_OK;
+  CROC_REQUIRE(vis, CROC_EINVAL, "null vis");
+  p.svis = (float2 *)ws_get_in(ctx, plan->ws, WS_SVIS, sizeof(float2) * p.a.nv);
+  if (!p.svis) return CROC_ENOMEM;
+  k_tw_vis_order<<<(unsigned)((p.a.nv + 255) / 256), 256, 0, ctx->stream>>>(
+      p.a.nv, p.rec, (const float2 *)vis, p.svis);
+  CROC_HIP(hipGetLastError());
+  return tw_grid_run(ctx, p, grid, kern, separable);
+}
+
+extern "C" int croc_wtowers_degrid_plan(croc_ctx *ctx, croc_tw_plan *plan, const void *grid,
+                                        void *vis_out, const void *kern, int separable) {
+  int rc = tw_plan_check(ctx, plan, kern);
+  if (rc) return rc;
+  CROC_REQUIRE(grid, CROC_EINVAL, "null grid");
+  if (plan->p.njobs == 0) return CROC_OK;
+  CROC_REQUIRE(vis_out, CROC_EINVAL, "null vis_out");
+  return tw_degrid_run(ctx, plan->p, grid, vis_out, kern, separable);
 }

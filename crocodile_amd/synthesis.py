@@ -1026,3 +1026,72 @@ def w_towers_predict(theta, lam, uvw, src, guv, wstep=100.0, subgrid_size=128, m
         _lib.ctx(), _lib.ptr(gt), N, _lib.ptr(uv), uv.shape[0], uv.shape[1], float(theta),
         _lib.ptr(out), float(wstep), int(subgrid_size), int(margin), _lib.ptr(kern), Q, S, sep))
     return _out(out, _torch_in(uvw, guv))
+
+
+class WTowersPlan:
+    """(chunk, w-plane) binning of one set of coordinates, reused by repeated
+    w-towers imaging and predict of the same visibilities (the C ABI's
+    croc_tw_plan; :func:`w_towers_imaging` / :func:`w_towers_predict` bin on
+    every call).  The arguments are theirs; the kernel ``kv`` may change
+    between calls but keeps the shape it had at binning (or stays None).
+    bin() validates every coordinate as those calls do."""
+
+    def __init__(self, theta, lam, uvw, wstep=100.0, subgrid_size=128, margin=32, kv=None):
+        self.theta, self.wstep = float(theta), float(wstep)
+        self.subgrid_size, self.margin = int(subgrid_size), int(margin)
+        self.N = _grid_size(theta, lam)
+        self.nv = 0
+        h = _lib._P()
+        _lib.check(_lib.load().croc_wtowers_plan_create(_lib.ctx(), ctypes.byref(h)))
+        self._h = h
+        self.bin(uvw, kv)
+
+    def bin(self, uvw, kv=None):
+        """Bin the coordinates ``uvw`` (wavelengths, with a w column)."""
+        uv = _uvw(uvw)
+        if uv.shape[1] < 3:
+            raise ValueError('w-towers need uvw with a w column')
+        kern, Q, S, _ = _tower_kernel(kv)
+        self.nv = 0
+        _lib.check(_lib.load().croc_wtowers_plan_bin(
+            _lib.ctx(), self._h, self.N, _lib.ptr(uv), uv.shape[0], uv.shape[1], self.theta,
+            self.wstep, self.subgrid_size, self.margin, int(kern is not None), Q, S))
+        self._shape = (kern is not None, Q, S)
+        self.nv, self._device = int(uv.shape[0]), uv.device
+
+    def _kernel(self, kv):
+        kern, Q, S, sep = _tower_kernel(kv)
+        if (kern is not None, Q, S) != self._shape:
+            raise ValueError('kernel shape differs from the planned one')
+        return kern, sep
+
+    def imaging(self, vis, kv=None):
+        """The grid of w_towers_imaging for the binned coordinates."""
+        kern, sep = self._kernel(kv)
+        vt = _vis_dev(vis)
+        if vt.shape[0] != self.nv:
+            raise ValueError('visibility count differs from the plan')
+        guv = torch.zeros((self.N, self.N), dtype=torch.complex64, device=self._device)
+        _lib.check(_lib.load().croc_wtowers_grid_plan(
+            _lib.ctx(), self._h, _lib.ptr(guv), _lib.ptr(vt), _lib.ptr(kern), sep))
+        return _out(guv, _torch_in(vis))
+
+    def predict(self, guv, kv=None):
+        """The visibilities of w_towers_predict for the binned coordinates."""
+        kern, sep = self._kernel(kv)
+        gt = _dev(guv, torch.complex64)
+        if _square(gt) != self.N:
+            raise ValueError('grid size does not match round(theta * lam)')
+        out = torch.empty(self.nv, dtype=torch.complex64, device=self._device)
+        _lib.check(_lib.load().croc_wtowers_degrid_plan(
+            _lib.ctx(), self._h, _lib.ptr(gt), _lib.ptr(out), _lib.ptr(kern), sep))
+        return _out(out, _torch_in(guv))
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and _lib._lib is not None:
+            try:
+                _lib.load().croc_wtowers_plan_destroy(_lib.ctx(), h)
+            except Exception:
+                pass
+            self._h = None

@@ -217,6 +217,29 @@ int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N,
                         int margin, const void *kern, int Q, int S,
                         int separable);
 
+/* W-towers plans: the (chunk, w-plane) binning of one set of coordinates,
+ * reused by any number of croc_wtowers_grid_plan / croc_wtowers_degrid_plan
+ * calls (the one-shot calls above bin on every call).  No counterpart in the
+ * reference, whose notebooks bin inside each imaging / predict cell: an
+ * extension for callers that image and predict the same visibilities
+ * repeatedly.  croc_wtowers_plan_bin takes the coordinate and geometry
+ * arguments of the one-shot calls, has_kernel (their kern != NULL) and the
+ * kernel shape Q, S; it validates every coordinate.  The plan calls then
+ * take kernel values of that shape (kern NULL exactly when has_kernel was
+ * 0).  uvw may change after binning.  Calls with one plan must not overlap. */
+typedef struct croc_tw_plan croc_tw_plan;
+int croc_wtowers_plan_create(croc_ctx *ctx, croc_tw_plan **out);
+int croc_wtowers_plan_destroy(croc_ctx *ctx, croc_tw_plan *plan);
+int croc_wtowers_plan_bin(croc_ctx *ctx, croc_tw_plan *plan, int N,
+                          const double *uvw, int64_t nv, int64_t stride,
+                          double theta, double wstep, int subgrid, int margin,
+                          int has_kernel, int Q, int S);
+int croc_wtowers_grid_plan(croc_ctx *ctx, croc_tw_plan *plan, void *grid,
+                           const void *vis, const void *kern, int separable);
+int croc_wtowers_degrid_plan(croc_ctx *ctx, croc_tw_plan *plan,
+                             const void *grid, void *vis_out,
+                             const void *kern, int separable);
+
 /* 2D convolution, scipy.signal.convolve2d(a, b[n], mode='same') for nb
  * kernels b sharing one a (cf64, [ah][aw] and [nb][bh][bw]; out
  * [nb][ah][aw]), FP64 direct sums.  The building block of aw_kernel_fn,

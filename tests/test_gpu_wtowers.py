@@ -2,8 +2,10 @@
 (oracle/wtowers_ref.py), nearest-cell and convolutional sub-grid kernels."""
 import numpy
 import pytest
+import torch
 
 import crocodile_amd.synthesis as G
+from crocodile_amd import _lib
 from oracle import synthesis_ref as R
 from oracle import wtowers_ref as WT
 
@@ -92,3 +94,54 @@ def test_wtowers_job_longer_than_a_part_window(gpu):
     g = G.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, kv)
     ref = G.conv_imaging(THETA, LAM, uvw, None, vis, kv)
     assert_close(g, ref, what='w-towers long job vs conv_imaging')
+
+
+@pytest.mark.parametrize('kind', ['nearest', 'separable', '2d'])
+def test_wtowers_plan_matches_one_shot(gpu, kind):
+    """A plan bins once; its imaging and predict calls equal the one-shot
+    calls bit for bit (the same records, in the same order), on repeated
+    calls, after re-binning other coordinates and with new kernel values of
+    the planned shape."""
+    rng, uvw, vis = case(6, 5000, 400.0)
+    kv = kernels(rng)[kind]
+    grid = rng.standard_normal((512, 512)) + 1j * rng.standard_normal((512, 512))
+    plan = G.WTowersPlan(THETA, LAM, uvw, 40.0, SG, MARGIN, kv)
+    one_g = G.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, kv)
+    one_v = G.w_towers_predict(THETA, LAM, uvw, None, grid, 40.0, SG, MARGIN, kv)
+    for _ in range(2):
+        assert (plan.imaging(vis, kv) == one_g).all()
+        assert (plan.predict(grid, kv) == one_v).all()
+    assert_close(plan.imaging(vis, kv),
+                 WT.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, kv),
+                 what='planned w_towers_imaging ' + kind)
+    _, uvw2, vis2 = case(7, 3000, 300.0)
+    kv2 = None if kv is None else kv * (0.5 + 0.25j)
+    plan.bin(uvw2, kv2)
+    assert (plan.imaging(vis2, kv2) ==
+            G.w_towers_imaging(THETA, LAM, uvw2, None, vis2, 40.0, SG, MARGIN, kv2)).all()
+    assert (plan.predict(grid, kv2) ==
+            G.w_towers_predict(THETA, LAM, uvw2, None, grid, 40.0, SG, MARGIN, kv2)).all()
+
+
+def test_wtowers_plan_errors_and_empty(gpu):
+    rng, uvw, vis = case(8, 1000, 100.0)
+    kv = kernels(rng)['2d']
+    plan = G.WTowersPlan(THETA, LAM, uvw, 40.0, SG, MARGIN, kv)
+    with pytest.raises(ValueError):
+        plan.imaging(vis, None)            # binned for a kernel
+    with pytest.raises(ValueError):
+        plan.imaging(vis[:10], kv)         # visibility count
+    with pytest.raises(ValueError):        # the C ABI refuses a missing kernel too
+        _lib.check(_lib.load().croc_wtowers_grid_plan(
+            _lib.ctx(), plan._h, _lib.ptr(torch.zeros((512, 512), dtype=torch.complex64,
+                                                      device='cuda')),
+            _lib.ptr(torch.zeros(1000, dtype=torch.complex64, device='cuda')), None, 0))
+    bad = uvw.copy()
+    bad[3, 0] = 2.0 * LAM                  # beyond the last chunk
+    with pytest.raises((ValueError, AssertionError)):
+        plan.bin(bad, kv)
+    with pytest.raises(ValueError):        # a failed binning leaves nothing to use
+        plan.imaging(vis, kv)
+    empty = G.WTowersPlan(THETA, LAM, numpy.zeros((0, 3)), 40.0, SG, MARGIN, None)
+    assert not empty.imaging(numpy.zeros(0, complex)).any()
+    assert empty.predict(numpy.ones((512, 512), complex)).shape == (0,)
