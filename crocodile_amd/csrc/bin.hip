@@ -288,12 +288,12 @@ int bin_visibilities(croc_ctx *ctx, WsSet &set, const BinArgs &a, int max_suppor
   p.status = small;
   p.nitems = small + 1;
   p.nitems_g = small + 2;
-  p.qhead = small + 8;  // XCD_QUEUES queue heads
+  p.qhead = small + 8;  // XCD_QUEUES per-XCD queue heads, then the shared one
 
   hipStream_t s = ctx->stream;
   int rc = zero_u32_async(ctx, p.count, p.nbins + 1);
   if (rc) return rc;
-  rc = zero_u32_async(ctx, small, 16);
+  rc = zero_u32_async(ctx, small, 24);
   if (rc) return rc;
   const BinGeom g{T, p.ntx, p.ngroups, p.group_elems};
   const int nblk = (int)((a.nv + BIN_VPB - 1) / BIN_VPB);

@@ -39,6 +39,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R == 63 ? 8 
       if (tid == 0) s_item = lo + (int)atomicAdd(qhead + x, 1u);
       __syncthreads();
       const int it = s_item;
+      __syncthreads();  // every wave has read s_item before the next queue rewrites it
       if (it >= hi) break;
       const int4 item = items[it];
       const int tile = __builtin_amdgcn_readfirstlane(item.x);

@@ -73,75 +73,127 @@ __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
 }
 
 
+// One work item: the item's visibilities spread into the LDS region, which
+// is then flushed (touched cells only) and cleared for the next item.
+template <int NT, bool SEPARABLE, int R>
+__device__ __forceinline__ void grid_item(unsigned long long *sacc, const int4 item,
+                                          unsigned long long *__restrict__ acc, int N, int T,
+                                          int ntx, const uint4 *__restrict__ rec,
+                                          const float2 *__restrict__ vis,
+                                          const float2 *__restrict__ kern, double scale) {
+  constexpr int CELLS = R * R;
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tile = __builtin_amdgcn_readfirstlane(item.x);
+  const int tx0 = (tile % ntx) * T, ty0 = (tile / ntx) * T;
+
+  // Each wave walks its visibilities j = first + wave, + NW, ..., records
+  // and values fetched 64 at a time (lane l: the l-th of the batch; the
+  // value is gathered from the caller's array by the record's index).
+  const int jend = __builtin_amdgcn_readfirstlane(item.z);
+  const int jfirst = __builtin_amdgcn_readfirstlane(item.y) + wave;
+  for (int jb = jfirst; jb < jend; jb += 64 * NW) {
+    const int jl = jb + lane * NW;
+    uint4 rb = make_uint4(0, 0, 0, 0);
+    float2 vb = make_float2(0.f, 0.f);
+    if (jl < jend) {
+      rb = rec[jl];
+      vb = vis[rb.w];
+    }
+    const int nb = min(64, (jend - jb + NW - 1) / NW);
+    for (int it = 0; it < nb; it++) {
+      const uint4 r = lane_u4(rb, it);
+      const float2 v = lane_f2(vb, it);
+      const int S = (int)(r.y & 0xffu);
+      const int xf = (int)((r.y >> 8) & 0xfffu), yf = (int)(r.y >> 20);
+      const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
+      const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
+      const float2 *ky = kp + yf * S, *kx = kp + xf * S;
+      spread_range<SEPARABLE>(sacc, base, R, S, 0, S * S, lane, kp, ky, kx,
+                              (double)v.x * scale, (double)v.y * scale);
+    }
+  }
+  barrier_after_lds_atomics();
+  // Flush the region (touched cells only) and clear it for the next item.
+  for (int c = tid; c < CELLS; c += NT) {
+    const unsigned long long re = sacc[2 * c], im = sacc[2 * c + 1];
+    if (re | im) {
+      const int cy = c / R, cx = c - cy * R;
+      const int gy = ty0 + cy, gx = tx0 + cx;
+      if (gy < N && gx < N) {
+        unsigned long long *g = acc + 2 * ((int64_t)gy * N + gx);
+        if (re) atomicAdd(g, re);
+        if (im) atomicAdd(g + 1, im);
+      }
+      sacc[2 * c] = 0ull;
+      sacc[2 * c + 1] = 0ull;
+    }
+  }
+  __syncthreads();
+}
+
+// Persistent work queues, in two phases.  Phase A (plans, whose per-XCD item
+// lists exist, with plane groups): the items of tiles holding at least
+// dense_min visibilities, cut per plane-group range, each XCD draining the
+// queue of its own range first (its L2 then serves one slice of the kernel
+// bank, as in degridding: with uniform w every tile touches every plane, so
+// whole-tile items read the whole bank through every XCD's L2).  Dense tiles
+// carry most of the visibilities, and their range items are long enough to
+// pay for their extra region flushes.  Phase B: every other tile's
+// tile-major items, from one shared queue (qhead[XCD_QUEUES]).
 template <int NT, bool SEPARABLE, int R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_grid_tiles(
     unsigned long long *__restrict__ acc, int N, int T, int ntx,
     const uint4 *__restrict__ rec, const float2 *__restrict__ vis,
     const float2 *__restrict__ kern, const int4 *__restrict__ items,
-    const uint32_t *__restrict__ nitems, uint32_t *__restrict__ qhead,
-    const ScaleState *__restrict__ st) {
+    const uint32_t *__restrict__ nitems, const int4 *__restrict__ items_g,
+    const uint32_t *__restrict__ gstart, int ngroups, const uint32_t *__restrict__ offset,
+    int dense_min, uint32_t *__restrict__ qhead, const ScaleState *__restrict__ st) {
   constexpr int CELLS = R * R;
-  constexpr int NW = NT / 64;
   __shared__ unsigned long long sacc[2 * CELLS];  // re, im interleaved
   __shared__ int s_item;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nit = (int)*nitems;
+  const int tid = threadIdx.x;
   const double scale = st->scale;
+  const bool split = items_g != nullptr && ngroups > 1 && dense_min > 0;
 
   for (int c = tid; c < 2 * CELLS; c += NT) sacc[c] = 0ull;
+  if (split) {
+    const int nq = min(XCD_QUEUES, ngroups);
+    const int x0 = xcd_id() % nq;
+    for (int k = 0; k < nq; k++) {
+      const int x = (x0 + k) % nq;
+      const int lo = (int)gstart[(int64_t)x * ngroups / nq];
+      const int hi = (int)gstart[(int64_t)(x + 1) * ngroups / nq];
+      // Queue x starts x / nq of the way into its tile-major list, so that
+      // the XCDs do not flush the same tiles' regions at the same time.
+      const int len = hi - lo, rot = (int)((int64_t)len * x / nq);
+      for (;;) {
+        if (tid == 0) s_item = (int)atomicAdd(qhead + x, 1u);
+        __syncthreads();
+        const int q = s_item;
+        __syncthreads();  // every wave has read s_item before it is rewritten
+        if (q >= len) break;
+        const int it = lo + (q + rot < len ? q + rot : q + rot - len);
+        const int4 item = items_g[it];
+        const int t = item.x;
+        if (offset[(t + 1) * ngroups] - offset[t * ngroups] < (uint32_t)dense_min) continue;
+        grid_item<NT, SEPARABLE, R>(sacc, item, acc, N, T, ntx, rec, vis, kern, scale);
+      }
+    }
+  }
+  const int nit = (int)*nitems;
   for (;;) {
-    if (tid == 0) s_item = (int)atomicAdd(qhead, 1u);
+    if (tid == 0) s_item = (int)atomicAdd(qhead + XCD_QUEUES, 1u);
     __syncthreads();
     const int it = s_item;
+    __syncthreads();
     if (it >= nit) break;
     const int4 item = items[it];
-    const int tile = __builtin_amdgcn_readfirstlane(item.x);
-    const int tx0 = (tile % ntx) * T, ty0 = (tile / ntx) * T;
-
-    // Each wave walks its visibilities j = first + wave, + NW, ..., records
-    // and values fetched 64 at a time (lane l: the l-th of the batch; the
-    // value is gathered from the caller's array by the record's index).
-    const int jend = __builtin_amdgcn_readfirstlane(item.z);
-    const int jfirst = __builtin_amdgcn_readfirstlane(item.y) + wave;
-    for (int jb = jfirst; jb < jend; jb += 64 * NW) {
-      const int jl = jb + lane * NW;
-      uint4 rb = make_uint4(0, 0, 0, 0);
-      float2 vb = make_float2(0.f, 0.f);
-      if (jl < jend) {
-        rb = rec[jl];
-        vb = vis[rb.w];
-      }
-      const int nb = min(64, (jend - jb + NW - 1) / NW);
-      for (int it = 0; it < nb; it++) {
-        const uint4 r = lane_u4(rb, it);
-        const float2 v = lane_f2(vb, it);
-        const int S = (int)(r.y & 0xffu);
-        const int xf = (int)((r.y >> 8) & 0xfffu), yf = (int)(r.y >> 20);
-        const int base = ((int)(r.x >> 16) - ty0) * R + ((int)(r.x & 0xffffu) - tx0);
-        const float2 *kp = kern + r.z;  // 2D table of (yf, xf) / separable [Q][S]
-        const float2 *ky = kp + yf * S, *kx = kp + xf * S;
-        spread_range<SEPARABLE>(sacc, base, R, S, 0, S * S, lane, kp, ky, kx,
-                                (double)v.x * scale, (double)v.y * scale);
-      }
-    }
-    barrier_after_lds_atomics();
-    // Flush the region (touched cells only) and clear it for the next item.
-    for (int c = tid; c < CELLS; c += NT) {
-      const unsigned long long re = sacc[2 * c], im = sacc[2 * c + 1];
-      if (re | im) {
-        const int cy = c / R, cx = c - cy * R;
-        const int gy = ty0 + cy, gx = tx0 + cx;
-        if (gy < N && gx < N) {
-          unsigned long long *g = acc + 2 * ((int64_t)gy * N + gx);
-          if (re) atomicAdd(g, re);
-          if (im) atomicAdd(g + 1, im);
-        }
-        sacc[2 * c] = 0ull;
-        sacc[2 * c + 1] = 0ull;
-      }
-    }
-    __syncthreads();
+    const int t = item.x;
+    if (split && offset[(t + 1) * ngroups] - offset[t * ngroups] >= (uint32_t)dense_min)
+      continue;
+    grid_item<NT, SEPARABLE, R>(sacc, item, acc, N, T, ntx, rec, vis, kern, scale);
   }
 }
 
@@ -184,24 +236,29 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, void *grid, int N,
   k_scale_pick<<<1, 1, 0, ctx->stream>>>(st, nv, separable);
   CROC_HIP(hipGetLastError());
 
+  // Tiles of at least one chunk are split by plane-group range (measured on
+  // wproj2048: grid 1.42 -> 1.24 ms; 1024 and 2048 visibilities 1.25 and
+  // 1.33 ms, 256 1.62 ms, where the range items get too short for their
+  // region flushes).
+  const int dense_min = plan.chunk;
   CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   const float2 *kt = (const float2 *)kern;
   if (plan.R == TILE_R_SMALL) {
     const int G = persistent_blocks(ctx, 2);  // 63.5 KiB of LDS each
     if (separable)
       k_grid_tiles<GRID_NT, true, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.items_g, plan.gstart, plan.ngroups, plan.offset, dense_min, plan.qhead, st);
     else
       k_grid_tiles<GRID_NT, false, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.items_g, plan.gstart, plan.ngroups, plan.offset, dense_min, plan.qhead, st);
   } else {
     const int G = persistent_blocks(ctx, 1);
     if (separable)
       k_grid_tiles<GRID_NT, true, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.items_g, plan.gstart, plan.ngroups, plan.offset, dense_min, plan.qhead, st);
     else
       k_grid_tiles<GRID_NT, false, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.items_g, plan.gstart, plan.ngroups, plan.offset, dense_min, plan.qhead, st);
   }
   CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));

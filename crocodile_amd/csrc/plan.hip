@@ -96,7 +96,7 @@ extern "C" int croc_convgrid_plan(croc_ctx *ctx, croc_plan *plan, void *grid,
   CROC_REQUIRE(grid && kern, CROC_EINVAL, "null array argument");
   if (plan->nv == 0) return CROC_OK;
   CROC_REQUIRE(vis != nullptr, CROC_EINVAL, "null vis");
-  rc = zero_u32_async(ctx, plan->bp.qhead, XCD_QUEUES);  // work queue restarts
+  rc = zero_u32_async(ctx, plan->bp.qhead, XCD_QUEUES + 1);  // work queues restart
   if (rc) return rc;
   return grid_binned(ctx, plan->bp, grid, plan->N, plan->nv, vis, kern, plan->Q,
                      plan->separable, plan->nplanes, plan->plane_off, plan->plane_s);
@@ -109,7 +109,7 @@ extern "C" int croc_convdegrid_plan(croc_ctx *ctx, croc_plan *plan, const void *
   CROC_REQUIRE(grid && kern, CROC_EINVAL, "null array argument");
   if (plan->nv == 0) return CROC_OK;
   CROC_REQUIRE(vis_out != nullptr, CROC_EINVAL, "null vis_out");
-  rc = zero_u32_async(ctx, plan->bp.qhead, XCD_QUEUES);  // work queue restarts
+  rc = zero_u32_async(ctx, plan->bp.qhead, XCD_QUEUES + 1);  // work queues restart
   if (rc) return rc;
   return degrid_binned(ctx, plan->bp, grid, plan->N, vis_out, kern, plan->separable);
 }
