@@ -292,7 +292,8 @@ extern "C" int croc_convgrid(croc_ctx *ctx, void *grid, int N,
   BinArgs a{N, uvw, nv, stride, lam, Q, nplanes, plane_s, plane_off, separable,
             plane_mode, vplane, wstep, wbin0, max_support};
   BinPlan plan;
-  rc = bin_visibilities(ctx, ctx->ws, a, max_support, ITEMS_BY_TILE, &plan);
+  // both lists: the per-XCD one carries the dense tiles (see k_grid_tiles)
+  rc = bin_visibilities(ctx, ctx->ws, a, max_support, ITEMS_BY_TILE | ITEMS_BY_GROUP, &plan);
   if (rc) return rc;
   rc = grid_binned(ctx, plan, grid, N, nv, vis, kern, Q, separable, nplanes,
                    plane_off, plane_s);
