@@ -31,10 +31,12 @@ def kernels(rng):
     sep = R.kernel_oversample(aaf, 16, 8)
     sep /= sep[0].sum()
     k2 = rng.standard_normal((4, 4, 6, 6)) + 1j * rng.standard_normal((4, 4, 6, 6))
-    return {'nearest': None, 'separable': sep, '2d': k2}
+    # S = 16, the support of the AW workloads (its own degridding walk)
+    k16 = rng.standard_normal((4, 4, 16, 16)) + 1j * rng.standard_normal((4, 4, 16, 16))
+    return {'nearest': None, 'separable': sep, '2d': k2, '2d16': k16}
 
 
-@pytest.mark.parametrize('kind', ['nearest', 'separable', '2d'])
+@pytest.mark.parametrize('kind', ['nearest', 'separable', '2d', '2d16'])
 def test_wtowers_imaging(gpu, kind):
     rng, uvw, vis = case(1, 6000, 400.0)
     kv = kernels(rng)[kind]
@@ -43,7 +45,7 @@ def test_wtowers_imaging(gpu, kind):
     assert_close(g, ref, what='w_towers_imaging ' + kind)
 
 
-@pytest.mark.parametrize('kind', ['nearest', 'separable', '2d'])
+@pytest.mark.parametrize('kind', ['nearest', 'separable', '2d', '2d16'])
 def test_wtowers_predict(gpu, kind):
     rng, uvw, vis = case(2, 6000, 400.0)
     kv = kernels(rng)[kind]
