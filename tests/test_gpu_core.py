@@ -379,6 +379,34 @@ def test_plane_groups_do_not_change_results(gpu):
         assert (g == grids[0]).all()
 
 
+def test_dense_tile_split_is_exact(gpu):
+    """Tiles holding at least one chunk of visibilities are gridded per
+    plane-group range from per-XCD queues, the others whole from a shared
+    queue.  With a 64-visibility chunk nearly every tile takes the split
+    path, with 4096 none does; with one plane group nothing is split.  Every
+    grid is bit-identical, and equal to the oracle's."""
+    rng = numpy.random.default_rng(21)
+    N, S, Q, nv, planes = 256, 12, 4, 30000, 12
+    tables, p, v = random_case(rng, N, S, Q, nv, planes)
+    vplane = rng.integers(0, planes, nv).astype(numpy.int32)
+    bank = cgrid.Bank(tables, False)
+    ref = numpy.zeros((N, N), dtype=complex)
+    cgrid.convgrid(bank, ref, p, v, vplane)
+    gcf, src = numpy.array(tables), vplane[:, None]
+    grids = []
+    try:
+        for chunk, group_kib in ((4096, 0), (64, 2), (256, 2), (64, 8), (4096, 2)):
+            _lib.set_tuning(0, chunk, group_kib)
+            a = numpy.zeros((N, N), dtype=complex)
+            G.convgrid(gcf, a, p, src, v)
+            assert_close(a, ref, what='grid chunk=%d group_kib=%d' % (chunk, group_kib))
+            grids.append(a)
+    finally:
+        _lib.set_tuning()
+    for g in grids[1:]:
+        assert (g == grids[0]).all()
+
+
 def test_gridding_is_bit_reproducible(gpu):
     """Fixed-point accumulation is order independent: different tilings,
     chunkings and repeated runs give bit-identical grids."""
