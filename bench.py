@@ -413,6 +413,25 @@ def run_diff(args, torch, G, W, _lib, dev):
     print(json.dumps(line), flush=True)
 
 
+def measured_copy_gbs(torch, dev, mib=1024, reps=5):
+    """HBM bandwidth re-measured on this device: a 1 GiB device-to-device
+    copy (torch copy_), read + write bytes over the mean of ``reps`` copies
+    timed with events; reported beside the 8 TB/s spec peak."""
+    src = torch.ones(mib << 18, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return round(2.0 * (mib << 20) / (ms * 1e-3) / 1e9, 1)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -461,6 +480,7 @@ def main():
     guv = torch.zeros((N, N), dtype=torch.complex64, device=dev)
     out = torch.empty(nv, dtype=torch.complex64, device=dev)
 
+    copy_gbs = measured_copy_gbs(torch, dev)
     if args.copy_probe_mib > 0:
         src = torch.full((args.copy_probe_mib << 18,), 1.0, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -557,7 +577,8 @@ def main():
                      'lds_bytes': lds_bytes,
                      'lds_frac': round(lds_bytes / (t_ms * 1e-3) / (LDS_PEAK_TBS * 1e12), 5),
                      'kernel_ms': {'k_grid_tiles': round(g_avg, 4),
-                                   'k_degrid_tiles': round(d_avg, 4)}})
+                                   'k_degrid_tiles': round(d_avg, 4)},
+                     'hbm_copy_measured_gbs': copy_gbs})
 
     cpu, parity = None, None
     if args.cpu_sample > 0 and config['workload'] == 'wproj2048':
