@@ -147,19 +147,26 @@ __global__ __launch_bounds__(DFT_NT) void k_dft2d(
                  *Dil = dsplit + 3 * NN;
   _Float16 *Lrh = lds, *Lrl = lds + PLANE, *Lih = lds + 2 * PLANE, *Lil = lds + 3 * PLANE;
 
-  // ---- load X (optionally modulated), scale to the fp16 range, split ----
+  // ---- load X (optionally modulated) once into registers, scale to the
+  // fp16 range, split ----
+  constexpr int PER = NN / DFT_NT;
+  static_assert(PER * DFT_NT == NN, "threads cover the sub-grid exactly");
+  float2 xv[PER];
   float m = 0.f;
-  for (int i = tid; i < NN; i += DFT_NT) {
-    const float2 v = modulate(X[i], mod_in, b, i);
-    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = tid + k * DFT_NT;
+    xv[k] = modulate(X[i], mod_in, b, i);
+    m = fmaxf(m, fmaxf(fabsf(xv[k].x), fabsf(xv[k].y)));
   }
   const float s1 = pow2_scale(block_max(m, red));
-  for (int i = tid; i < NN; i += DFT_NT) {
-    const float2 v = modulate(X[i], mod_in, b, i);
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = tid + k * DFT_NT;
     const int y = i / n, x = i - y * n;
     const int o = y * LDS_STRIDE + x;
-    split16(v.x * s1, Lrh[o], Lrl[o]);
-    split16(v.y * s1, Lih[o], Lil[o]);
+    split16(xv[k].x * s1, Lrh[o], Lrl[o]);
+    split16(xv[k].y * s1, Lih[o], Lil[o]);
   }
   __syncthreads();
 
