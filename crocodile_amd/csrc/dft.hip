@@ -27,6 +27,7 @@
 namespace {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int DFT_N = 128;
@@ -43,9 +44,21 @@ struct ModArgs {
 
 __device__ __forceinline__ half8 neg8(half8 a) { return -a; }
 
-__device__ __forceinline__ void split16(float x, _Float16 &h, _Float16 &l) {
-  h = (_Float16)x;
-  l = (_Float16)(x - (float)h);
+// Split four consecutive values (already scaled) and store them as one
+// 8-byte word per plane at element o of hi / lo (o a multiple of 4).
+__device__ __forceinline__ void split4_store(float a0, float a1, float a2, float a3,
+                                             _Float16 *hi, _Float16 *lo, int o) {
+  half4 h, l;
+  h[0] = (_Float16)a0;
+  l[0] = (_Float16)(a0 - (float)h[0]);
+  h[1] = (_Float16)a1;
+  l[1] = (_Float16)(a1 - (float)h[1]);
+  h[2] = (_Float16)a2;
+  l[2] = (_Float16)(a2 - (float)h[2]);
+  h[3] = (_Float16)a3;
+  l[3] = (_Float16)(a3 - (float)h[3]);
+  *reinterpret_cast<half4 *>(hi + o) = h;
+  *reinterpret_cast<half4 *>(lo + o) = l;
 }
 
 __device__ __forceinline__ floatx4 mfma(half8 a, half8 b, floatx4 c) {
@@ -149,24 +162,34 @@ __global__ __launch_bounds__(DFT_NT) void k_dft2d(
 
   // ---- load X (optionally modulated) once into registers, scale to the
   // fp16 range, split ----
-  constexpr int PER = NN / DFT_NT;
-  static_assert(PER * DFT_NT == NN, "threads cover the sub-grid exactly");
-  float2 xv[PER];
+  // (each thread holds four consecutive elements of a row per step, loaded
+  // as two 16-byte words and stored as one 8-byte word per split plane)
+  constexpr int PER4 = NN / (4 * DFT_NT);
+  static_assert(PER4 * 4 * DFT_NT == NN, "threads cover the sub-grid exactly");
+  float2 xv[4 * PER4];
   float m = 0.f;
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int i = tid + k * DFT_NT;
-    xv[k] = modulate(X[i], mod_in, b, i);
-    m = fmaxf(m, fmaxf(fabsf(xv[k].x), fabsf(xv[k].y)));
+  for (int k = 0; k < PER4; k++) {
+    const int i = 4 * (tid + k * DFT_NT);
+    const float4 p0 = reinterpret_cast<const float4 *>(X + i)[0];
+    const float4 p1 = reinterpret_cast<const float4 *>(X + i)[1];
+    xv[4 * k] = modulate(make_float2(p0.x, p0.y), mod_in, b, i);
+    xv[4 * k + 1] = modulate(make_float2(p0.z, p0.w), mod_in, b, i + 1);
+    xv[4 * k + 2] = modulate(make_float2(p1.x, p1.y), mod_in, b, i + 2);
+    xv[4 * k + 3] = modulate(make_float2(p1.z, p1.w), mod_in, b, i + 3);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      m = fmaxf(m, fmaxf(fabsf(xv[4 * k + j].x), fabsf(xv[4 * k + j].y)));
   }
   const float s1 = pow2_scale(block_max(m, red));
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int i = tid + k * DFT_NT;
+  for (int k = 0; k < PER4; k++) {
+    const int i = 4 * (tid + k * DFT_NT);
     const int y = i / n, x = i - y * n;
     const int o = y * LDS_STRIDE + x;
-    split16(xv[k].x * s1, Lrh[o], Lrl[o]);
-    split16(xv[k].y * s1, Lih[o], Lil[o]);
+    const float2 *v = xv + 4 * k;
+    split4_store(v[0].x * s1, v[1].x * s1, v[2].x * s1, v[3].x * s1, Lrh, Lrl, o);
+    split4_store(v[0].y * s1, v[1].y * s1, v[2].y * s1, v[3].y * s1, Lih, Lil, o);
   }
   __syncthreads();
 
@@ -194,16 +217,13 @@ __global__ __launch_bounds__(DFT_NT) void k_dft2d(
 #pragma unroll
     for (int r = 0; r < 4; r++) m = fmaxf(m, fmaxf(fabsf(tr[t][r]), fabsf(ti[t][r])));
   const float s2 = pow2_scale(block_max(m, red));  // (barrier: pass 1 reads done)
-  // store T^T: element (row y = 16t + 4(lane>>4) + r, col x = 16w + lr) at [x][y]
+  // store T^T: element (row y = 16t + 4(lane>>4) + r, col x = 16w + lr) at
+  // [x][y]; r = 0..3 are consecutive in y, one 8-byte word per plane
 #pragma unroll
   for (int t = 0; t < DFT_NW; t++) {
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int y = 16 * t + 4 * (lane >> 4) + r, x = 16 * w + lr;
-      const int o = x * LDS_STRIDE + y;
-      split16(tr[t][r] * s2, Lrh[o], Lrl[o]);
-      split16(ti[t][r] * s2, Lih[o], Lil[o]);
-    }
+    const int o = (16 * w + lr) * LDS_STRIDE + 16 * t + 4 * (lane >> 4);
+    split4_store(tr[t][0] * s2, tr[t][1] * s2, tr[t][2] * s2, tr[t][3] * s2, Lrh, Lrl, o);
+    split4_store(ti[t][0] * s2, ti[t][1] * s2, ti[t][2] * s2, ti[t][3] * s2, Lih, Lil, o);
   }
   __syncthreads();
 
