@@ -198,11 +198,19 @@ __global__ __launch_bounds__(DFT_NT) void k_dft2d(
   floatx4 tr[DFT_NW], ti[DFT_NW];
 #pragma unroll
   for (int t = 0; t < DFT_NW; t++) tr[t] = ti[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // B = D[k0 + lk .. +8][16w + lr]: D symmetric, read row 16w + lr; the
+  // next k-step's fragments are loaded before this one's products.
+  int dro = (16 * w + lr) * n + lk;
+  half8 nrh = ld8(Drh + dro), nrl = ld8(Drl + dro), nih = ld8(Dih + dro), nil = ld8(Dil + dro);
   for (int k0 = 0; k0 < n; k0 += 32) {
-    // B = D[k0 + lk .. +8][16w + lr]: D symmetric, read row 16w + lr.
-    const int dro = (16 * w + lr) * n + k0 + lk;
-    const half8 brh = ld8(Drh + dro), brl = ld8(Drl + dro);
-    const half8 bih = ld8(Dih + dro), bil = ld8(Dil + dro);
+    const half8 brh = nrh, brl = nrl, bih = nih, bil = nil;
+    if (k0 + 32 < n) {
+      dro += 32;
+      nrh = ld8(Drh + dro);
+      nrl = ld8(Drl + dro);
+      nih = ld8(Dih + dro);
+      nil = ld8(Dil + dro);
+    }
 #pragma unroll
     for (int t = 0; t < DFT_NW; t++) {
       const int xo = (16 * t + lr) * LDS_STRIDE + k0 + lk;  // A = X rows of tile-row t
@@ -231,10 +239,17 @@ __global__ __launch_bounds__(DFT_NT) void k_dft2d(
   floatx4 yr[DFT_NW], yi[DFT_NW];
 #pragma unroll
   for (int t = 0; t < DFT_NW; t++) yr[t] = yi[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  int dao = (16 * w + lr) * n + lk;  // A = D rows of tile-row w, one k-step ahead
+  half8 nah = ld8(Drh + dao), nal = ld8(Drl + dao), nbh = ld8(Dih + dao), nbl = ld8(Dil + dao);
   for (int k0 = 0; k0 < n; k0 += 32) {
-    const int dao = (16 * w + lr) * n + k0 + lk;  // A = D rows of tile-row w
-    const half8 arh = ld8(Drh + dao), arl = ld8(Drl + dao);
-    const half8 aih = ld8(Dih + dao), ail = ld8(Dil + dao);
+    const half8 arh = nah, arl = nal, aih = nbh, ail = nbl;
+    if (k0 + 32 < n) {
+      dao += 32;
+      nah = ld8(Drh + dao);
+      nal = ld8(Drl + dao);
+      nbh = ld8(Dih + dao);
+      nbl = ld8(Dil + dao);
+    }
 #pragma unroll
     for (int t = 0; t < DFT_NW; t++) {
       const int to = (16 * t + lr) * LDS_STRIDE + k0 + lk;  // B = T[k][16t + lr] = T^T row
