@@ -30,6 +30,10 @@ struct ScaleState {
   unsigned int max_k_bits;    // max |re|,|im| of the kernel bank (float bits)
   double scale;               // 2^e
   double inv_scale;           // 2^-e
+  // gridding work-list lengths: dense tiles' range items per XCD queue, then
+  // the sparse tiles' items
+  unsigned int qcount[XCD_QUEUES + 1];
+  unsigned int pad;
 };
 
 
@@ -72,6 +76,16 @@ __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
   st->inv_scale = ldexp(1.0, -e);
 }
 
+
+// Queue position q of a list of len items (tile-major) -> list index: a
+// walk with a prime stride, so that workgroups running at the same time
+// take tiles far apart, whose regions do not overlap and whose flushes do
+// not meet on the same cells; rot offsets the walk's start.
+__device__ __forceinline__ int list_pos(int q, int len, int rot) {
+  constexpr int64_t STRIDE = 7919;
+  const int64_t step = len % STRIDE ? STRIDE : 1;
+  return (int)(((int64_t)q * step + rot) % len);
+}
 
 // One work item: the item's visibilities spread into the LDS region, which
 // is then flushed (touched cells only) and cleared for the next item.
@@ -141,60 +155,116 @@ __device__ __forceinline__ void grid_item(unsigned long long *sacc, const int4 i
 // whole-tile items read the whole bank through every XCD's L2).  Dense tiles
 // carry most of the visibilities, and their range items are long enough to
 // pay for their extra region flushes.  Phase B: every other tile's
-// tile-major items, from one shared queue (qhead[XCD_QUEUES]).
+// tile-major items, from one shared queue (qhead[XCD_QUEUES]).  Both lists
+// are compacted beforehand (k_grid_lists), so a queue holds only its work.
 template <int NT, bool SEPARABLE, int R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_grid_tiles(
     unsigned long long *__restrict__ acc, int N, int T, int ntx,
     const uint4 *__restrict__ rec, const float2 *__restrict__ vis,
     const float2 *__restrict__ kern, const int4 *__restrict__ items,
-    const uint32_t *__restrict__ nitems, const int4 *__restrict__ items_g,
-    const uint32_t *__restrict__ gstart, int ngroups, const uint32_t *__restrict__ offset,
-    int dense_min, uint32_t *__restrict__ qhead, const ScaleState *__restrict__ st) {
+    const int4 *__restrict__ dense_g, const uint32_t *__restrict__ gstart, int ngroups,
+    uint32_t *__restrict__ qhead, const ScaleState *__restrict__ st) {
   constexpr int CELLS = R * R;
   __shared__ unsigned long long sacc[2 * CELLS];  // re, im interleaved
   __shared__ int s_item;
   const int tid = threadIdx.x;
   const double scale = st->scale;
-  const bool split = items_g != nullptr && ngroups > 1 && dense_min > 0;
 
   for (int c = tid; c < 2 * CELLS; c += NT) sacc[c] = 0ull;
-  if (split) {
+  if (ngroups > 1) {
     const int nq = min(XCD_QUEUES, ngroups);
     const int x0 = xcd_id() % nq;
     for (int k = 0; k < nq; k++) {
       const int x = (x0 + k) % nq;
       const int lo = (int)gstart[(int64_t)x * ngroups / nq];
-      const int hi = (int)gstart[(int64_t)(x + 1) * ngroups / nq];
       // Queue x starts x / nq of the way into its tile-major list, so that
       // the XCDs do not flush the same tiles' regions at the same time.
-      const int len = hi - lo, rot = (int)((int64_t)len * x / nq);
+      const int len = (int)st->qcount[x], rot = (int)((int64_t)len * x / nq);
       for (;;) {
         if (tid == 0) s_item = (int)atomicAdd(qhead + x, 1u);
         __syncthreads();
         const int q = s_item;
         __syncthreads();  // every wave has read s_item before it is rewritten
         if (q >= len) break;
-        const int it = lo + (q + rot < len ? q + rot : q + rot - len);
-        const int4 item = items_g[it];
-        const int t = item.x;
-        if (offset[(t + 1) * ngroups] - offset[t * ngroups] < (uint32_t)dense_min) continue;
+        const int4 item = dense_g[lo + list_pos(q, len, rot)];
         grid_item<NT, SEPARABLE, R>(sacc, item, acc, N, T, ntx, rec, vis, kern, scale);
       }
     }
   }
-  const int nit = (int)*nitems;
+  const int nit = (int)st->qcount[XCD_QUEUES];
   for (;;) {
     if (tid == 0) s_item = (int)atomicAdd(qhead + XCD_QUEUES, 1u);
     __syncthreads();
     const int it = s_item;
     __syncthreads();
     if (it >= nit) break;
-    const int4 item = items[it];
-    const int t = item.x;
-    if (split && offset[(t + 1) * ngroups] - offset[t * ngroups] >= (uint32_t)dense_min)
-      continue;
-    grid_item<NT, SEPARABLE, R>(sacc, item, acc, N, T, ntx, rec, vis, kern, scale);
+    grid_item<NT, SEPARABLE, R>(sacc, items[list_pos(it, nit, 0)], acc, N, T, ntx, rec, vis,
+                                kern, scale);
   }
+}
+
+// Slot in the list counted by *count for every lane with take set: one
+// atomic per wavefront (the lowest taking lane adds the wave's count),
+// slots in lane order.
+__device__ __forceinline__ uint32_t wave_claim(unsigned int *count, bool take) {
+  const unsigned long long m = __ballot(take);
+  const int lane = threadIdx.x & 63;
+  if (!m) return 0u;
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0u;
+  if (lane == leader) base = atomicAdd(count, (unsigned int)__popcll(m));
+  base = __shfl(base, leader, 64);
+  return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// The gridding work lists.  A tile holding at least dense_min visibilities
+// (with more than one plane group) is dense: its per-XCD range items are
+// kept, compacted within each range's span of items_g into dense_g, with
+// their count per queue; every other tile's tile-major items go to sparse.
+// Order within a list follows the atomics; gridding sums are exact, so it
+// does not change the result.  Every lane of a wavefront runs to the end
+// (the claims are wave-wide).
+__global__ void k_grid_lists(const int4 *__restrict__ items, const uint32_t *__restrict__ nitems,
+                             const int4 *__restrict__ items_g,
+                             const uint32_t *__restrict__ nitems_g,
+                             const uint32_t *__restrict__ gstart, int ngroups,
+                             const uint32_t *__restrict__ offset, int dense_min,
+                             int4 *__restrict__ dense_g, int4 *__restrict__ sparse,
+                             ScaleState *__restrict__ st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool split = ngroups > 1;
+  const int nq = min(XCD_QUEUES, ngroups);
+  // dense tiles' range items, per queue
+  int4 item = make_int4(0, 0, 0, 0);
+  int x = 0;  // the queue whose range holds item i
+  bool want = false;
+  if (split && i < (int)*nitems_g) {
+    item = items_g[i];
+    const int t = item.x;
+    want = offset[(t + 1) * ngroups] - offset[t * ngroups] >= (uint32_t)dense_min;
+    while (x + 1 < nq && (int)gstart[(int64_t)(x + 1) * ngroups / nq] <= i) x++;
+  }
+  for (;;) {  // one round per distinct queue among the wave's items
+    const unsigned long long m = __ballot(want);
+    if (!m) break;
+    const int xl = __shfl(x, __ffsll((long long)m) - 1, 64);
+    const bool mine = want && x == xl;
+    const uint32_t pos = wave_claim(&st->qcount[xl], mine);
+    if (mine) {
+      dense_g[gstart[(int64_t)xl * ngroups / nq] + pos] = item;
+      want = false;
+    }
+  }
+  // the other tiles' tile-major items
+  bool take = false;
+  int4 it2 = make_int4(0, 0, 0, 0);
+  if (i < (int)*nitems) {
+    it2 = items[i];
+    const int t = it2.x;
+    take = !split || offset[(t + 1) * ngroups] - offset[t * ngroups] < (uint32_t)dense_min;
+  }
+  const uint32_t pos = wave_claim(&st->qcount[XCD_QUEUES], take);
+  if (take) sparse[pos] = it2;
 }
 
 // grid[c] += acc[c] / 2^e, clearing nothing (the workspace is zeroed per call).
@@ -241,24 +311,32 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, void *grid, int N,
   // 1.33 ms, 256 1.62 ms, where the range items get too short for their
   // region flushes).
   const int dense_min = plan.chunk;
+  int4 *lists = (int4 *)ws_get(ctx, WS_GRIDQ, 2 * sizeof(int4) * plan.max_items);
+  if (!lists) return CROC_ENOMEM;
+  int4 *dense_g = lists, *sparse = lists + plan.max_items;
+  const int ng_lists = plan.items_g ? plan.ngroups : 1;
+  k_grid_lists<<<(unsigned)((plan.max_items + 255) / 256), 256, 0, ctx->stream>>>(
+      plan.items, plan.nitems, plan.items_g, plan.nitems_g, plan.gstart, ng_lists, plan.offset,
+      dense_min, dense_g, sparse, st);
+  CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   const float2 *kt = (const float2 *)kern;
   if (plan.R == TILE_R_SMALL) {
     const int G = persistent_blocks(ctx, 2);  // 63.5 KiB of LDS each
     if (separable)
       k_grid_tiles<GRID_NT, true, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.items_g, plan.gstart, plan.ngroups, plan.offset, dense_min, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, dense_g, plan.gstart, ng_lists, plan.qhead, st);
     else
       k_grid_tiles<GRID_NT, false, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.items_g, plan.gstart, plan.ngroups, plan.offset, dense_min, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, dense_g, plan.gstart, ng_lists, plan.qhead, st);
   } else {
     const int G = persistent_blocks(ctx, 1);
     if (separable)
       k_grid_tiles<GRID_NT, true, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.items_g, plan.gstart, plan.ngroups, plan.offset, dense_min, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, dense_g, plan.gstart, ng_lists, plan.qhead, st);
     else
       k_grid_tiles<GRID_NT, false, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, plan.items, plan.nitems, plan.items_g, plan.gstart, plan.ngroups, plan.offset, dense_min, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, dense_g, plan.gstart, ng_lists, plan.qhead, st);
   }
   CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));

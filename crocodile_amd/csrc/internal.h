@@ -62,6 +62,7 @@ enum WsSlot {
   WS_SUB,       // w-towers sub-grid stack
   WS_TOWER,     // w-towers job / chunk tables
   WS_SORT,      // w-towers radix sort keys, indices, run tables, temporaries
+  WS_GRIDQ,     // gridding work lists: dense tiles' range items, sparse tiles' items
   WS_COUNT
 };
 
