@@ -162,8 +162,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     unsigned long long *__restrict__ acc, int N, int T, int ntx,
     const uint4 *__restrict__ rec, const float2 *__restrict__ vis,
     const float2 *__restrict__ kern, const int4 *__restrict__ items,
-    const int4 *__restrict__ dense_g, const uint32_t *__restrict__ gstart, int ngroups,
-    uint32_t *__restrict__ qhead, const ScaleState *__restrict__ st) {
+    const uint32_t *__restrict__ nitems, const int4 *__restrict__ dense_g,
+    const uint32_t *__restrict__ gstart, int ngroups, uint32_t *__restrict__ qhead,
+    const ScaleState *__restrict__ st) {
   constexpr int CELLS = R * R;
   __shared__ unsigned long long sacc[2 * CELLS];  // re, im interleaved
   __shared__ int s_item;
@@ -191,7 +192,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
       }
     }
   }
-  const int nit = (int)st->qcount[XCD_QUEUES];
+  // with one plane group the tile-major list is used as it is (nitems)
+  const int nit = (int)(ngroups > 1 ? st->qcount[XCD_QUEUES] : *nitems);
   for (;;) {
     if (tid == 0) s_item = (int)atomicAdd(qhead + XCD_QUEUES, 1u);
     __syncthreads();
@@ -306,37 +308,41 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, void *grid, int N,
   k_scale_pick<<<1, 1, 0, ctx->stream>>>(st, nv, separable);
   CROC_HIP(hipGetLastError());
 
-  // Tiles of at least one chunk are split by plane-group range (measured on
-  // wproj2048: grid 1.42 -> 1.24 ms; 1024 and 2048 visibilities 1.25 and
-  // 1.33 ms, 256 1.62 ms, where the range items get too short for their
-  // region flushes).
-  const int dense_min = plan.chunk;
+  // Tiles of at least 1.5 chunks are split by plane-group range (measured on
+  // wproj2048 with chunk 512: grid 1.18 / 1.15 / 1.17 ms for thresholds of
+  // 512 / 768 / 1024 visibilities, 1.42 ms without the split; 256 is far
+  // slower, the range items getting too short for their region flushes).
+  const int dense_min = plan.chunk + plan.chunk / 2;
   int4 *lists = (int4 *)ws_get(ctx, WS_GRIDQ, 2 * sizeof(int4) * plan.max_items);
   if (!lists) return CROC_ENOMEM;
   int4 *dense_g = lists, *sparse = lists + plan.max_items;
   const int ng_lists = plan.items_g ? plan.ngroups : 1;
-  k_grid_lists<<<(unsigned)((plan.max_items + 255) / 256), 256, 0, ctx->stream>>>(
-      plan.items, plan.nitems, plan.items_g, plan.nitems_g, plan.gstart, ng_lists, plan.offset,
-      dense_min, dense_g, sparse, st);
-  CROC_HIP(hipGetLastError());
+  if (ng_lists > 1) {
+    k_grid_lists<<<(unsigned)((plan.max_items + 255) / 256), 256, 0, ctx->stream>>>(
+        plan.items, plan.nitems, plan.items_g, plan.nitems_g, plan.gstart, ng_lists,
+        plan.offset, dense_min, dense_g, sparse, st);
+    CROC_HIP(hipGetLastError());
+  } else {
+    sparse = plan.items;  // nothing to split
+  }
   CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   const float2 *kt = (const float2 *)kern;
   if (plan.R == TILE_R_SMALL) {
     const int G = persistent_blocks(ctx, 2);  // 63.5 KiB of LDS each
     if (separable)
       k_grid_tiles<GRID_NT, true, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, dense_g, plan.gstart, ng_lists, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, plan.nitems, dense_g, plan.gstart, ng_lists, plan.qhead, st);
     else
       k_grid_tiles<GRID_NT, false, TILE_R_SMALL><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, dense_g, plan.gstart, ng_lists, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, plan.nitems, dense_g, plan.gstart, ng_lists, plan.qhead, st);
   } else {
     const int G = persistent_blocks(ctx, 1);
     if (separable)
       k_grid_tiles<GRID_NT, true, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, dense_g, plan.gstart, ng_lists, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, plan.nitems, dense_g, plan.gstart, ng_lists, plan.qhead, st);
     else
       k_grid_tiles<GRID_NT, false, TILE_R><<<G, GRID_NT, 0, ctx->stream>>>(
-          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, dense_g, plan.gstart, ng_lists, plan.qhead, st);
+          acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, plan.nitems, dense_g, plan.gstart, ng_lists, plan.qhead, st);
   }
   CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));

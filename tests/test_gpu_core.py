@@ -380,7 +380,7 @@ def test_plane_groups_do_not_change_results(gpu):
 
 
 def test_dense_tile_split_is_exact(gpu):
-    """Tiles holding at least one chunk of visibilities are gridded per
+    """Tiles holding at least 1.5 chunks of visibilities are gridded per
     plane-group range from per-XCD queues, the others whole from a shared
     queue.  With a 64-visibility chunk nearly every tile takes the split
     path, with 4096 none does; with one plane group nothing is split.  Every
