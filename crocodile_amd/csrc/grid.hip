@@ -77,16 +77,6 @@ __global__ void k_scale_pick(ScaleState *st, int64_t nv, int separable) {
 }
 
 
-// Queue position q of a list of len items (tile-major) -> list index: a
-// walk with a prime stride, so that workgroups running at the same time
-// take tiles far apart, whose regions do not overlap and whose flushes do
-// not meet on the same cells; rot offsets the walk's start.
-__device__ __forceinline__ int list_pos(int q, int len, int rot) {
-  constexpr int64_t STRIDE = 7919;
-  const int64_t step = len % STRIDE ? STRIDE : 1;
-  return (int)(((int64_t)q * step + rot) % len);
-}
-
 // One work item: the item's visibilities spread into the LDS region, which
 // is then flushed (touched cells only) and cleared for the next item.
 template <int NT, bool SEPARABLE, int R>
@@ -187,7 +177,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
         const int q = s_item;
         __syncthreads();  // every wave has read s_item before it is rewritten
         if (q >= len) break;
-        const int4 item = dense_g[lo + list_pos(q, len, rot)];
+        const int4 item = dense_g[lo + (q + rot < len ? q + rot : q + rot - len)];
         grid_item<NT, SEPARABLE, R>(sacc, item, acc, N, T, ntx, rec, vis, kern, scale);
       }
     }
@@ -200,8 +190,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int it = s_item;
     __syncthreads();
     if (it >= nit) break;
-    grid_item<NT, SEPARABLE, R>(sacc, items[list_pos(it, nit, 0)], acc, N, T, ntx, rec, vis,
-                                kern, scale);
+    grid_item<NT, SEPARABLE, R>(sacc, items[it], acc, N, T, ntx, rec, vis, kern, scale);
   }
 }
 
