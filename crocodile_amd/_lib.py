@@ -66,6 +66,7 @@ SIGNATURES = {
 _lib = None
 _lock = threading.Lock()
 _ctx = {}
+_tuning = (0, 512, 0)  # tile, chunk, group_kib of croc_ctx_set_tuning
 
 
 class CrocError(RuntimeError):
@@ -115,16 +116,28 @@ def device():
 
 
 def ctx():
-    """Per-device library context, bound to torch's current stream."""
+    """Library context of torch's current stream on the current device.
+
+    A context owns the scratch buffers of its calls (fixed-point
+    accumulator, scale state, work lists, w-towers sub-grids), and calls
+    return without waiting for the device.  One context per (device,
+    stream) keeps calls queued on different streams from sharing those
+    buffers; calls on one stream are ordered by the stream itself."""
     dev = device()
     L = load()
-    c = _ctx.get(dev.index)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    key = (dev.index, stream)
+    c = _ctx.get(key)
     if c is None:
-        h = _P()
-        check(L.croc_ctx_create(dev.index, ctypes.byref(h)))
-        c = h
-        _ctx[dev.index] = c
-    check(L.croc_ctx_set_stream(c, _P(torch.cuda.current_stream(dev).cuda_stream)))
+        with _lock:
+            c = _ctx.get(key)
+            if c is None:
+                h = _P()
+                check(L.croc_ctx_create(dev.index, ctypes.byref(h)))
+                check(L.croc_ctx_set_stream(h, _P(stream)))
+                check(L.croc_ctx_set_tuning(h, *_tuning))
+                c = h
+                _ctx[key] = c
     return c
 
 
@@ -135,7 +148,13 @@ def last_kernel_ms():
 
 
 def set_tuning(tile=0, chunk=512, group_kib=0):
-    check(load().croc_ctx_set_tuning(ctx(), tile, chunk, group_kib))
+    """Tuning knobs (croc_ctx_set_tuning) for every context, present and
+    future: results do not depend on them."""
+    global _tuning
+    _tuning = (int(tile), int(chunk), int(group_kib))
+    ctx()
+    for c in list(_ctx.values()):
+        check(load().croc_ctx_set_tuning(c, *_tuning))
 
 
 def ptr(t):

@@ -305,7 +305,7 @@ def w_conj_kernel_fn(kernel_fn):
     def fn(theta, w, *args, **kw):
         if w < 0:
             k = kernel_fn(theta, -w, *args, **kw)
-            return k.conj() if _is_torch(k) else numpy.conj(k)
+            return torch.conj_physical(k) if _is_torch(k) else numpy.conj(k)
         return kernel_fn(theta, w, *args, **kw)
     return fn
 
@@ -335,8 +335,8 @@ class KernelBank:
     def __init__(self, kern, plane_off, plane_s, Q, separable):
         dev = _lib.device()
         self.kern = kern
-        self.plane_off_h = numpy.asarray(plane_off, dtype=numpy.int64)
-        self.plane_s_h = numpy.asarray(plane_s, dtype=numpy.int32)
+        self.plane_off_h = numpy.array(plane_off, dtype=numpy.int64)
+        self.plane_s_h = numpy.array(plane_s, dtype=numpy.int32)
         self.plane_off = torch.as_tensor(self.plane_off_h, device=dev)
         self.plane_s = torch.as_tensor(self.plane_s_h, device=dev)
         self.Q = int(Q)
@@ -439,10 +439,50 @@ def kernel_bank(gcf, src, nv, for_grid):
 # --------------------------------------------------------------------------
 
 
+def _check_device(t, what, dtype):
+    """Raise ValueError unless ``t`` is a contiguous tensor of ``dtype`` on
+    the current device (the C ABI takes raw pointers and trusts sizes)."""
+    if not _is_torch(t):
+        raise ValueError('%s must be a torch tensor on the device' % what)
+    if t.dtype != dtype:
+        raise ValueError('%s must be %s, not %s' % (what, dtype, t.dtype))
+    if t.device != _lib.device():
+        raise ValueError('%s must be on %s, not %s' % (what, _lib.device(), t.device))
+    if not t.is_contiguous():
+        raise ValueError('%s must be contiguous' % what)
+
+
+def _check_grid_args(grid_t, uvw_t, data_t, bank, mode, vplane, nv=None):
+    """Shapes and types of a gridding / degridding call: grid complex64
+    [N, N]; uvw float64 [nv, >=2] (>=3 with w-bins); visibilities complex64
+    with nv entries; bank complex64 on the device; vplane int32 [nv]."""
+    _check_device(grid_t, 'grid', torch.complex64)
+    if grid_t.ndim != 2 or grid_t.shape[0] != grid_t.shape[1]:
+        raise ValueError('grid must be square [N, N], not %s' % (tuple(grid_t.shape),))
+    if uvw_t is not None:
+        _check_device(uvw_t, 'uvw', torch.float64)
+        if uvw_t.ndim != 2 or uvw_t.shape[1] < (3 if mode == _lib.PLANE_WBIN else 2):
+            raise ValueError('uvw must have shape [nv, >=%d]'
+                             % (3 if mode == _lib.PLANE_WBIN else 2))
+        nv = uvw_t.shape[0]
+    _check_device(data_t, 'visibilities', torch.complex64)
+    if data_t.numel() != nv:
+        raise ValueError('visibilities hold %d values for %d coordinates'
+                         % (data_t.numel(), nv))
+    _check_device(bank.kern, 'kernel bank', torch.complex64)
+    if mode == _lib.PLANE_INDEX:
+        if vplane is None:
+            raise ValueError('CROC_PLANE_INDEX needs vplane')
+        _check_device(vplane, 'vplane', torch.int32)
+        if vplane.numel() != nv:
+            raise ValueError('vplane holds %d values for %d coordinates' % (vplane.numel(), nv))
+
+
 def _grid_call(fn, grid_t, N, uvw_t, lam, data_t, bank, mode, vplane, wstep=1.0, wbin0=0):
-    for t in (grid_t, uvw_t, data_t):
-        if not t.is_contiguous():
-            raise ValueError('device arrays must be contiguous')
+    bank = bank.as_complex64()
+    _check_grid_args(grid_t, uvw_t, data_t, bank, mode, vplane)
+    if int(N) != grid_t.shape[0]:
+        raise ValueError('grid size %d differs from N = %d' % (grid_t.shape[0], N))
     _lib.check(fn(_lib.ctx(), _lib.ptr(grid_t), int(N), _lib.ptr(uvw_t), uvw_t.shape[0],
                   uvw_t.shape[1], float(lam), _lib.ptr(data_t), _lib.ptr(bank.kern),
                   bank.Q, int(bank.separable), bank.nplanes, _lib.ptr(bank.plane_off),
@@ -498,7 +538,7 @@ class GridPlan:
 
     def __init__(self, bank, N, uvw=None, lam=None, mode=_lib.PLANE_SINGLE, vplane=None,
                  wstep=1.0, wbin0=0):
-        self.bank, self.N, self.nv = bank, int(N), 0
+        self.bank, self.N, self.nv = bank.as_complex64(), int(N), 0
         h = _lib._P()
         _lib.check(_lib.load().croc_plan_create(_lib.ctx(), ctypes.byref(h)))
         self._h = h
@@ -507,8 +547,17 @@ class GridPlan:
 
     def bin(self, uvw, lam, mode=_lib.PLANE_SINGLE, vplane=None, wstep=1.0, wbin0=0):
         """Bin coordinates ``uvw / lam`` (plane selection as convgrid)."""
-        if not uvw.is_contiguous():
-            raise ValueError('device arrays must be contiguous')
+        _check_device(uvw, 'uvw', torch.float64)
+        if uvw.ndim != 2 or uvw.shape[1] < (3 if mode == _lib.PLANE_WBIN else 2):
+            raise ValueError('uvw must have shape [nv, >=%d]'
+                             % (3 if mode == _lib.PLANE_WBIN else 2))
+        if mode == _lib.PLANE_INDEX:
+            if vplane is None:
+                raise ValueError('CROC_PLANE_INDEX needs vplane')
+            _check_device(vplane, 'vplane', torch.int32)
+            if vplane.numel() != uvw.shape[0]:
+                raise ValueError('vplane holds %d values for %d coordinates'
+                                 % (vplane.numel(), uvw.shape[0]))
         bank = self.bank
         self.nv = 0
         _lib.check(_lib.load().croc_plan_bin(
@@ -527,19 +576,21 @@ class GridPlan:
 
     def grid(self, guv, vis, bank=None):
         """guv (complex64 device [N, N]) += convolution of vis (complex64 [nv])."""
-        bank = self.bank if bank is None else bank
+        bank = (self.bank if bank is None else bank).as_complex64()
         self._same_layout(bank)
-        if guv.shape != (self.N, self.N) or vis.shape[0] != self.nv:
-            raise ValueError('grid or visibility shape differs from the plan')
+        _check_grid_args(guv, None, vis, bank, None, None, self.nv)
+        if guv.shape != (self.N, self.N):
+            raise ValueError('grid shape differs from the plan')
         _lib.check(_lib.load().croc_convgrid_plan(_lib.ctx(), self._h, _lib.ptr(guv),
                                                   _lib.ptr(vis), _lib.ptr(bank.kern)))
 
     def degrid(self, guv, out, bank=None):
         """out (complex64 device [nv]) = degridded visibilities of guv."""
-        bank = self.bank if bank is None else bank
+        bank = (self.bank if bank is None else bank).as_complex64()
         self._same_layout(bank)
-        if guv.shape != (self.N, self.N) or out.shape[0] != self.nv:
-            raise ValueError('grid or visibility shape differs from the plan')
+        _check_grid_args(guv, None, out, bank, None, None, self.nv)
+        if guv.shape != (self.N, self.N):
+            raise ValueError('grid shape differs from the plan')
         _lib.check(_lib.load().croc_convdegrid_plan(_lib.ctx(), self._h, _lib.ptr(guv),
                                                     _lib.ptr(out), _lib.ptr(bank.kern)))
 

@@ -48,7 +48,9 @@ def test_plan_calls_replay_in_a_graph(gpu):
     torch.cuda.current_stream().wait_stream(s)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    # captured on the warmed-up stream: library contexts are per stream, and
+    # the warm-up sized that stream's context workspace
+    with torch.cuda.graph(g, stream=s):
         body()
     for rep in range(4):
         guv.fill_(1.0)

@@ -382,3 +382,182 @@ def test_w_slice_composition():
         sel = order[i:i + 7]
         R.convgrid(kfn(theta, uvw[sel, 2].mean()), ref, uvw[sel] / lam, None, vis[sel])
     assert_allclose(g, ref, rtol=1e-13)
+
+
+# ------------------------------------- shifted / transformed w-kernels
+# tests/test_synthesis.py:283-356 transcribed: a point source simulated at a
+# shifted (and linearly transformed) position must image back to its grid
+# cell when the visibilities are phase-shifted and gridded with w-kernels
+# built on the shifted / transformed kernel coordinates.
+
+SHIFT_TS = [
+    numpy.array([[1, 0], [0, 1]]),
+    numpy.array([[2, 0], [0, 3]]),
+    numpy.array([[0, 1], [-1, 0]]),
+    numpy.array([[1 / numpy.sqrt(2), 1 / numpy.sqrt(2)], [-1 / numpy.sqrt(2), 1 / numpy.sqrt(2)]]),
+    numpy.array([[1e5, -5e3], [6e4, -1e6]]),
+    numpy.array([[0, .5], [-.5, 0]]),
+]
+
+
+def shift_w_peaks(M, lam=10, Ns=range(1, 6)):
+    """Peaks of test_grid_shift_w (tests/test_synthesis.py:283-310) with the
+    oracle or product module M: every value must be 1."""
+    peaks = []
+    for uw, vw in [(.5, 0), (0, .5), (-1, 0), (0, -1)]:
+        for N in Ns:
+            for dl, dm in [(1 / lam, 1 / lam), (-1 / lam, 2 / lam), (5 / lam, 0)]:
+                theta = N / lam
+                uvw_all = uvw_grid(N, uw, vw) * lam
+                slices = R.slice_vis(N, R.sort_vis_w(uvw_all))
+                gcfs = [M.w_kernel(theta, numpy.mean(u[:, 2]), N, N, 1, dl=-dl, dm=-dm) / N ** 2
+                        for u in slices]
+                for x, y in itertools.product(range(-(N // 2), (N + 1) // 2), repeat=2):
+                    a = numpy.zeros((2 * N, 2 * N), dtype=complex)
+                    for u, gcf in zip(slices, gcfs):
+                        vis = R.simulate_point(u, x / lam - dl, y / lam - dm)
+                        viss = R.visibility_shift(u, vis, dl, dm)
+                        assert_allclose(R.visibility_shift(u, viss, -dl, -dm), vis)
+                        M.convgrid(gcf, a, u / lam / 2, None, viss)
+                    a2 = numpy.fft.fftshift(a[:N, :N] + a[:N, N:] + a[N:, :N] + a[N:, N:])
+                    img = numpy.real(R.ifft(a2))
+                    peaks.append(img[N // 2 + y, N // 2 + x])
+    return numpy.array(peaks)
+
+
+def transform_shift_w_peaks(M, lam=10, Ns=range(1, 6)):
+    """Peaks of test_grid_transform_shift_w (tests/test_synthesis.py:312-356)
+    with module M: every value must be 1."""
+    peaks = []
+    for T in SHIFT_TS:
+        Ti = numpy.linalg.inv(T)
+        for uw, vw in [(.5, 0), (0, .5), (0, -1)]:
+            for dl, dm in [(0, 0), (1 / lam, 1 / lam), (-1 / lam, 4 / lam)]:
+                for N in Ns:
+                    uvwt_all = uvw_grid(N, uw, vw) * lam
+                    uvw_all = R.uvw_transform(uvwt_all, Ti)
+                    slices = R.slice_vis(N, *R.sort_vis_w(uvwt_all, uvw_all))
+                    gcfs = [M.w_kernel(N / lam, numpy.mean(ut[:, 2]), N, N, 1, T=Ti, dl=-dl,
+                                       dm=-dm) / N ** 2 for ut, _ in slices]
+                    for xt, yt in itertools.product(range(-(N // 2), (N + 1) // 2), repeat=2):
+                        x, y = numpy.dot([xt, yt], Ti)
+                        assert_allclose(numpy.dot([x, y], T), [xt, yt], atol=1e-14)
+                        a = numpy.zeros((2 * N, 2 * N), dtype=complex)
+                        for (ut, u), gcf in zip(slices, gcfs):
+                            vis = R.simulate_point(u, x / lam - dl, y / lam - dm)
+                            vis = R.visibility_shift(u, vis, dl, dm)
+                            M.convgrid(gcf, a, ut / lam / 2, None, vis)
+                        a = numpy.fft.fftshift(a[:N, :N] + a[:N, N:] + a[N:, :N] + a[N:, N:])
+                        img = numpy.real(R.ifft(a))
+                        peaks.append(img[N // 2 + yt, N // 2 + xt])
+    return numpy.array(peaks)
+
+
+def test_grid_shift_w():
+    """tests/test_synthesis.py:283-310 (w_kernel with dl, dm; visibility_shift)."""
+    assert_allclose(shift_w_peaks(R), 1)
+
+
+def test_grid_transform_shift_w():
+    """tests/test_synthesis.py:312-356 (w_kernel with T, dl, dm; uvw_transform)."""
+    assert_allclose(transform_shift_w_peaks(R), 1)
+
+
+def test_shift_w_fails_without_kernel_shift():
+    """The shift test is sensitive: kernels built without the (dl, dm) offset
+    leave the peak off 1 (so a dropped or sign-flipped shift is caught)."""
+    class NoShift:
+        w_kernel = staticmethod(lambda theta, w, F, S, Q, dl=0, dm=0: R.w_kernel(theta, w, F, S, Q))
+        convgrid = staticmethod(R.convgrid)
+
+    class SignFlip:
+        w_kernel = staticmethod(lambda theta, w, F, S, Q, dl=0, dm=0:
+                                R.w_kernel(theta, w, F, S, Q, dl=-dl, dm=-dm))
+        convgrid = staticmethod(R.convgrid)
+    for M in (NoShift, SignFlip):
+        peaks = shift_w_peaks(M, Ns=[3, 4])
+        assert numpy.abs(peaks - 1).max() > 1e-3
+
+
+def test_uvw_transform_and_shift_invert():
+    """simulate.py:259-300: uvw_transform by T then by T^-1 is the identity,
+    and it maps the image by T in the reference's row-vector convention (a
+    point at x images at x T, test_synthesis.py:218-223: the uv phase is
+    preserved, u' . (x T) = u . x); visibility_shift by (dl, dm)
+    then (-dl, -dm) is the identity."""
+    rng = numpy.random.default_rng(7)
+    uvw = rng.standard_normal((20, 3)) * 100
+    for T in SHIFT_TS:
+        Ti = numpy.linalg.inv(T)
+        assert_allclose(R.uvw_transform(R.uvw_transform(uvw, T), Ti), uvw, rtol=1e-9,
+                        atol=1e-9 * numpy.abs(uvw).max())
+        x = rng.standard_normal(2)
+        ut = R.uvw_transform(uvw, T)
+        assert_allclose(ut[:, :2] @ (x @ T), uvw[:, :2] @ x, rtol=1e-9, atol=1e-6)
+        assert_allclose(ut[:, 2], uvw[:, 2])
+    vis = rng.standard_normal(20) + 1j * rng.standard_normal(20)
+    assert_allclose(R.visibility_shift(uvw, R.visibility_shift(uvw, vis, .01, -.02), -.01, .02),
+                    vis)
+
+
+def test_kernel_transform_eigenstructure():
+    """synthesis.py:220-248: T is symmetric, maps the shift direction d to
+    dn d (foreshortening by the cosine dn = sqrt(1 - |d|^2)) and leaves the
+    perpendicular direction unchanged; the identity at d = 0."""
+    assert_allclose(R.kernel_transform(0, 0), numpy.eye(2))
+    for dl, dm in [(0.1, 0.0), (0.0, -0.3), (0.2, 0.5), (-0.6, 0.7), (1e-4, 2e-4)]:
+        T = R.kernel_transform(dl, dm)
+        dn = numpy.sqrt(1 - dl ** 2 - dm ** 2)
+        assert_allclose(T, T.T)
+        assert_allclose(T @ [dl, dm], dn * numpy.array([dl, dm]), atol=1e-15)
+        assert_allclose(T @ [-dm, dl], [-dm, dl], atol=1e-15)
+        assert_allclose(numpy.linalg.det(T), dn, rtol=1e-12)
+
+
+def test_w_conj_kernel_fn_dispatch():
+    """synthesis.py:682-699: negative w is served by conj(k(theta, -w));
+    w >= 0 passes through; extra arguments are forwarded."""
+    calls = []
+
+    def kfn(theta, w, *a, **kw):
+        calls.append((theta, w, a, kw))
+        return numpy.array([[1 + 2j, w + 1j]])
+    fn = R.w_conj_kernel_fn(kfn)
+    assert_allclose(fn(0.1, -3.0, 7, k=1), numpy.conj(kfn(0.1, 3.0)))
+    assert calls[0] == (0.1, 3.0, (7,), {'k': 1})
+    assert_allclose(fn(0.1, 2.0), kfn(0.1, 2.0))
+    assert calls[2][1] == 2.0
+
+
+def test_w_kernel_conjugate_symmetry():
+    """k(-w) = conj(k(w)) holds exactly for odd far-field sizes at Q = 1
+    (symmetric coordinates), the identity w_conj_kernel_fn folds on."""
+    for F, w in [(5, 30.0), (9, 250.0), (15, 1000.0)]:
+        assert_allclose(R.w_kernel(0.1, -w, F, F, 1), numpy.conj(R.w_kernel(0.1, w, F, F, 1)),
+                        atol=1e-12)
+
+
+def test_doweight_hand_built():
+    """synthesis.py:533-549 on a hand-built case: theta 0.1, lam 100 (N =
+    10, cells of 10 wavelengths); the weight cell is round-half-even(N//2 +
+    u theta) as frac_coord with Qpx = 1 gives, so u = 5 (5.5 -> 6) shares
+    cell 6 with u = 15 (6.5 -> 6), and u = 25, 26 share cell 8."""
+    uvw = numpy.array([[0, 0, 0], [1, 1, 0], [-2, 3, 0], [15, 0, 0], [5, 0, 0],
+                       [25, 0, 0], [26, 0, 0], [0, 31, 0]], dtype=float)
+    want = numpy.array([1 / 3, 1 / 3, 1 / 3, 1 / 2, 1 / 2, 1 / 2, 1 / 2, 1.0])
+    assert_allclose(R.doweight(0.1, 100.0, uvw, numpy.ones(8)), want)
+    v = numpy.arange(1, 9) * (1 + 1j)
+    assert_allclose(R.doweight(0.1, 100.0, uvw, v), v * want)
+
+
+def test_mirror_uvw_hand_built():
+    """synthesis.py:831-852: rows with v < 0 are negated (u, v and w) and
+    their visibilities conjugated; v >= 0 (including -0.0) is kept."""
+    uvw = numpy.array([[1., -2., 3.], [4., 5., -6.], [-7., 0., 8.], [9., -0.0, 1.],
+                       [-1., -1e-9, -2.]])
+    vis = numpy.array([1 + 2j, 3 - 4j, 5 + 6j, 7 + 8j, -1 - 1j])
+    u2, v2 = R.mirror_uvw(uvw, vis)
+    assert_allclose(u2, [[-1., 2., -3.], [4., 5., -6.], [-7., 0., 8.], [9., 0., 1.],
+                         [1., 1e-9, 2.]])
+    assert_allclose(v2, [1 - 2j, 3 - 4j, 5 + 6j, 7 + 8j, -1 + 1j])
+    assert uvw[0, 1] == -2.0 and vis[0] == 1 + 2j  # inputs untouched
