@@ -46,6 +46,8 @@ SIGNATURES = {
     'croc_plan_bin': [_P, _P, _I, _P, _I64, _I64, _D, _I, _I, _I, _P, _P, _I, _I, _P, _D, _I],
     'croc_convgrid_plan': [_P, _P, _P, _P, _P],
     'croc_convdegrid_plan': [_P, _P, _P, _P, _P],
+    'croc_convdegrid_noalloc': [_P, _P, _I, _P, _I64, _I64, _D, _P, _P, _I, _I, _I, _P, _P,
+                                _I, _I, _P, _D, _I],
     'croc_grid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
     'croc_degrid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
     'croc_w_kernels': [_P, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P, _I],

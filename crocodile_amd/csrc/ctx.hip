@@ -50,6 +50,18 @@ int ctx_enter(croc_ctx *ctx) {
 void *ws_get_in(croc_ctx *ctx, WsSet &set, WsSlot slot, size_t bytes) {
   if (bytes == 0) bytes = 256;
   if (set.ws_size[slot] >= bytes) return set.ws[slot];
+  if (set.arena) {  // fixed arena: carve, never allocate
+    const size_t off = (set.arena_used + 255) & ~(size_t)255;
+    if (off > set.arena_size || bytes > set.arena_size - off) {
+      croc_set_error("scratch needs more than the %zu-byte static scratch array "
+                     "(%zu bytes requested at offset %zu)", set.arena_size, bytes, off);
+      return nullptr;
+    }
+    set.ws[slot] = set.arena + off;
+    set.ws_size[slot] = bytes;
+    set.arena_used = off + bytes;
+    return set.ws[slot];
+  }
   if (set.ws[slot]) {
     // The stream may still use the old buffer.
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -76,6 +88,7 @@ void *ws_get(croc_ctx *ctx, WsSlot slot, size_t bytes) {
 }
 
 void ws_free(WsSet &set) {
+  if (set.arena) return;  // not ours
   for (int i = 0; i < WS_COUNT; i++) {
     if (set.ws[i]) (void)hipFree(set.ws[i]);
     set.ws[i] = nullptr;
@@ -90,6 +103,8 @@ extern "C" int croc_ctx_create(int device, croc_ctx **out) {
   CROC_REQUIRE(device >= 0 && device < ndev, CROC_EINVAL,
                "device %d not present (%d devices)", device, ndev);
   CROC_HIP(hipSetDevice(device));
+  int rc = noalloc_init(device);
+  if (rc) return rc;
   croc_ctx *ctx = new croc_ctx();
   ctx->device = device;
   hipDeviceProp_t prop;

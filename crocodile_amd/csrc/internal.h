@@ -66,10 +66,14 @@ enum WsSlot {
   WS_COUNT
 };
 
-// A set of grow-only device buffers, one per slot.
+// A set of grow-only device buffers, one per slot.  With an arena the slots
+// are carved from that fixed device buffer instead (no allocation ever; a
+// request beyond its end fails with CROC_ENOMEM and a message).
 struct WsSet {
   void *ws[WS_COUNT] = {};
   size_t ws_size[WS_COUNT] = {};
+  char *arena = nullptr;
+  size_t arena_size = 0, arena_used = 0;
 };
 
 struct croc_ctx {
@@ -95,6 +99,10 @@ void ws_free(WsSet &set);
 
 // Make the context's device current.
 int ctx_enter(croc_ctx *ctx);
+
+// Device address of the allocation-free entry points' static scratch array
+// on the current device, looked up once per device (noalloc.hip).
+int noalloc_init(int device);
 
 // ---------------------------------------------------------------------------
 // FP64 coordinate arithmetic.  Every product and sum is rounded separately

@@ -149,6 +149,30 @@ int croc_convgrid_plan(croc_ctx *ctx, croc_plan *plan, void *grid,
 int croc_convdegrid_plan(croc_ctx *ctx, croc_plan *plan, const void *grid,
                          void *vis_out, const void *kern);
 
+/* convdegrid without device allocation.  No counterpart in the reference
+ * (whose convdegrid allocates NumPy temporaries freely): for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same arguments, results
+ * and errors as croc_convdegrid (crocodile/synthesis.py:484-503), through
+ * the same binning and tile kernels; every scratch buffer (records, bin
+ * counters, scan space, work items, status) is carved from one static
+ * device array of CROC_NOALLOC_SCRATCH_MIB MiB per GPU, held from library
+ * load until the process ends.  Limits: at most CROC_NOALLOC_MAX_NV
+ * visibilities, and the binning of the grid's tiles must fit the array
+ * (grids up to 8192^2 do at supports >= 16); beyond either, CROC_EINVAL
+ * with a message, outputs untouched.  All calls on one GPU share the
+ * array: calls that overlap on one GPU (two streams or threads) race and
+ * must be serialised by the caller.  Allocates nothing, ever. */
+#define CROC_NOALLOC_SCRATCH_MIB 32
+#define CROC_NOALLOC_MAX_NV 524288
+int croc_convdegrid_noalloc(croc_ctx *ctx, const void *grid, int N,
+                            const double *uvw, int64_t nv, int64_t stride,
+                            double lam, void *vis_out, const void *kern, int Q,
+                            int separable, int nplanes, const int64_t *plane_off,
+                            const int32_t *plane_s, int max_support,
+                            int plane_mode, const int32_t *vplane, double wstep,
+                            int wbin0);
+
 /* grid / degrid (nearest cell): replace crocodile/synthesis.py:389-401 and
  * :404-416, cell N//2 + floor(0.5 + N*p) (round-half-up). */
 int croc_grid_nearest(croc_ctx *ctx, void *grid, int N, const double *uvw,
