@@ -50,17 +50,20 @@ SIGNATURES = {
                                 _I, _I, _P, _D, _I],
     'croc_grid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
     'croc_degrid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
-    'croc_w_kernels': [_P, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P, _I],
+    'croc_w_kernels': [_P, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P, _I, _P],
     'croc_kernel_oversample': [_P, _P, _I, _I, _I, _I, _P],
     'croc_make_hermitian': [_P, _P, _I, _I],
     'croc_dft2d_batched': [_P, _P, _P, _I, _I64, _I],
     'croc_fft2d_batched': [_P, _P, _P, _I, _I64, _I],
     'croc_convolve_same': [_P, _P, _I, _I, _P, _I, _I, _I64, _P],
-    'croc_wtowers_grid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I],
-    'croc_wtowers_degrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I],
+    'croc_wtowers_grid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I,
+                          _I, _D, _D],
+    'croc_wtowers_degrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _D, _I, _I, _P, _I, _I, _I,
+                            _I, _D, _D],
     'croc_wtowers_plan_create': [_P, ctypes.POINTER(_P)],
     'croc_wtowers_plan_destroy': [_P, _P],
-    'croc_wtowers_plan_bin': [_P, _P, _I, _P, _I64, _I64, _D, _D, _I, _I, _I, _I, _I],
+    'croc_wtowers_plan_bin': [_P, _P, _I, _P, _I64, _I64, _D, _D, _I, _I, _I, _I, _I, _I, _D,
+                              _D],
     'croc_wtowers_grid_plan': [_P, _P, _P, _P, _P, _I],
     'croc_wtowers_degrid_plan': [_P, _P, _P, _P, _P, _I],
 }

@@ -46,7 +46,8 @@ struct FFArgs {
 // kernel_coordinates: l = theta * ((l_idx - F//2) / F), likewise m; then the
 // optional transform (l, m) <- (T00 l + T10 m, T01 l + T11 m) and shift.
 __global__ void k_wfarfield(FFArgs a, const double *__restrict__ w,
-                            double2 *__restrict__ ff, uint32_t *status) {
+                            const double *__restrict__ aterm, double2 *__restrict__ ff,
+                            uint32_t *status) {
 #pragma clang fp contract(off)
   const int F = a.F;
   const int p = blockIdx.y;
@@ -83,6 +84,10 @@ __global__ void k_wfarfield(FFArgs a, const double *__restrict__ w,
       const double rt = c / s, den = s + c * rt;
       v = make_double2(rt / den, -1.0 / den);
     }
+  }
+  if (aterm) {  // image-plane A-term and taper: the AW kernel's far field
+    const double t = aterm[idx];
+    v = make_double2(v.x * t, v.y * t);
   }
   ff[(int64_t)p * F * F + idx] = v;
 }
@@ -188,7 +193,7 @@ extern "C" int croc_w_kernels(croc_ctx *ctx, double theta, const double *w,
                               int nw, int npix_ff, const int32_t *S, int Q,
                               int invert, double dl, double dm,
                               const double *T, const int64_t *plane_off,
-                              void *out, int out_double) {
+                              void *out, int out_double, const double *aterm) {
   int rc = ctx_enter(ctx);
   if (rc) return rc;
   CROC_REQUIRE(w && S && plane_off && out, CROC_EINVAL, "null argument");
@@ -235,7 +240,7 @@ extern "C" int croc_w_kernels(croc_ctx *ctx, double theta, const double *w,
   fa.invert = invert;
   fa.F = F;
   dim3 g((unsigned)((ffsz + 255) / 256), nw);
-  k_wfarfield<<<g, 256, 0, ctx->stream>>>(fa, dw, ff, dst);
+  k_wfarfield<<<g, 256, 0, ctx->stream>>>(fa, dw, aterm, ff, dst);
   CROC_HIP(hipGetLastError());
   rc = oversample_planes(ctx, ff, ffsz, F, Q, nw, dS, maxS, doff, out,
                          out_double);

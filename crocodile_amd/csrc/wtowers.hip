@@ -44,6 +44,11 @@ struct TowerArgs {
   double theta, wstep;
   int wp_min, nwp;
   int Q, S, nearest;
+  // w-kernel set of the residual w - wp wstep (grid.c's w_kernel_data):
+  // plane floor((w - wp wstep - wk_min) / wk_step + 0.5) of nwk; one plane
+  // (nwk = 1) serves every visibility
+  int nwk;
+  double wk_min, wk_step;
   const double *uvw;
   int64_t nv, stride;
 };
@@ -137,10 +142,21 @@ __device__ uint4 tw_locate(const TowerArgs &a, int64_t i, uint32_t *st) {
       return make_uint4(0, 0, 0, INVALID);
     }
   }
+  // w-kernel plane of the residual w relative to the plane's w (grid.c:286-289)
+  int wk = 0;
+  if (a.nwk > 1) {
+    const double dw = pu[2] - (double)wp * a.wstep;
+    const double q = floor((dw - a.wk_min) / a.wk_step + 0.5);
+    if (!(q >= 0.0 && q < (double)a.nwk)) {
+      *st |= ST_PLANE;
+      return make_uint4(0, 0, 0, INVALID);
+    }
+    wk = (int)q;
+  }
   const uint32_t key =
       (uint32_t)(((int64_t)cy * a.ncx + cx) * a.nwp + (wp - a.wp_min));
   return make_uint4((uint32_t)wx | ((uint32_t)wy << 16), (uint32_t)xf | ((uint32_t)yf << 16),
-                    0u, key);
+                    (uint32_t)wk, key);
 }
 
 // Located records, and (job key, visibility index) pairs for the sort; an
@@ -159,8 +175,9 @@ __global__ void k_tw_locate(TowerArgs a, uint32_t bad, uint4 *__restrict__ loc,
   if (st) atomicOr(status, st);
 }
 
-// Records (and visibilities) in job order: rec[pos] = (window, offsets, 0,
-// visibility index) of the pos-th visibility of the sorted order.
+// Records (and visibilities) in job order: rec[pos] = (window, offsets,
+// w-kernel plane, visibility index) of the pos-th visibility of the sorted
+// order.
 __global__ void k_tw_gather(int64_t nv, const uint32_t *__restrict__ sidx,
                             const uint4 *__restrict__ loc, const float2 *__restrict__ vis,
                             uint4 *__restrict__ rec, float2 *__restrict__ svis) {
@@ -168,7 +185,7 @@ __global__ void k_tw_gather(int64_t nv, const uint32_t *__restrict__ sidx,
   if (pos >= nv) return;
   const uint32_t i = sidx[pos];
   const uint4 L = loc[i];
-  rec[pos] = make_uint4(L.x, L.y, 0u, i);
+  rec[pos] = make_uint4(L.x, L.y, L.z, i);
   if (vis) svis[pos] = vis[i];
 }
 
@@ -230,7 +247,7 @@ template <bool SEPARABLE>
 __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
     const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
     const float2 *__restrict__ svis, const float2 *__restrict__ kern, int Q, int S,
-    int nearest, const TwScale *__restrict__ fs, float2 *__restrict__ sub,
+    int64_t tsize, int nearest, const TwScale *__restrict__ fs, float2 *__restrict__ sub,
     uint32_t *__restrict__ qc) {
   constexpr int ROWS = GRID_ROWS, HC = ROWS * SG;
   __shared__ unsigned long long acc[ROWS * GRID_RU64];  // re, im interleaved (128 KiB + pads)
@@ -311,8 +328,9 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
               // window rows wy + i in the pass: i in [ilo, ihi)
               const int ilo = max(0, r0 - wy), ihi = min(S, r0 + ROWS - wy);
               const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
-              const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
-              const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
+              const float2 *kt = kern + (int64_t)r.z * tsize;  // its w-kernel plane
+              const float2 *k2 = kt + (int64_t)(yf * Q + xf) * SS;
+              const float2 *ky = kt + (int64_t)yf * S, *kx = kt + (int64_t)xf * S;
               // cell of tap t: (wy - r0) * SG + wx + t + i * (SG - S), in
               // [0, HC) for the pass's rows, at byte 16 * cell + 8 * its row
               spread_range<SEPARABLE>(acc, (wy - r0) * SG + wx, SG, S, ilo * S, ihi * S, lane,
@@ -430,8 +448,8 @@ __global__ void k_tw_cut(const int32_t *__restrict__ chunks, const float2 *__res
 template <bool SEPARABLE>
 __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
     const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
-    const float2 *__restrict__ sub, const float2 *__restrict__ kern, int Q, int S, int nearest,
-    float2 *__restrict__ vis_out, uint32_t *__restrict__ qc) {
+    const float2 *__restrict__ sub, const float2 *__restrict__ kern, int Q, int S,
+    int64_t tsize, int nearest, float2 *__restrict__ vis_out, uint32_t *__restrict__ qc) {
   // Rows padded by 128 bytes (DEG_PITCH cells): the bare 1 KiB pitch would
   // put the rows of a 64-tap block on the same LDS banks.
   constexpr int DEG_PITCH = SG + 16;
@@ -480,8 +498,9 @@ __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
           const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
           const int wy = (int)(r.x >> 16);
           const int base = wy * SG + (int)(r.x & 0xffffu);
-          const float2 *k2 = kern + (int64_t)(yf * Q + xf) * SS;
-          const float2 *ky = kern + (int64_t)yf * S, *kx = kern + (int64_t)xf * S;
+          const float2 *kt = kern + (int64_t)r.z * tsize;  // its w-kernel plane
+          const float2 *k2 = kt + (int64_t)(yf * Q + xf) * SS;
+          const float2 *ky = kt + (int64_t)yf * S, *kx = kt + (int64_t)xf * S;
           float sr = 0.f, si = 0.f;
           gather_range<SEPARABLE>(sg, base, SG, S, 0, SS, lane, k2, ky, kx, sr, si,
                                   8u * (DEG_PITCH - SG), 8u * (DEG_PITCH - SG) * (uint32_t)wy);
@@ -614,7 +633,7 @@ int tw_bin(croc_ctx *ctx, Plan &p, const float2 *vis) {
 
 int tw_setup(croc_ctx *ctx, Plan &p, int N, const double *uvw, int64_t nv, int64_t stride,
              double theta, double wstep, int subgrid, int margin, bool has_kernel, int Q,
-             int S) {
+             int S, int nwk, double wk_min, double wk_step) {
   CROC_REQUIRE(subgrid == SG, CROC_EINVAL, "subgrid size must be %d", SG);
   CROC_REQUIRE(margin >= 0 && margin % 2 == 0 && margin <= subgrid / 2, CROC_EINVAL,
                "margin must be even and in [0, subgrid/2]");
@@ -622,6 +641,13 @@ int tw_setup(croc_ctx *ctx, Plan &p, int N, const double *uvw, int64_t nv, int64
   CROC_REQUIRE(nv < (int64_t)INT32_MAX, CROC_EINVAL, "too many visibilities");
   CROC_REQUIRE(!has_kernel || (Q >= 1 && Q <= 65535 && S >= 1 && S <= margin),
                CROC_EINVAL, "kernel support %d must not exceed the margin %d", S, margin);
+  CROC_REQUIRE(nwk >= 1 && (has_kernel || nwk == 1), CROC_EINVAL,
+               "w-kernel plane count must be >= 1 (1 without a kernel)");
+  // the set must cover the whole w-increment (grid.c:493-496)
+  CROC_REQUIRE(nwk == 1 || (wk_step > 0 && wk_min <= -wstep / 2 &&
+                            wk_min + (nwk - 1) * wk_step >= wstep / 2),
+               CROC_EINVAL, "the w-kernel set [%g, %g] does not cover a whole w-increment "
+               "[%g, %g]", wk_min, wk_min + (nwk - 1) * wk_step, -wstep / 2, wstep / 2);
   TowerArgs &a = p.a;
   a.N = N;
   a.sg = subgrid;
@@ -633,6 +659,9 @@ int tw_setup(croc_ctx *ctx, Plan &p, int N, const double *uvw, int64_t nv, int64
   a.Q = has_kernel ? Q : 1;
   a.S = has_kernel ? S : 1;
   a.nearest = !has_kernel;
+  a.nwk = nwk;
+  a.wk_min = wk_min;
+  a.wk_step = wk_step > 0 ? wk_step : 1.0;
   a.uvw = uvw;
   a.nv = nv;
   a.stride = stride;
@@ -655,7 +684,8 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
 extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double *uvw,
                                  int64_t nv, int64_t stride, double theta, const void *vis,
                                  double wstep, int subgrid, int margin, const void *kern,
-                                 int Q, int S, int separable) {
+                                 int Q, int S, int separable, int nwk, double wk_min,
+                                 double wk_step) {
   int rc = ctx_enter(ctx);
   if (rc) return rc;
   CROC_REQUIRE(grid, CROC_EINVAL, "null grid");
@@ -663,7 +693,7 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
   CROC_REQUIRE(uvw && vis, CROC_EINVAL, "null uvw / vis");
   Plan p;
   if ((rc = tw_setup(ctx, p, N, uvw, nv, stride, theta, wstep, subgrid, margin,
-                     kern != nullptr, Q, S)))
+                     kern != nullptr, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if ((rc = tw_bin(ctx, p, (const float2 *)vis))) return rc;
   return tw_grid_run(ctx, p, grid, kern, separable);
@@ -722,7 +752,8 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
 
   // fixed-point scale, Fresnel phase table, per-job plane multipliers
   const int64_t Q = a.Q, S = a.S;
-  const int64_t nk = kern ? (separable ? Q * S : Q * Q * S * S) : 0;
+  const int64_t tsize = separable ? Q * S : Q * Q * S * S;  // elements per w-kernel plane
+  const int64_t nk = kern ? tsize * a.nwk : 0;
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
                                                 sizeof(int32_t) * (5 * (size_t)p.njobs + 64) +
                                                 sizeof(uint32_t) * (batches.size() + 1));
@@ -770,12 +801,12 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
     const int G = (int)std::min<size_t>(bt.np, (size_t)ctx->num_cu);  // one per CU (LDS)
     if (a.nearest || !separable)
       k_tw_grid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
-                                                (const float2 *)kern, a.Q, a.S, a.nearest, fs,
-                                                sub, qc + bi);
+                                                (const float2 *)kern, a.Q, a.S, tsize,
+                                                a.nearest, fs, sub, qc + bi);
     else
       k_tw_grid_jobs<true><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
-                                               (const float2 *)kern, a.Q, a.S, a.nearest, fs,
-                                               sub, qc + bi);
+                                               (const float2 *)kern, a.Q, a.S, tsize,
+                                               a.nearest, fs, sub, qc + bi);
     CROC_HIP(hipGetLastError());
     if (bt.nm) {
       k_tw_sum_parts<<<dim3(ACC_SLICES, (unsigned)bt.nm), 256, 0, s>>>(dmulti + bt.m0, sub);
@@ -819,7 +850,8 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
 extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const double *uvw,
                                    int64_t nv, int64_t stride, double theta, void *vis_out,
                                    double wstep, int subgrid, int margin, const void *kern,
-                                   int Q, int S, int separable) {
+                                   int Q, int S, int separable, int nwk, double wk_min,
+                                   double wk_step) {
   int rc = ctx_enter(ctx);
   if (rc) return rc;
   CROC_REQUIRE(grid, CROC_EINVAL, "null grid");
@@ -827,7 +859,7 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
   CROC_REQUIRE(uvw && vis_out, CROC_EINVAL, "null uvw / vis_out");
   Plan p;
   if ((rc = tw_setup(ctx, p, N, uvw, nv, stride, theta, wstep, subgrid, margin,
-                     kern != nullptr, Q, S)))
+                     kern != nullptr, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if ((rc = tw_bin(ctx, p, nullptr))) return rc;
   return tw_degrid_run(ctx, p, grid, vis_out, kern, separable);
@@ -869,6 +901,7 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
   int32_t *dchunks = jchunk + p.njobs;
   uint32_t *qc = (uint32_t *)(kw + 3 * (size_t)p.njobs + 64);  // a part queue per batch
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
+  const int64_t tsize = separable ? (int64_t)a.Q * a.S : (int64_t)a.Q * a.Q * a.S * a.S;
   std::vector<int32_t> hk(p.njobs), hc(p.njobs);
   for (int j = 0; j < p.njobs; j++) {
     hk[j] = p.hjobs[j].x % a.nwp + a.wp_min;
@@ -898,12 +931,12 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
     const int G = std::min(npart, ctx->num_cu);  // one per CU (LDS)
     if (a.nearest || !separable)
       k_tw_degrid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
-                                                  (const float2 *)kern, a.Q, a.S, a.nearest,
-                                                  (float2 *)vis_out, qc + bi);
+                                                  (const float2 *)kern, a.Q, a.S, tsize,
+                                                  a.nearest, (float2 *)vis_out, qc + bi);
     else
       k_tw_degrid_jobs<true><<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
-                                                 (const float2 *)kern, a.Q, a.S, a.nearest,
-                                                 (float2 *)vis_out, qc + bi);
+                                                 (const float2 *)kern, a.Q, a.S, tsize,
+                                                 a.nearest, (float2 *)vis_out, qc + bi);
     CROC_HIP(hipGetLastError());
   }
   CROC_HIP(hipEventRecord(ctx->ev[3], s));
@@ -971,7 +1004,8 @@ extern "C" int croc_wtowers_plan_destroy(croc_ctx *ctx, croc_tw_plan *plan) {
 extern "C" int croc_wtowers_plan_bin(croc_ctx *ctx, croc_tw_plan *plan, int N,
                                      const double *uvw, int64_t nv, int64_t stride,
                                      double theta, double wstep, int subgrid, int margin,
-                                     int has_kernel, int Q, int S) {
+                                     int has_kernel, int Q, int S, int nwk, double wk_min,
+                                     double wk_step) {
   int rc = ctx_enter(ctx);
   if (rc) return rc;
   CROC_REQUIRE(plan != nullptr, CROC_EINVAL, "null plan");
@@ -983,7 +1017,7 @@ extern "C" int croc_wtowers_plan_bin(croc_ctx *ctx, croc_tw_plan *plan, int N,
   p = Plan();
   p.keep = &plan->ws;
   if ((rc = tw_setup(ctx, p, N, uvw, nv, stride, theta, wstep, subgrid, margin,
-                     has_kernel != 0, Q, S)))
+                     has_kernel != 0, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if (nv > 0 && (rc = tw_bin(ctx, p, nullptr))) return rc;
   plan->ready = true;

@@ -257,12 +257,15 @@ def _w_kernel_args(kwargs):
 
 
 def w_kernel_bank(theta, ws, NpixFF, supports, Qpx, invert=True, scale=1.0,
-                  double=False, **kwargs):
+                  double=False, aterm=None, **kwargs):
     """Generate the w-kernels of many planes in one device launch.
 
     Returns a :class:`KernelBank` (complex64 unless ``double``) whose plane k
     is ``scale * w_kernel(theta, ws[k], NpixFF, supports[k], Qpx, invert,
     **kwargs)``.  Same arithmetic as :func:`w_kernel` (synthesis.py:357-377).
+    ``aterm`` (real [NpixFF, NpixFF], optional) multiplies every plane's far
+    field before oversampling: an anti-aliasing taper times an A-term gives
+    AW kernels, the image-plane form of aw_kernel_fn (synthesis.py:701-734).
     """
     ws = numpy.ascontiguousarray(numpy.asarray(ws, dtype=numpy.float64).reshape(-1))
     S = numpy.ascontiguousarray(numpy.broadcast_to(
@@ -274,12 +277,17 @@ def w_kernel_bank(theta, ws, NpixFF, supports, Qpx, invert=True, scale=1.0,
     off = numpy.concatenate([[0], numpy.cumsum(sizes)[:-1]]).astype(numpy.int64)
     dtype = torch.complex128 if double else torch.complex64
     out = torch.empty(int(sizes.sum()), dtype=dtype, device=_lib.device())
+    at = None
+    if aterm is not None:
+        at = _dev(aterm, torch.float64)
+        if tuple(at.shape) != (int(NpixFF), int(NpixFF)):
+            raise ValueError('aterm must have shape [NpixFF, NpixFF]')
     L = _lib.load()
     _lib.check(L.croc_w_kernels(
         _lib.ctx(), float(theta), ws.ctypes.data_as(_lib._P), len(ws), int(NpixFF),
         S.ctypes.data_as(_lib._P), int(Qpx), int(bool(invert)), dl, dm,
         None if Tv is None else Tv.ctypes.data_as(_lib._P),
-        off.ctypes.data_as(_lib._P), _lib.ptr(out), int(bool(double))))
+        off.ctypes.data_as(_lib._P), _lib.ptr(out), int(bool(double)), _lib.ptr(at)))
     if scale != 1.0:
         out.mul_(scale)
     return KernelBank(out, off, S, Qpx, separable=False)
@@ -1021,19 +1029,75 @@ def sample_image(image, x, y):
 # --------------------------------------------------------------------------
 
 
+class WKernelSet:
+    """Kernels of the w-towers sub-grid gridding for the residual w of each
+    visibility relative to its w-plane (grid.c's w_kernel_data, grid.h:
+    114-122, as used by w_project, grid.c:277-305): ``tables[k]`` serves
+    residuals near ``w_min + k * w_step``; a visibility of w-plane wp takes
+    table floor((w - wp * wstep - w_min) / w_step + 0.5).  Tables are all
+    [Q, Q, S, S] (2D) or all [Q, S] (separable).  The set must cover the
+    w-increment [-wstep/2, wstep/2] (grid.c:493-496)."""
+
+    def __init__(self, tables, w_min, w_step):
+        t = _dev(tables, torch.complex64)
+        if t.ndim not in (3, 5):
+            raise AssertionError('w-kernel set tables must be [n, Q, S] or [n, Q, Q, S, S]')
+        self.tables = t.contiguous()
+        self.w_min, self.w_step = float(w_min), float(w_step)
+
+    @property
+    def nplanes(self):
+        return self.tables.shape[0]
+
+    @property
+    def w_max(self):
+        return self.w_min + (self.nplanes - 1) * self.w_step
+
+
+def w_towers_kernel_set(theta, wincrement, NpixFF, NpixKern, Qpx, step, taper=None,
+                        invert=True):
+    """Separable w-kernel set for the w-towers drivers: table k is the
+    [Qpx, NpixKern] oversampled kernel (kernel_oversample, synthesis.py:
+    330-354) of the 1D far field taper(l) * exp(-+ i pi w_k l^2), l =
+    theta * coordinates(NpixFF), w_k = w_min + k * step covering
+    [-wincrement/2, wincrement/2].  The product over both axes is the
+    w-kernel of the residual w_k in the paraxial form w (1 - n) ~ w (l^2 +
+    m^2) / 2 (its error, w r^4 / 8, stays below 2e-7 w at r^2 <= 1.25e-3)
+    times the separable taper.  ``invert`` gives the imaging kernel
+    (exp(-i ...), as w_kernel's 1 / w_kernel_function), else the predict
+    kernel."""
+    half = 0.5 * float(wincrement)
+    n = int(math.ceil(2 * half / step - 1e-9)) + 1
+    w_min = -(n - 1) * step / 2
+    l = coordinates(NpixFF) * theta
+    a = numpy.ones(NpixFF) if taper is None else numpy.asarray(taper, dtype=float)
+    sign = -1.0 if invert else 1.0
+    ff = numpy.array([a * numpy.exp(sign * 1j * numpy.pi * (w_min + k * step) * l ** 2)
+                      for k in range(n)])
+    tabs = [kernel_oversample(torch.as_tensor(f, device=_lib.device()), Qpx, NpixKern)
+            for f in ff]
+    return WKernelSet(torch.stack(tabs), w_min, step)
+
+
 def _tower_kernel(kv):
-    """(device table or None, Q, S, separable) for the sub-grid kernel."""
+    """(device tables or None, Q, S, separable, nwk, wk_min, wk_step) for the
+    sub-grid kernel: one table, or a :class:`WKernelSet`."""
     if kv is None:
-        return None, 1, 1, 0
-    t = _dev(kv, torch.complex64)
-    if t.ndim == 2:
-        Q, S = t.shape
-        return t.reshape(-1), int(Q), int(S), 1
-    if t.ndim == 4:
-        Q, Q2, gh, gw = t.shape
+        return None, 1, 1, 0, 1, 0.0, 0.0
+    if isinstance(kv, WKernelSet):
+        t, n, w_min, w_step = kv.tables, kv.nplanes, kv.w_min, kv.w_step
+        shape = tuple(t.shape[1:])
+    else:
+        t, n, w_min, w_step = _dev(kv, torch.complex64), 1, 0.0, 0.0
+        shape = tuple(t.shape)
+    if len(shape) == 2:
+        Q, S = shape
+        return t.reshape(-1), int(Q), int(S), 1, n, w_min, w_step
+    if len(shape) == 4:
+        Q, Q2, gh, gw = shape
         if gh != gw or Q != Q2:
             raise NotImplementedError('w-towers need a square [Q,Q,S,S] kernel')
-        return t.reshape(-1), int(Q), int(gw), 0
+        return t.reshape(-1), int(Q), int(gw), 0, n, w_min, w_step
     raise AssertionError('unsupported kernel table rank')
 
 
@@ -1043,19 +1107,21 @@ def w_towers_imaging(theta, lam, uvw, src, vis, wstep=100.0, subgrid_size=128, m
     subgrid_size - margin cells and w-planes round(w / wstep); every
     (chunk, plane) is gridded on its own sub-grid (nearest cell when ``kv``
     is None, as the notebook's simple_imaging, else convolution with the
-    oversampled kernel ``kv``), inverse transformed, multiplied by
+    oversampled kernel ``kv``, or with a :class:`WKernelSet` the kernel of
+    each visibility's residual w as grid.c's w_project), inverse transformed, multiplied by
     exp(-2 pi i wp wstep (1 - n)), summed per chunk, transformed back and
     added into the grid.  wtowers.ipynb imaging cell; grid.c:482-710."""
     N = _grid_size(theta, lam)
     uv = _uvw(uvw)
     if uv.shape[1] < 3:
         raise ValueError('w-towers need uvw with a w column')
-    kern, Q, S, sep = _tower_kernel(kv)
+    kern, Q, S, sep, nwk, wk0, wks = _tower_kernel(kv)
     guv = torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
     vt = _vis_dev(vis)
     _lib.check(_lib.load().croc_wtowers_grid(
         _lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uv), uv.shape[0], uv.shape[1], float(theta),
-        _lib.ptr(vt), float(wstep), int(subgrid_size), int(margin), _lib.ptr(kern), Q, S, sep))
+        _lib.ptr(vt), float(wstep), int(subgrid_size), int(margin), _lib.ptr(kern), Q, S, sep,
+        nwk, wk0, wks))
     return _out(guv, _torch_in(uvw, vis))
 
 
@@ -1071,11 +1137,12 @@ def w_towers_predict(theta, lam, uvw, src, guv, wstep=100.0, subgrid_size=128, m
     gt = _dev(guv, torch.complex64)
     if _square(gt) != N:
         raise ValueError('grid size does not match round(theta * lam)')
-    kern, Q, S, sep = _tower_kernel(kv)
+    kern, Q, S, sep, nwk, wk0, wks = _tower_kernel(kv)
     out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
     _lib.check(_lib.load().croc_wtowers_degrid(
         _lib.ctx(), _lib.ptr(gt), N, _lib.ptr(uv), uv.shape[0], uv.shape[1], float(theta),
-        _lib.ptr(out), float(wstep), int(subgrid_size), int(margin), _lib.ptr(kern), Q, S, sep))
+        _lib.ptr(out), float(wstep), int(subgrid_size), int(margin), _lib.ptr(kern), Q, S, sep,
+        nwk, wk0, wks))
     return _out(out, _torch_in(uvw, guv))
 
 
@@ -1102,18 +1169,19 @@ class WTowersPlan:
         uv = _uvw(uvw)
         if uv.shape[1] < 3:
             raise ValueError('w-towers need uvw with a w column')
-        kern, Q, S, _ = _tower_kernel(kv)
+        kern, Q, S, _, nwk, wk0, wks = _tower_kernel(kv)
         self.nv = 0
         _lib.check(_lib.load().croc_wtowers_plan_bin(
             _lib.ctx(), self._h, self.N, _lib.ptr(uv), uv.shape[0], uv.shape[1], self.theta,
-            self.wstep, self.subgrid_size, self.margin, int(kern is not None), Q, S))
-        self._shape = (kern is not None, Q, S)
+            self.wstep, self.subgrid_size, self.margin, int(kern is not None), Q, S,
+            nwk, wk0, wks))
+        self._shape = (kern is not None, Q, S, nwk, wk0, wks)
         self.nv, self._device = int(uv.shape[0]), uv.device
 
     def _kernel(self, kv):
-        kern, Q, S, sep = _tower_kernel(kv)
-        if (kern is not None, Q, S) != self._shape:
-            raise ValueError('kernel shape differs from the planned one')
+        kern, Q, S, sep, nwk, wk0, wks = _tower_kernel(kv)
+        if (kern is not None, Q, S, nwk, wk0, wks) != self._shape:
+            raise ValueError('kernel shape or w-kernel set differs from the planned one')
         return kern, sep
 
     def imaging(self, vis, kv=None):

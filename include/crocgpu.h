@@ -188,11 +188,16 @@ int croc_degrid_nearest(croc_ctx *ctx, const void *grid, int N,
  * (NpixKern per plane); T: host 2x2 matrix row-major or NULL.  Output
  * written at out + plane_off[k] (host offsets, complex elements), layout
  * [Q][Q][S][S]; out_double selects cf64 (1) or cf32 (0) output.
+ * aterm: NULL, or a device array double [npix_ff][npix_ff] multiplying
+ * every plane's far field (an anti-aliasing taper times an A-term), which
+ * makes AW kernels: the image-plane product whose uv-plane form
+ * aw_kernel_fn (synthesis.py:701-734) builds by convolution.
  * Computed in FP64 (phase and the oversampled DFT). */
 int croc_w_kernels(croc_ctx *ctx, double theta, const double *w, int nw,
                    int npix_ff, const int32_t *S, int Q, int invert,
                    double dl, double dm, const double *T,
-                   const int64_t *plane_off, void *out, int out_double);
+                   const int64_t *plane_off, void *out, int out_double,
+                   const double *aterm);
 
 /* kernel_oversample: replaces crocodile/synthesis.py:330-354 for a device
  * far field ff (cf64, [n][n] when ndim == 2, [n] when ndim == 1):
@@ -219,17 +224,24 @@ int croc_fft2d_batched(croc_ctx *ctx, const void *in, void *out, int n,
 /* W-towers imaging: replaces the w-stacking-by-sub-grid drivers of
  * examples/notebooks/wtowers.ipynb (imaging cell) and grid.c:482-710
  * (grid_wtowers).  Visibilities are binned into (chunk, w-plane) jobs, chunk
- * = subgrid - margin cells, w-plane = rint(w / wstep); each job is gridded
- * on a subgrid x subgrid sub-grid (kern == NULL: nearest cell as
+ * = subgrid - margin cells, w-plane wp = rint(w / wstep); each job is
+ * gridded on a subgrid x subgrid sub-grid (kern == NULL: nearest cell as
  * simple_imaging; else the Q-oversampled kernel, separable [Q][S] or 2D
  * [Q][Q][S][S]), inverse-transformed, multiplied by
  * exp(-2 pi i wp wstep (1 - n)), summed per chunk, transformed back and
  * added into grid (cf32 [N][N], in place).  subgrid must be 128, margin
- * even, S <= margin <= subgrid/2.  uvw in wavelengths, stride >= 3. */
+ * even, S <= margin <= subgrid/2.  uvw in wavelengths, stride >= 3.
+ * kern may hold a w-kernel set (grid.c's w_kernel_data, grid.h:114-122,
+ * used by w_project grid.c:277-305): nwk tables of the shape above, table k
+ * for the residual w - wp wstep = wk_min + k wk_step, chosen per visibility
+ * as floor((w - wp wstep - wk_min) / wk_step + 0.5); the set must cover
+ * [-wstep/2, wstep/2] (grid.c:493-496).  nwk = 1 uses the one table for
+ * every visibility (the notebook's form; wk_min, wk_step ignored). */
 int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double *uvw,
                       int64_t nv, int64_t stride, double theta, const void *vis,
                       double wstep, int subgrid, int margin, const void *kern,
-                      int Q, int S, int separable);
+                      int Q, int S, int separable, int nwk, double wk_min,
+                      double wk_step);
 
 /* W-towers predict: replaces examples/notebooks/wtowers-predict.ipynb
  * (predict cell): per chunk, inverse transform of the grid window; per
@@ -239,7 +251,7 @@ int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N,
                         const double *uvw, int64_t nv, int64_t stride,
                         double theta, void *vis_out, double wstep, int subgrid,
                         int margin, const void *kern, int Q, int S,
-                        int separable);
+                        int separable, int nwk, double wk_min, double wk_step);
 
 /* W-towers plans: the (chunk, w-plane) binning of one set of coordinates,
  * reused by any number of croc_wtowers_grid_plan / croc_wtowers_degrid_plan
@@ -257,7 +269,8 @@ int croc_wtowers_plan_destroy(croc_ctx *ctx, croc_tw_plan *plan);
 int croc_wtowers_plan_bin(croc_ctx *ctx, croc_tw_plan *plan, int N,
                           const double *uvw, int64_t nv, int64_t stride,
                           double theta, double wstep, int subgrid, int margin,
-                          int has_kernel, int Q, int S);
+                          int has_kernel, int Q, int S, int nwk, double wk_min,
+                          double wk_step);
 int croc_wtowers_grid_plan(croc_ctx *ctx, croc_tw_plan *plan, void *grid,
                            const void *vis, const void *kern, int separable);
 int croc_wtowers_degrid_plan(croc_ctx *ctx, croc_tw_plan *plan,

@@ -28,6 +28,12 @@ grid.c's grid-index chunking is used):
 The imaging notebook skips the Fresnel step for the second plane when a
 bin's first plane is the global first one (`if last_wp != wplanes[0]`); that
 is a bug (grid.c always steps), not reproduced here.
+
+The kernel ``kv`` is one table (the notebooks' form: every visibility of a
+job uses it) or a w-kernel set ``(tables, w_min, w_step)`` as grid.c's
+w_kernel_data: a visibility of plane wp uses table
+floor((w - wp wstep - w_min) / w_step + 0.5) (w_project, grid.c:277-305,
+with the residual w relative to the plane's w, grid.c:609-617).
 """
 
 import numpy
@@ -58,6 +64,18 @@ def fresnel_phase(subgrid_size, theta):
     return 1 - numpy.sqrt(1 - l ** 2 - m ** 2)
 
 
+def _kernel_planes(kv, uvw, wp, wstep):
+    """(tables, plane index per visibility) for a single table or a
+    w-kernel set (grid.c:286-289)."""
+    if isinstance(kv, tuple):
+        tables, w_min, w_step = kv
+        dw = uvw[:, 2] - wp * wstep
+        k = numpy.floor((dw - w_min) / w_step + 0.5).astype(int)
+        assert (k >= 0).all() and (k < len(tables)).all(), 'w-kernel set too small'
+        return tables, k
+    return [kv], numpy.zeros(len(uvw), dtype=int)
+
+
 def _sub_coords(uvw, theta, N, sg, x0, y0):
     u_mid = (x0 + sg // 2 - N // 2) / theta
     v_mid = (y0 + sg // 2 - N // 2) / theta
@@ -72,6 +90,8 @@ def w_towers_imaging(theta, lam, uvw, src, vis, wstep, subgrid_size, margin, kv=
     sg = subgrid_size
     ph = fresnel_phase(sg, theta)
     guv = numpy.zeros((N, N), dtype=complex)
+    if kv is not None:
+        tables, kidx = _kernel_planes(kv, uvw, wp, wstep)
     keys = sorted(set(zip(cy.tolist(), cx.tolist())))
     for (ky, kx) in keys:
         in_chunk = (cx == kx) & (cy == ky)
@@ -85,7 +105,9 @@ def w_towers_imaging(theta, lam, uvw, src, vis, wstep, subgrid_size, margin, kv=
             if kv is None:
                 R.grid(sub, p, vis[sel])
             else:
-                R.convgrid(kv, sub, p, None, vis[sel])
+                for k in sorted(set(kidx[sel].tolist())):
+                    sk = kidx[sel] == k
+                    R.convgrid(tables[k], sub, p[sk], None, vis[sel][sk])
             img += R.ifft(sub) * numpy.exp(-2j * numpy.pi * (w * wstep) * ph)
         sub = R.fft(img)
         xa, xb = max(x0, 0), min(x0 + sg, N)
@@ -99,6 +121,8 @@ def w_towers_predict(theta, lam, uvw, src, guv, wstep, subgrid_size, margin, kv=
     sg = subgrid_size
     ph = fresnel_phase(sg, theta)
     out = numpy.zeros(len(uvw), dtype=complex)
+    if kv is not None:
+        tables, kidx = _kernel_planes(kv, uvw, wp, wstep)
     keys = sorted(set(zip(cy.tolist(), cx.tolist())))
     for (ky, kx) in keys:
         in_chunk = (cx == kx) & (cy == ky)
@@ -116,5 +140,8 @@ def w_towers_predict(theta, lam, uvw, src, guv, wstep, subgrid_size, margin, kv=
             if kv is None:
                 out[sel] = R.degrid(psub, p)
             else:
-                out[sel] = R.convdegrid(kv, psub, p)
+                idx = numpy.flatnonzero(sel)
+                for k in sorted(set(kidx[sel].tolist())):
+                    sk = kidx[sel] == k
+                    out[idx[sk]] = R.convdegrid(tables[k], psub, p[sk])
     return out

@@ -147,3 +147,62 @@ def test_wtowers_plan_errors_and_empty(gpu):
     empty = G.WTowersPlan(THETA, LAM, numpy.zeros((0, 3)), 40.0, SG, MARGIN, None)
     assert not empty.imaging(numpy.zeros(0, complex)).any()
     assert empty.predict(numpy.ones((512, 512), complex)).shape == (0,)
+
+
+def kernel_sets(rng, wstep):
+    """A separable residual w-kernel set (PSWF taper times the paraxial
+    residual w-term, G.w_towers_kernel_set) and a random 2D set, both
+    covering [-wstep/2, wstep/2]; each as the product's WKernelSet and the
+    oracle's (tables, w_min, w_step)."""
+    aaf = R.anti_aliasing_function(512, 0, 8)
+    sep = G.w_towers_kernel_set(THETA, wstep, 512, 8, 16, 2.0, taper=aaf, invert=True)
+    sep_ref = (sep.tables.cpu().numpy().astype(complex), sep.w_min, sep.w_step)
+    n = 7
+    t2 = rng.standard_normal((n, 4, 4, 6, 6)) + 1j * rng.standard_normal((n, 4, 4, 6, 6))
+    w_min = -(n - 1) * (wstep / (n - 1)) / 2
+    two = G.WKernelSet(t2, w_min, wstep / (n - 1))
+    two_ref = (two.tables.cpu().numpy().astype(complex), two.w_min, two.w_step)
+    return {'separable-set': (sep, sep_ref), '2d-set': (two, two_ref)}
+
+
+def test_wtowers_kernel_set_generation(gpu):
+    """w_towers_kernel_set: table k is kernel_oversample of taper(l)
+    exp(-i pi w_k l^2) (1D, FP64 on the device), against the oracle's
+    NumPy FFT route."""
+    aaf = R.anti_aliasing_function(256, 0, 8)
+    ks = G.w_towers_kernel_set(0.1, 40.0, 256, 8, 8, 4.0, taper=aaf, invert=False)
+    assert ks.nplanes == 11 and ks.w_min == -20.0 and ks.w_max == 20.0
+    l = R.coordinates(256) * 0.1
+    for k in (0, 3, 10):
+        w = ks.w_min + k * ks.w_step
+        ref = R.kernel_oversample(aaf * numpy.exp(1j * numpy.pi * w * l ** 2), 8, 8)
+        got = ks.tables[k].cpu().numpy()
+        assert numpy.abs(got - ref).max() <= 1e-6 * numpy.abs(ref).max()
+
+
+@pytest.mark.parametrize('kind', ['separable-set', '2d-set'])
+def test_wtowers_kernel_set_parity(gpu, kind):
+    """Imaging and predict with a residual w-kernel set (w_project's table
+    choice per visibility) against the FP64 oracle, one-shot and planned."""
+    rng, uvw, vis = case(9, 6000, 400.0)
+    kset, kref = kernel_sets(rng, 40.0)[kind]
+    g = G.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, kset)
+    ref = WT.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, kref)
+    assert_close(g, ref, what='w_towers_imaging ' + kind)
+    grid = rng.standard_normal((512, 512)) + 1j * rng.standard_normal((512, 512))
+    v = G.w_towers_predict(THETA, LAM, uvw, None, grid, 40.0, SG, MARGIN, kset)
+    vref = WT.w_towers_predict(THETA, LAM, uvw, None, grid, 40.0, SG, MARGIN, kref)
+    assert_close(v, vref, what='w_towers_predict ' + kind)
+    plan = G.WTowersPlan(THETA, LAM, uvw, 40.0, SG, MARGIN, kset)
+    assert (plan.imaging(vis, kset) == g).all()
+    assert (plan.predict(grid, kset) == v).all()
+    with pytest.raises(ValueError):  # another set layout than the planned one
+        plan.imaging(vis, G.WKernelSet(kset.tables, kset.w_min - 1.0, kset.w_step))
+
+
+def test_wtowers_kernel_set_must_cover_increment(gpu):
+    rng, uvw, vis = case(10, 500, 100.0)
+    kset, _ = kernel_sets(rng, 40.0)['2d-set']
+    narrow = G.WKernelSet(kset.tables, kset.w_min / 2, kset.w_step / 2)
+    with pytest.raises(ValueError):
+        G.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, narrow)
