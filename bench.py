@@ -1,7 +1,14 @@
 #!/usr/bin/env python3
-"""Grid + degrid throughput of the w-projection workload on one MI355X.
+"""Grid + degrid throughput of crocodile's gridding path on one MI355X.
 
-Workload ``wproj2048`` (the default): N=2048 grid, theta=0.1, lam=20480,
+Workload ``towers8192`` (the default, the largest configuration and the one
+the w-towers pipeline targets): N=8192 grid, theta=0.05, 5e7 visibilities
+of 256 antennas over 8 h at 150 MHz, 256 w-planes, AW kernels of support 16
+and oversampling 32.  One step = one w-towers round trip (bin, image,
+predict) through the C ABI with inputs resident in HBM; see run_towers.
+``towers4096`` is the same on the 4096^2 configuration.
+
+Workload ``wproj2048``: N=2048 grid, theta=0.1, lam=20480,
 2e6 visibilities with surface density ~ r^-1.5 out to 0.45 lam, w uniform in
 +-1500 wavelengths, one w-kernel per 20-wavelength w-bin (151 bins, support
 16..32 growing with |w|, oversampling 8, far field 64 px), visibilities from
@@ -22,6 +29,7 @@ loops, one core, on a bounded sample of the same workload).
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -37,14 +45,15 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='wproj2048',
+    ap.add_argument('--workload', default='towers8192',
                     choices=['wproj2048', 'plain512', 'towers4096', 'towers8192', 'diff1024',
                              'dft128'])
     ap.add_argument('--nv', type=int, default=0,
                     help='towers workloads: visibilities (0 = the workload\'s own count)')
-    ap.add_argument('--cpu-sample', type=int, default=300_000,
-                    help='visibilities in the CPU-baseline sample, about 12 s of one '
-                         'core (0 disables it)')
+    ap.add_argument('--cpu-sample', type=int, default=0,
+                    help='visibilities in the CPU-baseline sample (0: the workload default, '
+                         '20000 for the towers workloads, 300000 for wproj2048, about 10-15 s '
+                         'of one core; negative disables it)')
     ap.add_argument('--copy-probe-mib', type=int, default=0,
                     help='before warm-up, copy a buffer of this many MiB once (a plain copy '
                          'of known size, to scale HBM counters of a profiled run)')
@@ -152,15 +161,28 @@ def cpu_baseline(G, W, torch, dev, n):
 
 
 def run_towers(args, torch, G, W, _lib, dev):
-    """W-towers round trip (croc_wtowers_grid + croc_wtowers_degrid, one
-    AW-style kernel S=16 for every visibility, w by stacking over sub-grid
-    planes) against the plain per-visibility path (convgrid + convdegrid
-    with one w-kernel plane per w-bin, the same 256 w-bins, S=16) on the
-    same antenna-track visibilities."""
+    """The north-star workload: the w-towers grid + degrid round trip on one
+    GPU (towers8192: N = 8192, 5e7 antenna-track visibilities, 256 w-planes,
+    AW kernels S = 16, Q = 32), next to the plain paths on the same data.
+
+    One step = bin the coordinates into (chunk, w-plane) jobs
+    (croc_wtowers_plan_bin), w-towers imaging into a cleared grid
+    (croc_wtowers_grid_plan) and w-towers predict from it
+    (croc_wtowers_degrid_plan).  The sub-grid gridding uses grid.c's
+    w_project rule: each visibility takes the kernel of its residual w
+    relative to its plane from a separable residual w-kernel set.
+
+    Comparisons on the same visibilities: the binned tile path with one 2D
+    AW w-kernel per w-plane (PLANE_WBIN) at supports growing with |w|
+    ('plain') and at the fixed S ('plain_s16'), and the plain per-visibility
+    kernels ('per_visibility').  Each path's predict error against the DFT
+    of a point-source model is measured at full size; the FP64 oracle
+    (oracle/wtowers_ref.py) is timed on a seeded sample of the same
+    distribution and the device result compared with it there."""
     import numpy
     name = args.workload
     c = W.SPEC[name]
-    N, theta = c['N'], c['theta']
+    N, theta, S, Q = c['N'], c['theta'], c['S'], c['Q']
     lam = N / theta
     nv = args.nv or c['nv']
     print('[bench] generating %s (%d visibilities)' % (name, nv), file=sys.stderr, flush=True)
@@ -168,141 +190,258 @@ def run_towers(args, torch, G, W, _lib, dev):
     vis_d = vis64.to(torch.complex64).contiguous()
     del vis64
     w = uvw_d[:, 2]
-    wmin, wmax = float(w.min()), float(w.max())
-    nplanes = 256
-    wstep = (wmax - wmin) / (nplanes - 1)
-    Q, S, npix_ff = c['Q'], c['S'], 64
-    ff = W.aw_far_field(npix_ff, theta)
-    kv = G.kernel_oversample(torch.as_tensor(ff, device=dev), Q, S)
-    kv = (kv / kv[0, 0].sum().real).to(torch.complex64).contiguous()
+    wstep = (float(w.max()) - float(w.min())) / (W.NPLANES - 1)
     wb = torch.round(w / wstep)
     lo, hi = int(wb.min()), int(wb.max())
-    ws = [wstep * b for b in range(lo, hi + 1)]
-    t0 = time.perf_counter()
-    bank = G.w_kernel_bank(theta, ws, npix_ff, S, Q, True, 1.0 / npix_ff ** 2)
-    torch.cuda.synchronize()
-    kgen_ms = 1e3 * (time.perf_counter() - t0)
+    ws_plain = [wstep * b for b in range(lo, hi + 1)]
+    print('[bench] kernels: %d w-planes, wstep %.2f' % (len(ws_plain), wstep), file=sys.stderr,
+          flush=True)
+    K = W.towers_kernels(c, wstep, ws_plain)
+    kimg, kpred = K['set_img'], K['set_pred']
     guv = torch.zeros((N, N), dtype=torch.complex64, device=dev)
     out = torch.empty(nv, dtype=torch.complex64, device=dev)
     L = _lib.load()
-    margin = 32
 
-    def towers_step():
-        guv.zero_()
-        _lib.check(L.croc_wtowers_grid(
-            _lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uvw_d), nv, 3, theta, _lib.ptr(vis_d),
-            wstep, 128, margin, _lib.ptr(kv), Q, S, 0))
-        g_ms, _ = _lib.last_kernel_ms()
-        _lib.check(L.croc_wtowers_degrid(
-            _lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uvw_d), nv, 3, theta, _lib.ptr(out),
-            wstep, 128, margin, _lib.ptr(kv), Q, S, 0))
-        _, d_ms = _lib.last_kernel_ms()
-        return g_ms, d_ms
+    def kargs(ks):
+        return (_lib.ptr(ks.tables), int(ks.nplanes), float(ks.w_min), float(ks.w_step))
 
-    # planned round trips: the coordinates binned once per step and used by
-    # both calls (croc_wtowers_plan_*, croc_plan_*)
     twp = _lib._P()
     _lib.check(L.croc_wtowers_plan_create(_lib.ctx(), ctypes.byref(twp)))
-    gplan = G.GridPlan(bank, N)
 
-    def towers_plan_step():
+    def tw_bin():
         _lib.check(L.croc_wtowers_plan_bin(_lib.ctx(), twp, N, _lib.ptr(uvw_d), nv, 3, theta,
-                                           wstep, 128, margin, 1, Q, S))
-        guv.zero_()
+                                           wstep, 128, W.MARGIN, 1, Q, S, kimg.nplanes,
+                                           kimg.w_min, kimg.w_step))
+
+    def tw_grid(ks):
         _lib.check(L.croc_wtowers_grid_plan(_lib.ctx(), twp, _lib.ptr(guv), _lib.ptr(vis_d),
-                                            _lib.ptr(kv), 0))
-        g_ms, _ = _lib.last_kernel_ms()
-        _lib.check(L.croc_wtowers_degrid_plan(_lib.ctx(), twp, _lib.ptr(guv), _lib.ptr(out),
-                                              _lib.ptr(kv), 0))
-        _, d_ms = _lib.last_kernel_ms()
-        return g_ms, d_ms
+                                            _lib.ptr(ks.tables), 1))
+        g = _lib.last_tap_kernel_ms()
+        return g[0], g[1]
 
-    def plain_plan_step():
-        gplan.bin(uvw_d, lam, _lib.PLANE_WBIN, None, wstep, lo)
-        guv.zero_()
-        gplan.grid(guv, vis_d)
-        g_ms, _ = _lib.last_kernel_ms()
-        gplan.degrid(guv, out)
-        _, d_ms = _lib.last_kernel_ms()
-        return g_ms, d_ms
+    def tw_degrid(ks, dst):
+        _lib.check(L.croc_wtowers_degrid_plan(_lib.ctx(), twp, _lib.ptr(guv), _lib.ptr(dst),
+                                              _lib.ptr(ks.tables), 1))
+        d = _lib.last_tap_kernel_ms()
+        return d[2], d[3]
 
-    def plain_step():
+    def towers_step():
+        tw_bin()
         guv.zero_()
-        G.grid_device(bank, guv, uvw_d, lam, vis_d, _lib.PLANE_WBIN, None, wstep, lo)
-        g_ms, _ = _lib.last_kernel_ms()
-        G.degrid_device(bank, guv, uvw_d, lam, out, _lib.PLANE_WBIN, None, wstep, lo)
-        _, d_ms = _lib.last_kernel_ms()
-        return g_ms, d_ms
+        gms, gn = tw_grid(kimg)
+        dms, dn = tw_degrid(kpred, out)
+        return gms, gn, dms, dn
+
+    def towers_one_shot():
+        guv.zero_()
+        _lib.check(L.croc_wtowers_grid(_lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uvw_d), nv, 3,
+                                       theta, _lib.ptr(vis_d), wstep, 128, W.MARGIN,
+                                       _lib.ptr(kimg.tables), Q, S, 1, *kargs(kimg)[1:]))
+        _lib.check(L.croc_wtowers_degrid(_lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uvw_d), nv, 3,
+                                         theta, _lib.ptr(out), wstep, 128, W.MARGIN,
+                                         _lib.ptr(kpred.tables), Q, S, 1, *kargs(kpred)[1:]))
+        t = _lib.last_tap_kernel_ms()
+        return t[0], t[1], t[2], t[3]
+
+    plans = {}
+
+    def plain_step(which):
+        bi, bp = K['banks'][which]
+        if which not in plans:
+            plans[which] = G.GridPlan(bi, N)
+        plan = plans[which]
+        plan.bin(uvw_d, lam, _lib.PLANE_WBIN, None, wstep, lo)
+        guv.zero_()
+        plan.grid(guv, vis_d)
+        plan.degrid(guv, out, bp)
+        t = _lib.last_tap_kernel_ms()
+        return t[0], t[1], t[2], t[3]
 
     def pervis_step():
+        bi, bp = K['banks']['rule']
         guv.zero_()
-        G.grid_device_pervis(bank, guv, uvw_d, lam, vis_d, _lib.PLANE_WBIN, None, wstep, lo)
-        g_ms, _ = _lib.last_kernel_ms()
-        G.degrid_device_pervis(bank, guv, uvw_d, lam, out, _lib.PLANE_WBIN, None, wstep, lo)
-        _, d_ms = _lib.last_kernel_ms()
-        return g_ms, d_ms
+        G.grid_device_pervis(bi, guv, uvw_d, lam, vis_d, _lib.PLANE_WBIN, None, wstep, lo)
+        G.degrid_device_pervis(bp, guv, uvw_d, lam, out, _lib.PLANE_WBIN, None, wstep, lo)
+        t = _lib.last_tap_kernel_ms()
+        return t[0], t[1], t[2], t[3]
 
-    def timed(step, label):
-        for _ in range(args.warmup):
+    def timed(step, label, steps, warmup):
+        for _ in range(warmup):
             step()
         torch.cuda.synchronize()
-        print('[bench] timing %d %s steps' % (args.steps, label), file=sys.stderr, flush=True)
-        gs, ds = [], []
+        print('[bench] timing %d %s steps' % (steps, label), file=sys.stderr, flush=True)
+        rec = []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            g, d = step()
-            gs.append(g)
-            ds.append(d)
+        for _ in range(steps):
+            rec.append(step())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        return nv * args.steps / dt, 1e3 * dt / args.steps, sum(gs) / len(gs), sum(ds) / len(ds)
+        a = numpy.array(rec, dtype=float)
+        return dict(value=nv * steps / dt, ms=1e3 * dt / steps, grid_tap_ms=a[:, 0].mean(),
+                    grid_launches=int(a[:, 1].mean()), degrid_tap_ms=a[:, 2].mean(),
+                    degrid_launches=int(a[:, 3].mean()))
 
-    tv, tms, tg, td = timed(towers_step, 'w-towers')
-    tpv, tpms, _, _ = timed(towers_plan_step, 'planned w-towers')
-    _lib.check(L.croc_wtowers_plan_destroy(_lib.ctx(), twp))
-    pv, pms, pg, pd = timed(plain_step, 'plain')
-    ppv, ppms, _, _ = timed(plain_plan_step, 'planned plain')
-    del gplan
-    qv, qms, qg, qd = timed(pervis_step, 'per-visibility')
-    # algorithmic work: taps (8 flop each way) and sub-grid DFTs (16 n^3 flop
-    # per 128^2 transform, three fp16 products each on the matrix cores)
-    taps = float(nv) * S * S
-    key = (torch.floor(uvw_d[:, 0] * theta + 0.5).to(torch.int64) + N // 2) // (128 - margin)
+    res = timed(towers_step, 'w-towers (planned)', args.steps, args.warmup)
+    few = max(2, args.steps // 4)
+    one = timed(towers_one_shot, 'w-towers (one-shot)', few, 1)
+    plain = timed(lambda: plain_step('rule'), 'plain tile path (support rule)', few, 1)
+    plain16 = timed(lambda: plain_step('fixed'), 'plain tile path (S fixed)', few, 1)
+    plans.clear()
+    pv = timed(pervis_step, 'per-visibility', 2, 1)
+
+    # job structure of the headline
+    key = (torch.floor(uvw_d[:, 0] * theta + 0.5).to(torch.int64) + N // 2) // (128 - W.MARGIN)
     key = key * 100000 + (torch.floor(uvw_d[:, 1] * theta + 0.5).to(torch.int64) + N // 2) \
-        // (128 - margin)
+        // (128 - W.MARGIN)
     key = key * 4096 + (wb.to(torch.int64) - lo)
     njobs = int(torch.unique(key).numel())
     nchunks = int(torch.unique(key // 4096).numel())
-    dft_flops = 16.0 * 128 ** 3 * (2 * njobs + 2 * nchunks)
+    del key
+
+    # predict accuracy against the DFT, full size
+    print('[bench] predict error against the DFT', file=sys.stderr, flush=True)
+    single = G.WKernelSet(kpred.tables[kpred.nplanes // 2:kpred.nplanes // 2 + 1], 0.0, 1.0)
+    nores = G.WTowersPlan(theta, lam, uvw_d, wstep, 128, W.MARGIN, single.tables[0])
+
+    def towers_pred(grid):
+        guv.copy_(grid)
+        v = torch.empty(nv, dtype=torch.complex64, device=dev)
+        tw_degrid(kpred, v)
+        return v
+
+    def plain_pred(which):
+        def fn(grid):
+            v = torch.empty(nv, dtype=torch.complex64, device=dev)
+            G.degrid_device(K['banks'][which][1], grid, uvw_d, lam, v, _lib.PLANE_WBIN, None,
+                            wstep, lo)
+            return v
+        return fn
+
+    tw_bin()
+    dft_err = W.dft_predict_error(c, uvw_d, {
+        'towers': towers_pred,
+        'towers_no_residual_kernels': lambda g: nores.predict(g, single.tables[0]),
+        'plain': plain_pred('rule'), 'plain_s16': plain_pred('fixed')}, c['seed'] + 99)
+    del nores
+    _lib.check(L.croc_wtowers_plan_destroy(_lib.ctx(), twp))
+
+    # roofline of the dominant tap kernel, from the live HIP events of the
+    # timed steps: 8 FP32 flop per kernel tap (one complex multiply-add),
+    # S^2 taps per visibility and direction
+    taps = float(nv) * S * S
+    if res['grid_tap_ms'] >= res['degrid_tap_ms']:
+        kname, kms, kn, lds_per_tap = 'k_tw_grid_jobs', res['grid_tap_ms'], res['grid_launches'], 16.0
+    else:
+        kname, kms, kn, lds_per_tap = ('k_tw_degrid_jobs', res['degrid_tap_ms'],
+                                       res['degrid_launches'], 8.0)
+    achieved = 8.0 * taps / (kms * 1e-3) / 1e12
+    roofline = {
+        'bound': 'valu', 'achieved': round(achieved, 3), 'peak': FP32_PEAK_TFLOPS,
+        'unit': 'TFLOP/s', 'frac': round(achieved / FP32_PEAK_TFLOPS, 5),
+        'bound_note': 'complex multiply-add per kernel tap on the vector ALUs, 8 FP32 flop/tap '
+                      '(the reference convention); no matrix cores on this path',
+        'kernel': kname, 'launches_per_step': kn,
+        'kernel_ms_per_launch': round(kms / max(kn, 1), 4),
+        'kernel_ms_per_step': {'k_tw_grid_jobs': round(res['grid_tap_ms'], 3),
+                               'k_tw_degrid_jobs': round(res['degrid_tap_ms'], 3)},
+        'algorithmic_flops_per_launch': 8.0 * taps / max(kn, 1),
+        'lds_bytes_per_tap': lds_per_tap,
+        'lds_frac': round(lds_per_tap * taps / (kms * 1e-3) / (LDS_PEAK_TBS * 1e12), 5),
+        'traffic': None,
+    }
+    pmc = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                       '%s_%s_pmc.json' % (kname, name))
+    if os.path.exists(pmc):
+        prof = json.load(open(pmc))
+        roofline['traffic'] = prof.get('hbm_bytes_per_launch')
+        roofline['traffic_profile'] = {'file': os.path.relpath(pmc, os.path.dirname(
+            os.path.abspath(__file__))), 'kernel_us': prof.get('kernel_us'),
+            'note': 'calibrated FETCH_SIZE + WRITE_SIZE per launch from rocprofv3 --pmc of the '
+                    'same command; compare kernel_us with kernel_ms_per_launch for staleness'}
+
+    cpu, parity = None, None
+    n_cpu = args.cpu_sample or 20000
+    if n_cpu > 0:
+        cpu, parity = towers_cpu_and_parity(G, W, c, name, wstep, kimg, kpred, n_cpu)
+
     line = {
-        'metric': 'visibilities/s grid+degrid', 'value': round(tv, 1),
+        'metric': 'visibilities/s grid+degrid', 'value': round(res['value'], 1),
         'unit': 'visibilities/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
-        'ms_per_step': round(tms, 3), 'higher_is_better': True, 'scaling': 'weak',
+        'ms_per_step': round(res['ms'], 3), 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': name, 'N': N, 'theta': theta, 'nv': nv, 'w_planes': nplanes,
+        'config': {'workload': name, 'N': N, 'theta': theta, 'nv': nv, 'w_planes': W.NPLANES,
                    'wstep': round(wstep, 3), 'support': S, 'oversampling': Q,
-                   'subgrid': 128, 'margin': margin, 'jobs': njobs, 'chunks': nchunks},
-        'towers': {'ms_per_step': round(tms, 3), 'grid_ms': round(tg, 3),
-                   'degrid_ms': round(td, 3), 'tap_tflops': round(2 * 8 * taps / tms / 1e9, 3),
-                   'dft_tflops': round(dft_flops / tms / 1e9, 3)},
-        'towers_planned': {'value': round(tpv, 1), 'ms_per_step': round(tpms, 3),
-                           'note': 'croc_wtowers_plan_bin + croc_wtowers_grid_plan + '
-                                   'croc_wtowers_degrid_plan: coordinates binned once per step'},
-        'plain_planned': {'value': round(ppv, 1), 'ms_per_step': round(ppms, 3),
-                          'note': 'croc_plan_bin + croc_convgrid_plan + croc_convdegrid_plan'},
-        'plain': {'value': round(pv, 1), 'ms_per_step': round(pms, 3),
-                  'grid_ms': round(pg, 3), 'degrid_ms': round(pd, 3),
-                  'kernel_bank_mib': round(8 * bank.kern.numel() / 2 ** 20, 1),
-                  'kernel_gen_ms': round(kgen_ms, 1)},
-        'per_visibility': {'value': round(qv, 1), 'ms_per_step': round(qms, 3),
-                           'grid_ms': round(qg, 3), 'degrid_ms': round(qd, 3),
-                           'note': 'croc_convgrid_pervis + croc_convdegrid_pervis: one '
-                                   'wavefront per visibility, FP32 atomics, no binning, '
-                                   'same kernel bank as plain'},
-        'speedup_vs_plain': round(tv / pv, 3),
-        'speedup_vs_per_visibility': round(tv / qv, 3),
+                   'subgrid': 128, 'margin': W.MARGIN, 'jobs': njobs, 'chunks': nchunks,
+                   'residual_w_kernels': kimg.nplanes, 'residual_w_step': W.RES_STEP,
+                   'kernels': 'separable AW (PSWF c=1.3S x Gaussian beam) x residual w-term',
+                   'step': 'croc_wtowers_plan_bin + croc_wtowers_grid_plan + '
+                           'croc_wtowers_degrid_plan',
+                   'kernel_gen_ms': round(K['gen_ms'], 1)},
+        'roofline': roofline, 'cpu_baseline': cpu, 'parity_vs_fp64_oracle': parity,
+        'predict_error_vs_dft': {k: float('%.4g' % v) for k, v in dft_err.items()},
+        'towers_one_shot': {'value': round(one['value'], 1), 'ms_per_step': round(one['ms'], 3),
+                            'note': 'croc_wtowers_grid + croc_wtowers_degrid, each binning'},
+        'plain': {'value': round(plain['value'], 1), 'ms_per_step': round(plain['ms'], 3),
+                  'grid_tap_ms': round(plain['grid_tap_ms'], 3),
+                  'degrid_tap_ms': round(plain['degrid_tap_ms'], 3),
+                  'supports': [min(K['supports']), max(K['supports'])],
+                  'kernel_bank_mib': round(8 * K['banks']['rule'][0].kern.numel() / 2 ** 20, 1),
+                  'note': 'binned tile path, croc_plan_bin + croc_convgrid_plan + '
+                          'croc_convdegrid_plan, 2D AW w-kernel per w-plane, support '
+                          'S + 2 ceil(|w| theta^2 / 2) capped at 64'},
+        'plain_s16': {'value': round(plain16['value'], 1), 'ms_per_step': round(plain16['ms'], 3),
+                      'note': 'the same at support 16 for every w-plane'},
+        'per_visibility': {'value': round(pv['value'], 1), 'ms_per_step': round(pv['ms'], 3),
+                           'note': 'croc_convgrid_pervis + croc_convdegrid_pervis (one '
+                                   'wavefront per visibility, FP32 atomics), support rule bank'},
+        'speedup_vs_plain': round(res['value'] / plain['value'], 3),
+        'speedup_vs_plain_s16': round(res['value'] / plain16['value'], 3),
+        'speedup_vs_per_visibility': round(res['value'] / pv['value'], 3),
     }
     print(json.dumps(line), flush=True)
+
+
+def towers_cpu_and_parity(G, W, c, name, wstep, kimg, kpred, n):
+    """The FP64 NumPy restatement of the w-towers drivers
+    (oracle/wtowers_ref.py) timed on one core over a seeded n-visibility
+    sample of the same distribution (W.towers_sample), with the device
+    kernel-set tables; and the device result on the same sample against
+    it."""
+    import numpy
+    from oracle import wtowers_ref as WT
+    theta, N = c['theta'], c['N']
+    lam = N / theta
+    uvw, vis = W.towers_sample(name, n, c['seed'] + 77)
+    ki = (kimg.tables.cpu().numpy().astype(numpy.complex128), kimg.w_min, kimg.w_step)
+    kp = (kpred.tables.cpu().numpy().astype(numpy.complex128), kpred.w_min, kpred.w_step)
+    print('[bench] cpu baseline: FP64 oracle w-towers on %d visibilities' % n, file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    g64 = WT.w_towers_imaging(theta, lam, uvw, None, vis, wstep, 128, W.MARGIN, ki)
+    t1 = time.perf_counter()
+    v64 = WT.w_towers_predict(theta, lam, uvw, None, g64, wstep, 128, W.MARGIN, kp)
+    t2 = time.perf_counter()
+    g32 = G.w_towers_imaging(theta, lam, uvw, None, vis, wstep, 128, W.MARGIN, kimg)
+    v32 = G.w_towers_predict(theta, lam, uvw, None, g64, wstep, 128, W.MARGIN, kpred)
+
+    def err(x, ref):
+        d = x - ref
+        return (float(numpy.linalg.norm(d) / numpy.linalg.norm(ref)),
+                float(numpy.abs(d).max() / numpy.abs(ref).max()))
+
+    ge, ve = err(g32, g64), err(v32, v64)
+    del g32, g64
+    base = {'value': n / (t2 - t0), 'unit': 'visibilities/s', 'cores': 1, 'kind': 'port',
+            'sample': '%d visibilities of %s (towers_sample seed %d): FP64 NumPy w-towers '
+                      'imaging + predict (oracle/wtowers_ref.py) with the same residual w-kernel '
+                      'sets; grid %.2fs + degrid %.2fs; the rate is per visibility of this '
+                      'sample (its per-job sub-grid FFTs grow more slowly than nv, so it does '
+                      'not extrapolate linearly to the full set)'
+                      % (n, name, c['seed'] + 77, t1 - t0, t2 - t1),
+            'host_cpus': os.cpu_count()}
+    parity = {'sample_nv': n, 'grid_rms_rel': ge[0], 'grid_max_rel': ge[1],
+              'vis_rms_rel': ve[0], 'vis_max_rel': ve[1]}
+    return base, parity
 
 
 def run_dft(args, torch, _lib, dev):
@@ -550,10 +689,11 @@ def main():
     t_flop, t_byte = flops / FP32_PEAK_TFLOPS / 1e12, alg_bytes / HBM_PEAK_GBS / 1e9
     if t_flop >= t_byte:
         achieved = flops / (t_ms * 1e-3) / 1e12
-        roofline = {'bound': 'mfma', 'achieved': round(achieved, 3), 'peak': FP32_PEAK_TFLOPS,
+        roofline = {'bound': 'valu', 'achieved': round(achieved, 3), 'peak': FP32_PEAK_TFLOPS,
                     'unit': 'TFLOP/s', 'frac': round(achieved / FP32_PEAK_TFLOPS, 5),
-                    'compute': 'fp32 complex multiply-add on the vector ALUs (8 flop/tap); '
-                               'the fp32 peak is the bound, no matrix cores are used'}
+                    'bound_note': 'complex multiply-add per kernel tap on the vector ALUs, 8 '
+                                  'FP32 flop/tap (the reference convention); no matrix cores '
+                                  'on this path'}
     else:
         achieved = alg_bytes / (t_ms * 1e-3) / 1e9
         roofline = {'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS,
@@ -581,10 +721,11 @@ def main():
                      'hbm_copy_measured_gbs': copy_gbs})
 
     cpu, parity = None, None
-    if args.cpu_sample > 0 and config['workload'] == 'wproj2048':
-        print('[bench] cpu baseline on %d visibilities' % args.cpu_sample, file=sys.stderr,
+    n_cpu = args.cpu_sample or 300_000
+    if n_cpu > 0 and config['workload'] == 'wproj2048':
+        print('[bench] cpu baseline on %d visibilities' % n_cpu, file=sys.stderr,
               flush=True)
-        cpu, parity = cpu_baseline(G, W, torch, dev, args.cpu_sample)
+        cpu, parity = cpu_baseline(G, W, torch, dev, n_cpu)
 
     line = {
         'metric': 'visibilities/s grid+degrid', 'value': round(value, 1),

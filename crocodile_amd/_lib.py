@@ -32,6 +32,8 @@ SIGNATURES = {
     'croc_ctx_set_stream': [_P, _P],
     'croc_ctx_set_tuning': [_P, _I, _I, _I],
     'croc_last_kernel_ms': [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    'croc_last_tap_kernel_ms': [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I),
+                                ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I)],
     'croc_frac_coords': [_P, _P, _I64, _I64, _D, _I, _I, _I, _P, _P, _P, _P],
     'croc_convgrid': [_P, _P, _I, _P, _I64, _I64, _D, _P, _P, _I, _I, _I, _P, _P,
                       _I, _I, _P, _D, _I],
@@ -150,6 +152,16 @@ def last_kernel_ms():
     a, b = ctypes.c_float(), ctypes.c_float()
     check(load().croc_last_kernel_ms(ctx(), ctypes.byref(a), ctypes.byref(b)))
     return a.value, b.value
+
+
+def last_tap_kernel_ms():
+    """(grid ms, grid launches, degrid ms, degrid launches) of the tap
+    kernels of the last gridding / degridding call on this stream."""
+    a, b = ctypes.c_float(), ctypes.c_float()
+    na, nb = _I(), _I()
+    check(load().croc_last_tap_kernel_ms(ctx(), ctypes.byref(a), ctypes.byref(na),
+                                         ctypes.byref(b), ctypes.byref(nb)))
+    return a.value, na.value, b.value, nb.value
 
 
 def set_tuning(tile=0, chunk=512, group_kib=0):

@@ -133,6 +133,8 @@ extern "C" int croc_ctx_destroy(croc_ctx *ctx) {
   ws_free(ctx->ws);
   for (int i = 0; i < 4; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  for (int d = 0; d < 2; d++)
+    for (hipEvent_t e : ctx->tev[d]) (void)hipEventDestroy(e);
   if (ctx->host_status) (void)hipHostFree(ctx->host_status);
   delete ctx;
   return CROC_OK;
@@ -172,6 +174,46 @@ extern "C" int croc_last_kernel_ms(croc_ctx *ctx, float *grid_ms,
   }
   if (grid_ms) *grid_ms = g;
   if (degrid_ms) *degrid_ms = d;
+  return CROC_OK;
+}
+
+void tap_timer_reset(croc_ctx *ctx, int dir) { ctx->tev_n[dir] = 0; }
+
+int tap_timer_begin(croc_ctx *ctx, int dir) {
+  std::vector<hipEvent_t> &v = ctx->tev[dir];
+  const size_t k = (size_t)ctx->tev_n[dir];
+  while (v.size() < 2 * k + 2) {  // events are made once and reused
+    hipEvent_t e = nullptr;
+    CROC_HIP(hipEventCreate(&e));
+    v.push_back(e);
+  }
+  CROC_HIP(hipEventRecord(v[2 * k], ctx->stream));
+  return CROC_OK;
+}
+
+int tap_timer_end(croc_ctx *ctx, int dir) {
+  CROC_HIP(hipEventRecord(ctx->tev[dir][2 * (size_t)ctx->tev_n[dir] + 1], ctx->stream));
+  ctx->tev_n[dir]++;
+  return CROC_OK;
+}
+
+extern "C" int croc_last_tap_kernel_ms(croc_ctx *ctx, float *grid_ms, int *grid_launches,
+                                       float *degrid_ms, int *degrid_launches) {
+  CROC_REQUIRE(ctx != nullptr, CROC_EINVAL, "null context");
+  CROC_HIP(hipSetDevice(ctx->device));
+  float tot[2] = {0.f, 0.f};
+  for (int d = 0; d < 2; d++) {
+    for (int k = 0; k < ctx->tev_n[d]; k++) {
+      float ms = 0.f;
+      CROC_HIP(hipEventSynchronize(ctx->tev[d][2 * (size_t)k + 1]));
+      CROC_HIP(hipEventElapsedTime(&ms, ctx->tev[d][2 * (size_t)k], ctx->tev[d][2 * (size_t)k + 1]));
+      tot[d] += ms;
+    }
+  }
+  if (grid_ms) *grid_ms = tot[0];
+  if (grid_launches) *grid_launches = ctx->tev_n[0];
+  if (degrid_ms) *degrid_ms = tot[1];
+  if (degrid_launches) *degrid_launches = ctx->tev_n[1];
   return CROC_OK;
 }
 

@@ -96,6 +96,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(R == 63 ? 8 
 int degrid_binned(croc_ctx *ctx, const BinPlan &plan, const void *grid, int N, void *vis_out,
                   const void *kern, int separable) {
   CROC_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+  tap_timer_reset(ctx, 1);
+  int rc = tap_timer_begin(ctx, 1);
+  if (rc) return rc;
   const float2 *g = (const float2 *)grid;
   const float2 *kt = (const float2 *)kern;
   float2 *vo = (float2 *)vis_out;
@@ -117,6 +120,7 @@ int degrid_binned(croc_ctx *ctx, const BinPlan &plan, const void *grid, int N, v
           g, N, plan.T, plan.ntx, plan.rec, vo, kt, plan.items_g, plan.gstart, plan.ngroups, plan.qhead);
   }
   CROC_HIP(hipGetLastError());
+  if ((rc = tap_timer_end(ctx, 1))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
   ctx->ev_degrid = true;
   return CROC_OK;

@@ -315,6 +315,8 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, void *grid, int N,
     sparse = plan.items;  // nothing to split
   }
   CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  tap_timer_reset(ctx, 0);
+  if ((rc = tap_timer_begin(ctx, 0))) return rc;
   const float2 *kt = (const float2 *)kern;
   if (plan.R == TILE_R_SMALL) {
     const int G = persistent_blocks(ctx, 2);  // 63.5 KiB of LDS each
@@ -334,6 +336,7 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, void *grid, int N,
           acc, N, plan.T, plan.ntx, plan.rec, (const float2 *)vis, kt, sparse, plan.nitems, dense_g, plan.gstart, ng_lists, plan.qhead, st);
   }
   CROC_HIP(hipGetLastError());
+  if ((rc = tap_timer_end(ctx, 0))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
   ctx->ev_grid = true;
   k_grid_finish<<<(unsigned)((cells + 255) / 256), 256, 0, ctx->stream>>>(

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 
 #include "../../include/crocgpu.h"
 
@@ -85,6 +86,12 @@ struct croc_ctx {
   // ev[0], ev[1] bracket the last gridding kernel(s), ev[2], ev[3] the last
   // degridding; the durations are read on demand (croc_last_kernel_ms)
   bool ev_grid = false, ev_degrid = false;
+  // Event pairs around every launch of the tap kernels (the gridding /
+  // degridding kernels proper: k_grid_tiles, k_tw_grid_jobs, ...) of the
+  // last gridding (dir 0) and degridding (dir 1) call: tev[dir][2k],
+  // tev[dir][2k + 1] bracket launch k of tev_n[dir].
+  std::vector<hipEvent_t> tev[2];
+  int tev_n[2] = {0, 0};
   int num_cu = 256;
   int tile = 0;     // 0: automatic
   int chunk = 512;  // visibilities per work item
@@ -99,6 +106,13 @@ void ws_free(WsSet &set);
 
 // Make the context's device current.
 int ctx_enter(croc_ctx *ctx);
+
+// Tap-kernel launch timers: tap_timer_reset at the start of a gridding
+// (dir 0) or degridding (dir 1) call, tap_timer_begin / tap_timer_end
+// around each launch of its tap kernel.
+void tap_timer_reset(croc_ctx *ctx, int dir);
+int tap_timer_begin(croc_ctx *ctx, int dir);
+int tap_timer_end(croc_ctx *ctx, int dir);
 
 // Device address of the allocation-free entry points' static scratch array
 // on the current device, looked up once per device (noalloc.hip).

@@ -127,6 +127,8 @@ extern "C" int croc_convgrid_pervis(croc_ctx *ctx, void *grid, int N, const doub
   if ((rc = pv_check(ctx, a))) return rc;
   const int G = persistent_blocks(ctx, 8);
   CROC_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  tap_timer_reset(ctx, 0);
+  if ((rc = tap_timer_begin(ctx, 0))) return rc;
   if (separable)
     k_pv_grid<true><<<G, PV_NT, 0, ctx->stream>>>(a, (float2 *)grid, (const float2 *)vis,
                                                   (const float2 *)kern);
@@ -134,6 +136,7 @@ extern "C" int croc_convgrid_pervis(croc_ctx *ctx, void *grid, int N, const doub
     k_pv_grid<false><<<G, PV_NT, 0, ctx->stream>>>(a, (float2 *)grid, (const float2 *)vis,
                                                    (const float2 *)kern);
   CROC_HIP(hipGetLastError());
+  if ((rc = tap_timer_end(ctx, 0))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
   ctx->ev_grid = true;
   return CROC_OK;
@@ -154,6 +157,8 @@ extern "C" int croc_convdegrid_pervis(croc_ctx *ctx, const void *grid, int N,
   if ((rc = pv_check(ctx, a))) return rc;
   const int G = persistent_blocks(ctx, 8);
   CROC_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+  tap_timer_reset(ctx, 1);
+  if ((rc = tap_timer_begin(ctx, 1))) return rc;
   if (separable)
     k_pv_degrid<true><<<G, PV_NT, 0, ctx->stream>>>(a, (const float2 *)grid, (float2 *)vis_out,
                                                     (const float2 *)kern);
@@ -161,6 +166,7 @@ extern "C" int croc_convdegrid_pervis(croc_ctx *ctx, const void *grid, int N,
     k_pv_degrid<false><<<G, PV_NT, 0, ctx->stream>>>(a, (const float2 *)grid,
                                                      (float2 *)vis_out, (const float2 *)kern);
   CROC_HIP(hipGetLastError());
+  if ((rc = tap_timer_end(ctx, 1))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
   ctx->ev_degrid = true;
   return CROC_OK;

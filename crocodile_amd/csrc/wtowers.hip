@@ -796,9 +796,11 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
 
   if ((rc = zero_u32_async(ctx, qc, (int64_t)batches.size()))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[0], s));
+  tap_timer_reset(ctx, 0);
   for (size_t bi = 0; bi < batches.size(); bi++) {
     const Batch &bt = batches[bi];
     const int G = (int)std::min<size_t>(bt.np, (size_t)ctx->num_cu);  // one per CU (LDS)
+    if ((rc = tap_timer_begin(ctx, 0))) return rc;
     if (a.nearest || !separable)
       k_tw_grid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
                                                 (const float2 *)kern, a.Q, a.S, tsize,
@@ -808,6 +810,7 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
                                                (const float2 *)kern, a.Q, a.S, tsize,
                                                a.nearest, fs, sub, qc + bi);
     CROC_HIP(hipGetLastError());
+    if ((rc = tap_timer_end(ctx, 0))) return rc;
     if (bt.nm) {
       k_tw_sum_parts<<<dim3(ACC_SLICES, (unsigned)bt.nm), 256, 0, s>>>(dmulti + bt.m0, sub);
       CROC_HIP(hipGetLastError());
@@ -916,6 +919,7 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
   if ((rc = zero_u32_async(ctx, qc, (int64_t)poff.size()))) return rc;
   dim3 g(16, (unsigned)p.chunks.size());
   CROC_HIP(hipEventRecord(ctx->ev[2], s));
+  tap_timer_reset(ctx, 1);
   k_tw_cut<<<g, 256, 0, s>>>(dchunks, (const float2 *)grid, cimg, a.N, a.chunk, a.margin, a.ncx);
   CROC_HIP(hipGetLastError());
   rc = fft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, +1, 1.0f / (float)SGN,
@@ -929,6 +933,7 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
     if (rc) return rc;
     const int npart = (int)(poff[bi + 1] - poff[bi]);
     const int G = std::min(npart, ctx->num_cu);  // one per CU (LDS)
+    if ((rc = tap_timer_begin(ctx, 1))) return rc;
     if (a.nearest || !separable)
       k_tw_degrid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
                                                   (const float2 *)kern, a.Q, a.S, tsize,
@@ -938,6 +943,7 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
                                                  (const float2 *)kern, a.Q, a.S, tsize,
                                                  a.nearest, (float2 *)vis_out, qc + bi);
     CROC_HIP(hipGetLastError());
+    if ((rc = tap_timer_end(ctx, 1))) return rc;
   }
   CROC_HIP(hipEventRecord(ctx->ev[3], s));
   ctx->ev_degrid = true;

@@ -210,6 +210,59 @@ def towers(name, nv=None, seed=None, device=None, margin_cells=48):
     return ut, vis
 
 
+def _towers_antennas(rng, name):
+    """Antenna layout of the ``towers4096`` / ``towers8192`` workloads (the
+    first draws of towers()' generator)."""
+    c = SPEC[name]
+    if name == 'towers8192':
+        return antenna_layout(rng, c['nant'], sigma_m=1500.0, arms_to_m=40000.0)
+    return antenna_layout(rng, c['nant'], sigma_m=1500.0)
+
+
+def towers_sample(name, n, seed, margin_cells=48):
+    """A seeded uniform subsample of ``n`` visibilities of the full
+    ``towers4096`` / ``towers8192`` workload: random (dump, baseline) pairs of
+    the full track set (the same antennas, sources, hour-angle sampling and
+    edge filter as towers() at its full size), each computed directly, with
+    noise from ``seed``.  Returns (uvw [n, 3], vis [n] complex128) on the
+    host: the full set's distribution at a cost proportional to n, for
+    oracle checks and CPU timings."""
+    c = SPEC[name]
+    grng = numpy.random.default_rng(c['seed'])
+    xyz = _towers_antennas(grng, name)
+    ls, ms, fs = point_sources(grng, c['nsrc'], c['theta'])
+    nant = xyz.shape[0]
+    nbl = nant * (nant - 1) // 2
+    a1, a2 = numpy.triu_indices(nant, 1)
+    bl = xyz_at_latitude(xyz, LATITUDE)
+    bl = bl[a2] - bl[a1]
+    lim = (c['N'] // 2 - margin_cells) / c['theta']
+    lam_m = C_LIGHT / c['freq']
+    dec = c['dec'] * numpy.pi / 180
+
+    def hour_angle(t, ndump):
+        return ((t + 0.5) / ndump - 0.5) * c['hours'] / 24 * 2 * numpy.pi
+
+    # the dump count of the full set: towers() takes enough dumps for nv
+    # samples to survive the edge filter (estimated from 32 hour angles)
+    keep = numpy.mean([((numpy.abs(u[:, 0]) < lim) & (numpy.abs(u[:, 1]) < lim)).mean()
+                       for u in (xyz_to_uvw(bl, hour_angle(t, 32), dec) / lam_m
+                                 for t in range(32))])
+    ndump = max(1, int(numpy.ceil(c['nv'] / (nbl * max(keep, 1e-9)))))
+    rng = numpy.random.default_rng(seed)
+    parts, have = [], 0
+    while have < n:
+        k = 2 * (n - have) + 64
+        t = rng.integers(0, ndump, k)
+        b = rng.integers(0, nbl, k)
+        u = xyz_to_uvw(bl[b], hour_angle(t, ndump), dec) / lam_m
+        u = u[(numpy.abs(u[:, 0]) < lim) & (numpy.abs(u[:, 1]) < lim)]
+        parts.append(u)
+        have += u.shape[0]
+    uvw = numpy.concatenate(parts)[:n]
+    return uvw, add_noise(rng, dft_vis(uvw, ls, ms, fs))
+
+
 def aw_far_field(n, theta, c_pswf=None):
     """Far field [n, n] of an AW-style kernel without its w part: the PSWF
     anti-aliasing function times a Gaussian primary-beam A-term (FWHM =
@@ -222,6 +275,24 @@ def aw_far_field(n, theta, c_pswf=None):
     beam = numpy.exp(-4 * numpy.log(2) * (ll / theta) ** 2)
     a1 = aa * beam
     return numpy.outer(a1, a1)
+
+
+def aw_taper(n, theta, support):
+    """1D image-plane taper of the AW kernels of the towers workloads,
+    sampled on n pixels of the field theta: crocodile's PSWF anti-aliasing
+    function (synthesis.py:199-218, c = 1.3 * support as in
+    wtowers-predict.ipynb) times a Gaussian primary beam whose FWHM is the
+    field of view.  The 2D taper is its outer product."""
+    import scipy.special
+    x = (numpy.arange(n) - n // 2) / n
+    aa = scipy.special.pro_ang1(0, 0, 1.3 * support, (2 - 1 / n / 4) * x)[0]
+    return aa * beam_taper(x * theta, theta)
+
+
+def beam_taper(l, theta):
+    """Gaussian primary beam exp(-4 ln 2 (l / theta)^2): half power at the
+    field edge l = +-theta/2."""
+    return numpy.exp(-4 * numpy.log(2) * (numpy.asarray(l) / theta) ** 2)
 
 
 def gaussian_blobs(rng, N, nblob=50, sigma=(1.0, 4.0)):
@@ -259,3 +330,92 @@ def diff1024(nv=None, N=None, seed=None):
     image = gaussian_blobs(rng, N, c['nblob'])
     uvw = diff_uvw(rng, c['nv'] if nv is None else nv, lam, c['wmax'])
     return image, uvw
+
+
+# --------------------------------------------------------------------------
+# Kernels and the DFT accuracy check of the towers workloads
+# --------------------------------------------------------------------------
+
+RES_STEP = 0.5      # residual w-kernel step of the w-towers sets (wavelengths)
+NPLANES = 256       # w-planes of the towers workloads
+MARGIN = 32         # w-towers sub-grid margin (cells); sub-grids are 128
+PLAIN_FF = 128      # far-field pixels of the plain path's 2D AW w-kernels
+
+
+def plain_support(w, theta, S, cap=64):
+    """Support of the plain path's w-kernel at w: the AW kernel's S plus
+    the w-term's spread, 2 ceil(|w| theta^2 / 2) cells (the phase gradient
+    w l at the field edge l = theta/2, in cells of 1/theta), at most the
+    library's 64."""
+    return int(min(cap, S + 2 * math.ceil(abs(w) * theta * theta / 2 - 1e-9)))
+
+
+def towers_kernels(c, wstep, ws_plain):
+    """Kernels of the towers workloads, all normalised to unit DC gain.
+
+    * w-towers: separable residual w-kernel sets (imaging, predict) of the
+      AW taper sampled on the full grid's N pixels, residuals in steps of
+      RES_STEP over [-wstep/2, wstep/2] (G.w_towers_kernel_set);
+    * plain path: 2D AW w-kernels (taper on PLAIN_FF pixels times the
+      w-term of each w-plane's centre) with supports growing with |w|
+      (plain_support), and the same at the fixed support S."""
+    import time
+    import torch
+    from . import synthesis as G
+    theta, S, Q, N = c['theta'], c['S'], c['Q'], c['N']
+    t0 = time.perf_counter()
+    taper = aw_taper(N, theta, S)
+    norm1 = float(G.kernel_oversample(taper, Q, S)[0].sum().real)
+    kset_img = G.w_towers_kernel_set(theta, wstep, N, S, Q, RES_STEP, taper / norm1, True)
+    kset_pred = G.w_towers_kernel_set(theta, wstep, N, S, Q, RES_STEP, taper / norm1, False)
+    tp = aw_taper(PLAIN_FF, theta, S)
+    norm2 = float(G.kernel_oversample(tp, Q, S)[0].sum().real) ** 2
+    aterm = numpy.outer(tp, tp)
+    sup = [plain_support(w, theta, S) for w in ws_plain]
+    banks = {}
+    for name, supports in (('rule', sup), ('fixed', [S] * len(ws_plain))):
+        banks[name] = tuple(G.w_kernel_bank(theta, ws_plain, PLAIN_FF, supports, Q, inv,
+                                            1.0 / norm2, aterm=aterm) for inv in (True, False))
+    torch.cuda.synchronize()
+    return dict(set_img=kset_img, set_pred=kset_pred, banks=banks, supports=sup,
+                gen_ms=1e3 * (time.perf_counter() - t0), taper_norm=norm1)
+
+
+def dft_predict_error(c, uvw_d, predict_fns, seed):
+    """Accuracy of predict against the direct Fourier transform, the metric
+    of wtowers-predict.ipynb (error = sqrt(mean |V - V_dft|^2) / mean
+    |V_dft|): a model image of 20 pixel-centred point sources within 3/8 of
+    the field of its centre, divided by the PSWF anti-aliasing function and
+    FFT'd to a grid; each predict function degrids that grid; the reference
+    is the phase-tracked DFT of the sources seen through the primary beam
+    (which the AW kernels apply)."""
+    import torch
+    from . import synthesis as Gs
+    dev = uvw_d.device
+    N, theta, S = c['N'], c['theta'], c['S']
+    lam = N / theta
+    rng = numpy.random.default_rng(seed)
+    lim = 3 * N // 8
+    px = rng.integers(-lim, lim + 1, 20)
+    py = rng.integers(-lim, lim + 1, 20)
+    flux = rng.lognormal(0.0, 1.0, 20)
+    aa = Gs.anti_aliasing_function(N, 0, 1.3 * S)
+    image = torch.zeros((N, N), dtype=torch.float64, device=dev)
+    for x, y, f in zip(px, py, flux):
+        image[N // 2 + int(y), N // 2 + int(x)] += float(f / (aa[N // 2 + y] * aa[N // 2 + x]))
+    grid = Gs.fft(image.to(torch.complex128)).to(torch.complex64).contiguous()
+    del image
+    ref = torch.zeros(uvw_d.shape[0], dtype=torch.complex128, device=dev)
+    for x, y, f in zip(px, py, flux):
+        l, m = x / lam, y / lam
+        n1 = math.sqrt(1.0 - l * l - m * m) - 1.0
+        a = f * float(beam_taper(l, theta) * beam_taper(m, theta))
+        ref += a * torch.exp(-2j * math.pi * (uvw_d[:, 0] * l + uvw_d[:, 1] * m + uvw_d[:, 2] * n1))
+    mref = float(ref.abs().mean())
+    out = {}
+    for name, fn in predict_fns.items():
+        v = fn(grid).to(torch.complex128)
+        out[name] = float((v - ref).abs().pow(2).mean().sqrt()) / mref
+        del v
+    del grid, ref
+    return out

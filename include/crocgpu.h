@@ -70,6 +70,13 @@ int croc_ctx_set_tuning(croc_ctx *ctx, int tile, int chunk, int group_kib);
 /* Device time in milliseconds of the last gridding / degridding kernel
  * (HIP events around its launch on the context stream). */
 int croc_last_kernel_ms(croc_ctx *ctx, float *grid_ms, float *degrid_ms);
+/* Device time in milliseconds summed over the launches of the tap kernels
+ * proper (the kernels that spread / gather kernel taps: k_grid_tiles,
+ * k_tw_grid_jobs, k_degrid_tiles, k_tw_degrid_jobs, ...) of the last
+ * gridding and degridding call on this context, and their launch counts
+ * (HIP events around each launch, on the context stream). */
+int croc_last_tap_kernel_ms(croc_ctx *ctx, float *grid_ms, int *grid_launches,
+                            float *degrid_ms, int *degrid_launches);
 
 /* frac_coords: replaces crocodile/synthesis.py:438-449 (frac_coords) and
  * :419-435 (frac_coord) -- x = N//2 + p*N, fx = round-half-even(-x*Q),
