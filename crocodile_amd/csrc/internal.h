@@ -64,6 +64,7 @@ enum WsSlot {
   WS_TOWER,     // w-towers job / chunk tables
   WS_SORT,      // w-towers radix sort keys, indices, run tables, temporaries
   WS_GRIDQ,     // gridding work lists: dense tiles' range items, sparse tiles' items
+  WS_KSPLIT,    // w-towers: fp16 (hi, lo) split of a separable kernel set
   WS_COUNT
 };
 

@@ -358,6 +358,302 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Gridding of job parts with separable kernels on the matrix cores.
+//
+// With a separable kernel a visibility adds the outer product
+// (v ky) (x) kx to its S x S window, so the part's sub-grid is
+//   G = sum_k a_k b_k^T,  a_k = ky_k placed at rows [wy_k, wy_k + S),
+//                         b_k = v_k kx_k placed at columns [wx_k, wx_k + S),
+// a product of a 128 x K and a K x 128 matrix over the K visibilities.  Cut
+// into 16 x 16 blocks, block (by, bx) only needs the visibilities whose
+// window meets it (at most 2 x 2 blocks each for S <= 16), and for those
+// G_blk = A_blk B_blk is a dense complex product with K = the block's
+// visibilities: 16 x 16 x 32 steps of v_mfma_f32_16x16x32_f16, each FP32
+// operand split as hi + lo in fp16 (x = hi + lo to 22 bits; the lo * lo
+// term, 2^-22 relative, is dropped) and accumulated in FP32.  No LDS atomics:
+// every block belongs to one wavefront, which writes it once.  Operands:
+// A (ky) comes from a copy of the kernel set pre-split into fp16 (hi, lo)
+// pairs scaled by sk; B = v kx is split on the fly, scaled by sv sk.
+// Blocks are taken dynamically by the part's wavefronts; a wavefront lists
+// the block's visibilities in index order into a ring (ballot compaction)
+// and runs the matrix products 32 at a time, so the sums are deterministic.
+typedef _Float16 tw_half8 __attribute__((ext_vector_type(8)));
+typedef float tw_floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int MG_WIN = 1024;   // visibilities staged in LDS per window of a part
+constexpr int MG_RING = 128;   // per-wavefront ring of listed visibilities
+constexpr int MG_NW = TW_NT / 64;  // wavefronts per workgroup
+constexpr int MG_NB = 64;      // 8 x 8 blocks of 16 x 16 cells
+
+// Largest power of two 2^-e with x * 2^-e <= 1 (x > 0 finite), else 1.
+__device__ __forceinline__ float unit_scale(float x) {
+  if (!(x > 0.f) || !isfinite(x)) return 1.f;
+  int e;
+  frexpf(x, &e);  // x < 2^e
+  return ldexpf(1.f, -e);
+}
+
+// hi / lo fp16 split of the kernel set, scaled by sk = unit_scale(max |k|):
+// ksplit[i] = (re hi, re lo, im hi, im lo).
+__global__ void k_tw_split_tables(const float2 *__restrict__ kern, int64_t n,
+                                  const TwScale *__restrict__ fs, uint2 *__restrict__ ksplit) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float sk = unit_scale(__uint_as_float(fs->max_k_bits));
+  const float2 k = kern[i];
+  const float re = k.x * sk, im = k.y * sk;
+  const _Float16 rh = (_Float16)re, ih = (_Float16)im;
+  const _Float16 rl = (_Float16)(re - (float)rh), il = (_Float16)(im - (float)ih);
+  union {
+    _Float16 h[4];
+    uint2 u;
+  } o;
+  o.h[0] = rh;
+  o.h[1] = rl;
+  o.h[2] = ih;
+  o.h[3] = il;
+  ksplit[i] = o.u;
+}
+
+__device__ __forceinline__ tw_floatx4 tw_mfma(tw_half8 a, tw_half8 b, tw_floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// One workgroup per part, in windows of MG_WIN visibilities (S <= 16).
+// Per window:
+//   1. stage the records in LDS, with each visibility's B row, v kx split
+//      into fp16 (hi, lo) pairs (16 entries, zero beyond S), and count the
+//      visibilities by bucket, the block holding their window's first cell
+//      (per wavefront, each owning a contiguous range of the window);
+//   2. bucket starts, and each wavefront's offset within each bucket;
+//   3. stable scatter: bucket lists in visibility order (ballots per
+//      bucket within each batch of 64, wavefront ranges in order);
+//   4. blocks taken dynamically by the wavefronts: the visibilities that
+//      can reach block (by, bx) start in buckets (by - 1 .. by, bx - 1 ..
+//      bx); each is tested against the block and listed in a ring, 32 at a
+//      time going through the matrix products, A from the split kernel set
+//      in memory, B from LDS.
+// The lists' order depends only on the data, so the sums are
+// deterministic; a block is written (first window) or added to (later
+// windows) by the one wavefront holding it.
+__global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs_mfma(
+    const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
+    const float2 *__restrict__ svis, const float2 *__restrict__ kern,
+    const uint2 *__restrict__ ksplit, int S, int64_t tsize, const TwScale *__restrict__ fs,
+    float2 *__restrict__ sub, uint32_t *__restrict__ qc) {
+  __shared__ uint2 s_rec[MG_WIN];       // {wx | wy << 16, ky offset}
+  __shared__ uint2 s_bv[MG_WIN][16];    // B rows: v kx (hi, lo) pairs, scaled by sv sk
+  __shared__ uint8_t s_key[MG_WIN];
+  __shared__ uint16_t s_idx[MG_WIN];    // window positions, bucket by bucket
+  // listed visibilities: {wx | wy << 7 | position << 14, ky offset}
+  __shared__ uint2 s_ring[MG_NW][MG_RING];
+  __shared__ int s_cnt[MG_NW][MG_NB];   // per-wavefront bucket counts, then cursors
+  __shared__ int s_bstart[MG_NB + 1];
+  __shared__ int s_b, s_blk;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int m = lane & 15;          // A row / B column of this lane
+  const int kk = 8 * (lane >> 4);   // first of the lane's 8 K slots
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const float sk = unit_scale(__uint_as_float(fs->max_k_bits));
+  const float sv = unit_scale(__uint_as_float(fs->max_vis_bits));
+  const float out_scale = 1.f / (sv * sk * sk);
+  uint2 *ring = s_ring[wave];
+  ring[lane] = make_uint2(0u, 0u);  // slots past the listed ones are read, then ignored
+  ring[lane + 64] = make_uint2(0u, 0u);
+  for (;;) {  // parts taken from the launch's queue counter qc
+    if (tid == 0) s_b = (int)atomicAdd(qc, 1u);
+    __syncthreads();
+    const int b = s_b;
+    if (b >= nb) break;
+    const int4 job = parts[b];
+    float2 *out = sub + (int64_t)job.x * SGN;
+    for (int jb = job.y, win = 0; jb < job.z; jb += MG_WIN, win++) {
+      const int n = min(job.z - jb, MG_WIN);
+      const int per = (n + MG_NW - 1) / MG_NW;
+      const int lo = min(n, wave * per), hi = min(n, lo + per);  // this wavefront's range
+      s_cnt[tid >> 6][tid & 63] = 0;  // MG_NW x MG_NB = TW_NT counters
+      __syncthreads();
+      // 1a. records and bucket counts of the wavefront's range
+      for (int j = lo + lane; j < hi; j += 64) {
+        const uint4 r = rec[jb + j];
+        const uint32_t yf = r.y >> 16;
+        s_rec[j] = make_uint2(r.x, r.z * (uint32_t)tsize + yf * (uint32_t)S);
+        const int key = (int)((r.x >> 20) << 3) + (int)((r.x & 0xffffu) >> 4);  // (wy/16, wx/16)
+        s_key[j] = (uint8_t)key;
+        atomicAdd(&s_cnt[wave][key], 1);
+      }
+      // 1b. B rows, 16 lanes per visibility
+      for (int e = tid; e < 16 * n; e += TW_NT) {
+        const int j = e >> 4, c = e & 15;
+        uint2 o = make_uint2(0u, 0u);
+        if (c < S) {
+          const uint4 r = rec[jb + j];
+          const float2 v = svis[jb + j];
+          const uint32_t xf = r.y & 0xffffu;
+          const float2 k = kern[(int64_t)r.z * tsize + xf * S + c];
+          const float br = (v.x * k.x - v.y * k.y) * (sv * sk);
+          const float bi = (v.x * k.y + v.y * k.x) * (sv * sk);
+          union {
+            _Float16 h[4];
+            uint2 u;
+          } q;
+          q.h[0] = (_Float16)br;
+          q.h[1] = (_Float16)(br - (float)q.h[0]);
+          q.h[2] = (_Float16)bi;
+          q.h[3] = (_Float16)(bi - (float)q.h[2]);
+          o = q.u;
+        }
+        s_bv[j][c] = o;
+      }
+      barrier_after_lds_atomics();
+      // 2. bucket starts (exclusive scan over buckets) and wavefront offsets
+      if (tid < MG_NB) {
+        int tot = 0;
+        for (int w = 0; w < MG_NW; w++) tot += s_cnt[w][tid];
+        int inc = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int y = __shfl_up(inc, off, 64);
+          if (lane >= off) inc += y;
+        }
+        int run = inc - tot;
+        s_bstart[tid] = run;
+        if (tid == MG_NB - 1) s_bstart[MG_NB] = inc;
+        for (int w = 0; w < MG_NW; w++) {
+          const int c = s_cnt[w][tid];
+          s_cnt[w][tid] = run;
+          run += c;
+        }
+      }
+      __syncthreads();
+      // 3. stable scatter of the wavefront's range
+      for (int j0 = lo; j0 < hi; j0 += 64) {
+        const int j = j0 + lane;
+        bool todo = j < hi;
+        const int key = todo ? (int)s_key[j] : 0;
+        for (;;) {
+          const unsigned long long pend = __ballot(todo);
+          if (!pend) break;
+          const int k =
+              __builtin_amdgcn_readfirstlane(__shfl(key, __ffsll((long long)pend) - 1, 64));
+          const unsigned long long mk = __ballot(todo && key == k);
+          const int cur = __builtin_amdgcn_readfirstlane(s_cnt[wave][k]);
+          if (todo && key == k) {
+            s_idx[cur + __popcll(mk & lt)] = (uint16_t)j;
+            todo = false;
+          }
+          if (lane == 0) s_cnt[wave][k] = cur + __popcll(mk);
+        }
+      }
+      if (tid == 0) s_blk = 0;
+      __syncthreads();
+      // 4. blocks
+      for (;;) {
+        int blk = 0;
+        if (lane == 0) blk = atomicAdd(&s_blk, 1);
+        blk = __builtin_amdgcn_readfirstlane(__shfl(blk, 0, 64));
+        if (blk >= MG_NB) break;
+        const int by = blk >> 3, bx = blk & 7;
+        const int r0 = 16 * by, c0 = 16 * bx;
+        tw_floatx4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+        uint32_t head = 0, tail = 0;
+        // K-group of 32 listed visibilities starting at ring position h
+        // (nvalid of them real, the rest zero)
+        auto group = [&](uint32_t h, int nvalid) {
+          // branch-free: every slot loads (clamped addresses), then selects
+          uint2 ka[8], kb[8];
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            const uint2 e = ring[(h + kk + i) & (MG_RING - 1)];
+            const int wx = (int)(e.x & 127u), wy = (int)((e.x >> 7) & 127u);
+            const int j = (int)(e.x >> 14);
+            const int ry = r0 + m - wy, cx = c0 + m - wx;
+            const bool ok = kk + i < nvalid;
+            const uint2 a = ksplit[e.y + (uint32_t)min(max(ry, 0), S - 1)];
+            const uint2 c = s_bv[j][cx & 15];
+            const bool va = ok && ry >= 0 && ry < S, vb = ok && cx >= 0 && cx < 16;
+            ka[i] = make_uint2(va ? a.x : 0u, va ? a.y : 0u);
+            kb[i] = make_uint2(vb ? c.x : 0u, vb ? c.y : 0u);
+          }
+          tw_half8 arh, arl, aih, ail, brh, brl, bih, bil;
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            union {
+              uint2 u;
+              _Float16 h[4];
+            } a, c;
+            a.u = ka[i];
+            c.u = kb[i];
+            arh[i] = a.h[0];
+            arl[i] = a.h[1];
+            aih[i] = a.h[2];
+            ail[i] = a.h[3];
+            brh[i] = c.h[0];
+            brl[i] = c.h[1];
+            bih[i] = c.h[2];
+            bil[i] = c.h[3];
+          }
+          const tw_half8 naih = -aih, nail = -ail;
+          cr = tw_mfma(arh, brh, cr);
+          cr = tw_mfma(arh, brl, cr);
+          cr = tw_mfma(arl, brh, cr);
+          cr = tw_mfma(naih, bih, cr);
+          cr = tw_mfma(naih, bil, cr);
+          cr = tw_mfma(nail, bih, cr);
+          ci = tw_mfma(arh, bih, ci);
+          ci = tw_mfma(arh, bil, ci);
+          ci = tw_mfma(arl, bih, ci);
+          ci = tw_mfma(aih, brh, ci);
+          ci = tw_mfma(aih, brl, ci);
+          ci = tw_mfma(ail, brh, ci);
+        };
+        for (int sy = max(0, by - 1); sy <= by; sy++) {
+          for (int sx = max(0, bx - 1); sx <= bx; sx++) {
+            const int key = 8 * sy + sx;
+            const int p0 = s_bstart[key], p1 = s_bstart[key + 1];
+            for (int q0 = p0; q0 < p1; q0 += 64) {
+              const int q = q0 + lane;
+              bool mem = false;
+              int j = 0;
+              uint2 rj = make_uint2(0u, 0u);
+              if (q < p1) {
+                j = s_idx[q];
+                rj = s_rec[j];
+                const int wx = (int)(rj.x & 0xffffu), wy = (int)(rj.x >> 16);
+                mem = wy < r0 + 16 && wy + S > r0 && wx < c0 + 16 && wx + S > c0;
+                rj.x = (uint32_t)wx | ((uint32_t)wy << 7) | ((uint32_t)j << 14);
+              }
+              const unsigned long long bal = __ballot(mem);
+              if (mem) ring[(tail + __popcll(bal & lt)) & (MG_RING - 1)] = rj;
+              tail += (uint32_t)__popcll(bal);
+              while (tail - head >= 32) {
+                group(head, 32);
+                head += 32;
+              }
+            }
+          }
+        }
+        if (tail > head) group(head, (int)(tail - head));
+        // D element (row 4 (lane / 16) + q, column lane % 16)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          float2 *o = out + (r0 + 4 * (lane >> 4) + q) * SG + c0 + m;
+          float2 val = make_float2(cr[q] * out_scale, ci[q] * out_scale);
+          if (win) {
+            const float2 old = *o;
+            val.x += old.x;
+            val.y += old.y;
+          }
+          *o = val;
+        }
+      }
+      __syncthreads();  // the window's records are read; the next one rewrites them
+    }
+  }
+}
+
 // sub[first] += sub[first + 1 .. first + n - 1], parts in order: multi[m] =
 // (first slot, n) of a job gridded in n parts.  blockIdx.y = job,
 // blockIdx.x = slice of the pixels (two per thread).
@@ -788,6 +1084,16 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
   if ((rc = zero_u32_async(ctx, (uint32_t *)fs, sizeof(TwScale) / 4))) return rc;
   k_tw_maxima<<<ctx->num_cu, 1024, 0, s>>>(p.svis, a.nv, (const float2 *)kern, nk, fs);
   k_tw_pick<<<1, 1, 0, s>>>(fs, a.nv, a.nearest, separable);
+  // Separable kernels of support <= 16 grid on the matrix cores
+  // (k_tw_grid_jobs_mfma), from an fp16 (hi, lo) copy of the kernel set.
+  uint2 *ksplit = nullptr;
+  if (!a.nearest && separable && a.S <= 16) {
+    CROC_REQUIRE(nk < ((int64_t)1 << 32), CROC_EINVAL, "kernel set beyond 2^32 elements");
+    ksplit = (uint2 *)ws_get(ctx, WS_KSPLIT, sizeof(uint2) * (size_t)nk);
+    if (!ksplit) return CROC_ENOMEM;
+    k_tw_split_tables<<<(unsigned)((nk + 255) / 256), 256, 0, s>>>((const float2 *)kern, nk, fs,
+                                                                  ksplit);
+  }
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
   CROC_HIP(hipGetLastError());
   // zero the chunk images that will receive data
@@ -801,7 +1107,11 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
     const Batch &bt = batches[bi];
     const int G = (int)std::min<size_t>(bt.np, (size_t)ctx->num_cu);  // one per CU (LDS)
     if ((rc = tap_timer_begin(ctx, 0))) return rc;
-    if (a.nearest || !separable)
+    if (ksplit)
+      k_tw_grid_jobs_mfma<<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
+                                              (const float2 *)kern, ksplit, a.S, tsize, fs, sub,
+                                              qc + bi);
+    else if (a.nearest || !separable)
       k_tw_grid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
                                                 (const float2 *)kern, a.Q, a.S, tsize,
                                                 a.nearest, fs, sub, qc + bi);
