@@ -61,6 +61,9 @@ def parse():
                     help='measure a few tile / chunk settings during warm-up and keep the '
                          'fastest (the default is the library default: automatic tile, '
                          'chunk 512)')
+    ap.add_argument('--headline-only', action='store_true',
+                    help='towers workloads: time the headline w-towers steps only (no '
+                         'comparison paths, DFT accuracy or CPU baseline), for profiled runs')
     ap.add_argument('--tile', type=int, default=0)
     ap.add_argument('--chunk', type=int, default=512)
     ap.add_argument('--group-kib', type=int, default=0)
@@ -282,6 +285,15 @@ def run_towers(args, torch, G, W, _lib, dev):
                     degrid_launches=int(a[:, 3].mean()))
 
     res = timed(towers_step, 'w-towers (planned)', args.steps, args.warmup)
+    if args.headline_only:
+        _lib.check(L.croc_wtowers_plan_destroy(_lib.ctx(), twp))
+        print(json.dumps({'metric': 'visibilities/s grid+degrid', 'value': round(res['value'], 1),
+                          'unit': 'visibilities/s', 'ms_per_step': round(res['ms'], 3),
+                          'kernel_ms_per_step': {'k_tw_grid_jobs': round(res['grid_tap_ms'], 3),
+                                                 'k_tw_degrid_jobs': round(res['degrid_tap_ms'], 3)},
+                          'launches_per_step': [res['grid_launches'], res['degrid_launches']],
+                          'config': {'workload': name, 'headline_only': True}}), flush=True)
+        return
     few = max(2, args.steps // 4)
     one = timed(towers_one_shot, 'w-towers (one-shot)', few, 1)
     plain = timed(lambda: plain_step('rule'), 'plain tile path (support rule)', few, 1)

@@ -144,25 +144,40 @@ __global__ void k_zoom_cols(const double2 *__restrict__ tmp, int64_t tmp_stride,
   out[o].y = si;
 }
 
-// 1D: out[xf][x] = sum_l ff[l] E(Q x + xf - Q(S//2), l).
+// 1D: out[xf][x] = sum_l ff[l] E(Q x + xf - Q(S//2), l); one workgroup per
+// output sample, its threads summing strided terms (FP64), then a
+// workgroup reduction: the far field may be a whole grid row (F in the
+// thousands) while there are only Q * S outputs.
+constexpr int ZOOM1D_NT = 256;
 template <typename OutT>
-__global__ void k_zoom_1d(const double2 *__restrict__ ff, int F, int Q, int S,
-                          OutT *__restrict__ out) {
-  const int64_t QS = (int64_t)Q * S;
-  const int64_t kidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (kidx >= QS) return;
+__global__ __launch_bounds__(ZOOM1D_NT) void k_zoom_1d(const double2 *__restrict__ ff, int F,
+                                                      int Q, int S, OutT *__restrict__ out) {
+  __shared__ double s_re[ZOOM1D_NT], s_im[ZOOM1D_NT];
+  const int64_t kidx = blockIdx.x;
   const int64_t k = kidx - (int64_t)Q * (S / 2);
   const int64_t M = (int64_t)F * Q;
   double sr = 0.0, si = 0.0;
-  for (int l = 0; l < F; l++) {
+  for (int l = threadIdx.x; l < F; l += ZOOM1D_NT) {
     const double2 t = twiddle(mod64(k * (l - F / 2), M), M);
     const double2 f = ff[l];
     sr += f.x * t.x - f.y * t.y;
     si += f.x * t.y + f.y * t.x;
   }
-  const int x = (int)(kidx / Q), xf = (int)(kidx - (int64_t)x * Q);
-  out[(int64_t)xf * S + x].x = sr;
-  out[(int64_t)xf * S + x].y = si;
+  s_re[threadIdx.x] = sr;
+  s_im[threadIdx.x] = si;
+  __syncthreads();
+  for (int h = ZOOM1D_NT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s_re[threadIdx.x] += s_re[threadIdx.x + h];
+      s_im[threadIdx.x] += s_im[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int x = (int)(kidx / Q), xf = (int)(kidx - (int64_t)x * Q);
+    out[(int64_t)xf * S + x].x = s_re[0];
+    out[(int64_t)xf * S + x].y = s_im[0];
+  }
 }
 
 int oversample_planes(croc_ctx *ctx, const double2 *ff, int64_t ff_stride,
@@ -270,7 +285,7 @@ extern "C" int croc_kernel_oversample(croc_ctx *ctx, const void *ff, int n,
                CROC_EINVAL, "support %d too large for far field %d", s, n);
   if (ndim == 1) {
     const int64_t QS = (int64_t)Q * s;
-    k_zoom_1d<double2><<<(unsigned)((QS + 255) / 256), 256, 0, ctx->stream>>>(
+    k_zoom_1d<double2><<<(unsigned)QS, ZOOM1D_NT, 0, ctx->stream>>>(
         (const double2 *)ff, n, Q, s, (double2 *)out);
     CROC_HIP(hipGetLastError());
     CROC_HIP(hipStreamSynchronize(ctx->stream));
