@@ -413,6 +413,7 @@ def kernel_bank(gcf, src, nv, for_grid):
     elif d == 4:
         separable = False
         Q, gh, gw = gcf.shape[-3:]
+        S = gw
         if gh != gw:
             if for_grid:
                 # convgrid unpacks make_gcf's (Q, gw, gh) as (Q, gh, gw), so
@@ -420,8 +421,17 @@ def kernel_bank(gcf, src, nv, for_grid):
                 # broadcasting fails (synthesis.py:458, :477, :480-481).
                 raise ValueError('operands could not be broadcast: convgrid '
                                  'with a non-square %dx%d kernel' % (gh, gw))
-            raise NotImplementedError('non-square 2D kernel tables')
-        S = gw
+            # convdegrid's gh x gw window (synthesis.py:497-503) is the
+            # S x S window of S = max(gh, gw) with the table zero-padded by
+            # S//2 - gh//2 rows above and S//2 - gw//2 columns to the left:
+            # row i of the table lands on grid row y - gh//2 + i either way.
+            S = max(gh, gw)
+            ro, co = S // 2 - gh // 2, S // 2 - gw // 2
+            t = _dev(gcf, torch.complex64)
+            padded = torch.zeros(tuple(gcf.shape[:-2]) + (S, S), dtype=torch.complex64,
+                                 device=t.device)
+            padded[..., ro:ro + gh, co:co + gw] = t
+            gcf = padded
     else:
         raise AssertionError('unsupported kernel table rank')
     prefix = tuple(gcf.shape[:k])
@@ -844,37 +854,73 @@ def _slice_planes(uv, wstep):
     return means, vplane, order
 
 
+def _slice_bank(kernel_fn, theta, means, vplane, src, nv, kwargs):
+    """Bank and plane index of the w-slices.  Without source columns, one
+    plane per slice.  With source columns (k of them) each slice's table
+    kernel_fn(theta, w, **kwargs) carries k leading dimensions indexed by a
+    visibility's source row (make_gcf, synthesis.py:451-461): the planes
+    are every slice's sub-tables, plane = slice * prod(prefix) + the row's
+    flat index (negative indices wrap as NumPy's)."""
+    src_rows = _src_rows(src, nv)
+    if src_rows is None or src_rows.shape[1] == 0:
+        return _bank_from_fn(kernel_fn, theta, means, None, kwargs).as_complex64(), vplane
+    k = src_rows.shape[1]
+    tables = [kernel_fn(theta, w, **kwargs) for w in means]
+    shape = tuple(tables[0].shape)
+    d = len(shape) - k
+    if d not in (2, 4):
+        raise AssertionError('unsupported kernel table rank')
+    prefix = shape[:k]
+    subs = []
+    for t in tables:
+        if tuple(t.shape) != shape:
+            raise ValueError('w-slice kernel tables differ in shape')
+        t = t if _is_torch(t) else numpy.asarray(t)
+        subs.extend(t.reshape((-1,) + shape[k:])[i] for i in range(int(numpy.prod(prefix))))
+    if d == 4 and shape[-1] != shape[-2]:
+        raise NotImplementedError('non-square 2D kernel tables')
+    bank = KernelBank.from_tables(subs, separable=(d == 2))
+    idx = src_rows.astype(numpy.int64)
+    dims = numpy.array(prefix, dtype=numpy.int64)
+    idx = numpy.where(idx < 0, idx + dims, idx)
+    if (idx < 0).any() or (idx >= dims).any():
+        raise IndexError('src index out of bounds for the kernel table')
+    flat = torch.as_tensor(numpy.ravel_multi_index(tuple(idx.T), prefix).astype(numpy.int32),
+                           device=vplane.device)
+    return bank, (vplane * int(numpy.prod(prefix)) + flat).to(torch.int32).contiguous()
+
+
 def w_slice_imaging(theta, lam, uvw, src, vis, wstep=2000, kernel_fn=w_kernel,
                     **kwargs):
     """W-projection by slices of ``wstep`` visibilities sorted by w, kernel at
     each slice's mean w.  The reference refers to undefined ``p, v``
     (NameError, synthesis.py:638); the intended ``uvw, vis`` are used, with
-    a stable sort.  synthesis.py:616-644."""
+    a stable sort, and ``src`` rows travel with their visibilities (the
+    reference passes the unsorted ``src`` to every slice's convgrid).
+    synthesis.py:616-644."""
     N = _grid_size(theta, lam)
     uv = _uvw(uvw)
     means, vplane, order = _slice_planes(uv, wstep)
-    src_rows = _src_rows(src, uv.shape[0])
-    if src_rows is not None and src_rows.shape[1]:
-        raise NotImplementedError('w_slice_imaging with source columns')
-    bank = _bank_from_fn(kernel_fn, theta, means, None, kwargs)
+    bank, vplane = _slice_bank(kernel_fn, theta, means, vplane, src, uv.shape[0], kwargs)
     guv = torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
-    grid_device(bank.as_complex64(), guv, uv, lam, _vis_dev(vis), _lib.PLANE_INDEX, vplane)
+    grid_device(bank, guv, uv, lam, _vis_dev(vis), _lib.PLANE_INDEX, vplane)
     return _out(guv, _torch_in(uvw, vis))
 
 
 def w_slice_predict(theta, lam, uvw, src, guv, wstep=2000, kernel_fn=w_kernel,
                     **kwargs):
-    """Predict counterpart of w_slice_imaging, results in input order.  The
+    """Predict counterpart of w_slice_imaging, results in input order, with
+    the source rows selecting sub-tables as in w_slice_imaging.  The
     reference uses undefined ``p`` (NameError, synthesis.py:672).
     synthesis.py:647-679."""
     _grid_size(theta, lam)
     uv = _uvw(uvw)
     means, vplane, order = _slice_planes(uv, wstep)
-    bank = _bank_from_fn(kernel_fn, theta, means, None, kwargs)
+    bank, vplane = _slice_bank(kernel_fn, theta, means, vplane, src, uv.shape[0], kwargs)
     gt = _dev(guv, torch.complex64)
     _square(gt)
     out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
-    degrid_device(bank.as_complex64(), gt, uv, lam, out, _lib.PLANE_INDEX, vplane)
+    degrid_device(bank, gt, uv, lam, out, _lib.PLANE_INDEX, vplane)
     return _out(out, _torch_in(uvw, guv))
 
 

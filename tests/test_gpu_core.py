@@ -519,3 +519,23 @@ def test_pervis_kernels_parity(gpu, N, S, Q, nv, planes, separable):
     with pytest.raises(ValueError):
         G.grid_device_pervis(bank, a, bad, 1.0, vis, _lib.PLANE_INDEX, vp)
     assert torch.equal(a, before)
+
+
+@pytest.mark.parametrize('gh,gw,src', [(6, 10, False), (11, 4, False), (7, 8, True)])
+def test_convdegrid_non_square_kernel(gpu, gh, gw, src):
+    """convdegrid with a non-square [Q, Q, gh, gw] table (its window is
+    gh rows x gw columns, synthesis.py:497-503), optionally indexed by a
+    source column, against the oracle; windows kept off the grid edge."""
+    rng = numpy.random.default_rng(gh * 31 + gw)
+    N, Q, nv = 128, 4, 3000
+    shape = ((3,) if src else ()) + (Q, Q, gh, gw)
+    gcf = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    lim = 0.5 - (max(gh, gw) // 2 + 2) / N
+    p = rng.uniform(-lim, lim, (nv, 2))
+    s = rng.integers(-3, 3, (nv, 1)) if src else None
+    a = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    got = G.convdegrid(gcf, a, p, s)
+    ref = R.convdegrid(gcf, a, p, s)
+    assert_close(got, ref, what='convdegrid %dx%d' % (gh, gw))
+    with pytest.raises(ValueError):  # convgrid transposes the window (synthesis.py:477)
+        G.convgrid(gcf, numpy.zeros((N, N), complex), p, s, numpy.ones(nv))

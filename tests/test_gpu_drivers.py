@@ -223,3 +223,30 @@ def test_wproj2048_full_size(gpu):
     G.degrid_device(bank_p.as_complex64(), torch.as_tensor(ref, device=dev).to(torch.complex64),
                     uvw_d, c['lam'], out, 2, None, c['wstep'], lo)
     assert_close(out.cpu().numpy(), vref, what='wproj2048 degrid')
+
+
+@pytest.mark.parametrize('separable', [False, True])
+def test_w_slice_with_source_columns(gpu, separable):
+    """w_slice_* with source columns: each slice's table carries leading
+    dimensions indexed by the visibility's source row (make_gcf,
+    synthesis.py:451-461; negative rows wrap), against the oracle."""
+    theta, lam = 0.1, 3000.0
+    rng = numpy.random.default_rng(12)
+    nv = 4000
+    uvw = numpy.stack([rng.uniform(-1000, 1000, nv), rng.uniform(-1000, 1000, nv),
+                       rng.uniform(-300, 300, nv)], axis=1)
+    vis = rng.standard_normal(nv) + 1j * rng.standard_normal(nv)
+    src = numpy.stack([rng.integers(-2, 2, nv), rng.integers(0, 3, nv)], axis=1)
+    f = rng.standard_normal((2, 3)) + 1j * rng.standard_normal((2, 3))
+
+    def kfn(theta, w, invert=True):
+        k = R.w_kernel(theta, w, 16, 8, 4, invert=invert) / 256
+        if separable:
+            k = k[:, 0, 4, :]  # a [Q, S] table, made up but fixed per w
+        return numpy.array([[k * f[i, j] for j in range(3)] for i in range(2)])
+    g = G.w_slice_imaging(theta, lam, uvw, src, vis, wstep=700, kernel_fn=kfn)
+    ref = R.w_slice_imaging(theta, lam, uvw, src, vis, wstep=700, kernel_fn=kfn)
+    assert_close(g, ref, what='w_slice_imaging src sep=%d' % separable)
+    v = G.w_slice_predict(theta, lam, uvw, src, ref, wstep=700, kernel_fn=kfn, invert=False)
+    vref = R.w_slice_predict(theta, lam, uvw, src, ref, wstep=700, kernel_fn=kfn, invert=False)
+    assert_close(v, vref, what='w_slice_predict src sep=%d' % separable)
