@@ -23,6 +23,18 @@ _I = ctypes.c_int
 _I64 = ctypes.c_int64
 _D = ctypes.c_double
 
+
+class BlSetC(ctypes.Structure):
+    """croc_bl_set: a packed set of baselines (device arrays)."""
+    _fields_ = [('nbl', _I64), ('uvw_rows', _I64), ('freq_len', _I64), ('nvis', _I64),
+                ('time_count', _P), ('freq_count', _P), ('uvw_first', _P),
+                ('freq_first', _P), ('vis_first', _P), ('uvw_m', _P), ('freq', _P),
+                ('vis', _P)]
+
+
+_BL = ctypes.POINTER(BlSetC)
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+
 # name -> argtypes, in the order of include/crocgpu.h
 SIGNATURES = {
     'croc_version': [],
@@ -68,6 +80,16 @@ SIGNATURES = {
                               _D],
     'croc_wtowers_grid_plan': [_P, _P, _P, _P, _P, _I],
     'croc_wtowers_degrid_plan': [_P, _P, _P, _P, _P, _I],
+    'croc_bl_weight': [_P, _P, _I, _D, _BL],
+    'croc_bl_grid_simple': [_P, _P, _I, _D, _BL, _U64P],
+    'croc_bl_degrid_simple': [_P, _P, _I, _D, _BL, _U64P],
+    'croc_degrid_conv_uv': [_P, _P, _I, _D, _P, _I64, _P, _I, _I, _P, _U64P],
+    'croc_degrid_conv_bl': [_P, _P, _I, _D, _D, _D, _D, _D, _D, _D, _BL, _I, _I, _I, _I, _P,
+                            _I, _I, _U64P],
+    'croc_degrid_conv': [_P, _P, _I, _D, _D, _D, _BL, _P, _I, _I, _U64P],
+    'croc_grid_wprojection': [_P, _P, _I, _D, _BL, _P, _I, _D, _D, _I, _I, _U64P],
+    'croc_streamer_degrid': [_P, _P, _I, _D, _D, _D, _D, _D, _D, _D, _BL, _P, _I64, _I, _I,
+                             _P, _I, _I, _P, _P, _U64P],
 }
 
 _lib = None

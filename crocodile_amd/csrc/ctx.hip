@@ -26,6 +26,10 @@ int croc_status_to_code(uint32_t st) {
     croc_set_error("w-bin / plane index outside the kernel bank");
     return CROC_EPLANE;
   }
+  if (st & ST_RANGE) {
+    croc_set_error("time / frequency window or array range outside a baseline's data");
+    return CROC_EINVAL;
+  }
   if (st & ST_BANK) {
     croc_set_error("kernel bank larger than 2^32 complex elements");
     return CROC_EINVAL;

@@ -38,6 +38,7 @@ enum : uint32_t {
   ST_PLANE = 4u,   // plane index outside the bank
   ST_SUPPORT = 8u, // a plane's support exceeds the declared maximum
   ST_BANK = 16u,   // a kernel table lies beyond element 2^32 of the bank
+  ST_RANGE = 32u,  // a baseline's time / frequency window or array range is invalid
 };
 
 int croc_status_to_code(uint32_t st);
@@ -65,6 +66,8 @@ enum WsSlot {
   WS_SORT,      // w-towers radix sort keys, indices, run tables, temporaries
   WS_GRIDQ,     // gridding work lists: dense tiles' range items, sparse tiles' items
   WS_KSPLIT,    // w-towers: fp16 (hi, lo) split of a separable kernel set
+  WS_BL,        // baseline layout: flattened coordinates, planes, visibilities
+  WS_BL_BANK,   // baseline layout: kernel-bank plane tables, unit kernel
   WS_COUNT
 };
 

@@ -296,6 +296,118 @@ int croc_convolve_same(croc_ctx *ctx, const void *a, int ah, int aw,
  * Native counterpart: examples/grid/grid.c:794-817. */
 int croc_make_hermitian(croc_ctx *ctx, void *grid, int N, int is_double);
 
+/* ------------------------------------------------------------------------
+ * Baseline / time / frequency layout: the native counterparts of
+ * examples/grid/grid.h:14-40 (struct bl_data, struct vis_data) and the
+ * functions of grid.c / streamer.c over it.
+ *
+ * A set of baselines in device memory, packed: baseline b has
+ * time_count[b] dumps and freq_count[b] channels; its uvw (metres) are rows
+ * uvw_first[b] .. + time_count[b] - 1 of uvw_m [uvw_rows][3]; its channel
+ * frequencies (Hz) freq[freq_first[b] ..]; its visibilities
+ * vis[vis_first[b] + time * freq_count[b] + freq] (cf32 [nvis]).  A
+ * visibility's coordinate in wavelengths is uvw_m * f / c (grid.h:53-60,
+ * c = 299792458 m/s, FP64, rounded as written).  The struct itself is host
+ * memory; every array it points to is device memory.  Ranges are checked
+ * on the device (CROC_EINVAL when a baseline's arrays or a time / frequency
+ * window leave their arrays).
+ *
+ * Coordinates follow grid.c's frac_coord (:35-53): ox = rint((N/2 - x) Q)
+ * (round-half-even: the fesetround(3) call there is rejected by glibc),
+ * cell N - ox/Q, sub-offset ox % Q, where x = theta * u.  A kernel window
+ * that leaves the grid is CROC_EBOUNDS (grid.c reads or writes the
+ * neighbouring row, or outside the array, there).  Separable kernels here
+ * are real FP64 tables [Q][S] (grid.h:101-106 sep_kernel_data; sub-pixel
+ * offset outer).  All of these synchronise the stream before returning and
+ * leave their outputs unmodified on error.  flops (host, may be NULL)
+ * receives the reference's flop count of the call.
+ * ------------------------------------------------------------------------ */
+typedef struct croc_bl_set {
+  int64_t nbl;                /* baselines (vis_data.bl_count)            */
+  int64_t uvw_rows;           /* rows of uvw_m                            */
+  int64_t freq_len;           /* entries of freq                          */
+  int64_t nvis;               /* entries of vis                           */
+  const int32_t *time_count;  /* [nbl]                                    */
+  const int32_t *freq_count;  /* [nbl]                                    */
+  const int64_t *uvw_first;   /* [nbl]                                    */
+  const int64_t *freq_first;  /* [nbl]                                    */
+  const int64_t *vis_first;   /* [nbl]                                    */
+  const double *uvw_m;        /* [uvw_rows][3], metres                    */
+  const double *freq;         /* [freq_len], Hz                           */
+  void *vis;                  /* cf32 [nvis]                              */
+} croc_bl_set;
+
+/* weight (grid.c:119-141): counts the visibilities of every nearest cell
+ * (floor(theta * u + .5) + N/2, grid.c:22-32) into wgrid (uint32 [N][N],
+ * overwritten), then divides each visibility by its cell's count. */
+int croc_bl_weight(croc_ctx *ctx, uint32_t *wgrid, int N, double theta,
+                   const croc_bl_set *vis);
+
+/* grid_simple (grid.c:161-177): uvgrid (cf32 [N][N]) += each visibility at
+ * its nearest cell, accumulated exactly (int64 fixed point) in any order.
+ * degrid_simple (grid.c:143-159): each visibility = uvgrid at its cell. */
+int croc_bl_grid_simple(croc_ctx *ctx, void *uvgrid, int N, double theta,
+                        const croc_bl_set *vis, uint64_t *flops);
+int croc_bl_degrid_simple(croc_ctx *ctx, const void *uvgrid, int N, double theta,
+                          const croc_bl_set *vis, uint64_t *flops);
+
+/* degrid_conv_uv (grid.c:179-232) for n coordinates uv (double [n][2],
+ * wavelengths): vis_out[i] (cf32) = sum_y k[fy][y] sum_x k[fx][x]
+ * uvgrid[iy - S/2 + y][ix - S/2 + x], kernel k real double [Q][S]. */
+int croc_degrid_conv_uv(croc_ctx *ctx, const void *uvgrid, int N, double theta,
+                        const double *uv, int64_t n, const double *kern, int S,
+                        int Q, void *vis_out, uint64_t *flops);
+
+/* degrid_conv_bl (grid.c:234-259), for every baseline of the set: the
+ * visibilities (time, freq) of [time0, time1) x [freq0, freq1) whose
+ * (u, v) lie in [min_u, max_u) x [min_v, max_v) are degridded at
+ * (u - d_u, v - d_v) into vis[vis_first[b] + (time - time0) * (freq1 -
+ * freq0) + freq - freq0] (the reference's chunk-local index); the others
+ * are left untouched. */
+int croc_degrid_conv_bl(croc_ctx *ctx, const void *uvgrid, int N, double theta,
+                        double d_u, double d_v, double min_u, double max_u,
+                        double min_v, double max_v, const croc_bl_set *bl,
+                        int time0, int time1, int freq0, int freq1,
+                        const double *kern, int S, int Q, uint64_t *flops);
+
+/* degrid_conv (grid.c:261-275): degrid_conv_bl of every baseline over all
+ * its dumps and channels with the grid's own window
+ * [d - N/theta/2, d + N/theta/2). */
+int croc_degrid_conv(croc_ctx *ctx, const void *uvgrid, int N, double theta,
+                     double d_u, double d_v, const croc_bl_set *vis,
+                     const double *kern, int S, int Q, uint64_t *flops);
+
+/* grid_wprojection (grid.c:307-322 over w_project :277-305): every
+ * visibility adds v * conj(k) over its S x S window, k the table
+ * (yf, xf) of w-plane floor((w - w_min) / w_step + .5) of wkern (cf32
+ * [plane_count][Q][Q][S][S], w_kernel_data grid.h:114-122), into uvgrid
+ * (cf32 [N][N], in place), accumulated exactly (int64 fixed point).  A
+ * plane outside [0, plane_count) is CROC_EPLANE. */
+int croc_grid_wprojection(croc_ctx *ctx, void *uvgrid, int N, double theta,
+                          const croc_bl_set *vis, const void *wkern,
+                          int plane_count, double w_min, double w_step, int S,
+                          int Q, uint64_t *flops);
+
+/* streamer_degrid_worker (streamer.c:366-486) for nchunks chunks of one
+ * subgrid (cf32 [subgrid_size][subgrid_size], centred at (mid_u, mid_v)):
+ * chunk c = chunks[5c .. 5c + 4] = (baseline, it0, it1, if0, if1), its
+ * visibility (time, freq) written to vis_queue[c * time_chunk * freq_chunk
+ * + (time - it0) * freq_chunk + freq - if0] (cf32): degridded at
+ * (u - mid_u, v - mid_v) when u >= max(min_u, 0), u < max_u, v in
+ * [min_v, max_v); else the conjugate of the degrid at (-u - mid_u,
+ * -v - mid_v) when -u >= max(min_u, DBL_MIN) and (-u, -v) lies in the
+ * window; else 0 (streamer.c:418-438).  Slot entries beyond a chunk's
+ * extent are not written.  ndegrid (device int64 [nchunks], may be NULL)
+ * receives each chunk's count of degridded visibilities (the reference's
+ * "flops == 0" test for an empty chunk). */
+int croc_streamer_degrid(croc_ctx *ctx, const void *subgrid, int subgrid_size,
+                         double theta, double mid_u, double mid_v, double min_u,
+                         double max_u, double min_v, double max_v,
+                         const croc_bl_set *bl, const int32_t *chunks,
+                         int64_t nchunks, int time_chunk, int freq_chunk,
+                         const double *kern, int S, int Q, void *vis_queue,
+                         int64_t *ndegrid, uint64_t *flops);
+
 #ifdef __cplusplus
 }
 #endif
