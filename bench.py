@@ -47,13 +47,14 @@ def parse():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='towers8192',
                     choices=['wproj2048', 'plain512', 'towers4096', 'towers8192', 'diff1024',
-                             'dft128'])
+                             'dft128', 'stream32k'])
     ap.add_argument('--nv', type=int, default=0,
                     help='towers workloads: visibilities (0 = the workload\'s own count)')
     ap.add_argument('--cpu-sample', type=int, default=0,
                     help='visibilities in the CPU-baseline sample (0: the workload default, '
                          '20000 for the towers workloads, 300000 for wproj2048, about 10-15 s '
-                         'of one core; negative disables it)')
+                         'of one core; for stream32k a count of chunk slots, default 32000; '
+                         'negative disables it)')
     ap.add_argument('--copy-probe-mib', type=int, default=0,
                     help='before warm-up, copy a buffer of this many MiB once (a plain copy '
                          'of known size, to scale HBM counters of a profiled run)')
@@ -564,6 +565,150 @@ def run_diff(args, torch, G, W, _lib, dev):
     print(json.dumps(line), flush=True)
 
 
+def run_stream(args, torch, W, _lib, dev):
+    """Subgrid-streaming degrid (examples/grid/streamer.c:366-600) on the
+    stream32k workload: 130816 baselines x 128 dumps x one 64-channel
+    frequency chunk (1.07e9 visibilities, uvw in metres, frequencies in Hz:
+    the streamer's bl_data layout) against the subgrids of the 32k-8k-1k
+    recombination (xM = 1024 spaced xA = 832, iu >= 0 half of the uv plane).
+    One step = every subgrid of the work assignment degridded into queue
+    slots (separable PSWF kernel S = 8, Q = 1024), so each visibility is
+    degridded exactly once (directly or as the conjugate of its mirror
+    image): all subgrids in one croc_streamer_degrid_batch launch (the
+    headline), and the reference-shaped one croc_streamer_degrid call per
+    subgrid (reported beside it).  The subgrids are random complex64 arrays
+    resident in HBM (the facet recombination that would produce them is not
+    part of this path); chunk lists are selected before timing by
+    streamer_degrid_chunk's bounding-box test.  The queue writer (HDF5
+    output) is not timed."""
+    import numpy
+    from crocodile_amd import bl as BL
+    c = W.SPEC['stream32k']
+    S, Q, tc, fc = c['S'], c['Q'], c['time_chunk'], c['freq_chunk']
+    theta, xA, xM = c['theta'], c['xA'], c['xM']
+    print('[bench] building workload stream32k', file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    uvw_h, freq_h = W.stream()
+    vis = BL.VisData.uniform(uvw_h, freq_h, device=dev)
+    kern_h = W.stream_kernel()
+    kern = BL.SepKernelData(kern_h)
+    st = BL.Streamer(vis, theta, xA, xM, tc, fc, kern)
+    umax = float(numpy.abs(uvw_h[..., :2]).max() * freq_h.max() / W.C_LIGHT)
+    offs = [o for o in W.stream_subgrids(umax=umax) if len(st.chunks_for(*o))]
+    wins, ch6 = st.batch(offs)
+    nch = ch6.shape[0]
+    per_sub = numpy.bincount(ch6[:, 0].cpu().numpy(), minlength=len(offs))
+    first = numpy.concatenate([[0], numpy.cumsum(per_sub)])
+    ch5 = ch6[:, 1:].contiguous()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(c['seed'])
+    subs = torch.randn((len(offs), xM, xM), dtype=torch.complex64, device=dev, generator=gen)
+    queue = torch.empty((nch, tc * fc), dtype=torch.complex64, device=dev)
+    nd = torch.empty(nch, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    per_vis = 4 * (1 + S) * S      # grid.c:222: flops per degridded visibility
+    nvis_total = int(vis.vis.shape[0])
+
+    def step_batch():
+        BL.streamer_degrid_batch(vis, subs, theta, wins, ch6, kern, tc, fc, vis_queue=queue,
+                                 ndegrid=nd, want_flops=False)
+        return _lib.last_tap_kernel_ms()[2]
+
+    def step_single():
+        for i in range(len(offs)):
+            a, b = int(first[i]), int(first[i + 1])
+            w = wins[i]
+            BL.streamer_degrid_worker(vis, subs[i], xM, theta, w[0], w[1], ch5[a:b], w[2], w[3],
+                                      w[4], w[5], kern, tc, fc, vis_queue=queue[a:b],
+                                      ndegrid=nd[a:b], want_flops=False)
+
+    print('[bench] stream32k: %d subgrids, %d chunk slots, warm-up' % (len(offs), nch),
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step_batch()
+        step_single()
+    torch.cuda.synchronize()
+    kms = 0.0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        kms += step_batch()
+        print('[bench] step %d done' % i, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n_step = int(nd.sum())
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        step_single()
+    torch.cuda.synchronize()
+    dt_single = time.perf_counter() - t1
+    assert int(nd.sum()) == n_step
+    kernel_ms = kms / args.steps
+    achieved = n_step * per_vis / (kernel_ms * 1e-3) / 1e12
+    line = {
+        'metric': 'visibilities/s degridded from streamed subgrids',
+        'value': round(n_step * args.steps / dt, 1), 'unit': 'visibilities/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(1e3 * dt / args.steps, 3), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32 (f64 coordinates)',
+        'data': 'synthetic (random subgrids, station tracks)',
+        'config': {'workload': 'stream32k', 'image_size': c['image_size'],
+                   'theta': round(theta, 6), 'xA': xA, 'xM': xM, 'baselines': vis.bl_count,
+                   'dumps': c['ndump'], 'channels': c['nchan'], 'time_chunk': tc,
+                   'freq_chunk': fc, 'support': S, 'oversampling': Q,
+                   'subgrids': len(offs), 'chunk_slots': nch, 'visibilities': nvis_total,
+                   'degridded_per_step': n_step,
+                   # chunks are selected by the bounding box of their corner
+                   # dumps and channels (streamer.c:520-535): a visibility of
+                   # a curved track outside that box is missed, as there
+                   'degridded_fraction': round(n_step / nvis_total, 6),
+                   'step': 'all subgrids in one croc_streamer_degrid_batch launch',
+                   'setup_s': round(setup_s, 1)},
+        'per_subgrid_calls': {
+            'entry': 'croc_streamer_degrid, one call per subgrid (streamer_degrid_worker)',
+            'ms_per_step': round(1e3 * dt_single / args.steps, 3),
+            'visibilities_per_s': round(n_step * args.steps / dt_single, 1)},
+        'roofline': {'bound': 'valu', 'achieved': round(achieved, 3),
+                     'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': round(achieved / FP32_PEAK_TFLOPS, 5), 'kernel': 'k_bl_degrid',
+                     'kernel_ms_per_step': round(kernel_ms, 3),
+                     'flop_per_visibility': per_vis, 'traffic': None,
+                     'note': 'algorithmic flops of degrid_conv_uv (grid.c:222, 4 (1 + S) S '
+                             'per visibility) / k_bl_degrid launch time (HIP events)'},
+    }
+    if args.cpu_sample >= 0:
+        from oracle import blgrid as OB
+        # the busiest subgrid's chunks, a prefix of them: about 5-10 s of one core
+        i = int(numpy.argmax(per_sub))
+        a = int(first[i])
+        ncs = min(int(per_sub[i]), args.cpu_sample if args.cpu_sample > 0 else 32000)
+        ch_h = ch5[a:a + ncs].cpu().numpy()
+        bls = numpy.unique(ch_h[:, 0])
+        hs = OB.BlSet.uniform(uvw_h[bls], freq_h, with_vis=False)
+        chs = ch_h.copy()
+        chs[:, 0] = numpy.searchsorted(bls, ch_h[:, 0])
+        sub_h = subs[i].cpu().numpy().astype(numpy.complex128)
+        t1 = time.perf_counter()
+        ref, nd_ref = OB.streamer_degrid(sub_h, theta, *wins[i], hs, chs, tc, fc, kern_h)
+        cpu_s = time.perf_counter() - t1
+        q = queue[a:a + ncs].cpu().numpy()
+        zero = ref == 0
+        d = q - ref
+        line['cpu_baseline'] = {
+            'value': round(float(nd_ref.sum()) / cpu_s, 1), 'unit': 'visibilities/s',
+            'cores': 1, 'kind': 'port',
+            'sample': '%d chunk slots (%d degridded visibilities) of the busiest subgrid, '
+                      'oracle/blgrid.c streamer_degrid (FP64 C restatement of '
+                      'streamer.c:366-486), %.1f s' % (ncs, int(nd_ref.sum()), cpu_s)}
+        line['parity'] = {
+            'vs': 'oracle/blgrid.c on the CPU-baseline sample (slots of the timed run)',
+            'selection_exact': bool((nd[a:a + ncs].cpu().numpy() == nd_ref).all() and
+                                    ((q == 0) == zero).all()),
+            'rms_rel': float(numpy.linalg.norm(d) / numpy.linalg.norm(ref)),
+            'max_rel': float(numpy.abs(d).max() / numpy.abs(ref).max())}
+    print(json.dumps(line), flush=True)
+
+
 def measured_copy_gbs(torch, dev, mib=1024, reps=5):
     """HBM bandwidth re-measured on this device: a 1 GiB device-to-device
     copy (torch copy_), read + write bytes over the mean of ``reps`` copies
@@ -598,6 +743,8 @@ def main():
         return run_towers(args, torch, G, W, _lib, dev)
     if args.workload == 'dft128':
         return run_dft(args, torch, _lib, dev)
+    if args.workload == 'stream32k':
+        return run_stream(args, torch, W, _lib, dev)
     if args.workload == 'diff1024':
         return run_diff(args, torch, G, W, _lib, dev)
     print('[bench] building workload %s' % args.workload, file=sys.stderr, flush=True)

@@ -90,6 +90,8 @@ SIGNATURES = {
     'croc_grid_wprojection': [_P, _P, _I, _D, _BL, _P, _I, _D, _D, _I, _I, _U64P],
     'croc_streamer_degrid': [_P, _P, _I, _D, _D, _D, _D, _D, _D, _D, _BL, _P, _I64, _I, _I,
                              _P, _I, _I, _P, _P, _U64P],
+    'croc_streamer_degrid_batch': [_P, _P, _I, _I, _D, _P, _BL, _P, _I64, _I, _I, _P, _I, _I,
+                                   _P, _P, _U64P],
 }
 
 _lib = None

@@ -305,13 +305,27 @@ def grid_wprojection(uvgrid, grid_size, theta, vis, wkern):
 
 def streamer_degrid_worker(vis, subgrid, subgrid_size, theta, mid_u, mid_v, chunks,
                            min_u, max_u, min_v, max_v, kernel, time_chunk, freq_chunk,
-                           vis_queue=None, ndegrid=None):
+                           vis_queue=None, ndegrid=None, want_flops=True):
     """streamer.c:366-486 for a list of chunks (int32 [n, 5]: baseline, it0,
     it1, if0, if1) of one subgrid: returns (vis_queue complex64 [n,
     time_chunk * freq_chunk], ndegrid int64 [n], flops).  Slot entries beyond
-    a chunk's extent are left as they were (zeros when allocated here)."""
+    a chunk's extent are left as they were (zeros when allocated here).
+    With ``want_flops`` False the call returns as soon as its inputs are
+    validated (flops None; ndegrid and the slots fill in on the stream)."""
     chunks = _t(chunks, torch.int32, vis.device).reshape(-1, 5)
     n = chunks.shape[0]
+    vis_queue, ndegrid = _queue(vis, n, time_chunk, freq_chunk, vis_queue, ndegrid)
+    f = _flops_ref()
+    s = vis.c_struct()
+    _lib.check(_lib.load().croc_streamer_degrid(
+        _lib.ctx(), _grid_arg(subgrid, subgrid_size), subgrid_size, theta, mid_u, mid_v,
+        min_u, max_u, min_v, max_v, ctypes.byref(s), _lib.ptr(chunks), n, time_chunk,
+        freq_chunk, _lib.ptr(kernel.data), kernel.size, kernel.oversampling,
+        _lib.ptr(vis_queue), _lib.ptr(ndegrid), ctypes.byref(f) if want_flops else None))
+    return vis_queue, ndegrid, (f.value if want_flops else None)
+
+
+def _queue(vis, n, time_chunk, freq_chunk, vis_queue, ndegrid):
     if vis_queue is None:
         vis_queue = torch.zeros((n, time_chunk * freq_chunk), dtype=torch.complex64,
                                 device=vis.device)
@@ -320,14 +334,36 @@ def streamer_degrid_worker(vis, subgrid, subgrid_size, theta, mid_u, mid_v, chun
     if vis_queue.dtype != torch.complex64 or vis_queue.numel() < n * time_chunk * freq_chunk \
             or not vis_queue.is_contiguous():
         raise ValueError('vis_queue must hold n * time_chunk * freq_chunk complex64')
+    if ndegrid.dtype != torch.int64 or ndegrid.numel() < n or not ndegrid.is_contiguous():
+        raise ValueError('ndegrid must hold n int64')
+    return vis_queue, ndegrid
+
+
+def streamer_degrid_batch(vis, subgrids, theta, windows, chunks, kernel, time_chunk,
+                          freq_chunk, vis_queue=None, ndegrid=None, want_flops=True):
+    """streamer_degrid_worker for many subgrids in one launch (an extension:
+    the reference degrids one subgrid per call).  ``subgrids`` complex64
+    [nsub, xM, xM] device tensor; ``windows`` [nsub, 6] = (mid_u, mid_v,
+    min_u, max_u, min_v, max_v); ``chunks`` int32 [n, 6] = (subgrid,
+    baseline, it0, it1, if0, if1).  Returns (vis_queue, ndegrid, flops)."""
+    if not isinstance(subgrids, torch.Tensor) or subgrids.dtype != torch.complex64 or \
+            subgrids.ndim != 3 or subgrids.shape[1] != subgrids.shape[2] or \
+            not subgrids.is_contiguous() or subgrids.device.type != 'cuda':
+        raise ValueError('subgrids must be a contiguous complex64 [nsub, xM, xM] device tensor')
+    win = numpy.ascontiguousarray(windows, dtype=numpy.float64).reshape(-1, 6)
+    if win.shape[0] != subgrids.shape[0]:
+        raise ValueError('one window per subgrid')
+    chunks = _t(chunks, torch.int32, vis.device).reshape(-1, 6)
+    n = chunks.shape[0]
+    vis_queue, ndegrid = _queue(vis, n, time_chunk, freq_chunk, vis_queue, ndegrid)
     f = _flops_ref()
     s = vis.c_struct()
-    _lib.check(_lib.load().croc_streamer_degrid(
-        _lib.ctx(), _grid_arg(subgrid, subgrid_size), subgrid_size, theta, mid_u, mid_v,
-        min_u, max_u, min_v, max_v, ctypes.byref(s), _lib.ptr(chunks), n, time_chunk,
-        freq_chunk, _lib.ptr(kernel.data), kernel.size, kernel.oversampling,
-        _lib.ptr(vis_queue), _lib.ptr(ndegrid), ctypes.byref(f)))
-    return vis_queue, ndegrid, f.value
+    _lib.check(_lib.load().croc_streamer_degrid_batch(
+        _lib.ctx(), _lib.ptr(subgrids), subgrids.shape[0], subgrids.shape[1], theta,
+        win.ctypes.data_as(ctypes.c_void_p), ctypes.byref(s), _lib.ptr(chunks), n,
+        time_chunk, freq_chunk, _lib.ptr(kernel.data), kernel.size, kernel.oversampling,
+        _lib.ptr(vis_queue), _lib.ptr(ndegrid), ctypes.byref(f) if want_flops else None))
+    return vis_queue, ndegrid, (f.value if want_flops else None)
 
 
 class Streamer:
@@ -408,6 +444,26 @@ class Streamer:
             max_v, self.kernel, self.time_chunk, self.freq_chunk)
         return ch, q, nd, flops
 
+    def batch(self, offsets):
+        """Work of many subgrids at once: (windows [nsub, 6], chunks int32
+        [n, 6] device tensor with the subgrid index first)."""
+        wins, parts = [], []
+        for i, (ou, ov) in enumerate(offsets):
+            wins.append(self.window(ou, ov))
+            ch = self.chunks_for(ou, ov)
+            parts.append(numpy.concatenate(
+                [numpy.full((len(ch), 1), i, dtype=numpy.int32), ch], axis=1))
+        ch = numpy.concatenate(parts) if parts else numpy.zeros((0, 6), numpy.int32)
+        return numpy.array(wins).reshape(-1, 6), _t(ch, torch.int32, self.vis.device)
+
+    def degrid_batch(self, subgrids, offsets, work=None):
+        """Degrid subgrids [nsub, xM, xM] at cell offsets in one launch:
+        (chunks [n, 6], vis_queue, ndegrid, flops)."""
+        wins, ch = work if work is not None else self.batch(offsets)
+        q, nd, flops = streamer_degrid_batch(self.vis, subgrids, self.theta, wins, ch,
+                                             self.kernel, self.time_chunk, self.freq_chunk)
+        return ch, q, nd, flops
+
     def write(self, chunks, vis_queue):
         """The writer (streamer.c:322-337): copy every nonzero slot entry into
         the visibility set; raises if one lands on an entry written before."""
@@ -415,6 +471,8 @@ class Streamer:
             return
         tc, fc = self.time_chunk, self.freq_chunk
         ch = chunks.to(torch.int64)
+        if ch.shape[1] == 6:  # batched: drop the subgrid column
+            ch = ch[:, 1:]
         b, t0, t1, f0, f1 = ch.unbind(1)
         tt = torch.arange(tc, device=ch.device)
         ff = torch.arange(fc, device=ch.device)

@@ -42,15 +42,28 @@ __device__ __forceinline__ double bl_lambda(const BlView &s, int64_t row, int64_
 
 // grid.c:35-53 for x = theta * u: ox = rint((N/2 - x) Q), cell N - ox/Q,
 // sub-offset ox % Q (C truncating division).  False when x lies beyond the
-// grid's upper edge (ox < 0: a negative sub-offset) or is not finite.
+// grid's upper edge (ox < 0: a negative sub-offset) or so far outside that
+// no grid of at most 65536 cells with Q <= 2^20 could hold it.  The
+// quotient comes from the FP64 reciprocal of Q (within 2^-12 of the true
+// quotient for ox < 2^40), corrected by the remainder: exact.
 __device__ __forceinline__ bool frac_coord_c(int N, int Q, double x, int64_t *ix, int *fx) {
 #pragma clang fp contract(off)
   const double s = ((double)(N / 2) - x) * (double)Q;
-  if (!(fabs(s) < 4.0e18)) return false;
-  const int64_t ox = (int64_t)rint(s);
-  if (ox < 0) return false;
-  *ix = (int64_t)N - ox / Q;
-  *fx = (int)(ox % Q);
+  if (!(fabs(s) < 0x1p40)) return false;
+  const double so = rint(s);
+  if (so < 0.0) return false;
+  const int64_t ox = (int64_t)so;
+  int64_t q = (int64_t)(so * (1.0 / (double)Q));
+  int64_t rem = ox - q * Q;
+  if (rem < 0) {
+    q -= 1;
+    rem += Q;
+  } else if (rem >= Q) {
+    q += 1;
+    rem -= Q;
+  }
+  *ix = (int64_t)N - q;
+  *fx = (int)rem;
   return true;
 }
 
@@ -87,7 +100,7 @@ __device__ __forceinline__ bool bl_cell(int N, double theta, double u, int *c) {
 enum ChunkMode {
   CH_FULL = 0,   // chunk b = baseline b, all dumps and channels; out = vis layout
   CH_WINDOW,     // chunk b = baseline b, [t0, t1) x [f0, f1); chunk-local out index
-  CH_LIST,       // chunks from a device list (b, t0, t1, f0, f1); out = queue slots
+  CH_LIST,       // chunks from a device list ([sub,] b, t0, t1, f0, f1); out = queue slots
   CH_ROWS,       // a plain coordinate list uv[n][2] cut into chunks of `rows` entries
 };
 
@@ -101,6 +114,8 @@ struct Chunks {
   int mode;
   int64_t n;
   const int32_t *list;
+  int cols;                 // CH_LIST: 5 (b, t0, t1, f0, f1) or 6 (subgrid first)
+  int nsub;                 // CH_LIST with 6 columns: subgrids
   int t0, t1, f0, f1;       // CH_WINDOW
   int64_t slot;             // CH_LIST: queue elements per chunk
   int slot_row;             // CH_LIST: channels per slot row (freq_chunk)
@@ -111,6 +126,7 @@ struct Chunks {
 
 struct Chunk {
   int64_t b;
+  int sub;       // subgrid of the chunk (6-column lists), else 0
   int t0, t1, f0, f1;
   int64_t out0;  // output index of (t0, f0)
   int64_t orow;  // output elements per dump
@@ -118,7 +134,7 @@ struct Chunk {
 };
 
 __device__ __forceinline__ Chunk chunk_at(const BlView &s, const Chunks &c, int64_t k) {
-  Chunk r{0, 0, 0, 0, 0, 0, 0, true};
+  Chunk r{0, 0, 0, 0, 0, 0, 0, 0, true};
   if (c.mode == CH_ROWS) {
     r.t0 = 0;
     r.t1 = (int)min(c.rows, c.nrows - k * c.rows);
@@ -130,7 +146,8 @@ __device__ __forceinline__ Chunk chunk_at(const BlView &s, const Chunks &c, int6
     return r;
   }
   if (c.mode == CH_LIST) {
-    const int32_t *e = c.list + 5 * k;
+    const int32_t *e = c.list + c.cols * k;
+    if (c.cols == 6) r.sub = *e++;
     r.b = e[0];
     r.t0 = e[1];
     r.t1 = e[2];
@@ -138,7 +155,8 @@ __device__ __forceinline__ Chunk chunk_at(const BlView &s, const Chunks &c, int6
     r.f1 = e[4];
     r.out0 = k * c.slot;
     r.orow = c.slot_row;
-    r.ok = r.t1 - r.t0 <= c.slot_t && r.f1 - r.f0 <= c.slot_f;
+    r.ok = r.t1 - r.t0 <= c.slot_t && r.f1 - r.f0 <= c.slot_f && r.sub >= 0 &&
+           (c.cols == 5 || r.sub < c.nsub);
   } else {
     r.b = k;
     if (c.mode == CH_FULL) {
@@ -224,14 +242,25 @@ struct DegridArgs {
   BlView s;
   Chunks c;
   Box box;
-  const float2 *grid;
+  const double *win;   // [nsub][6] per-subgrid mirror windows (6-column lists) or null
+  const float2 *grid;  // subgrid 0; subgrid s at grid + s * N * N
   int N, S, Q;
   double theta;
   const double *kern;  // real [Q][S]
+  const float *kf;     // the same rounded to FP32 (degrid pass)
   float2 *out;
-  unsigned long long *count;  // [n chunks] degridded visibilities (check pass)
+  unsigned long long *count;  // [n chunks] degridded visibilities
+  unsigned long long *total;  // their sum
+  int count_in_degrid;        // counts made by the degrid pass (else the check pass)
   uint32_t *status;
 };
+
+// Window and subgrid of a chunk.
+__device__ __forceinline__ Box chunk_box(const DegridArgs &a, const Chunk &r) {
+  if (!a.win) return a.box;
+  const double *w = a.win + 6 * r.sub;
+  return Box{SEL_MIRROR, w[0], w[1], w[2], w[3], w[4], w[5]};
+}
 
 __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long x,
                                                             unsigned long long *red) {
@@ -250,8 +279,10 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long x
 __global__ __launch_bounds__(BL_NT) void k_bl_degrid_check(DegridArgs a) {
   __shared__ unsigned long long red[BL_NT / 64];
   uint32_t st = 0;
+  unsigned long long tot = 0;
   for (int64_t k = blockIdx.x; k < a.c.n; k += gridDim.x) {
     const Chunk r = chunk_at(a.s, a.c, k);
+    const Box box = chunk_box(a, r);
     unsigned long long n = 0;
     if (!r.ok) {
       st |= ST_RANGE;
@@ -262,7 +293,7 @@ __global__ __launch_bounds__(BL_NT) void k_bl_degrid_check(DegridArgs a) {
         const int t = r.t0 + (int)(j / nfw), f = r.f0 + (int)(j % nfw);
         double u, v, pu, pv;
         chunk_uv(a.s, a.c, r, t, f, &u, &v);
-        const int kind = classify(a.box, u, v, &pu, &pv);
+        const int kind = classify(box, u, v, &pu, &pv);
         if (kind == 1 || kind == 2) {
           int x0, y0, fx, fy;
           if (!bl_window(a.N, a.Q, a.S, a.theta, pu, pv, &x0, &y0, &fx, &fy)) st |= ST_BOUNDS;
@@ -271,60 +302,117 @@ __global__ __launch_bounds__(BL_NT) void k_bl_degrid_check(DegridArgs a) {
       }
     }
     n = block_sum_u64(n, red);
-    if (threadIdx.x == 0 && a.count) a.count[k] = n;
+    if (threadIdx.x == 0) {
+      a.count[k] = n;
+      tot += n;
+    }
   }
+  if (threadIdx.x == 0 && tot) atomicAdd(a.total, tot);
   if (st) atomicOr(a.status, st);
 }
 
-// Degridding: G lanes per visibility (G a power of two >= S); lane x sums
-// column x of the window over the rows weighted by k[fy][y], the group then
-// sums the columns weighted by k[fx][x].  Rows are read as G consecutive
-// complex64 cells.  FP32 arithmetic on the FP64 table's values.
+// Validation of the chunk ranges only, one thread per chunk (when every
+// window the chunks can select provably lies inside its subgrid).
+__global__ void k_bl_chunk_check(DegridArgs a) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.c.n) return;
+  const Chunk r = chunk_at(a.s, a.c, k);
+  a.count[k] = 0;
+  if (!r.ok) atomicOr(a.status, (uint32_t)ST_RANGE);
+}
+
+// Degridding.  Each wavefront takes 64 consecutive visibilities of a chunk:
+// first every lane works out the coordinates of one of them (FP64 uvw
+// scaling, window rule, frac_coord), then groups of G lanes (G a power of
+// two >= S) degrid them, 64 / G at a time, fetching each visibility's
+// window, sub-offsets and output index from the lane that computed them.
+// In a group, lane x sums column x of the window over the rows weighted by
+// k[fy][y]; the group then sums the columns weighted by k[fx][x].  Rows are
+// read as S consecutive complex64 cells.  FP32 arithmetic on the table's
+// values rounded to FP32 (kf).
 template <int G>
 __global__ __launch_bounds__(BL_NT) void k_bl_degrid(DegridArgs a) {
-  constexpr int V = BL_NT / G;
-  const int lane = threadIdx.x & (G - 1);
-  const int slot = threadIdx.x / G;
+  constexpr int V = 64 / G;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = lane & (G - 1), grp = lane / G;
   const int S = a.S, N = a.N;
+  unsigned long long tot = 0;
   for (int64_t k = blockIdx.x; k < a.c.n; k += gridDim.x) {
     const Chunk r = chunk_at(a.s, a.c, k);
+    const Box box = chunk_box(a, r);
+    const float2 *grid = a.grid + (int64_t)r.sub * N * N;
     const int nfw = r.f1 - r.f0;
     const int64_t m = (int64_t)(r.t1 - r.t0) * nfw;
-    for (int64_t j = slot; j < m; j += V) {
-      const int t = r.t0 + (int)(j / nfw), f = r.f0 + (int)(j % nfw);
-      const int64_t o = r.out0 + (int64_t)(t - r.t0) * r.orow + (f - r.f0);
-      double u, v, pu, pv;
-      chunk_uv(a.s, a.c, r, t, f, &u, &v);
-      const int kind = classify(a.box, u, v, &pu, &pv);
-      if (kind == 0) continue;
-      if (kind == 3) {
-        if (lane == 0) a.out[o] = make_float2(0.f, 0.f);
-        continue;
+    unsigned long long nk = 0;
+    for (int64_t jb = (int64_t)wave * 64; jb < m; jb += BL_NT) {
+      // this lane's visibility: window origin, sub-offsets, kind, output
+      uint32_t xy = 0u, fxv = 0u, kfy = 0u;
+      int64_t o = 0;
+      const int64_t j = jb + lane;
+      if (j < m) {
+        const int t = r.t0 + (int)(j / nfw), f = r.f0 + (int)(j % nfw);
+        o = r.out0 + (int64_t)(t - r.t0) * r.orow + (f - r.f0);
+        double u, v, pu, pv;
+        chunk_uv(a.s, a.c, r, t, f, &u, &v);
+        const int kind = classify(box, u, v, &pu, &pv);
+        int x0 = 0, y0 = 0, fx = 0, fy = 0;
+        if (kind == 1 || kind == 2) bl_window(N, a.Q, S, a.theta, pu, pv, &x0, &y0, &fx, &fy);
+        xy = (uint32_t)x0 | ((uint32_t)y0 << 16);
+        fxv = (uint32_t)fx;
+        kfy = (uint32_t)fy | ((uint32_t)kind << 28);
       }
-      int x0 = 0, y0 = 0, fx = 0, fy = 0;
-      bl_window(N, a.Q, S, a.theta, pu, pv, &x0, &y0, &fx, &fy);
-      float ar = 0.f, ai = 0.f;
-      if (lane < S) {
-        const float2 *col = a.grid + (int64_t)y0 * N + x0 + lane;
-        const double *ky = a.kern + (int64_t)S * fy;
-        for (int y = 0; y < S; y++) {
-          const float ky_ = (float)ky[y];
-          const float2 g = col[(int64_t)y * N];
-          ar = fmaf(ky_, g.x, ar);
-          ai = fmaf(ky_, g.y, ai);
-        }
-        const float kx = (float)a.kern[(int64_t)S * fx + lane];
-        ar *= kx;
-        ai *= kx;
-      }
+      nk += (unsigned long long)__popcll(__ballot((kfy >> 28) == 1u || (kfy >> 28) == 2u));
+      const int nr = (int)((min<int64_t>(64, m - jb) + V - 1) / V);  // wave-uniform
+      for (int rr = 0; rr < nr; rr++) {
+        const int q = rr * V + grp;
+        const uint32_t xy_q = __shfl(xy, q), fx_q = __shfl(fxv, q), kfy_q = __shfl(kfy, q);
+        const uint32_t olo = __shfl((uint32_t)o, q), ohi = __shfl((uint32_t)(o >> 32), q);
+        const int kind = (int)(kfy_q >> 28);
+        float ar = 0.f, ai = 0.f;
+        if ((kind == 1 || kind == 2) && x < S) {
+          const int x0 = (int)(xy_q & 0xffffu), y0 = (int)(xy_q >> 16);
+          const float2 *col = grid + (int64_t)y0 * N + x0 + x;
+          const float *ky = a.kf + (int64_t)S * (kfy_q & 0xfffffffu);
+          // all G row loads issue before the products: rows past S re-read
+          // row S - 1 (in the window) with weight 0
+          float2 g[G];
+          float w[G];
 #pragma unroll
-      for (int w = G / 2; w > 0; w >>= 1) {
-        ar += __shfl_xor(ar, w);
-        ai += __shfl_xor(ai, w);
+          for (int y = 0; y < G; y++) {
+            const int yy = y < S ? y : S - 1;
+            g[y] = col[(int64_t)yy * N];
+            w[y] = y < S ? ky[yy] : 0.f;
+          }
+#pragma unroll
+          for (int y = 0; y < G; y++) {
+            ar = fmaf(w[y], g[y].x, ar);
+            ai = fmaf(w[y], g[y].y, ai);
+          }
+          const float kx = a.kf[(int64_t)S * fx_q + x];
+          ar *= kx;
+          ai *= kx;
+        }
+#pragma unroll
+        for (int w = G / 2; w > 0; w >>= 1) {
+          ar += __shfl_xor(ar, w);
+          ai += __shfl_xor(ai, w);
+        }
+        if (x == 0 && kind != 0) {
+          const int64_t oq = (int64_t)(((uint64_t)ohi << 32) | olo);
+          a.out[oq] = kind == 3 ? make_float2(0.f, 0.f)
+                                : make_float2(ar, kind == 2 ? -ai : ai);
+        }
       }
-      if (lane == 0) a.out[o] = make_float2(ar, kind == 2 ? -ai : ai);
     }
+    if (a.count_in_degrid && lane == 0 && nk) atomicAdd(&a.count[k], nk);
+    tot += nk;
   }
+  if (a.count_in_degrid && lane == 0 && tot) atomicAdd(a.total, tot);
+}
+
+__global__ void k_to_float(const double *in, float *out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -490,10 +578,11 @@ __global__ __launch_bounds__(BL_NT) void k_bl_cells(CellArgs a) {
   if (st) atomicOr(a.status, st);
 }
 
+// Status word (offset 0) and a 64-bit counter (offset 8), cleared.
 int status_word(croc_ctx *ctx, uint32_t **st) {
   *st = (uint32_t *)ws_get(ctx, WS_SMALL, 256);
   if (!*st) return CROC_ENOMEM;
-  return zero_u32_async(ctx, *st, 1);
+  return zero_u32_async(ctx, *st, 4);
 }
 
 int read_word(croc_ctx *ctx, uint32_t *st) {
@@ -528,7 +617,27 @@ int check_kernel(const double *kern, int S, int Q) {
 }
 
 // Check pass, then the degridding pass.  count: device [nchunks] or null.
-int degrid_run(croc_ctx *ctx, DegridArgs a, uint64_t *ndeg_total) {
+// True when every window a visibility selected by [min, max) x [min, max)
+// (shifted by (du, dv)) can take lies inside an N x N grid: then no window
+// needs a per-visibility bounds check.  The window of x = theta (u - du)
+// cells starts at N/2 + x - S/2 (within a cell) and ends S - 1 cells
+// later; the margins below cover that cell and the rounding.
+bool box_inside(int N, int S, double theta, double du, double dv, double min_u, double max_u,
+                double min_v, double max_v) {
+  const double lo = -(double)(N / 2) + S + 2, hi = (double)(N / 2) - S - 2;
+  const double b[4] = {theta * (min_u - du), theta * (max_u - du), theta * (min_v - dv),
+                       theta * (max_v - dv)};
+  for (double x : b)
+    if (!(x >= lo && x <= hi)) return false;
+  return theta > 0;
+}
+
+// Validation, then the degridding pass (queued; the call waits only for
+// the validation, and for the total when ndeg_total is wanted).  safe:
+// every selectable window lies inside its grid (box_inside), so only the
+// chunk ranges are validated and the degrid pass counts; otherwise a
+// per-visibility pass checks every window and counts.
+int degrid_run(croc_ctx *ctx, DegridArgs a, bool safe, uint64_t *ndeg_total) {
   if (a.c.n == 0) {
     if (ndeg_total) *ndeg_total = 0;
     return CROC_OK;
@@ -537,23 +646,24 @@ int degrid_run(croc_ctx *ctx, DegridArgs a, uint64_t *ndeg_total) {
   int rc = status_word(ctx, &st);
   if (rc) return rc;
   a.status = st;
-  const bool own = a.count == nullptr;
-  if (own) {
+  a.total = (unsigned long long *)((char *)st + 8);
+  if (!a.count) {
     a.count = (unsigned long long *)ws_get(ctx, WS_BL, sizeof(unsigned long long) * a.c.n);
     if (!a.count) return CROC_ENOMEM;
   }
+  a.count_in_degrid = safe ? 1 : 0;
   const int nb = blocks_for(ctx, a.c.n);
-  k_bl_degrid_check<<<nb, BL_NT, 0, ctx->stream>>>(a);
+  if (safe)
+    k_bl_chunk_check<<<(unsigned)((a.c.n + 255) / 256), 256, 0, ctx->stream>>>(a);
+  else
+    k_bl_degrid_check<<<nb, BL_NT, 0, ctx->stream>>>(a);
   if ((rc = read_word(ctx, st))) return rc;
-  if (ndeg_total) {
-    // the counts are small: sum them on the host
-    std::vector<unsigned long long> h(a.c.n);
-    CROC_HIP(hipMemcpyAsync(h.data(), a.count, sizeof(unsigned long long) * a.c.n,
-                            hipMemcpyDeviceToHost, ctx->stream));
-    CROC_HIP(hipStreamSynchronize(ctx->stream));
-    unsigned long long t = 0;
-    for (auto x : h) t += x;
-    *ndeg_total = t;
+  {
+    const int64_t nk = (int64_t)a.Q * a.S;
+    float *kf = (float *)ws_get(ctx, WS_BL_BANK, sizeof(float) * nk);
+    if (!kf) return CROC_ENOMEM;
+    k_to_float<<<(unsigned)((nk + 255) / 256), 256, 0, ctx->stream>>>(a.kern, kf, nk);
+    a.kf = kf;
   }
   CROC_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
   tap_timer_reset(ctx, 1);
@@ -573,7 +683,12 @@ int degrid_run(croc_ctx *ctx, DegridArgs a, uint64_t *ndeg_total) {
   if ((rc = tap_timer_end(ctx, 1))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
   ctx->ev_degrid = true;
-  CROC_HIP(hipStreamSynchronize(ctx->stream));
+  if (ndeg_total) {
+    unsigned long long t = 0;
+    CROC_HIP(hipMemcpyAsync(&t, a.total, sizeof(t), hipMemcpyDeviceToHost, ctx->stream));
+    CROC_HIP(hipStreamSynchronize(ctx->stream));
+    *ndeg_total = t;
+  }
   return CROC_OK;
 }
 
@@ -778,7 +893,7 @@ extern "C" int croc_degrid_conv_uv(croc_ctx *ctx, const void *uvgrid, int N, dou
   a.kern = kern;
   a.out = (float2 *)vis_out;
   uint64_t nd = 0;
-  if ((rc = degrid_run(ctx, a, &nd))) return rc;
+  if ((rc = degrid_run(ctx, a, false, flops ? &nd : nullptr))) return rc;
   if (flops) *flops = degrid_flops(nd, S);
   return CROC_OK;
 }
@@ -812,7 +927,8 @@ extern "C" int croc_degrid_conv_bl(croc_ctx *ctx, const void *uvgrid, int N, dou
   a.kern = kern;
   a.out = (float2 *)bl->vis;
   uint64_t nd = 0;
-  if ((rc = degrid_run(ctx, a, &nd))) return rc;
+  const bool safe = box_inside(N, S, theta, d_u, d_v, min_u, max_u, min_v, max_v);
+  if ((rc = degrid_run(ctx, a, safe, flops ? &nd : nullptr))) return rc;
   if (flops) *flops = degrid_flops(nd, S);
   return CROC_OK;
 }
@@ -842,7 +958,9 @@ extern "C" int croc_degrid_conv(croc_ctx *ctx, const void *uvgrid, int N, double
   a.kern = kern;
   a.out = (float2 *)vis->vis;
   uint64_t nd = 0;
-  if ((rc = degrid_run(ctx, a, &nd))) return rc;
+  const bool safe = box_inside(N, S, theta, d_u, d_v, a.box.min_u, a.box.max_u, a.box.min_v,
+                               a.box.max_v);
+  if ((rc = degrid_run(ctx, a, safe, flops ? &nd : nullptr))) return rc;
   if (flops) *flops = degrid_flops(nd, S);
   return CROC_OK;
 }
@@ -868,6 +986,7 @@ extern "C" int croc_streamer_degrid(croc_ctx *ctx, const void *subgrid, int subg
   a.c.mode = CH_LIST;
   a.c.n = nchunks;
   a.c.list = chunks;
+  a.c.cols = 5;
   a.c.slot = (int64_t)time_chunk * freq_chunk;
   a.c.slot_row = freq_chunk;
   a.c.slot_t = time_chunk;
@@ -882,7 +1001,61 @@ extern "C" int croc_streamer_degrid(croc_ctx *ctx, const void *subgrid, int subg
   a.out = (float2 *)vis_queue;
   a.count = (unsigned long long *)ndegrid;
   uint64_t nd = 0;
-  if ((rc = degrid_run(ctx, a, flops ? &nd : nullptr))) return rc;
+  const bool safe =
+      box_inside(subgrid_size, S, theta, mid_u, mid_v, min_u, max_u, min_v, max_v);
+  if ((rc = degrid_run(ctx, a, safe, flops ? &nd : nullptr))) return rc;
+  if (flops) *flops = degrid_flops(nd, S);
+  return CROC_OK;
+}
+
+extern "C" int croc_streamer_degrid_batch(croc_ctx *ctx, const void *subgrids, int nsub,
+                                          int subgrid_size, double theta,
+                                          const double *windows, const croc_bl_set *bl,
+                                          const int32_t *chunks, int64_t nchunks,
+                                          int time_chunk, int freq_chunk, const double *kern,
+                                          int S, int Q, void *vis_queue, int64_t *ndegrid,
+                                          uint64_t *flops) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if ((rc = check_set(bl))) return rc;
+  if ((rc = check_kernel(kern, S, Q))) return rc;
+  CROC_REQUIRE(subgrids && windows && nsub >= 1 && subgrid_size >= 1, CROC_EINVAL,
+               "subgrids, windows, nsub >= 1 and size >= 1 required");
+  CROC_REQUIRE(nchunks >= 0 && (nchunks == 0 || (chunks && vis_queue)), CROC_EINVAL,
+               "null chunk list / visibility queue");
+  CROC_REQUIRE(time_chunk >= 1 && freq_chunk >= 1, CROC_EINVAL,
+               "time_chunk and freq_chunk must be positive");
+  bool safe = true;
+  for (int k = 0; k < nsub; k++) {
+    const double *w = windows + 6 * k;
+    safe = safe && box_inside(subgrid_size, S, theta, w[0], w[1], w[2], w[3], w[4], w[5]);
+  }
+  double *win = (double *)ws_get(ctx, WS_BL_WIN, sizeof(double) * 6 * nsub);
+  if (!win) return CROC_ENOMEM;
+  CROC_HIP(hipMemcpyAsync(win, windows, sizeof(double) * 6 * nsub, hipMemcpyHostToDevice,
+                          ctx->stream));
+  DegridArgs a{};
+  a.s = view(bl);
+  a.c.mode = CH_LIST;
+  a.c.n = nchunks;
+  a.c.list = chunks;
+  a.c.cols = 6;
+  a.c.nsub = nsub;
+  a.c.slot = (int64_t)time_chunk * freq_chunk;
+  a.c.slot_row = freq_chunk;
+  a.c.slot_t = time_chunk;
+  a.c.slot_f = freq_chunk;
+  a.win = win;
+  a.grid = (const float2 *)subgrids;
+  a.N = subgrid_size;
+  a.S = S;
+  a.Q = Q;
+  a.theta = theta;
+  a.kern = kern;
+  a.out = (float2 *)vis_queue;
+  a.count = (unsigned long long *)ndegrid;
+  uint64_t nd = 0;
+  if ((rc = degrid_run(ctx, a, safe, flops ? &nd : nullptr))) return rc;
   if (flops) *flops = degrid_flops(nd, S);
   return CROC_OK;
 }

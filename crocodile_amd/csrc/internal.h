@@ -67,7 +67,8 @@ enum WsSlot {
   WS_GRIDQ,     // gridding work lists: dense tiles' range items, sparse tiles' items
   WS_KSPLIT,    // w-towers: fp16 (hi, lo) split of a separable kernel set
   WS_BL,        // baseline layout: flattened coordinates, planes, visibilities
-  WS_BL_BANK,   // baseline layout: kernel-bank plane tables, unit kernel
+  WS_BL_BANK,   // baseline layout: kernel-bank plane tables, unit kernel, FP32 table
+  WS_BL_WIN,    // baseline layout: per-subgrid windows of a batched streamer call
   WS_COUNT
 };
 

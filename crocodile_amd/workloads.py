@@ -22,6 +22,15 @@ Workload names:
                  (Gaussian core plus log-spiral arms to 40 km) over 8 h at
                  150 MHz, dec -45 deg, AW-style kernel S=16 Q=32, 256
                  w-planes.
+* ``stream32k``  subgrid-streaming degrid (examples/grid/streamer.c): the
+                 32k-8k-1k recombination geometry (image 32768, subgrids
+                 xM=1024 spaced xA=832, bench_recombine.c:154-165) with the
+                 LOWBD2-style visibility spec (bench_recombine.c:20-36: 512
+                 stations, 128 dumps in chunks of 64, channels from 260 MHz
+                 in steps of 40 MHz / 2048, chunks of 64, grid theta = field
+                 of view = image / 210000); one GPU's share is one 64-channel
+                 frequency chunk of every baseline, 1.07e9 visibilities;
+                 separable PSWF kernel S=8, Q=1024.
 """
 
 import math
@@ -42,6 +51,9 @@ SPEC = {
     'towers8192': dict(N=8192, theta=0.05, lam=163840.0, nv=50_000_000, Q=32,
                        S=16, nant=256, hours=8.0, freq=150e6, dec=-45.0,
                        subgrid=128, nsrc=20, seed=1005),
+    'stream32k': dict(image_size=32768, theta=32768 / 210000, xA=832, xM=1024, nant=512,
+                      ndump=128, time_chunk=64, nchan=64, freq_chunk=64, freq0=260e6,
+                      dfreq=40e6 / 2048, hours=4.0, dec=-45.0, S=8, Q=1024, seed=1006),
 }
 
 
@@ -419,3 +431,65 @@ def dft_predict_error(c, uvw_d, predict_fns, seed):
         del v
     del grid, ref
     return out
+
+
+# --------------------------------------------------------------------------
+# Subgrid-streaming degrid workload
+# --------------------------------------------------------------------------
+
+def stream(name='stream32k', nant=None, ndump=None, nchan=None, device=None):
+    """Visibility spec of the streaming degrid workload: (uvw_m [nbl, ndump,
+    3] metres of every station pair, time-major per baseline, freq [nchan]
+    Hz), as torch device tensors when ``device`` is given.  Stations: a
+    Gaussian core plus log-spiral arms to 40 km; dumps spread over
+    ``hours`` of hour angle centred on transit (ha_to_uvw, grid.h:64-89)."""
+    c = SPEC[name]
+    rng = numpy.random.default_rng(c['seed'])
+    nant = c['nant'] if nant is None else nant
+    ndump = c['ndump'] if ndump is None else ndump
+    nchan = c['nchan'] if nchan is None else nchan
+    xyz = xyz_at_latitude(antenna_layout(rng, nant, sigma_m=1500.0, arms_to_m=40000.0),
+                          LATITUDE)
+    a1, a2 = numpy.triu_indices(nant, 1)
+    bl = xyz[a2] - xyz[a1]
+    dec = c['dec'] * numpy.pi / 180
+    has = ((numpy.arange(ndump) + 0.5) / ndump - 0.5) * c['hours'] / 24 * 2 * numpy.pi
+    uvw = numpy.empty((bl.shape[0], ndump, 3))
+    for t, ha in enumerate(has):
+        uvw[:, t] = xyz_to_uvw(bl, ha, dec)
+    freq = c['freq0'] + c['dfreq'] * numpy.arange(nchan)
+    if device is None:
+        return uvw, freq
+    import torch
+    return (torch.as_tensor(uvw, device=device), torch.as_tensor(freq, device=device))
+
+
+def stream_kernel(name='stream32k'):
+    """Separable PSWF degridding kernel [Q, S] (FP64, rows normalised to
+    unit sum) and its 1D anti-aliasing function on xM pixels."""
+    import scipy.special
+    c = SPEC[name]
+    S, Q, n = c['S'], c['Q'], 64
+    x = (numpy.arange(n) - n // 2) / n
+    aa = scipy.special.pro_ang1(0, 0, 0.5 * numpy.pi * S, (2 - 1 / n / 4) * x)[0]
+    # kernel_oversample (synthesis.py:330-354): pad to n Q, centred FFT,
+    # every Q-th sample from n Q / 2 - Q (S // 2) + f
+    big = numpy.zeros(n * Q, dtype=complex)
+    big[n * Q // 2 - n // 2:n * Q // 2 - n // 2 + n] = aa
+    ft = numpy.fft.fftshift(numpy.fft.fft(numpy.fft.ifftshift(big))).real
+    start = n * Q // 2 - Q * (S // 2)
+    k = numpy.array([ft[start + f:start + f + Q * S:Q] for f in range(Q)])
+    return k / k.sum(axis=1, keepdims=True)
+
+
+def stream_subgrids(name='stream32k', umax=None):
+    """Subgrid offsets (iu * xA, iv * xA) cells of the streamer's work
+    assignment (config.c:505-516): iu >= 0 only (visibilities at u < 0 are
+    degridded as conjugates of their mirror image), every subgrid inside
+    the image and reaching |u|, |v| <= umax wavelengths."""
+    c = SPEC[name]
+    xA, xM, N, th = c['xA'], c['xM'], c['image_size'], c['theta']
+    kmax = (N // 2 - xM // 2) // xA
+    if umax is not None:
+        kmax = min(kmax, int(math.ceil((umax * th - xA / 2) / xA)))
+    return [(iu * xA, iv * xA) for iv in range(-kmax, kmax + 1) for iu in range(0, kmax + 1)]

@@ -318,9 +318,12 @@ int croc_make_hermitian(croc_ctx *ctx, void *grid, int N, int is_double);
  * that leaves the grid is CROC_EBOUNDS (grid.c reads or writes the
  * neighbouring row, or outside the array, there).  Separable kernels here
  * are real FP64 tables [Q][S] (grid.h:101-106 sep_kernel_data; sub-pixel
- * offset outer).  All of these synchronise the stream before returning and
- * leave their outputs unmodified on error.  flops (host, may be NULL)
- * receives the reference's flop count of the call.
+ * offset outer).  All of these validate every input they will read before
+ * writing anything (one stream synchronisation) and leave their outputs
+ * unmodified on error; the degridding functions then return without waiting
+ * for the device unless flops is requested.  flops (host, may be NULL)
+ * receives the reference's flop count of the call (4 (1 + S) S per
+ * degridded visibility, grid.c:222; 8 S^2 per gridded one, :304).
  * ------------------------------------------------------------------------ */
 typedef struct croc_bl_set {
   int64_t nbl;                /* baselines (vis_data.bl_count)            */
@@ -407,6 +410,21 @@ int croc_streamer_degrid(croc_ctx *ctx, const void *subgrid, int subgrid_size,
                          int64_t nchunks, int time_chunk, int freq_chunk,
                          const double *kern, int S, int Q, void *vis_queue,
                          int64_t *ndegrid, uint64_t *flops);
+
+/* croc_streamer_degrid for many subgrids in one launch.  No counterpart in
+ * the reference, whose streamer degrids one subgrid per call as the
+ * recombination delivers it: an extension for callers that hold a work
+ * assignment's subgrids in device memory together.  subgrids cf32
+ * [nsub][subgrid_size][subgrid_size]; windows host double [nsub][6] =
+ * (mid_u, mid_v, min_u, max_u, min_v, max_v) per subgrid; chunks int32
+ * [nchunks][6] = (subgrid, baseline, it0, it1, if0, if1); the queue slots,
+ * ndegrid and flops as for croc_streamer_degrid (slot c for chunk c). */
+int croc_streamer_degrid_batch(croc_ctx *ctx, const void *subgrids, int nsub,
+                               int subgrid_size, double theta, const double *windows,
+                               const croc_bl_set *bl, const int32_t *chunks,
+                               int64_t nchunks, int time_chunk, int freq_chunk,
+                               const double *kern, int S, int Q, void *vis_queue,
+                               int64_t *ndegrid, uint64_t *flops);
 
 #ifdef __cplusplus
 }

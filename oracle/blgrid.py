@@ -73,6 +73,26 @@ class BlSet:
             self.vis = numpy.ascontiguousarray(numpy.concatenate(vis), dtype=numpy.complex128)
         assert self.vis.shape[0] == int(nvis.sum())
 
+    @classmethod
+    def uniform(cls, uvw_m, freq, with_vis=True):
+        """Baselines sharing one channel axis: ``uvw_m`` [nbl, nt, 3],
+        ``freq`` [nf]; visibilities zero (none held without ``with_vis``:
+        for streamer_degrid, which writes queue slots only)."""
+        self = cls.__new__(cls)
+        uvw_m = numpy.ascontiguousarray(uvw_m, dtype=numpy.float64)
+        nbl, nt = uvw_m.shape[:2]
+        nf = len(freq)
+        self.nbl = nbl
+        self.nt = numpy.full(nbl, nt, dtype=numpy.int32)
+        self.nf = numpy.full(nbl, nf, dtype=numpy.int32)
+        self.uvw0 = numpy.arange(nbl, dtype=numpy.int64) * nt
+        self.f0 = numpy.zeros(nbl, dtype=numpy.int64)
+        self.v0 = numpy.arange(nbl, dtype=numpy.int64) * nt * nf
+        self.uvw_m = uvw_m.reshape(nbl * nt, 3)
+        self.freq = numpy.ascontiguousarray(freq, dtype=numpy.float64)
+        self.vis = numpy.zeros(nbl * nt * nf if with_vis else 1, dtype=numpy.complex128)
+        return self
+
     def c(self):
         return _BlSetC(self.nbl, _ptr(self.nt).value, _ptr(self.nf).value,
                        _ptr(self.uvw0).value, _ptr(self.f0).value, _ptr(self.v0).value,

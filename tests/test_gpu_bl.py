@@ -293,3 +293,65 @@ def test_errors_leave_outputs_unmodified(gpu):
     with pytest.raises(ValueError):  # chunk longer than a queue slot
         BL.streamer_degrid_worker(dev, gd, N, theta, 0, 0, [[0, 0, 6, 0, 5]], -1e9, 1e9,
                                   -1e9, 1e9, kern, 4, 5)
+
+
+def test_streamer_batch_matches_per_subgrid_calls(gpu):
+    """croc_streamer_degrid_batch over every subgrid of a tiling in one
+    launch gives the per-subgrid croc_streamer_degrid slots bit for bit."""
+    print('case streamer batch', flush=True)
+    rng = numpy.random.default_rng(31)
+    N, theta, S, Q, xA, xM = 512, 0.05, 8, 64, 64, 128
+    g, _ = rand_grid(rng, N)
+    kern = BL.SepKernelData(rng.normal(size=(Q, S)))
+    host, dev = make_set(rng, 50, 32, 64, 14000.0, with_vis=False)
+    st = BL.Streamer(dev, theta, xA, xM, 16, 32, kern)
+    offs = tiles(N, xA, xM)
+    subs = torch.stack([torch.as_tensor(numpy.ascontiguousarray(cut(g, ou, ov, xM))
+                                        .astype(numpy.complex64)) for ou, ov in offs]).cuda()
+    ch6, qb, ndb, flb = st.degrid_batch(subs, offs)
+    singles, nds, fls = [], [], 0
+    for i, (ou, ov) in enumerate(offs):
+        ch, q, nd, fl = st.degrid(subs[i].contiguous(), ou, ov)
+        singles.append(q)
+        nds.append(nd)
+        fls += fl
+    assert torch.equal(qb, torch.cat(singles))
+    assert torch.equal(ndb, torch.cat(nds))
+    assert flb == fls and flb > 0
+
+
+def test_streamer_windows_near_subgrid_edge(gpu):
+    """Windows that reach near the subgrid's edge take the per-visibility
+    bounds pass: still equal to the oracle when every window fits, and a
+    window over the edge raises with the queue untouched."""
+    print('case streamer near edge', flush=True)
+    rng = numpy.random.default_rng(32)
+    N, theta, S, Q, xM = 256, 0.05, 8, 32, 128
+    g, _ = rand_grid(rng, N)
+    kern = rng.normal(size=(Q, S))
+    host, dev = make_set(rng, 60, 16, 32, 4000.0, with_vis=False)
+    tc, fc = 8, 16
+    chunks = numpy.array([[b, t, t + tc, f, f + fc] for b in range(host.nbl)
+                          for t in range(0, 16, tc) for f in range(0, 32, fc)], dtype=numpy.int32)
+    sub = cut(g, 0, 0, xM)
+    subd = torch.as_tensor(numpy.ascontiguousarray(sub).astype(numpy.complex64)).cuda()
+    xA = 110  # |x| <= 55 cells: fits the 128 subgrid, beyond the provable margin
+    box = (-xA / 2 / theta, xA / 2 / theta, -xA / 2 / theta, xA / 2 / theta)
+    ref, nd_ref = B.streamer_degrid(sub, theta, 0.0, 0.0, *box, host, chunks, tc, fc, kern)
+    q, nd, _ = BL.streamer_degrid_worker(dev, subd, xM, theta, 0.0, 0.0, chunks, *box,
+                                         BL.SepKernelData(kern), tc, fc)
+    assert_array_equal(nd.cpu().numpy(), nd_ref)
+    assert nd_ref.sum() > 100
+    q = q.cpu().numpy()
+    zero = ref == 0
+    assert_array_equal(q == 0, zero)
+    assert_close(q[~zero], ref[~zero], what='streamer near edge')
+    xA = 124  # windows at |x| near 62 leave the subgrid
+    box = (-xA / 2 / theta, xA / 2 / theta, -xA / 2 / theta, xA / 2 / theta)
+    with pytest.raises(ValueError):
+        B.streamer_degrid(sub, theta, 0.0, 0.0, *box, host, chunks, tc, fc, kern)
+    queue = torch.full((len(chunks), tc * fc), 5.0, dtype=torch.complex64, device=gpu)
+    with pytest.raises(ValueError):
+        BL.streamer_degrid_worker(dev, subd, xM, theta, 0.0, 0.0, chunks, *box,
+                                  BL.SepKernelData(kern), tc, fc, vis_queue=queue)
+    assert bool((queue == 5.0).all())
