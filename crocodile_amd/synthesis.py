@@ -318,10 +318,18 @@ def w_conj_kernel_fn(kernel_fn):
     return fn
 
 
-def aw_kernel_fn(a_kernel_fn, w_kernel_fn=w_kernel):
+def aw_kernel_fn(a_kernel_fn, w_kernel_fn=w_kernel, method='auto'):
     """AW kernels: the two antennas' A-kernels convolved (mode 'same') and
     then convolved with every oversampled w sub-kernel, on the device in
-    FP64.  synthesis.py:701-734."""
+    FP64.  synthesis.py:701-734.  Returns a crocodile_amd.kernels.AWKernels
+    (A-kernels, w-kernels, antenna-pair kernels and their spectra made once
+    and kept across calls) with ``method`` 'fft' (Fourier convolutions with
+    shared spectra), 'direct' (direct sums) or 'auto' (direct for small
+    kernels, FFT for large); ``method='uncached'`` gives the plain
+    per-call direct sums with no state."""
+    if method != 'uncached':
+        from .kernels import AWKernels
+        return AWKernels(a_kernel_fn, w_kernel_fn, method=method)
     from .awkernel import convolve_same, convolve_same_batched
 
     def fn(theta, w, a1, a2, *src):
