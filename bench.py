@@ -209,6 +209,7 @@ def run_towers(args, torch, G, W, _lib, dev):
     def kargs(ks):
         return (_lib.ptr(ks.tables), int(ks.nplanes), float(ks.w_min), float(ks.w_step))
 
+    copy_probe(torch, dev, args.copy_probe_mib)
     twp = _lib._P()
     _lib.check(L.croc_wtowers_plan_create(_lib.ctx(), ctypes.byref(twp)))
 
@@ -342,24 +343,30 @@ def run_towers(args, torch, G, W, _lib, dev):
     # timed steps: 8 FP32 flop per kernel tap (one complex multiply-add),
     # S^2 taps per visibility and direction
     taps = float(nv) * S * S
+    # separable sets of support <= 16 run on the matrix-core job kernels
+    sfx = '_mfma' if S <= 16 else ''
     if res['grid_tap_ms'] >= res['degrid_tap_ms']:
-        kname, kms, kn, lds_per_tap = 'k_tw_grid_jobs', res['grid_tap_ms'], res['grid_launches'], 16.0
+        kname, kms, kn = 'k_tw_grid_jobs' + sfx, res['grid_tap_ms'], res['grid_launches']
     else:
-        kname, kms, kn, lds_per_tap = ('k_tw_degrid_jobs', res['degrid_tap_ms'],
-                                       res['degrid_launches'], 8.0)
+        kname, kms, kn = 'k_tw_degrid_jobs' + sfx, res['degrid_tap_ms'], res['degrid_launches']
     achieved = 8.0 * taps / (kms * 1e-3) / 1e12
+    # algorithmic HBM bytes of a job kernel: each visibility's 16-byte
+    # record and 8-byte value once, each (chunk, w-plane) job's 128^2
+    # complex64 sub-grid once (written by gridding, read by degridding)
+    alg_bytes = (24.0 * nv + 8.0 * 128 * 128 * njobs) / max(kn, 1)
     roofline = {
         'bound': 'valu', 'achieved': round(achieved, 3), 'peak': FP32_PEAK_TFLOPS,
         'unit': 'TFLOP/s', 'frac': round(achieved / FP32_PEAK_TFLOPS, 5),
-        'bound_note': 'complex multiply-add per kernel tap on the vector ALUs, 8 FP32 flop/tap '
-                      '(the reference convention); no matrix cores on this path',
+        'bound_note': 'complex multiply-add per kernel tap, 8 FP32 flop/tap (the reference '
+                      'convention), against the FP32 vector peak: the job kernels form the '
+                      'products on the matrix cores (fp16 hi/lo split, FP32 accumulation) but '
+                      'are bound by the vector work around them (counters: MFMA busy ~5 %)',
         'kernel': kname, 'launches_per_step': kn,
         'kernel_ms_per_launch': round(kms / max(kn, 1), 4),
-        'kernel_ms_per_step': {'k_tw_grid_jobs': round(res['grid_tap_ms'], 3),
-                               'k_tw_degrid_jobs': round(res['degrid_tap_ms'], 3)},
+        'kernel_ms_per_step': {'k_tw_grid_jobs' + sfx: round(res['grid_tap_ms'], 3),
+                               'k_tw_degrid_jobs' + sfx: round(res['degrid_tap_ms'], 3)},
         'algorithmic_flops_per_launch': 8.0 * taps / max(kn, 1),
-        'lds_bytes_per_tap': lds_per_tap,
-        'lds_frac': round(lds_per_tap * taps / (kms * 1e-3) / (LDS_PEAK_TBS * 1e12), 5),
+        'algorithmic_bytes_per_launch': round(alg_bytes),
         'traffic': None,
     }
     pmc = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
@@ -367,6 +374,7 @@ def run_towers(args, torch, G, W, _lib, dev):
     if os.path.exists(pmc):
         prof = json.load(open(pmc))
         roofline['traffic'] = prof.get('hbm_bytes_per_launch')
+        roofline['traffic_over_algorithmic'] = round(roofline['traffic'] / alg_bytes, 3)
         roofline['traffic_profile'] = {'file': os.path.relpath(pmc, os.path.dirname(
             os.path.abspath(__file__))), 'kernel_us': prof.get('kernel_us'),
             'note': 'calibrated FETCH_SIZE + WRITE_SIZE per launch from rocprofv3 --pmc of the '
@@ -607,6 +615,7 @@ def run_stream(args, torch, W, _lib, dev):
     nd = torch.empty(nch, dtype=torch.int64, device=dev)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
+    copy_probe(torch, dev, args.copy_probe_mib)
     per_vis = 4 * (1 + S) * S      # grid.c:222: flops per degridded visibility
     nvis_total = int(vis.vis.shape[0])
 
@@ -709,6 +718,18 @@ def run_stream(args, torch, W, _lib, dev):
     print(json.dumps(line), flush=True)
 
 
+def copy_probe(torch, dev, mib):
+    """One device copy of ``mib`` MiB (a blit kernel of known traffic, to
+    scale the HBM counters of a profiled run); nothing when mib <= 0."""
+    if mib <= 0:
+        return
+    src = torch.full((mib << 18,), 1.0, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def measured_copy_gbs(torch, dev, mib=1024, reps=5):
     """HBM bandwidth re-measured on this device: a 1 GiB device-to-device
     copy (torch copy_), read + write bytes over the mean of ``reps`` copies
@@ -779,12 +800,7 @@ def main():
     out = torch.empty(nv, dtype=torch.complex64, device=dev)
 
     copy_gbs = measured_copy_gbs(torch, dev)
-    if args.copy_probe_mib > 0:
-        src = torch.full((args.copy_probe_mib << 18,), 1.0, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    copy_probe(torch, dev, args.copy_probe_mib)
 
     # With --autotune the warm-up also measures a few tile / chunk settings on
     # this workload and keeps the fastest (results do not depend on them).
