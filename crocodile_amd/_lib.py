@@ -62,6 +62,7 @@ SIGNATURES = {
     'croc_convdegrid_plan': [_P, _P, _P, _P, _P],
     'croc_convdegrid_noalloc': [_P, _P, _I, _P, _I64, _I64, _D, _P, _P, _I, _I, _I, _P, _P,
                                 _I, _I, _P, _D, _I],
+    'croc_remap_coords': [_P, _P, _I64, _I64, _D, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
     'croc_grid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
     'croc_degrid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
     'croc_w_kernels': [_P, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P, _I, _P],

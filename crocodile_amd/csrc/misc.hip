@@ -115,6 +115,62 @@ __global__ void k_hermitian(C *__restrict__ g, int N) {
   g[mc].y = b.y - a.y;
 }
 
+// Non-square grids: each visibility's cell and sub-offset by synthesis.py's
+// rule for an [Nh][Nw] grid -- frac_coords (:438-449, per-axis sizes) for
+// the convolution, or the nearest cell of grid / degrid (:389-416, which
+// take N = Nh for both axes and index [y, x] with NumPy's negative wrap) --
+// checked against that grid (convolution window rows y - gh//2 ..
+// y + (gh+1)//2 - 1, columns likewise), then re-expressed as the coordinate
+// of a square [Nb][Nb] grid holding the [Nh][Nw] one at rows oy.., columns
+// ox..: the square-grid kernels land every tap on the same cell there.
+__global__ void k_remap(const double *__restrict__ uvw, int64_t nv, int64_t stride,
+                        double lam, int Nh, int Nw, int Q, int gh, int gw, int nearest,
+                        int Nb, int oy, int ox, double *__restrict__ p,
+                        uint32_t *status) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const double px = grid_fraction(uvw[i * stride], lam);
+  const double py = grid_fraction(uvw[i * stride + 1], lam);
+  int64_t cx, cy;
+  int fx = 0, fy = 0;
+  if (nearest) {
+    // numpy.max(p) < 0.5 covers every column of p, w included
+    const bool wok = stride < 3 || grid_fraction(uvw[i * stride + 2], lam) < 0.5;
+    if (!(px < 0.5 && py < 0.5 && wok)) {
+      atomicOr(status, (uint32_t)ST_DOMAIN);
+      return;
+    }
+    const double tx = floor(0.5 + (double)Nh * px), ty = floor(0.5 + (double)Nh * py);
+    if (!(fabs(tx) < 1e15 && fabs(ty) < 1e15)) {
+      atomicOr(status, (uint32_t)ST_BOUNDS);
+      return;
+    }
+    cx = (int64_t)(Nh / 2) + (int64_t)tx;
+    cy = (int64_t)(Nh / 2) + (int64_t)ty;
+    if (cx < 0) cx += Nw;
+    if (cy < 0) cy += Nh;
+    if (cx < 0 || cx >= Nw || cy < 0 || cy >= Nh) {
+      atomicOr(status, (uint32_t)ST_BOUNDS);
+      return;
+    }
+  } else {
+    int ix, iy;
+    if (!frac_coord_dev(Nw, Q, px, &ix, &fx) || !frac_coord_dev(Nh, Q, py, &iy, &fy)) {
+      atomicOr(status, (uint32_t)ST_DOMAIN);
+      return;
+    }
+    if (ix - gw / 2 < 0 || ix + (gw + 1) / 2 > Nw || iy - gh / 2 < 0 || iy + (gh + 1) / 2 > Nh) {
+      atomicOr(status, (uint32_t)ST_BOUNDS);
+      return;
+    }
+    cx = ix;
+    cy = iy;
+  }
+  p[2 * i] = ((double)(cx + ox - Nb / 2) - (double)fx / (double)Q) / (double)Nb;
+  p[2 * i + 1] = ((double)(cy + oy - Nb / 2) - (double)fy / (double)Q) / (double)Nb;
+}
+
 int small_status(croc_ctx *ctx, uint32_t **st) {
   *st = (uint32_t *)ws_get(ctx, WS_SMALL, 256);
   if (!*st) return CROC_ENOMEM;
@@ -144,6 +200,25 @@ extern "C" int croc_frac_coords(croc_ctx *ctx, const double *uvw, int64_t nv,
   if ((rc = small_status(ctx, &st))) return rc;
   k_frac_coords<<<(unsigned)((nv + 255) / 256), 256, 0, ctx->stream>>>(
       uvw, nv, stride, lam, Nh, Nw, Q, x, xf, y, yf, st);
+  CROC_HIP(hipGetLastError());
+  return finish_status(ctx, st);
+}
+
+extern "C" int croc_remap_coords(croc_ctx *ctx, const double *uvw, int64_t nv, int64_t stride,
+                                 double lam, int Nh, int Nw, int Q, int gh, int gw,
+                                 int nearest, int Nb, int oy, int ox, double *p_out) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  CROC_REQUIRE(Q >= 1 && Nh >= 1 && Nw >= 1 && stride >= 2 && lam > 0 && gh >= 1 && gw >= 1,
+               CROC_EINVAL, "bad arguments");
+  CROC_REQUIRE(Nb >= 2 && Nb % 2 == 0 && oy >= 0 && ox >= 0 && oy + Nh <= Nb && ox + Nw <= Nb,
+               CROC_EINVAL, "the square grid must be even and hold the [Nh][Nw] grid");
+  if (nv == 0) return CROC_OK;
+  CROC_REQUIRE(uvw && p_out, CROC_EINVAL, "null array");
+  uint32_t *st;
+  if ((rc = small_status(ctx, &st))) return rc;
+  k_remap<<<(unsigned)((nv + 255) / 256), 256, 0, ctx->stream>>>(
+      uvw, nv, stride, lam, Nh, Nw, Q, gh, gw, nearest, Nb, oy, ox, p_out, st);
   CROC_HIP(hipGetLastError());
   return finish_status(ctx, st);
 }

@@ -404,7 +404,8 @@ def kernel_bank(gcf, src, nv, for_grid):
     ``src`` rows (make_gcf semantics, synthesis.py:451-461).  Returns
     (bank, plane_mode, vplane)."""
     if isinstance(gcf, KernelBank):
-        return gcf.as_complex64(), _lib.PLANE_SINGLE, None
+        b = gcf.as_complex64()
+        return b, _lib.PLANE_SINGLE, None
     if src is None:
         src_a = numpy.zeros((nv, 0), dtype=numpy.int64)
     elif _is_torch(src):
@@ -421,6 +422,7 @@ def kernel_bank(gcf, src, nv, for_grid):
     elif d == 4:
         separable = False
         Q, gh, gw = gcf.shape[-3:]
+        table_hw = (gh, gw)
         S = gw
         if gh != gw:
             if for_grid:
@@ -449,6 +451,8 @@ def kernel_bank(gcf, src, nv, for_grid):
     kern = _dev(gcf, torch.complex64).reshape(-1)
     off = numpy.arange(nplanes, dtype=numpy.int64) * tsize
     bank = KernelBank(kern, off, numpy.full(nplanes, S, dtype=numpy.int32), Q, separable)
+    # the reference's window extents (rows, columns) of these tables
+    bank.window = (S, S) if separable else tuple(int(x) for x in table_hw)
     if k == 0:
         return bank, _lib.PLANE_SINGLE, None
     idx = src_a.astype(numpy.int64)
@@ -636,6 +640,40 @@ def _square(a):
     return a.shape[0]
 
 
+def _nonsquare(a):
+    return a.ndim == 2 and a.shape[0] != a.shape[1]
+
+
+def _embedding(Nh, Nw, S):
+    """Even square size holding an [Nh, Nw] grid with S + 2 cells of margin
+    on every side, and the grid's row / column offset in it."""
+    Nb = max(Nh, Nw) + 2 * (S + 2)
+    Nb += Nb % 2
+    return Nb, (Nb - Nh) // 2, (Nb - Nw) // 2
+
+
+def _remap(uv, Nh, Nw, Q, window, nearest, Nb, oy, ox):
+    """Coordinates of the [Nh, Nw] grid's cells in the square embedding
+    (croc_remap_coords)."""
+    out = torch.empty((uv.shape[0], 2), dtype=torch.float64, device=uv.device)
+    gh, gw = window
+    _lib.check(_lib.load().croc_remap_coords(
+        _lib.ctx(), _lib.ptr(uv), uv.shape[0], uv.shape[1], 1.0, int(Nh), int(Nw), int(Q),
+        int(gh), int(gw), int(nearest), int(Nb), int(oy), int(ox), _lib.ptr(out)))
+    return out
+
+
+def _window(bank):
+    return getattr(bank, 'window', None) or (bank.max_support, bank.max_support)
+
+
+def _add_into(a, part):
+    if _is_torch(a):
+        a += part.to(device=a.device, dtype=a.dtype)
+    else:
+        a += part.cpu().numpy()
+
+
 def frac_coord(N, Qpx, p):
     """(ix, xf) with ix - xf/Qpx the nearest 1/Qpx approximation of
     N//2 + p N, half-to-even at ties; int64 results.  synthesis.py:419-435."""
@@ -672,10 +710,18 @@ def convgrid(gcf, a, p, src, v):
     """Grid visibilities ``v`` at ``p`` (fraction of the grid) into ``a`` in
     place: a[y-S//2 : y+(S+1)//2, x-S//2 : x+(S+1)//2] += k(s, xf, yf) v.
     No conjugation.  synthesis.py:464-481."""
-    N = _square(a)
     uv = _uvw(p)
     bank, mode, vplane = kernel_bank(gcf, src, uv.shape[0], True)
     vt = _dev(v, torch.complex64).reshape(-1)
+    if _nonsquare(a):
+        Nh, Nw = a.shape
+        Nb, oy, ox = _embedding(Nh, Nw, bank.max_support)
+        pb = _remap(uv, Nh, Nw, bank.Q, _window(bank), 0, Nb, oy, ox)
+        acc = torch.zeros((Nb, Nb), dtype=torch.complex64, device=uv.device)
+        grid_device(bank, acc, pb, 1.0, vt, mode, vplane)
+        _add_into(a, acc[oy:oy + Nh, ox:ox + Nw])
+        return
+    N = _square(a)
     if _is_torch(a) and a.is_cuda and a.dtype == torch.complex64 and a.is_contiguous():
         grid_device(bank, a, uv, 1.0, vt, mode, vplane)
         return
@@ -690,11 +736,19 @@ def convgrid(gcf, a, p, src, v):
 def convdegrid(gcf, a, p, src=None):
     """v_i = sum(a[window_i] * k(s, xf, yf)), no conjugation.
     synthesis.py:484-503."""
-    N = _square(a)
     uv = _uvw(p)
     bank, mode, vplane = kernel_bank(gcf, src, uv.shape[0], False)
     gt = _dev(a, torch.complex64)
     out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
+    if _nonsquare(gt):
+        Nh, Nw = gt.shape
+        Nb, oy, ox = _embedding(Nh, Nw, bank.max_support)
+        pb = _remap(uv, Nh, Nw, bank.Q, _window(bank), 0, Nb, oy, ox)
+        big = torch.zeros((Nb, Nb), dtype=torch.complex64, device=uv.device)
+        big[oy:oy + Nh, ox:ox + Nw] = gt
+        degrid_device(bank, big, pb, 1.0, out, mode, vplane)
+        return _out(out, _torch_in(a, p))
+    _square(gt)
     degrid_device(bank, gt, uv, 1.0, out, mode, vplane)
     return _out(out, _torch_in(a, p))
 
@@ -702,9 +756,18 @@ def convdegrid(gcf, a, p, src=None):
 def grid(a, p, v):
     """Nearest-cell gridding, cell N//2 + floor(0.5 + N p).
     synthesis.py:389-401."""
-    N = _square(a)
     uv = _uvw(p)
     vt = _dev(v, torch.complex64).reshape(-1)
+    if _nonsquare(a):
+        Nh, Nw = a.shape
+        Nb, oy, ox = _embedding(Nh, Nw, 0)
+        pb = _remap(uv, Nh, Nw, 1, (1, 1), 1, Nb, oy, ox)
+        acc = torch.zeros((Nb, Nb), dtype=torch.complex64, device=uv.device)
+        _lib.check(_lib.load().croc_grid_nearest(_lib.ctx(), _lib.ptr(acc), Nb, _lib.ptr(pb),
+                                                 pb.shape[0], 2, 1.0, _lib.ptr(vt)))
+        _add_into(a, acc[oy:oy + Nh, ox:ox + Nw])
+        return
+    N = _square(a)
     direct = _is_torch(a) and a.is_cuda and a.dtype == torch.complex64 and a.is_contiguous()
     acc = a if direct else torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
     _lib.check(_lib.load().croc_grid_nearest(_lib.ctx(), _lib.ptr(acc), N, _lib.ptr(uv),
@@ -718,10 +781,19 @@ def grid(a, p, v):
 
 def degrid(a, p):
     """Nearest-cell degridding.  synthesis.py:404-416."""
-    N = _square(a)
     uv = _uvw(p)
     gt = _dev(a, torch.complex64)
     out = torch.empty(uv.shape[0], dtype=torch.complex64, device=uv.device)
+    if _nonsquare(gt):
+        Nh, Nw = gt.shape
+        Nb, oy, ox = _embedding(Nh, Nw, 0)
+        pb = _remap(uv, Nh, Nw, 1, (1, 1), 1, Nb, oy, ox)
+        big = torch.zeros((Nb, Nb), dtype=torch.complex64, device=uv.device)
+        big[oy:oy + Nh, ox:ox + Nw] = gt
+        _lib.check(_lib.load().croc_degrid_nearest(_lib.ctx(), _lib.ptr(big), Nb, _lib.ptr(pb),
+                                                   pb.shape[0], 2, 1.0, _lib.ptr(out)))
+        return _out(out, _torch_in(a, p))
+    N = _square(gt)
     _lib.check(_lib.load().croc_degrid_nearest(_lib.ctx(), _lib.ptr(gt), N, _lib.ptr(uv),
                                                uv.shape[0], uv.shape[1], 1.0, _lib.ptr(out)))
     return _out(out, _torch_in(a, p))

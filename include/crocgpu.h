@@ -180,6 +180,23 @@ int croc_convdegrid_noalloc(croc_ctx *ctx, const void *grid, int N,
                             int plane_mode, const int32_t *vplane, double wstep,
                             int wbin0);
 
+/* Non-square grids [Nh][Nw] (synthesis.py's convgrid / convdegrid / grid /
+ * degrid on any 2D array): each visibility's cell and sub-offset by the
+ * reference's rule for that shape -- frac_coords (:438-449, x from Nw, y
+ * from Nh) when nearest == 0, with the window rows y - gh//2 ..
+ * y + (gh+1)//2 - 1 and columns x - gw//2 .. checked inside the grid
+ * (CROC_EBOUNDS); the nearest cell of grid / degrid (:389-416: N = Nh for
+ * both axes, NumPy's negative-index wrap) when nearest == 1 -- written as
+ * the coordinate p_out (double [nv][2]) of a square [Nb][Nb] grid (Nb even)
+ * that holds the [Nh][Nw] grid at rows oy.., columns ox..: the square-grid
+ * entry points given p_out and lam = 1 then land every tap on the same cell
+ * of that embedding (exactly: the cell and sub-offset are recovered bit for
+ * bit).  No separate counterpart in the reference, whose NumPy functions
+ * take any shape. */
+int croc_remap_coords(croc_ctx *ctx, const double *uvw, int64_t nv, int64_t stride,
+                      double lam, int Nh, int Nw, int Q, int gh, int gw, int nearest,
+                      int Nb, int oy, int ox, double *p_out);
+
 /* grid / degrid (nearest cell): replace crocodile/synthesis.py:389-401 and
  * :404-416, cell N//2 + floor(0.5 + N*p) (round-half-up). */
 int croc_grid_nearest(croc_ctx *ctx, void *grid, int N, const double *uvw,

@@ -539,3 +539,54 @@ def test_convdegrid_non_square_kernel(gpu, gh, gw, src):
     assert_close(got, ref, what='convdegrid %dx%d' % (gh, gw))
     with pytest.raises(ValueError):  # convgrid transposes the window (synthesis.py:477)
         G.convgrid(gcf, numpy.zeros((N, N), complex), p, s, numpy.ones(nv))
+
+
+@pytest.mark.parametrize('Nh,Nw', [(96, 160), (150, 64)])
+@pytest.mark.parametrize('kind', ['separable', '2d', '2d_nonsquare_kernel', 'src'])
+def test_non_square_grids(gpu, Nh, Nw, kind):
+    """convgrid / convdegrid on an [h, w] grid (frac_coords with per-axis
+    sizes, synthesis.py:438-449) against the oracle, with separable, 2D,
+    non-square 2D (degrid only) and source-indexed tables; the edge rule
+    for windows; grid / degrid (nearest cell, N = h for both axes as
+    synthesis.py:398, :413)."""
+    rng = numpy.random.default_rng(Nh + Nw + len(kind))
+    Q, nv = 8, 4000
+    if kind == 'separable':
+        gcf, s, gh, gw = rng.standard_normal((Q, 7)) + 0j, None, 7, 7
+    elif kind == '2d':
+        gcf, s, gh, gw = rng.standard_normal((Q, Q, 8, 8)) + 1j, None, 8, 8
+    elif kind == '2d_nonsquare_kernel':
+        gcf, s, gh, gw = rng.standard_normal((Q, Q, 5, 9)) + 1j, None, 5, 9
+    else:
+        gcf = rng.standard_normal((3, Q, Q, 6, 6)) + 1j * rng.standard_normal((3, Q, Q, 6, 6))
+        s, gh, gw = rng.integers(0, 3, (nv, 1)), 6, 6
+    p = numpy.stack([rng.uniform(-0.5 + (gw // 2 + 2) / Nw, 0.5 - (gw // 2 + 2) / Nw, nv),
+                     rng.uniform(-0.5 + (gh // 2 + 2) / Nh, 0.5 - (gh // 2 + 2) / Nh, nv)], 1)
+    print('case non-square grid %dx%d %s' % (Nh, Nw, kind), flush=True)
+    a = rng.standard_normal((Nh, Nw)) + 1j * rng.standard_normal((Nh, Nw))
+    got = G.convdegrid(gcf, a, p, s)
+    assert_close(got, R.convdegrid(gcf, a, p, s), what='convdegrid %dx%d %s' % (Nh, Nw, kind))
+    v = rng.standard_normal(nv) + 1j * rng.standard_normal(nv)
+    if kind != '2d_nonsquare_kernel':
+        g = numpy.zeros((Nh, Nw), complex)
+        G.convgrid(gcf, g, p, s, v)
+        ref = numpy.zeros((Nh, Nw), complex)
+        R.convgrid(gcf, ref, p, s, v)
+        assert_close(g, ref, what='convgrid %dx%d %s' % (Nh, Nw, kind))
+        # a window over the bottom edge raises and leaves the grid as it was
+        bad = p.copy()
+        bad[5, 1] = 0.5 - 1.0 / Nh
+        before = g.copy()
+        with pytest.raises(ValueError):
+            G.convgrid(gcf, g, bad, s, v)
+        assert (g == before).all()
+    # nearest cell
+    # x = h//2 + floor(0.5 + h p_x) must index the w columns
+    q = numpy.stack([rng.uniform(-0.5, min(0.49, (Nw - Nh // 2 - 1) / Nh), nv),
+                     rng.uniform(-0.5, 0.49, nv)], 1)
+    gn = numpy.zeros((Nh, Nw), complex)
+    G.grid(gn, q, v)
+    refn = numpy.zeros((Nh, Nw), complex)
+    R.grid(refn, q, v)
+    assert_close(gn, refn, what='grid %dx%d' % (Nh, Nw))
+    assert_close(G.degrid(a, q), R.degrid(a, q), what='degrid %dx%d' % (Nh, Nw))
