@@ -681,10 +681,26 @@ def run_stream(args, torch, W, _lib, dev):
                      'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': round(achieved / FP32_PEAK_TFLOPS, 5), 'kernel': 'k_bl_degrid',
                      'kernel_ms_per_step': round(kernel_ms, 3),
-                     'flop_per_visibility': per_vis, 'traffic': None,
+                     'flop_per_visibility': per_vis,
+                     # queue slots written once (zeros included), subgrids read
+                     # once, each chunk's uvw rows and channel frequencies
+                     'algorithmic_bytes_per_launch': int(8 * nch * tc * fc + 8 * len(offs) * xM * xM
+                                                         + nch * (24 * tc + 8 * fc)),
+                     'traffic': None,
                      'note': 'algorithmic flops of degrid_conv_uv (grid.c:222, 4 (1 + S) S '
                              'per visibility) / k_bl_degrid launch time (HIP events)'},
     }
+    pmc = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                       'k_bl_degrid_stream32k_pmc.json')
+    if os.path.exists(pmc):
+        prof = json.load(open(pmc))
+        r = line['roofline']
+        r['traffic'] = prof.get('hbm_bytes_per_launch')
+        r['traffic_over_algorithmic'] = round(r['traffic'] / r['algorithmic_bytes_per_launch'], 3)
+        r['traffic_profile'] = {'file': 'profiles/k_bl_degrid_stream32k_pmc.json',
+                                'kernel_us': prof.get('kernel_us'),
+                                'note': 'calibrated FETCH_SIZE + WRITE_SIZE of the batched launch '
+                                        'from rocprofv3 --pmc of the same command'}
     if args.cpu_sample >= 0:
         from oracle import blgrid as OB
         # the busiest subgrid's chunks, a prefix of them: about 5-10 s of one core
