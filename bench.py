@@ -197,6 +197,7 @@ def run_towers(args, torch, G, W, _lib, dev):
     wstep = (float(w.max()) - float(w.min())) / (W.NPLANES - 1)
     wb = torch.round(w / wstep)
     lo, hi = int(wb.min()), int(wb.max())
+    margin = W.tower_margin(float(w.abs().max()), theta)
     ws_plain = [wstep * b for b in range(lo, hi + 1)]
     print('[bench] kernels: %d w-planes, wstep %.2f' % (len(ws_plain), wstep), file=sys.stderr,
           flush=True)
@@ -215,7 +216,7 @@ def run_towers(args, torch, G, W, _lib, dev):
 
     def tw_bin():
         _lib.check(L.croc_wtowers_plan_bin(_lib.ctx(), twp, N, _lib.ptr(uvw_d), nv, 3, theta,
-                                           wstep, 128, W.MARGIN, 1, Q, S, kimg.nplanes,
+                                           wstep, 128, margin, 1, Q, S, kimg.nplanes,
                                            kimg.w_min, kimg.w_step))
 
     def tw_grid(ks):
@@ -240,10 +241,10 @@ def run_towers(args, torch, G, W, _lib, dev):
     def towers_one_shot():
         guv.zero_()
         _lib.check(L.croc_wtowers_grid(_lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uvw_d), nv, 3,
-                                       theta, _lib.ptr(vis_d), wstep, 128, W.MARGIN,
+                                       theta, _lib.ptr(vis_d), wstep, 128, margin,
                                        _lib.ptr(kimg.tables), Q, S, 1, *kargs(kimg)[1:]))
         _lib.check(L.croc_wtowers_degrid(_lib.ctx(), _lib.ptr(guv), N, _lib.ptr(uvw_d), nv, 3,
-                                         theta, _lib.ptr(out), wstep, 128, W.MARGIN,
+                                         theta, _lib.ptr(out), wstep, 128, margin,
                                          _lib.ptr(kpred.tables), Q, S, 1, *kargs(kpred)[1:]))
         t = _lib.last_tap_kernel_ms()
         return t[0], t[1], t[2], t[3]
@@ -304,9 +305,9 @@ def run_towers(args, torch, G, W, _lib, dev):
     pv = timed(pervis_step, 'per-visibility', 2, 1)
 
     # job structure of the headline
-    key = (torch.floor(uvw_d[:, 0] * theta + 0.5).to(torch.int64) + N // 2) // (128 - W.MARGIN)
+    key = (torch.floor(uvw_d[:, 0] * theta + 0.5).to(torch.int64) + N // 2) // (128 - margin)
     key = key * 100000 + (torch.floor(uvw_d[:, 1] * theta + 0.5).to(torch.int64) + N // 2) \
-        // (128 - W.MARGIN)
+        // (128 - margin)
     key = key * 4096 + (wb.to(torch.int64) - lo)
     njobs = int(torch.unique(key).numel())
     nchunks = int(torch.unique(key // 4096).numel())
@@ -315,7 +316,7 @@ def run_towers(args, torch, G, W, _lib, dev):
     # predict accuracy against the DFT, full size
     print('[bench] predict error against the DFT', file=sys.stderr, flush=True)
     single = G.WKernelSet(kpred.tables[kpred.nplanes // 2:kpred.nplanes // 2 + 1], 0.0, 1.0)
-    nores = G.WTowersPlan(theta, lam, uvw_d, wstep, 128, W.MARGIN, single.tables[0])
+    nores = G.WTowersPlan(theta, lam, uvw_d, wstep, 128, margin, single.tables[0])
 
     def towers_pred(grid):
         guv.copy_(grid)
@@ -383,7 +384,7 @@ def run_towers(args, torch, G, W, _lib, dev):
     cpu, parity = None, None
     n_cpu = args.cpu_sample or 20000
     if n_cpu > 0:
-        cpu, parity = towers_cpu_and_parity(G, W, c, name, wstep, kimg, kpred, n_cpu)
+        cpu, parity = towers_cpu_and_parity(G, W, c, name, wstep, kimg, kpred, n_cpu, margin)
 
     line = {
         'metric': 'visibilities/s grid+degrid', 'value': round(res['value'], 1),
@@ -392,7 +393,7 @@ def run_towers(args, torch, G, W, _lib, dev):
         'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
         'config': {'workload': name, 'N': N, 'theta': theta, 'nv': nv, 'w_planes': W.NPLANES,
                    'wstep': round(wstep, 3), 'support': S, 'oversampling': Q,
-                   'subgrid': 128, 'margin': W.MARGIN, 'jobs': njobs, 'chunks': nchunks,
+                   'subgrid': 128, 'margin': margin, 'jobs': njobs, 'chunks': nchunks,
                    'residual_w_kernels': kimg.nplanes, 'residual_w_step': W.RES_STEP,
                    'kernels': 'separable AW (PSWF c=1.3S x Gaussian beam) x residual w-term',
                    'step': 'croc_wtowers_plan_bin + croc_wtowers_grid_plan + '
@@ -422,7 +423,7 @@ def run_towers(args, torch, G, W, _lib, dev):
     print(json.dumps(line), flush=True)
 
 
-def towers_cpu_and_parity(G, W, c, name, wstep, kimg, kpred, n):
+def towers_cpu_and_parity(G, W, c, name, wstep, kimg, kpred, n, margin):
     """The FP64 NumPy restatement of the w-towers drivers
     (oracle/wtowers_ref.py) timed on one core over a seeded n-visibility
     sample of the same distribution (W.towers_sample), with the device
@@ -438,12 +439,12 @@ def towers_cpu_and_parity(G, W, c, name, wstep, kimg, kpred, n):
     print('[bench] cpu baseline: FP64 oracle w-towers on %d visibilities' % n, file=sys.stderr,
           flush=True)
     t0 = time.perf_counter()
-    g64 = WT.w_towers_imaging(theta, lam, uvw, None, vis, wstep, 128, W.MARGIN, ki)
+    g64 = WT.w_towers_imaging(theta, lam, uvw, None, vis, wstep, 128, margin, ki)
     t1 = time.perf_counter()
-    v64 = WT.w_towers_predict(theta, lam, uvw, None, g64, wstep, 128, W.MARGIN, kp)
+    v64 = WT.w_towers_predict(theta, lam, uvw, None, g64, wstep, 128, margin, kp)
     t2 = time.perf_counter()
-    g32 = G.w_towers_imaging(theta, lam, uvw, None, vis, wstep, 128, W.MARGIN, kimg)
-    v32 = G.w_towers_predict(theta, lam, uvw, None, g64, wstep, 128, W.MARGIN, kpred)
+    g32 = G.w_towers_imaging(theta, lam, uvw, None, vis, wstep, 128, margin, kimg)
+    v32 = G.w_towers_predict(theta, lam, uvw, None, g64, wstep, 128, margin, kpred)
 
     def err(x, ref):
         d = x - ref

@@ -354,6 +354,17 @@ MARGIN = 32         # w-towers sub-grid margin (cells); sub-grids are 128
 PLAIN_FF = 128      # far-field pixels of the plain path's 2D AW w-kernels
 
 
+def tower_margin(wmax, theta):
+    """W-towers sub-grid margin for |w| up to wmax: the w-term moves a
+    visibility's contribution by up to |w| theta^2 / 2 cells (the phase
+    gradient w l at the field edge l = theta / 2), which must stay within
+    the margin of its 128-cell sub-grid: MARGIN (32) while that spread fits,
+    else 64, the most a 128 sub-grid takes with a chunk left (towers8192:
+    31.6 cells -> 32; towers4096: 63.8 -> 64, predict error against the DFT
+    0.126 -> 0.030)."""
+    return MARGIN if abs(wmax) * theta * theta / 2 <= MARGIN else 64
+
+
 def plain_support(w, theta, S, cap=64):
     """Support of the plain path's w-kernel at w: the AW kernel's S plus
     the w-term's spread, 2 ceil(|w| theta^2 / 2) cells (the phase gradient
