@@ -119,6 +119,11 @@ void tap_timer_reset(croc_ctx *ctx, int dir);
 int tap_timer_begin(croc_ctx *ctx, int dir);
 int tap_timer_end(croc_ctx *ctx, int dir);
 
+// The context's one-word status (WS_SMALL), cleared on its stream, and its
+// read-back: waits for the stream and maps the status bits to a CROC_E* code.
+int small_status(croc_ctx *ctx, uint32_t **st);
+int finish_status(croc_ctx *ctx, uint32_t *st);
+
 // Device address of the allocation-free entry points' static scratch array
 // on the current device, looked up once per device (noalloc.hip).
 int noalloc_init(int device);
@@ -159,6 +164,18 @@ __device__ __forceinline__ bool frac_coord_dev(int N, int Q, double p, int *ix,
 __device__ __forceinline__ int64_t wbin_dev(double w, double wstep) {
 #pragma clang fp contract(off)
   return (int64_t)rint(w / wstep);
+}
+
+// synthesis.py:399 / :414: N//2 + floor(0.5 + N p); a negative index wraps
+// once as NumPy indexing does, anything else outside the grid is an error.
+__device__ __forceinline__ bool nearest_cell(int N, double p, int *c) {
+#pragma clang fp contract(off)
+  const double f = floor(0.5 + (double)N * p);
+  int64_t k = (int64_t)(N / 2) + (int64_t)f;
+  if (k < 0) k += N;
+  if (k < 0 || k >= N) return false;
+  *c = (int)k;
+  return true;
 }
 
 // Workgroup barrier after LDS atomics whose results are not used

@@ -22,18 +22,6 @@ __global__ void k_frac_coords(const double *__restrict__ uvw, int64_t nv,
   yf[i] = iyf;
 }
 
-// synthesis.py:399 / :414: N//2 + floor(0.5 + N p); a negative index wraps
-// once as NumPy indexing does, anything else outside the grid is an error.
-__device__ __forceinline__ bool nearest_cell(int N, double p, int *c) {
-#pragma clang fp contract(off)
-  const double f = floor(0.5 + (double)N * p);
-  int64_t k = (int64_t)(N / 2) + (int64_t)f;
-  if (k < 0) k += N;
-  if (k < 0 || k >= N) return false;
-  *c = (int)k;
-  return true;
-}
-
 __global__ void k_grid_nearest(float2 *__restrict__ grid, int N,
                                const double *__restrict__ uvw, int64_t nv,
                                int64_t stride, double lam,
@@ -171,6 +159,8 @@ __global__ void k_remap(const double *__restrict__ uvw, int64_t nv, int64_t stri
   p[2 * i + 1] = ((double)(cy + oy - Nb / 2) - (double)fy / (double)Q) / (double)Nb;
 }
 
+}  // namespace
+
 int small_status(croc_ctx *ctx, uint32_t **st) {
   *st = (uint32_t *)ws_get(ctx, WS_SMALL, 256);
   if (!*st) return CROC_ENOMEM;
@@ -183,8 +173,6 @@ int finish_status(croc_ctx *ctx, uint32_t *st) {
   CROC_HIP(hipStreamSynchronize(ctx->stream));
   return croc_status_to_code(*ctx->host_status);
 }
-
-}  // namespace
 
 extern "C" int croc_frac_coords(croc_ctx *ctx, const double *uvw, int64_t nv,
                                 int64_t stride, double lam, int Nh, int Nw,

@@ -1340,3 +1340,146 @@ class WTowersPlan:
             except Exception:
                 pass
             self._h = None
+
+
+# --------------------------------------------------------------------------
+# W-stacking drivers (examples/notebooks/wstacking.ipynb cells 10 and 12)
+# --------------------------------------------------------------------------
+
+
+def _w_planes(uv, wstep):
+    """Per visibility its w-plane round-half-even(w / wstep) (numpy.around,
+    wstacking.ipynb cell 10) as int64, and per non-empty plane in ascending
+    order: (plane numbers on the host, counts and mean w on the device)."""
+    pl = torch.round(uv[:, 2] / float(wstep)).to(torch.int64)
+    lo, hi = (int(x) for x in torch.stack([pl.min(), pl.max()]).cpu())
+    idx = pl - lo
+    counts = torch.bincount(idx, minlength=hi - lo + 1)
+    sums = torch.zeros(hi - lo + 1, dtype=torch.float64, device=uv.device)
+    sums.index_add_(0, idx, uv[:, 2])
+    nz = torch.nonzero(counts).reshape(-1)
+    midw = sums[nz] / counts[nz].to(torch.float64)
+    return pl, lo, hi - lo + 1, (nz + lo).cpu().tolist(), counts[nz], midw
+
+
+def w_stacking_imaging(theta, lam, uvw, src, vis, wstep, max_stack_bytes=4 << 30):
+    """Full-grid w-stacking (wstacking.ipynb cell 10, JSON lines 191-240):
+    the visibilities of each w-plane round(w / wstep) are imaged nearest-cell
+    (simple_imaging) on the full grid, the image divided by the Fresnel
+    pattern w_kernel_function(l, m, mean w of the plane) and the images
+    summed.  Returns the image sum ([N, N], complex128).
+
+    Planes go through the device in batches of at most ``max_stack_bytes``
+    of [N, N] complex64 grids: one gridding launch per batch, a batched
+    inverse FFT and one launch that applies every plane's phase and adds the
+    batch into the FP64 image.  Like the notebook (cell 8), callers flip
+    visibilities to one half-plane and weight them first."""
+    N = _grid_size(theta, lam)
+    uv = _uvw(uvw)
+    if uv.shape[1] < 3:
+        raise ValueError('w-stacking needs uvw with a w column')
+    vt = _vis_dev(vis)
+    like_torch = _torch_in(uvw, vis)
+    image = torch.zeros((N, N), dtype=torch.complex128, device=uv.device)
+    if uv.shape[0] == 0:
+        return _out(image, like_torch)
+    _, lo, ntab, planes, _, midw = _w_planes(uv, wstep)
+    L = _lib.load()
+    centred = int(N % 2 == 0)
+    batch = max(1, min(len(planes), int(max_stack_bytes) // (N * N * 8)))
+    stack = torch.empty((batch, N, N), dtype=torch.complex64, device=uv.device)
+    slot_of = torch.empty(ntab, dtype=torch.int32, device=uv.device)
+    for b0 in range(0, len(planes), batch):
+        sel = planes[b0:b0 + batch]
+        P = len(sel)
+        slot_of.fill_(-1)
+        slot_of[torch.as_tensor(sel, device=uv.device) - lo] = torch.arange(
+            P, dtype=torch.int32, device=uv.device)
+        st = stack[:P]
+        st.zero_()
+        _lib.check(L.croc_wstack_grid(_lib.ctx(), _lib.ptr(st), P, N, _lib.ptr(uv),
+                                      uv.shape[0], uv.shape[1], float(lam), float(wstep),
+                                      lo, ntab, _lib.ptr(slot_of), _lib.ptr(vt), centred))
+        if centred:
+            img = torch.fft.ifftn(st, dim=(-2, -1))
+        else:
+            img = torch.fft.fftshift(torch.fft.ifftn(torch.fft.ifftshift(st, dim=(-2, -1)),
+                                                     dim=(-2, -1)), dim=(-2, -1))
+        mw = midw[b0:b0 + P].contiguous()
+        _lib.check(L.croc_wstack_add(_lib.ctx(), _lib.ptr(image), _lib.ptr(img), P, N,
+                                     float(theta), _lib.ptr(mw), centred))
+        del img
+    return _out(image, like_torch)
+
+
+def w_stacking_subgrid_imaging(theta, lam, uvw, src, vis, wstep, npixkern):
+    """Per-plane sub-grid w-stacking (wstacking.ipynb cell 12, JSON lines
+    249-300): a plane whose mean w lies within wstep / 2 of 0 is gridded
+    nearest-cell onto the grid directly; any other plane onto the square
+    sub-grid spanning its visibilities' cells plus ``npixkern``, centred at
+    the sub-grid middle and the plane's mean w, whose image is divided by the
+    Fresnel pattern of that w and transformed back before it is added to the
+    grid at the sub-grid's offset.  Returns the grid ([N, N], complex128).
+    A sub-grid that would leave the grid raises ValueError (the notebook's
+    slices would wrap or clip it silently)."""
+    N = _grid_size(theta, lam)
+    uv = _uvw(uvw)
+    if uv.shape[1] < 3:
+        raise ValueError('w-stacking needs uvw with a w column')
+    vt = _vis_dev(vis)
+    like_torch = _torch_in(uvw, vis)
+    grid_sum = torch.zeros((N, N), dtype=torch.complex128, device=uv.device)
+    if uv.shape[0] == 0:
+        return _out(grid_sum, like_torch)
+    pl, lo, ntab, planes, counts, midw = _w_planes(uv, wstep)
+    order = torch.sort(pl, stable=True).indices
+    us, vs = uv[order].contiguous(), vt[order].contiguous()
+    slot = torch.zeros(ntab, dtype=torch.int64, device=uv.device)
+    slot[torch.as_tensor(planes, device=uv.device) - lo] = torch.arange(
+        len(planes), device=uv.device)
+    s_of = slot[pl[order] - lo]
+    big = torch.tensor(float('inf'), dtype=torch.float64, device=uv.device)
+    mins = big.expand(len(planes), 2).clone().scatter_reduce_(
+        0, s_of[:, None].expand(-1, 2), us[:, :2], 'amin')
+    maxs = (-big).expand(len(planes), 2).clone().scatter_reduce_(
+        0, s_of[:, None].expand(-1, 2), us[:, :2], 'amax')
+    host = torch.cat([mins, maxs, midw[:, None], counts[:, None].to(torch.float64)],
+                     1).cpu().numpy()
+    mid = int(lam * theta) // 2
+    L = _lib.load()
+    start = 0
+    for k in range(len(planes)):
+        n = int(host[k, 5])
+        pu, pv = us[start:start + n], vs[start:start + n]
+        start += n
+        w_mid = float(host[k, 4])
+        if abs(w_mid) < wstep / 2:
+            g = torch.zeros((N, N), dtype=torch.complex64, device=uv.device)
+            _lib.check(L.croc_grid_nearest(_lib.ctx(), _lib.ptr(g), N, _lib.ptr(pu), n,
+                                           pu.shape[1], float(lam), _lib.ptr(pv)))
+            grid_sum += g
+            continue
+        xy_min = numpy.floor(host[k, 0:2] * theta).astype(int)
+        xy_max = numpy.ceil(host[k, 2:4] * theta).astype(int)
+        xy_min -= (npixkern + 1) // 2
+        xy_max += npixkern // 2
+        xy_size = int(numpy.max(xy_max - xy_min))
+        xy_maxq = numpy.amax([xy_max, xy_min + xy_size], axis=0)
+        uvw_size = xy_size / theta
+        uvw_mid = numpy.hstack([(xy_maxq + xy_min) // 2 / theta, w_mid])
+        x0, y0 = mid + xy_min
+        x1, y1 = mid + xy_max
+        if x0 < 0 or y0 < 0 or x1 > N or y1 > N:
+            raise ValueError('sub-grid of w-plane %d leaves the grid' % planes[k])
+        n_sub = _grid_size(theta, uvw_size)
+        rel = (pu[:, :3] - torch.as_tensor(uvw_mid, device=uv.device)).contiguous()
+        g = torch.zeros((n_sub, n_sub), dtype=torch.complex64, device=uv.device)
+        _lib.check(L.croc_grid_nearest(_lib.ctx(), _lib.ptr(g), n_sub, _lib.ptr(rel), n, 3,
+                                       float(uvw_size), _lib.ptr(pv)))
+        img = torch.zeros((n_sub, n_sub), dtype=torch.complex128, device=uv.device)
+        mw = torch.tensor([w_mid], dtype=torch.float64, device=uv.device)
+        gi = _fft2c(g, True)
+        _lib.check(L.croc_wstack_add(_lib.ctx(), _lib.ptr(img), _lib.ptr(gi), 1, n_sub,
+                                     float(theta), _lib.ptr(mw), 0))
+        grid_sum[y0:y1, x0:x1] += _fft2c(img, False)[0:y1 - y0, 0:x1 - x0]
+    return _out(grid_sum, like_torch)

@@ -205,6 +205,30 @@ int croc_degrid_nearest(croc_ctx *ctx, const void *grid, int N,
                         const double *uvw, int64_t nv, int64_t stride,
                         double lam, void *vis_out);
 
+/* Full-grid w-stacking, the per-plane loop of
+ * examples/notebooks/wstacking.ipynb cell 10 (JSON lines 191-240:
+ * simple_imaging per w-plane, ifft, divide by w_kernel_function(l, m, mean w),
+ * sum), split into two batched steps around a batched inverse FFT:
+ *
+ * croc_wstack_grid adds each visibility whose plane
+ * t = round-half-even(w / wstep) - plane0 has slot_of[t] >= 0 to
+ * stack[slot_of[t]] ([P][N][N] cf32) at its nearest cell (as
+ * croc_grid_nearest; stride >= 3).  A plane outside [0, ntab) is
+ * CROC_EPLANE.  With centred (even N only) each value is multiplied by
+ * (-1)^(x+y), the input half of the centred transform's shifts.
+ *
+ * croc_wstack_add adds sum_p stack[p] * conj(w_kernel_function(l, m, midw[p]))
+ * (midw: device [P] FP64; l, m = theta * coordinates2(N)) to image ([N][N]
+ * cf64), times (-1)^(x+y) when centred (the output half of the shifts).
+ * CROC_EDOMAIN when the image corner lies outside the unit circle
+ * (synthesis.py:290). */
+int croc_wstack_grid(croc_ctx *ctx, void *stack, int P, int N, const double *uvw,
+                     int64_t nv, int64_t stride, double lam, double wstep,
+                     int64_t plane0, int ntab, const int32_t *slot_of, const void *vis,
+                     int centred);
+int croc_wstack_add(croc_ctx *ctx, void *image, const void *stack, int P, int N,
+                    double theta, const double *midw, int centred);
+
 /* w-kernel bank generation: replaces crocodile/synthesis.py:357-377
  * (w_kernel) = :250-270 (kernel_coordinates) + :273-293
  * (w_kernel_function, with 1/k when invert) + :330-354 (kernel_oversample)
