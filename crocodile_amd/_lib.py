@@ -66,6 +66,9 @@ SIGNATURES = {
     'croc_grid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
     'croc_wstack_grid': [_P, _P, _I, _I, _P, _I64, _I64, _D, _D, _I64, _I, _P, _P, _I],
     'croc_wstack_add': [_P, _P, _P, _I, _I, _D, _P, _I],
+    'croc_awt_pregrid': [_P, _P, _I, _P, _I64, _I64, _D, _I, _I, _P, _P, _P, _I, _I],
+    'croc_awt_apply': [_P, _P, _I, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
+    'croc_awt_place': [_P, _P, _I, _P, _I, _I, _P],
     'croc_degrid_nearest': [_P, _P, _I, _P, _I64, _I64, _D, _P],
     'croc_w_kernels': [_P, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P, _I, _P],
     'croc_kernel_oversample': [_P, _P, _I, _I, _I, _I, _P],

@@ -1483,3 +1483,161 @@ def w_stacking_subgrid_imaging(theta, lam, uvw, src, vis, wstep, npixkern):
                                      float(theta), _lib.ptr(mw), 0))
         grid_sum[y0:y1, x0:x1] += _fft2c(img, False)[0:y1 - y0, 0:x1 - x0]
     return _out(grid_sum, like_torch)
+
+
+# --------------------------------------------------------------------------
+# AW-towers imaging (examples/notebooks/awtowers.ipynb cells 17, 19, 22)
+# --------------------------------------------------------------------------
+
+
+def _awt_chunks(theta, lam, subgrid_size, margin, ub, vb):
+    """uv_mid [n, 2] and sub-grid corners [n, 2] of chunks (ub, vb), with
+    awtowers.ipynb cell 22's float arithmetic (grid_tower's in-place
+    `xy_min -=` moves the corner the clipped add uses)."""
+    chunk = subgrid_size - margin
+    ub = numpy.asarray(ub, dtype=numpy.int64)
+    vb = numpy.asarray(vb, dtype=numpy.int64)
+    xy_min = chunk * numpy.stack([ub - 0.5, vb - 0.5], 1)
+    xy_max = chunk * numpy.stack([ub + 0.5, vb + 0.5], 1)
+    uv_mid = (xy_max + xy_min) // 2 / theta
+    xy_min -= (margin + 1) // 2
+    xy_max += margin // 2
+    if not numpy.all(numpy.max(xy_max - xy_min, axis=1) == subgrid_size):
+        raise AssertionError('chunk does not fill the sub-grid')
+    mid = int(lam * theta) // 2
+    corner = mid + xy_min.astype(int)
+    return uv_mid, corner
+
+
+def aw_towers_imaging(theta, lam, uvw, src, vis, wstep, subgrid_size, margin, aa_gcf,
+                      aa_subgrid, aa_subgrid2, aimage, wkern, dl=0.0, dm=0.0,
+                      max_stack_bytes=4 << 30):
+    """AW-towers imaging (awtowers.ipynb cells 17, 19 and 22, the notebook's
+    globals as arguments): chunks of subgrid_size - margin cells centred on
+    ub = floor(u theta / chunk + 0.5); per (chunk, w-plane round(w / wstep),
+    antenna pair src = (a1, a2)) the visibilities relative to (uv_mid,
+    wp wstep) are gridded with the separable anti-aliasing table ``aa_gcf``
+    [Q, K] on a 2 subgrid_size grid at 2 theta, inverse transformed,
+    divided by aa_subgrid2 (x) aa_subgrid2, cut to the middle, multiplied by
+    aa_subgrid (x) aa_subgrid and by aimage[a2] / aimage[a1] (``aimage``
+    [nant, S, S]); each chunk sums its planes times wkern^-wp (the
+    notebook's stepping by wkern^(wp - last_wp) and final division by
+    wkern^last_wp), is transformed and added into the grid, clipped at its
+    edges.  ``dl``, ``dm``: the visibilities are divided by
+    w_kernel_function(dl, dm, w - wp wstep) first (the notebook's shifted
+    field).  Returns grid_sum ([N, N], N = ceil(theta lam)); the notebook's
+    image is real(ifft(grid_sum)) / (aa (x) aa) / sum(wt).
+
+    The notebook skips the wkern step after a chunk's plane equal to the
+    global lowest plane (`if last_wp != wplanes[0]`); as in
+    w_towers_imaging that is not reproduced.  Antenna indices outside
+    [0, nant) raise IndexError."""
+    N = int(math.ceil(theta * lam))
+    S = int(subgrid_size)
+    if S % 2 or S < 2 or not 0 <= margin < S:
+        raise ValueError('AW-towers need an even subgrid_size above the margin')
+    M = 2 * S
+    uv = _uvw(uvw)
+    if uv.shape[1] < 3:
+        raise ValueError('AW-towers need uvw with a w column')
+    dev = uv.device
+    nv = uv.shape[0]
+    like_torch = _torch_in(uvw, vis)
+    grid = torch.zeros((N, N), dtype=torch.complex64, device=dev)
+    if nv == 0:
+        return _out(grid, like_torch)
+    st = _dev(src, torch.int64).reshape(nv, -1)[:, :2]
+    at = _dev(aimage, torch.complex64)
+    nant = at.shape[0]
+    if tuple(at.shape[1:]) != (S, S):
+        raise ValueError('aimage must be [nant, subgrid_size, subgrid_size]')
+    if bool(((st < 0) | (st >= nant)).any()):
+        raise IndexError('antenna index outside the A-screens')
+    gcf = _dev(aa_gcf, torch.complex64)
+    if gcf.ndim != 2:
+        raise ValueError('aa_gcf must be a separable [Q, K] table')
+    Q, K = int(gcf.shape[0]), int(gcf.shape[1])
+    chunk = S - margin
+    wp = torch.round(uv[:, 2] / float(wstep)).to(torch.int64)
+    ub = torch.floor(uv[:, 0] * theta / chunk + 0.5).to(torch.int64)
+    vb = torch.floor(uv[:, 1] * theta / chunk + 0.5).to(torch.int64)
+    lo = torch.stack([wp.min(), ub.min(), vb.min()]).cpu().tolist()
+    hi = torch.stack([wp.max(), ub.max(), vb.max()]).cpu().tolist()
+    P, nub, nvb = (hi[k] - lo[k] + 1 for k in range(3))
+    cid = (ub - lo[1]) * nvb + (vb - lo[2])
+    key = ((cid * P + (wp - lo[0])) * nant + st[:, 0]) * nant + st[:, 1]
+    key, order = torch.sort(key, stable=True)
+    gkey, gcount = torch.unique_consecutive(key, return_counts=True)
+    G = gkey.shape[0]
+    ga2 = (gkey % nant).to(torch.int32)
+    ga1 = ((gkey // nant) % nant).to(torch.int32)
+    gwp_rel = (gkey // (nant * nant)) % P
+    gcid = gkey // (nant * nant * P)
+    gwp = gwp_rel + lo[0]
+    # chunks in key order, and each group's chunk slot
+    ccid, ccount = torch.unique_consecutive(gcid, return_counts=True)
+    C = ccid.shape[0]
+    gchunk = torch.repeat_interleave(torch.arange(C, device=dev), ccount)
+    ccid_h = ccid.cpu().numpy()
+    uv_mid, corner = _awt_chunks(theta, lam, S, margin, ccid_h // nvb + lo[1],
+                                 ccid_h % nvb + lo[2])
+    mid_g = torch.cat([torch.as_tensor(uv_mid, device=dev)[gchunk],
+                       (gwp.to(torch.float64) * float(wstep))[:, None]], 1)
+    gid = torch.repeat_interleave(torch.arange(G, device=dev, dtype=torch.int32), gcount)
+    rel = (uv[order, :3] - mid_g[gid.long()]).contiguous()
+    vc = _dev(vis, torch.complex128).reshape(-1)[order]
+    if dl != 0 or dm != 0:
+        vc = vc / w_kernel_function(torch.tensor(float(dl), dtype=torch.float64, device=dev),
+                                    torch.tensor(float(dm), dtype=torch.float64, device=dev),
+                                    rel[:, 2])
+    vt = vc.to(torch.complex64).contiguous()
+    lam_sub = S / theta
+    L = _lib.load()
+    _lib.check(L.croc_awt_pregrid(_lib.ctx(), None, M, _lib.ptr(rel), 0, nv, float(lam_sub),
+                                  Q, K, _lib.ptr(gcf), _lib.ptr(vt), _lib.ptr(gid), 0, 1))
+    # image-space tables of the sub-grid
+    a2 = numpy.asarray(aa_subgrid2, dtype=float)[S // 2:S // 2 + S]
+    a1 = numpy.asarray(aa_subgrid, dtype=float)
+    win = torch.as_tensor(numpy.outer(a1, a1) / numpy.outer(a2, a2), dtype=torch.float32,
+                          device=dev).contiguous()
+    wk = numpy.asarray(wkern.cpu().numpy() if _is_torch(wkern) else wkern, dtype=complex)
+    turns = torch.as_tensor(numpy.angle(wk) / (2 * numpy.pi), device=dev).contiguous()
+    lmag = torch.as_tensor(numpy.log2(numpy.abs(wk)), dtype=torch.float32,
+                           device=dev).contiguous()
+    # group ranges: visibility offsets and chunk offsets
+    vstart = torch.zeros(G + 1, dtype=torch.int64, device=dev)
+    vstart[1:] = torch.cumsum(gcount, 0)
+    cstart = torch.zeros(C + 1, dtype=torch.int64, device=dev)
+    cstart[1:] = torch.cumsum(ccount, 0)
+    vstart_h, cstart_h = vstart.cpu().numpy(), cstart.cpu().numpy()
+    bins = torch.zeros((C, S, S), dtype=torch.complex64, device=dev)
+    gwp32 = gwp.to(torch.int32).contiguous()
+    gmax = max(1, int(max_stack_bytes) // (M * M * 8))
+    stack = torch.empty((min(G, gmax), M, M), dtype=torch.complex64, device=dev)
+    for g0 in range(0, G, gmax):
+        g1 = min(G, g0 + gmax)
+        sb = stack[:g1 - g0]
+        sb.zero_()
+        _lib.check(L.croc_awt_pregrid(_lib.ctx(), _lib.ptr(sb), M, _lib.ptr(rel),
+                                      int(vstart_h[g0]), int(vstart_h[g1]), float(lam_sub),
+                                      Q, K, _lib.ptr(gcf), _lib.ptr(vt), _lib.ptr(gid), g0, 0))
+        img = torch.fft.ifftn(sb, dim=(-2, -1))
+        c_lo = int(numpy.searchsorted(cstart_h, g0, side='right')) - 1
+        c_hi = int(numpy.searchsorted(cstart_h, g1, side='left'))
+        for c0 in range(c_lo, c_hi, 65535):
+            c1 = min(c_hi, c0 + 65535)
+            # group range of each chunk within the batch
+            starts = numpy.clip(cstart_h[c0:c1 + 1], g0, g1) - g0
+            st_t = torch.as_tensor(starts.astype(numpy.int32), device=dev)
+            _lib.check(L.croc_awt_apply(
+                _lib.ctx(), _lib.ptr(bins), c0, c1 - c0, _lib.ptr(img), S, _lib.ptr(st_t),
+                _lib.ptr(gwp32[g0:g1]), _lib.ptr(ga1[g0:g1]), _lib.ptr(ga2[g0:g1]),
+                _lib.ptr(at), _lib.ptr(win), _lib.ptr(turns), _lib.ptr(lmag)))
+        del img
+    fb = torch.fft.fftn(bins, dim=(-2, -1))
+    corner_t = torch.as_tensor(corner.astype(numpy.int32), device=dev).contiguous()
+    for c0 in range(0, C, 65535):
+        c1 = min(C, c0 + 65535)
+        _lib.check(L.croc_awt_place(_lib.ctx(), _lib.ptr(grid), N, _lib.ptr(fb[c0:c1]),
+                                    c1 - c0, S, _lib.ptr(corner_t[c0:c1])))
+    return _out(grid, like_torch)

@@ -229,6 +229,38 @@ int croc_wstack_grid(croc_ctx *ctx, void *stack, int P, int N, const double *uvw
 int croc_wstack_add(croc_ctx *ctx, void *image, const void *stack, int P, int N,
                     double theta, const double *midw, int centred);
 
+/* AW-towers imaging, examples/notebooks/awtowers.ipynb cells 17
+ * (pregrid_tower_plane), 19 (grid_tower_plane: A-screen ratio per antenna
+ * pair) and 22 (grid_tower, grid_towers), as three batched steps around
+ * batched FFTs.  Visibilities are sorted by group = (chunk, w-plane, antenna
+ * pair); gid[v] is visibility v's group, rel its [nv][3] coordinates
+ * relative to its group's (u_mid, v_mid, wp * wstep).
+ *
+ * croc_awt_pregrid grids visibilities [v0, v1) onto stack[gid - g0]
+ * ([G][M][M] cf32, M = 2 * subgrid_size) with the separable cf32 table gcf
+ * [Q][K] at p = rel / lam_sub (synthesis.py:451-481's convgrid, frac_coord
+ * on the M grid), each tap times (-1)^(x+y).  check_only: validate only
+ * (CROC_EDOMAIN for p outside [-0.5, 0.5), CROC_EBOUNDS for a window that
+ * leaves the M grid).
+ *
+ * croc_awt_apply adds to bins[c0 + k] ([C][S][S] cf32), k < nc, the sum
+ * over groups g in [start[k], start[k+1]) of the cut-out middle pixel of
+ * the (inverse transformed) stack[g] times ascreen[ga2[g]] / ascreen[ga1[g]]
+ * ([nant][S][S] cf32) times wkern^-gwp[g] (wkern = 2^lmag exp(2 pi i turns)
+ * per pixel), all times win ([S][S]).
+ *
+ * croc_awt_place adds (-1)^(x+y) bins[c] (forward transformed) into grid
+ * ([N][N] cf32) at corner[c] = (x0, y0), clipped at the grid's edges. */
+int croc_awt_pregrid(croc_ctx *ctx, void *stack, int M, const double *rel, int64_t v0,
+                     int64_t v1, double lam_sub, int Q, int K, const void *gcf,
+                     const void *vis, const int32_t *gid, int32_t g0, int check_only);
+int croc_awt_apply(croc_ctx *ctx, void *bins, int c0, int nc, const void *stack, int S,
+                   const int32_t *start, const int32_t *gwp, const int32_t *ga1,
+                   const int32_t *ga2, const void *ascreen, const float *win,
+                   const double *turns, const float *lmag);
+int croc_awt_place(croc_ctx *ctx, void *grid, int N, const void *bins, int nc, int S,
+                   const int32_t *corner);
+
 /* w-kernel bank generation: replaces crocodile/synthesis.py:357-377
  * (w_kernel) = :250-270 (kernel_coordinates) + :273-293
  * (w_kernel_function, with 1/k when invert) + :330-354 (kernel_oversample)
