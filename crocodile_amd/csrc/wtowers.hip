@@ -1058,6 +1058,35 @@ __global__ void k_tw_phi(double *phi, double theta, double wstep) {
   phi[i] = r2 < 1.0 ? 2.0 * M_PI * wstep * (1.0 - sqrt(1.0 - r2)) : 0.0;
 }
 
+// Gridding schedule of a binning, made on the host once and kept on the
+// device: jobs cut into parts of at most PART_VIS visibilities, batches of
+// whole jobs holding at most JOB_BATCH part slots.  Per batch: its parts
+// (slot, first, end), the multi-part jobs (first slot, parts) and the chunk
+// groups (chunk, first job, end job); per job its w-plane and slot; the
+// chunks in four parity classes for the final adds.
+struct GridSched {
+  struct Batch {
+    int j0, nj;
+    size_t p0, np, m0, nm, g0, ng;
+  };
+  std::vector<Batch> batches;
+  int max_slots = 0;
+  int32_t *kw = nullptr, *chunks = nullptr, *jslot = nullptr, *par = nullptr;
+  int4 *parts = nullptr, *groups = nullptr;
+  int2 *multi = nullptr;
+  size_t npar[4] = {0, 0, 0, 0};
+};
+
+// Degridding schedule: batches of at most JOB_BATCH jobs (job b of a batch in
+// slot b), every job cut into parts of at most PART_VIS visibilities sharing
+// its slot; per job its w-plane and chunk.
+struct DegridSched {
+  int jb = 0;
+  std::vector<size_t> poff;  // first part of each batch, and the end
+  int32_t *kw = nullptr, *jchunk = nullptr, *chunks = nullptr;
+  int4 *parts = nullptr;
+};
+
 struct Plan {
   TowerArgs a;
   WsSet *keep = nullptr;  // where rec, svis and jobs live (null: the context's set)
@@ -1068,6 +1097,9 @@ struct Plan {
   int njobs = 0;
   std::vector<int4> hjobs;
   std::vector<int32_t> chunks;  // non-empty chunk ids, ascending
+  GridSched gs;
+  DegridSched ds;
+  bool gs_ready = false, ds_ready = false;
 };
 
 int tw_sync_status(croc_ctx *ctx, uint32_t *st) {
@@ -1206,6 +1238,128 @@ int upload_i32(croc_ctx *ctx, int32_t *dst, const std::vector<int32_t> &v) {
   return CROC_OK;
 }
 
+// Build the gridding schedule of p and upload it into `set` (slot WS_TILES).
+int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
+  GridSched &g = p.gs;
+  g = GridSched();
+  const TowerArgs &a = p.a;
+  std::vector<int4> parts, groups;
+  std::vector<int2> multi;
+  std::vector<int32_t> jslot(p.njobs), hk(p.njobs);
+  int slots = 0;
+  for (int j = 0; j < p.njobs; j++) {
+    const int4 job = p.hjobs[j];
+    const int n = job.z - job.y;
+    const int np = std::min(std::max(1, (n + PART_VIS - 1) / PART_VIS), JOB_BATCH / 4);
+    if (g.batches.empty() || slots + np > JOB_BATCH) {
+      g.batches.push_back(GridSched::Batch{j, 0, parts.size(), 0, multi.size(), 0,
+                                           groups.size(), 0});
+      slots = 0;
+    }
+    GridSched::Batch &bt = g.batches.back();
+    jslot[j] = slots;
+    hk[j] = job.x % a.nwp + a.wp_min;
+    const int len = (n + np - 1) / np;
+    for (int k = 0; k < np; k++)
+      parts.push_back(make_int4(slots + k, job.y + k * len,
+                                std::min(job.z, job.y + (k + 1) * len), 0));
+    if (np > 1) multi.push_back(make_int2(slots, np));
+    const int c = job.x / a.nwp, lj = j - bt.j0;
+    if (bt.ng == 0 || groups.back().x != c) {
+      groups.push_back(make_int4(c, lj, lj + 1, 0));
+      bt.ng++;
+    } else {
+      groups.back().z = lj + 1;
+    }
+    slots += np;
+    g.max_slots = std::max(g.max_slots, slots);
+    bt.nj++;
+    bt.np = parts.size() - bt.p0;
+    bt.nm = multi.size() - bt.m0;
+  }
+  // parity classes of the chunks: one add launch each, stream order keeps
+  // overlapping margins of different classes apart
+  std::vector<int32_t> par[4], allpar;
+  for (int c : p.chunks) par[(c % a.ncx) % 2 + 2 * ((c / a.ncx) % 2)].push_back(c);
+  for (int q = 0; q < 4; q++) {
+    allpar.insert(allpar.end(), par[q].begin(), par[q].end());
+    g.npar[q] = par[q].size();
+  }
+  const size_t n32 = 2 * (size_t)p.njobs + 2 * p.chunks.size();
+  const size_t bytes = sizeof(int4) * (parts.size() + groups.size()) +
+                       sizeof(int2) * multi.size() + sizeof(int32_t) * n32 + 256;
+  char *d = (char *)ws_get_in(ctx, set, WS_TILES, bytes);
+  if (!d) return CROC_ENOMEM;
+  g.parts = (int4 *)d;
+  g.groups = g.parts + parts.size();
+  g.multi = (int2 *)(g.groups + groups.size());
+  g.kw = (int32_t *)(g.multi + multi.size());
+  g.jslot = g.kw + p.njobs;
+  g.chunks = g.jslot + p.njobs;
+  g.par = g.chunks + p.chunks.size();
+  hipStream_t s = ctx->stream;
+  if (!parts.empty())
+    CROC_HIP(hipMemcpyAsync(g.parts, parts.data(), sizeof(int4) * parts.size(),
+                            hipMemcpyHostToDevice, s));
+  if (!groups.empty())
+    CROC_HIP(hipMemcpyAsync(g.groups, groups.data(), sizeof(int4) * groups.size(),
+                            hipMemcpyHostToDevice, s));
+  if (!multi.empty())
+    CROC_HIP(hipMemcpyAsync(g.multi, multi.data(), sizeof(int2) * multi.size(),
+                            hipMemcpyHostToDevice, s));
+  int rc;
+  if ((rc = upload_i32(ctx, g.kw, hk))) return rc;
+  if ((rc = upload_i32(ctx, g.jslot, jslot))) return rc;
+  if ((rc = upload_i32(ctx, g.chunks, p.chunks))) return rc;
+  if ((rc = upload_i32(ctx, g.par, allpar))) return rc;
+  // the host vectors above end here: wait for their copies
+  CROC_HIP(hipStreamSynchronize(s));
+  p.gs_ready = true;
+  return CROC_OK;
+}
+
+// Build the degridding schedule of p and upload it into `set` (slot WS_GRIDQ).
+int tw_degrid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
+  DegridSched &g = p.ds;
+  g = DegridSched();
+  const TowerArgs &a = p.a;
+  g.jb = std::min(p.njobs, JOB_BATCH);
+  std::vector<int4> parts;
+  for (int j0 = 0; j0 < p.njobs; j0 += g.jb) {
+    g.poff.push_back(parts.size());
+    for (int b = 0; b < std::min(g.jb, p.njobs - j0); b++) {
+      const int4 job = p.hjobs[j0 + b];
+      for (int v = job.y; v < job.z; v += PART_VIS)
+        parts.push_back(make_int4(b, v, std::min(job.z, v + PART_VIS), 0));
+    }
+  }
+  g.poff.push_back(parts.size());
+  std::vector<int32_t> hk(p.njobs), hc(p.njobs);
+  for (int j = 0; j < p.njobs; j++) {
+    hk[j] = p.hjobs[j].x % a.nwp + a.wp_min;
+    hc[j] = p.hjobs[j].x / a.nwp;
+  }
+  const size_t bytes = sizeof(int4) * (parts.size() + 16) +
+                       sizeof(int32_t) * (2 * (size_t)p.njobs + p.chunks.size()) + 256;
+  char *d = (char *)ws_get_in(ctx, set, WS_GRIDQ, bytes);
+  if (!d) return CROC_ENOMEM;
+  g.parts = (int4 *)d;
+  g.kw = (int32_t *)(g.parts + parts.size() + 16);
+  g.jchunk = g.kw + p.njobs;
+  g.chunks = g.jchunk + p.njobs;
+  hipStream_t s = ctx->stream;
+  if (!parts.empty())
+    CROC_HIP(hipMemcpyAsync(g.parts, parts.data(), sizeof(int4) * parts.size(),
+                            hipMemcpyHostToDevice, s));
+  int rc;
+  if ((rc = upload_i32(ctx, g.kw, hk))) return rc;
+  if ((rc = upload_i32(ctx, g.jchunk, hc))) return rc;
+  if ((rc = upload_i32(ctx, g.chunks, p.chunks))) return rc;
+  CROC_HIP(hipStreamSynchronize(s));
+  p.ds_ready = true;
+  return CROC_OK;
+}
+
 int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int separable);
 int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
                   const void *kern, int separable);
@@ -1227,95 +1381,38 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
                      kern != nullptr, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if ((rc = tw_bin(ctx, p, (const float2 *)vis))) return rc;
+  if ((rc = tw_grid_sched(ctx, p, ctx->ws))) return rc;
   return tw_grid_run(ctx, p, grid, kern, separable);
 }
 
 namespace {
 
-// Imaging from binned jobs (p.svis holds the visibilities in record order).
+// Imaging from binned jobs (p.svis holds the visibilities in record order,
+// p.gs the schedule).  Enqueues kernels and event records only: no host
+// synchronisation, no allocation once the context's buffers have this
+// size, no memset.
 int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int separable) {
   int rc;
   if (p.njobs == 0) return CROC_OK;
+  CROC_REQUIRE(p.gs_ready, CROC_EINVAL, "gridding schedule missing");
   const TowerArgs &a = p.a;
+  const GridSched &gs = p.gs;
   hipStream_t s = ctx->stream;
 
-  // Work plan on the host: jobs are cut into parts of at most PART_VIS
-  // visibilities (at most JOB_BATCH / 4 parts; longer parts beyond that);
-  // batches hold whole jobs and at most JOB_BATCH part slots.  Per batch:
-  // parts (slot, first, end), the first slot of each job, the multi-part
-  // jobs (first slot, parts) and the chunk groups (chunk, first job, end job).
-  struct Batch {
-    int j0, nj;
-    size_t p0, np, m0, nm, g0, ng;
-  };
-  std::vector<Batch> batches;
-  std::vector<int4> parts, groups;
-  std::vector<int2> multi;
-  std::vector<int32_t> jslot(p.njobs);
-  int slots = 0, max_slots = 0;
-  for (int j = 0; j < p.njobs; j++) {
-    const int4 job = p.hjobs[j];
-    const int n = job.z - job.y;
-    const int np = std::min(std::max(1, (n + PART_VIS - 1) / PART_VIS), JOB_BATCH / 4);
-    if (batches.empty() || slots + np > JOB_BATCH) {
-      batches.push_back(Batch{j, 0, parts.size(), 0, multi.size(), 0, groups.size(), 0});
-      slots = 0;
-    }
-    Batch &bt = batches.back();
-    jslot[j] = slots;
-    const int len = (n + np - 1) / np;
-    for (int k = 0; k < np; k++)
-      parts.push_back(make_int4(slots + k, job.y + k * len, std::min(job.z, job.y + (k + 1) * len), 0));
-    if (np > 1) multi.push_back(make_int2(slots, np));
-    const int c = job.x / a.nwp, lj = j - bt.j0;
-    if (bt.ng == 0 || groups.back().x != c) {
-      groups.push_back(make_int4(c, lj, lj + 1, 0));
-      bt.ng++;
-    } else {
-      groups.back().z = lj + 1;
-    }
-    slots += np;
-    max_slots = std::max(max_slots, slots);
-    bt.nj++;
-    bt.np = parts.size() - bt.p0;
-    bt.nm = multi.size() - bt.m0;
-  }
-
-  // fixed-point scale, Fresnel phase table, per-job plane multipliers
+  // fixed-point scale, Fresnel phase table, a part queue per batch
   const int64_t Q = a.Q, S = a.S;
   const int64_t tsize = separable ? Q * S : Q * Q * S * S;  // elements per w-kernel plane
   const int64_t nk = kern ? tsize * a.nwk : 0;
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
-                                                sizeof(int32_t) * (5 * (size_t)p.njobs + 64) +
-                                                sizeof(uint32_t) * (batches.size() + 1));
+                                                sizeof(uint32_t) * (gs.batches.size() + 64));
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
   float2 *stack =
-      (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)max_slots);
-  char *plan = (char *)ws_get(ctx, WS_TILES, sizeof(int4) * (parts.size() + groups.size()) +
-                                                 sizeof(int2) * multi.size() + 256);
-  if (!aux || !stack || !plan) return CROC_ENOMEM;
+      (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)gs.max_slots);
+  if (!aux || !stack) return CROC_ENOMEM;
   TwScale *fs = (TwScale *)aux;
   double *phi = (double *)(aux + 256);
-  int32_t *kw = (int32_t *)(phi + SGN);
-  int32_t *dchunks = kw + p.njobs;
-  int32_t *djslot = dchunks + p.njobs;
-  uint32_t *qc = (uint32_t *)(kw + 5 * (size_t)p.njobs + 64);  // a part queue per batch
+  uint32_t *qc = (uint32_t *)(phi + SGN);
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
-  int4 *dparts = (int4 *)plan;
-  int4 *dgroups = dparts + parts.size();
-  int2 *dmulti = (int2 *)(dgroups + groups.size());
-  std::vector<int32_t> hk(p.njobs);
-  for (int j = 0; j < p.njobs; j++) hk[j] = p.hjobs[j].x % a.nwp + a.wp_min;
-  if ((rc = upload_i32(ctx, kw, hk))) return rc;
-  if ((rc = upload_i32(ctx, dchunks, p.chunks))) return rc;
-  if ((rc = upload_i32(ctx, djslot, jslot))) return rc;
-  CROC_HIP(hipMemcpyAsync(dparts, parts.data(), sizeof(int4) * parts.size(),
-                          hipMemcpyHostToDevice, s));
-  CROC_HIP(hipMemcpyAsync(dgroups, groups.data(), sizeof(int4) * groups.size(),
-                          hipMemcpyHostToDevice, s));
-  if (!multi.empty())
-    CROC_HIP(hipMemcpyAsync(dmulti, multi.data(), sizeof(int2) * multi.size(),
-                            hipMemcpyHostToDevice, s));
   if ((rc = zero_u32_async(ctx, (uint32_t *)fs, sizeof(TwScale) / 4))) return rc;
   k_tw_maxima<<<ctx->num_cu, 1024, 0, s>>>(p.svis, a.nv, (const float2 *)kern, nk, fs);
   k_tw_pick<<<1, 1, 0, s>>>(fs, a.nv, a.nearest, separable);
@@ -1332,64 +1429,57 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
   CROC_HIP(hipGetLastError());
   // zero the chunk images that will receive data
-  k_tw_zero_chunks<<<dim3(16, (unsigned)p.chunks.size()), 256, 0, s>>>(dchunks, cimg);
+  k_tw_zero_chunks<<<dim3(16, (unsigned)p.chunks.size()), 256, 0, s>>>(gs.chunks, cimg);
   CROC_HIP(hipGetLastError());
 
-  if ((rc = zero_u32_async(ctx, qc, (int64_t)batches.size()))) return rc;
+  if ((rc = zero_u32_async(ctx, qc, (int64_t)gs.batches.size()))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[0], s));
   tap_timer_reset(ctx, 0);
-  for (size_t bi = 0; bi < batches.size(); bi++) {
-    const Batch &bt = batches[bi];
+  for (size_t bi = 0; bi < gs.batches.size(); bi++) {
+    const GridSched::Batch &bt = gs.batches[bi];
     const int G = (int)std::min<size_t>(bt.np, (size_t)ctx->num_cu);  // one per CU (LDS)
     if ((rc = tap_timer_begin(ctx, 0))) return rc;
     if (ksplit)
-      k_tw_grid_jobs_mfma<<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
+      k_tw_grid_jobs_mfma<<<G, TW_NT, 0, s>>>(gs.parts + bt.p0, (int)bt.np, p.rec, p.svis,
                                               (const float2 *)kern, ksplit, a.S, tsize, fs, sub,
                                               qc + bi);
     else if (a.nearest || !separable)
-      k_tw_grid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
+      k_tw_grid_jobs<false><<<G, TW_NT, 0, s>>>(gs.parts + bt.p0, (int)bt.np, p.rec, p.svis,
                                                 (const float2 *)kern, a.Q, a.S, tsize,
                                                 a.nearest, fs, sub, qc + bi);
     else
-      k_tw_grid_jobs<true><<<G, TW_NT, 0, s>>>(dparts + bt.p0, (int)bt.np, p.rec, p.svis,
+      k_tw_grid_jobs<true><<<G, TW_NT, 0, s>>>(gs.parts + bt.p0, (int)bt.np, p.rec, p.svis,
                                                (const float2 *)kern, a.Q, a.S, tsize,
                                                a.nearest, fs, sub, qc + bi);
     CROC_HIP(hipGetLastError());
     if ((rc = tap_timer_end(ctx, 0))) return rc;
     if (bt.nm) {
-      k_tw_sum_parts<<<dim3(ACC_SLICES, (unsigned)bt.nm), 256, 0, s>>>(dmulti + bt.m0, sub);
+      k_tw_sum_parts<<<dim3(ACC_SLICES, (unsigned)bt.nm), 256, 0, s>>>(gs.multi + bt.m0, sub);
       CROC_HIP(hipGetLastError());
     }
     // image = ifft(sub[slot]) * exp(-i wp phi), in place in each job's slot
     rc = fft2d_launch(ctx, sub, sub, SG, bt.nj, SGN, +1, 1.0f / (float)SGN, nullptr, nullptr, 0,
-                      phi, kw + bt.j0, -1, djslot + bt.j0, djslot + bt.j0);
+                      phi, gs.kw + bt.j0, -1, gs.jslot + bt.j0, gs.jslot + bt.j0);
     if (rc) return rc;
     k_tw_chunk_accum<<<dim3(ACC_SLICES, (unsigned)bt.ng), 256, 0, s>>>(
-        dgroups + bt.g0, djslot + bt.j0, sub, cimg);
+        gs.groups + bt.g0, gs.jslot + bt.j0, sub, cimg);
     CROC_HIP(hipGetLastError());
   }
   // chunk sub-grids = fft(chunk image), then parity-ordered adds
   rc = fft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, -1, 1.0f, nullptr,
-                    nullptr, 0, nullptr, nullptr, 0, dchunks, dchunks);
+                    nullptr, 0, nullptr, nullptr, 0, gs.chunks, gs.chunks);
   if (rc) return rc;
-  // parity classes, uploaded once, one launch each (stream order keeps
-  // overlapping margins of different classes apart)
-  std::vector<int32_t> par[4], allpar;
-  for (int c : p.chunks) par[(c % a.ncx) % 2 + 2 * ((c / a.ncx) % 2)].push_back(c);
-  for (int q = 0; q < 4; q++) allpar.insert(allpar.end(), par[q].begin(), par[q].end());
-  int32_t *dpar = djslot + p.njobs;  // chunks <= njobs entries
-  if ((rc = upload_i32(ctx, dpar, allpar))) return rc;
   size_t po = 0;
   for (int q = 0; q < 4; q++) {
-    if (par[q].empty()) continue;
-    dim3 g(16, (unsigned)par[q].size());
-    k_tw_add<<<g, 256, 0, s>>>(dpar + po, cimg, (float2 *)grid, a.N, a.chunk, a.margin, a.ncx);
+    if (gs.npar[q] == 0) continue;
+    dim3 g(16, (unsigned)gs.npar[q]);
+    k_tw_add<<<g, 256, 0, s>>>(gs.par + po, cimg, (float2 *)grid, a.N, a.chunk, a.margin,
+                               a.ncx);
     CROC_HIP(hipGetLastError());
-    po += par[q].size();
+    po += gs.npar[q];
   }
   CROC_HIP(hipEventRecord(ctx->ev[1], s));
   ctx->ev_grid = true;
-  CROC_HIP(hipStreamSynchronize(s));
   return CROC_OK;
 }
 
@@ -1410,44 +1500,30 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
                      kern != nullptr, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if ((rc = tw_bin(ctx, p, nullptr))) return rc;
+  if ((rc = tw_degrid_sched(ctx, p, ctx->ws))) return rc;
   return tw_degrid_run(ctx, p, grid, vis_out, kern, separable);
 }
 
 namespace {
 
-// Predict from binned jobs.
+// Predict from binned jobs (p.ds the schedule).  Enqueues kernels and event
+// records only, like tw_grid_run.
 int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
                   const void *kern, int separable) {
   int rc;
   if (p.njobs == 0) return CROC_OK;
+  CROC_REQUIRE(p.ds_ready, CROC_EINVAL, "degridding schedule missing");
   const TowerArgs &a = p.a;
+  const DegridSched &ds = p.ds;
   hipStream_t s = ctx->stream;
-  // batches of at most JOB_BATCH jobs (job b of a batch in slot b); every
-  // job cut into parts of at most PART_VIS visibilities sharing its slot
-  const int jb = std::min(p.njobs, JOB_BATCH);
-  std::vector<int4> parts;
-  std::vector<size_t> poff;
-  for (int j0 = 0; j0 < p.njobs; j0 += jb) {
-    poff.push_back(parts.size());
-    for (int b = 0; b < std::min(jb, p.njobs - j0); b++) {
-      const int4 job = p.hjobs[j0 + b];
-      for (int v = job.y; v < job.z; v += PART_VIS)
-        parts.push_back(make_int4(b, v, std::min(job.z, v + PART_VIS), 0));
-    }
-  }
-  poff.push_back(parts.size());
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
-                                                sizeof(int32_t) * (3 * (size_t)p.njobs + 64) +
-                                                sizeof(uint32_t) * poff.size());
+                                                sizeof(uint32_t) * (ds.poff.size() + 64));
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
-  float2 *stack = (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)jb);
-  int4 *dparts = (int4 *)ws_get(ctx, WS_TILES, sizeof(int4) * (parts.size() + 16));
-  if (!aux || !stack || !dparts) return CROC_ENOMEM;
+  float2 *stack =
+      (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)ds.jb);
+  if (!aux || !stack) return CROC_ENOMEM;
   double *phi = (double *)(aux + 256);
-  int32_t *kw = (int32_t *)(phi + SGN);
-  int32_t *jchunk = kw + p.njobs;
-  int32_t *dchunks = jchunk + p.njobs;
-  uint32_t *qc = (uint32_t *)(kw + 3 * (size_t)p.njobs + 64);  // a part queue per batch
+  uint32_t *qc = (uint32_t *)(phi + SGN);  // a part queue per batch
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
   const int64_t tsize = separable ? (int64_t)a.Q * a.S : (int64_t)a.Q * a.Q * a.S * a.S;
   // Separable kernels of support <= 16 degrid on the matrix cores
@@ -1465,45 +1541,36 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
                                                                   ksplit);
     CROC_HIP(hipGetLastError());
   }
-  std::vector<int32_t> hk(p.njobs), hc(p.njobs);
-  for (int j = 0; j < p.njobs; j++) {
-    hk[j] = p.hjobs[j].x % a.nwp + a.wp_min;
-    hc[j] = p.hjobs[j].x / a.nwp;
-  }
-  if ((rc = upload_i32(ctx, kw, hk))) return rc;
-  if ((rc = upload_i32(ctx, jchunk, hc))) return rc;
-  if ((rc = upload_i32(ctx, dchunks, p.chunks))) return rc;
-  CROC_HIP(hipMemcpyAsync(dparts, parts.data(), sizeof(int4) * parts.size(),
-                          hipMemcpyHostToDevice, s));
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
-  if ((rc = zero_u32_async(ctx, qc, (int64_t)poff.size()))) return rc;
+  if ((rc = zero_u32_async(ctx, qc, (int64_t)ds.poff.size()))) return rc;
   dim3 g(16, (unsigned)p.chunks.size());
   CROC_HIP(hipEventRecord(ctx->ev[2], s));
   tap_timer_reset(ctx, 1);
-  k_tw_cut<<<g, 256, 0, s>>>(dchunks, (const float2 *)grid, cimg, a.N, a.chunk, a.margin, a.ncx);
+  k_tw_cut<<<g, 256, 0, s>>>(ds.chunks, (const float2 *)grid, cimg, a.N, a.chunk, a.margin,
+                             a.ncx);
   CROC_HIP(hipGetLastError());
   rc = fft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, +1, 1.0f / (float)SGN,
-                    nullptr, nullptr, 0, nullptr, nullptr, 0, dchunks, dchunks);
+                    nullptr, nullptr, 0, nullptr, nullptr, 0, ds.chunks, ds.chunks);
   if (rc) return rc;
-  for (int j0 = 0, bi = 0; j0 < p.njobs; j0 += jb, bi++) {
-    const int nb = std::min(jb, p.njobs - j0);
+  for (int j0 = 0, bi = 0; j0 < p.njobs; j0 += ds.jb, bi++) {
+    const int nb = std::min(ds.jb, p.njobs - j0);
     // plane sub-grid = fft(chunk image * exp(+i wp phi))
-    rc = fft2d_launch(ctx, cimg, sub, SG, nb, SGN, -1, 1.0f, phi, kw + j0, +1, nullptr, nullptr,
-                      0, jchunk + j0, nullptr);
+    rc = fft2d_launch(ctx, cimg, sub, SG, nb, SGN, -1, 1.0f, phi, ds.kw + j0, +1, nullptr, nullptr,
+                      0, ds.jchunk + j0, nullptr);
     if (rc) return rc;
-    const int npart = (int)(poff[bi + 1] - poff[bi]);
+    const int npart = (int)(ds.poff[bi + 1] - ds.poff[bi]);
     const int G = std::min(npart, ctx->num_cu);  // one per CU (LDS)
     if ((rc = tap_timer_begin(ctx, 1))) return rc;
     if (ksplit)
-      k_tw_degrid_jobs_mfma<<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
+      k_tw_degrid_jobs_mfma<<<G, TW_NT, 0, s>>>(ds.parts + ds.poff[bi], npart, p.rec, sub,
                                                 (const float2 *)kern, ksplit, a.S, tsize, fs,
                                                 (float2 *)vis_out, qc + bi);
     else if (a.nearest || !separable)
-      k_tw_degrid_jobs<false><<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
+      k_tw_degrid_jobs<false><<<G, TW_NT, 0, s>>>(ds.parts + ds.poff[bi], npart, p.rec, sub,
                                                   (const float2 *)kern, a.Q, a.S, tsize,
                                                   a.nearest, (float2 *)vis_out, qc + bi);
     else
-      k_tw_degrid_jobs<true><<<G, TW_NT, 0, s>>>(dparts + poff[bi], npart, p.rec, sub,
+      k_tw_degrid_jobs<true><<<G, TW_NT, 0, s>>>(ds.parts + ds.poff[bi], npart, p.rec, sub,
                                                  (const float2 *)kern, a.Q, a.S, tsize,
                                                  a.nearest, (float2 *)vis_out, qc + bi);
     CROC_HIP(hipGetLastError());
@@ -1511,7 +1578,6 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
   }
   CROC_HIP(hipEventRecord(ctx->ev[3], s));
   ctx->ev_degrid = true;
-  CROC_HIP(hipStreamSynchronize(s));
   return CROC_OK;
 }
 
@@ -1590,6 +1656,8 @@ extern "C" int croc_wtowers_plan_bin(croc_ctx *ctx, croc_tw_plan *plan, int N,
                      has_kernel != 0, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if (nv > 0 && (rc = tw_bin(ctx, p, nullptr))) return rc;
+  if (nv > 0 && (rc = tw_grid_sched(ctx, p, plan->ws))) return rc;
+  if (nv > 0 && (rc = tw_degrid_sched(ctx, p, plan->ws))) return rc;
   plan->ready = true;
   return CROC_OK;
 }

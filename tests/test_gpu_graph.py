@@ -81,3 +81,53 @@ def test_autotune_keeps_results(gpu):
         assert torch.equal(a, b)
     finally:
         _lib.set_tuning()
+
+
+@pytest.mark.parametrize('kind', ['separable-set', 'nearest', '2d'])
+def test_wtowers_plan_calls_replay_in_a_graph(gpu, kind):
+    """W-towers planned imaging and predict keep their job schedule on the
+    device from binning on, so a call enqueues kernels and event records
+    only: captured in a graph, every replay equals the eager result bit for
+    bit (the matrix-core separable path, nearest cell and 2D kernels)."""
+    from test_gpu_wtowers import LAM, MARGIN, SG, THETA, case, kernel_sets, kernels
+    rng, uvw, vis = case(21, 20000, 400.0)
+    if kind == 'separable-set':
+        kv = kernel_sets(rng, 40.0)['separable-set'][0]
+    else:
+        kv = kernels(rng)[kind]
+        kv = None if kv is None else torch.as_tensor(kv, device=gpu).to(torch.complex64)
+    uvw_d = torch.as_tensor(uvw, device=gpu)
+    vis_d = torch.as_tensor(vis, device=gpu).to(torch.complex64)
+    grid_in = torch.as_tensor(rng.standard_normal((512, 512)) +
+                              1j * rng.standard_normal((512, 512)), device=gpu).to(torch.complex64)
+    plan = G.WTowersPlan(THETA, LAM, uvw_d, 40.0, SG, MARGIN, kv)
+    out = {}
+
+    def body():
+        out['g'] = plan.imaging(vis_d, kv)
+        out['v'] = plan.predict(grid_in, kv)
+
+    body()
+    torch.cuda.synchronize()
+    ref_g, ref_v = out['g'].clone(), out['v'].clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        body()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    assert torch.equal(out['g'], ref_g) and torch.equal(out['v'], ref_v)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        body()
+    for rep in range(3):
+        out['g'].fill_(1.0)
+        out['v'].fill_(2.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out['g'], ref_g), 'grid differs on replay %d' % rep
+        assert torch.equal(out['v'], ref_v), 'visibilities differ on replay %d' % rep
+        # eager work of the program between replays (another context)
+        G.w_towers_imaging(THETA, LAM, uvw_d, None, vis_d, 40.0, SG, MARGIN, kv)
+        (vis_d * 2).sum().item()
