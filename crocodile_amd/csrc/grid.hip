@@ -33,12 +33,13 @@ struct ScaleState {
   // gridding work-list lengths: dense tiles' range items per XCD queue, then
   // the sparse tiles' items
   unsigned int qcount[XCD_QUEUES + 1];
-  unsigned int pad;
+  unsigned int nonfinite;     // bit 0: a visibility, bit 1: a kernel value is NaN or infinite
 };
 
 
 
-// Grid-stride maxima of the visibility and kernel component magnitudes.
+// Grid-stride maxima of the finite visibility and kernel component
+// magnitudes; NaN or infinite values are flagged in st->nonfinite instead.
 // The bank is scanned as one extent [0, max_k(plane_off[k] + table size)),
 // which the C ABI requires to be a single allocation.  Non-negative floats
 // order like their bit patterns, so atomicMax on the bits is a float max.
@@ -51,8 +52,10 @@ __global__ void k_scale_maxima(const float2 *__restrict__ vis, int64_t nv,
   const int64_t nk = bank_extent(plane_off, plane_s, nplanes, Q, separable, &s_nk);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  block_max_to(array_max(vis, nv, i0, stride), &st->max_vis_bits);
-  block_max_to(array_max(kern, nk, i0, stride), &st->max_k_bits);
+  bool bv = false, bk = false;
+  block_max_to(array_max(vis, nv, i0, stride, bv), &st->max_vis_bits);
+  block_max_to(array_max(kern, nk, i0, stride, bk), &st->max_k_bits);
+  if (bv || bk) atomicOr(&st->nonfinite, (bv ? 1u : 0u) | (bk ? 2u : 0u));
 }
 
 // 2^e with (nv + 1) * c * 2^e < 2^62 (no sum overflows) and c * 2^e < 2^50
@@ -274,6 +277,38 @@ __global__ void k_grid_finish(float2 *__restrict__ grid,
   }
 }
 
+// Non-finite values, as synthesis.py:479-480 computes them in FP64: a NaN or
+// infinite visibility makes every cell of its window non-finite, a NaN or
+// infinite tap the cell it lands on.  The fixed-point pass cannot hold them
+// (their contributions are meaningless there), so those cells are set to
+// NaN afterwards.  Runs only when k_scale_maxima flagged a value.
+__global__ void k_grid_nonfinite(float2 *__restrict__ grid, int N,
+                                 const uint4 *__restrict__ rec, int64_t n,
+                                 const float2 *__restrict__ vis,
+                                 const float2 *__restrict__ kern, int separable,
+                                 const ScaleState *__restrict__ st) {
+  const unsigned flags = st->nonfinite;
+  if (flags == 0) return;
+  const float nan = __builtin_nanf("");
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint4 r = rec[i];
+    const bool vbad = !finite2(vis[r.w]);
+    if (!vbad && !(flags & 2u)) continue;
+    const int S = (int)(r.y & 0xffu);
+    const int xf = (int)((r.y >> 8) & 0xfffu), yf = (int)(r.y >> 20);
+    const int x0 = (int)(r.x & 0xffffu), y0 = (int)(r.x >> 16);
+    const float2 *kp = kern + r.z;
+    for (int y = 0; y < S; ++y) {
+      for (int x = 0; x < S; ++x) {
+        const bool bad = vbad || (separable ? !(finite2(kp[yf * S + y]) && finite2(kp[xf * S + x]))
+                                            : !finite2(kp[y * S + x]));
+        if (bad) grid[(int64_t)(y0 + y) * N + x0 + x] = make_float2(nan, nan);
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // Gridding of binned visibilities into grid (record w: index into vis).  No
@@ -341,6 +376,8 @@ int grid_binned(croc_ctx *ctx, const BinPlan &plan, void *grid, int N,
   ctx->ev_grid = true;
   k_grid_finish<<<(unsigned)((cells + 255) / 256), 256, 0, ctx->stream>>>(
       (float2 *)grid, (const long long *)acc, cells, st);
+  k_grid_nonfinite<<<ctx->num_cu, 256, 0, ctx->stream>>>(
+      (float2 *)grid, N, plan.rec, nv, (const float2 *)vis, kt, separable, st);
   CROC_HIP(hipGetLastError());
   return CROC_OK;
 }

@@ -317,14 +317,24 @@ __device__ __forceinline__ void store_stream(float2 *p, float2 v) {
   __builtin_nontemporal_store(w, (croc_v2f *)p);
 }
 
-__device__ __forceinline__ float cabs_max(float2 a) {
-  return fmaxf(fabsf(a.x), fabsf(a.y));
+// max(|re|, |im|) of a finite value; 0 and `bad` set for a NaN or infinite
+// component (NaN fails every comparison).
+__device__ __forceinline__ float cabs_max(float2 a, bool &bad) {
+  const float x = fabsf(a.x), y = fabsf(a.y);
+  const bool fin = x <= 3.402823466e38f && y <= 3.402823466e38f;
+  bad |= !fin;
+  return fin ? fmaxf(x, y) : 0.f;
 }
 
-// max |re|, |im| of a[i0], a[i0 + stride], ...: eight independent loads in
-// flight per thread per round (the scan is latency-bound otherwise).
+__device__ __forceinline__ bool finite2(float2 a) {
+  return fabsf(a.x) <= 3.402823466e38f && fabsf(a.y) <= 3.402823466e38f;
+}
+
+// max |re|, |im| of the finite values among a[i0], a[i0 + stride], ...
+// (`bad` set when one is not finite): eight independent loads in flight per
+// thread per round (the scan is latency-bound otherwise).
 __device__ __forceinline__ float array_max(const float2 *__restrict__ a, int64_t n, int64_t i0,
-                                           int64_t stride) {
+                                           int64_t stride, bool &bad) {
   float m[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int64_t i = i0;
   for (; i + 7 * stride < n; i += 8 * stride) {
@@ -332,9 +342,9 @@ __device__ __forceinline__ float array_max(const float2 *__restrict__ a, int64_t
 #pragma unroll
     for (int k = 0; k < 8; k++) x[k] = a[i + k * stride];
 #pragma unroll
-    for (int k = 0; k < 8; k++) m[k] = fmaxf(m[k], cabs_max(x[k]));
+    for (int k = 0; k < 8; k++) m[k] = fmaxf(m[k], cabs_max(x[k], bad));
   }
-  for (; i < n; i += stride) m[0] = fmaxf(m[0], cabs_max(a[i]));
+  for (; i < n; i += stride) m[0] = fmaxf(m[0], cabs_max(a[i], bad));
 #pragma unroll
   for (int k = 1; k < 8; k++) m[0] = fmaxf(m[0], m[k]);
   return m[0];

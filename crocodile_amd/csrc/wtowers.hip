@@ -201,14 +201,18 @@ __global__ void k_tw_jobs(const uint32_t *__restrict__ ukeys, const uint32_t *__
 struct TwScale {
   unsigned int max_vis_bits, max_k_bits;  // float bits of max |re|, |im|
   double scale, inv_scale;                // 2^e, 2^-e
+  unsigned int nonfinite;  // bit 0: a visibility, bit 1: a kernel value is NaN or infinite
+  unsigned int pad;
 };
 
 __global__ void k_tw_maxima(const float2 *__restrict__ vis, int64_t nv,
                             const float2 *__restrict__ kern, int64_t nk, TwScale *fs) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  block_max_to(array_max(vis, nv, i0, stride), &fs->max_vis_bits);
-  block_max_to(array_max(kern, nk, i0, stride), &fs->max_k_bits);
+  bool bv = false, bk = false;
+  block_max_to(array_max(vis, nv, i0, stride, bv), &fs->max_vis_bits);
+  block_max_to(array_max(kern, nk, i0, stride, bk), &fs->max_k_bits);
+  if (bv || bk) atomicOr(&fs->nonfinite, (bv ? 1u : 0u) | (bk ? 2u : 0u));
 }
 
 // 2^e with (nv + 1) * c * 2^e < 2^62 (no sum overflows) and c * 2^e < 2^50
@@ -722,6 +726,51 @@ __global__ void k_tw_add(const int32_t *__restrict__ chunks, const float2 *__res
     g.x += s[i].x;
     g.y += s[i].y;
     grid[(int64_t)y * N + x] = g;
+  }
+}
+
+// Non-finite values: a NaN or infinite visibility makes its job's sub-grid
+// image non-finite after the inverse transform, and with it the chunk's
+// stacked image and the chunk's whole sub-grid region of the grid
+// (wtowers.ipynb imaging cell, grid.c:604-678, in FP64).  The passes below
+// mark such chunks and set their regions to NaN after the adds; a
+// non-finite kernel value marks every chunk.  They run only when
+// k_tw_maxima flagged a value.
+__global__ void k_tw_nonfinite_mark(const int4 *__restrict__ jobs, int njobs, int nwp,
+                                    const float2 *__restrict__ svis,
+                                    const TwScale *__restrict__ fs, uint32_t *__restrict__ cflag) {
+  const unsigned flags = fs->nonfinite;
+  if (flags == 0) return;
+  for (int j = blockIdx.x; j < njobs; j += gridDim.x) {
+    const int4 job = jobs[j];
+    const int c = job.x / nwp;
+    if (flags & 2u) {
+      if (threadIdx.x == 0) cflag[c] = 1u;
+      continue;
+    }
+    for (int i = job.y + (int)threadIdx.x; i < job.z; i += blockDim.x) {
+      if (!finite2(svis[i])) {
+        cflag[c] = 1u;
+        break;
+      }
+    }
+  }
+}
+
+__global__ void k_tw_nonfinite_fill(const int32_t *__restrict__ chunks,
+                                    const uint32_t *__restrict__ cflag, float2 *__restrict__ grid,
+                                    int N, int chunk, int margin, int ncx,
+                                    const TwScale *__restrict__ fs) {
+  if (fs->nonfinite == 0) return;
+  const int c = chunks[blockIdx.y];
+  if (!cflag[c]) return;
+  const int cx = c % ncx, cy = c / ncx;
+  const int x0 = cx * chunk - margin / 2, y0 = cy * chunk - margin / 2;
+  const float nan = __builtin_nanf("");
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < SGN; i += gridDim.x * blockDim.x) {
+    const int y = y0 + i / SG, x = x0 + i % SG;
+    if (x < 0 || y < 0 || x >= N || y >= N) continue;
+    grid[(int64_t)y * N + x] = make_float2(nan, nan);
   }
 }
 
@@ -1404,7 +1453,8 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
   const int64_t tsize = separable ? Q * S : Q * Q * S * S;  // elements per w-kernel plane
   const int64_t nk = kern ? tsize * a.nwk : 0;
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
-                                                sizeof(uint32_t) * (gs.batches.size() + 64));
+                                                sizeof(uint32_t) * (gs.batches.size() + 64) +
+                                                sizeof(uint32_t) * (size_t)a.ncx * a.ncx);
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
   float2 *stack =
       (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)gs.max_slots);
@@ -1412,6 +1462,7 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
   TwScale *fs = (TwScale *)aux;
   double *phi = (double *)(aux + 256);
   uint32_t *qc = (uint32_t *)(phi + SGN);
+  uint32_t *cflag = qc + gs.batches.size() + 64;  // chunks to set non-finite
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
   if ((rc = zero_u32_async(ctx, (uint32_t *)fs, sizeof(TwScale) / 4))) return rc;
   k_tw_maxima<<<ctx->num_cu, 1024, 0, s>>>(p.svis, a.nv, (const float2 *)kern, nk, fs);
@@ -1478,6 +1529,13 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
     CROC_HIP(hipGetLastError());
     po += gs.npar[q];
   }
+  // chunks of non-finite values (only when flagged)
+  if ((rc = zero_u32_async(ctx, cflag, (int64_t)a.ncx * a.ncx))) return rc;
+  k_tw_nonfinite_mark<<<(unsigned)std::min(p.njobs, 4 * ctx->num_cu), 256, 0, s>>>(
+      p.jobs, p.njobs, a.nwp, p.svis, fs, cflag);
+  k_tw_nonfinite_fill<<<dim3(16, (unsigned)p.chunks.size()), 256, 0, s>>>(
+      gs.chunks, cflag, (float2 *)grid, a.N, a.chunk, a.margin, a.ncx, fs);
+  CROC_HIP(hipGetLastError());
   CROC_HIP(hipEventRecord(ctx->ev[1], s));
   ctx->ev_grid = true;
   return CROC_OK;
