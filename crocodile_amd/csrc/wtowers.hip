@@ -479,37 +479,61 @@ __global__ __launch_bounds__(TW_NT) void k_tw_grid_jobs_mfma(
       const int lo = min(n, wave * per), hi = min(n, lo + per);  // this wavefront's range
       s_cnt[tid >> 6][tid & 63] = 0;  // MG_NW x MG_NB = TW_NT counters
       __syncthreads();
-      // 1a. records and bucket counts of the wavefront's range
-      for (int j = lo + lane; j < hi; j += 64) {
-        const uint4 r = rec[jb + j];
-        const uint32_t yf = r.y >> 16;
-        s_rec[j] = make_uint2(r.x, r.z * (uint32_t)tsize + yf * (uint32_t)S);
-        const int key = (int)((r.x >> 20) << 3) + (int)((r.x & 0xffffu) >> 4);  // (wy/16, wx/16)
-        s_key[j] = (uint8_t)key;
-        atomicAdd(&s_cnt[wave][key], 1);
-      }
-      // 1b. B rows, 16 lanes per visibility
-      for (int e = tid; e < 16 * n; e += TW_NT) {
-        const int j = e >> 4, c = e & 15;
-        uint2 o = make_uint2(0u, 0u);
-        if (c < S) {
-          const uint4 r = rec[jb + j];
-          const float2 v = svis[jb + j];
-          const uint32_t xf = r.y & 0xffffu;
-          const float2 k = kern[(int64_t)r.z * tsize + xf * S + c];
-          const float br = (v.x * k.x - v.y * k.y) * (sv * sk);
-          const float bi = (v.x * k.y + v.y * k.x) * (sv * sk);
-          union {
-            _Float16 h[4];
-            uint2 u;
-          } q;
-          q.h[0] = (_Float16)br;
-          q.h[1] = (_Float16)(br - (float)q.h[0]);
-          q.h[2] = (_Float16)bi;
-          q.h[3] = (_Float16)(bi - (float)q.h[2]);
-          o = q.u;
+      // 1. per 64 visibilities of the wavefront's range: every lane loads one
+      // record and value (records, bucket counts), then the B rows are
+      // built 16 lanes per visibility, the table rows' loads of eight
+      // visibilities issued back to back (offsets and values taken from the
+      // owner lane), so each batch waits for memory twice, not per row.
+      {
+        const int c = lane & 15, sub = lane >> 4;
+        const int cs = min(c, S - 1);
+        for (int j0 = lo; j0 < hi; j0 += 64) {
+          const int j = j0 + lane;
+          const int cnt = min(64, hi - j0);
+          uint4 r = make_uint4(0u, 0u, 0u, 0u);
+          float2 v = make_float2(0.f, 0.f);
+          if (j < hi) {
+            r = rec[jb + j];
+            v = svis[jb + j];
+            const uint32_t yf = r.y >> 16;
+            s_rec[j] = make_uint2(r.x, r.z * (uint32_t)tsize + yf * (uint32_t)S);
+            const int key = (int)((r.x >> 20) << 3) + (int)((r.x & 0xffffu) >> 4);  // (wy/16, wx/16)
+            s_key[j] = (uint8_t)key;
+            atomicAdd(&s_cnt[wave][key], 1);
+          }
+          const uint32_t kxo = r.z * (uint32_t)tsize + (r.y & 0xffffu) * (uint32_t)S;
+          for (int st = 0; st < 16; st += 8) {
+            float2 kv[8], vv[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+              const int src = 4 * (st + q) + sub;  // owner lane of visibility j0 + src
+              const uint32_t o = (uint32_t)__shfl((int)kxo, src, 64);
+              vv[q] = make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+              kv[q] = kern[o + (uint32_t)cs];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+              const int src = 4 * (st + q) + sub;
+              if (src >= cnt) continue;
+              uint2 o = make_uint2(0u, 0u);
+              if (c < S) {
+                const float2 k = kv[q], w = vv[q];
+                const float br = (w.x * k.x - w.y * k.y) * (sv * sk);
+                const float bi = (w.x * k.y + w.y * k.x) * (sv * sk);
+                union {
+                  _Float16 h[4];
+                  uint2 u;
+                } h;
+                h.h[0] = (_Float16)br;
+                h.h[1] = (_Float16)(br - (float)h.h[0]);
+                h.h[2] = (_Float16)bi;
+                h.h[3] = (_Float16)(bi - (float)h.h[2]);
+                o = h.u;
+              }
+              s_bv[j0 + src][c] = o;
+            }
+          }
         }
-        s_bv[j][c] = o;
       }
       barrier_after_lds_atomics();
       // 2. bucket starts (exclusive scan over buckets) and wavefront offsets
