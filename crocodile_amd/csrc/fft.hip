@@ -135,7 +135,7 @@ __device__ __forceinline__ void pass_b(const float2 *lds, int line_stride, int e
 // input straight from memory and the columns' pass B writes straight to
 // memory, so the sub-grid crosses LDS six times (three passes in place).
 // The LDS sub-grid allows one workgroup per compute unit, so the workgroups
-// are persistent (sub-grids b = blockIdx.x, + gridDim.x, ...) and load the
+// are persistent (each takes a contiguous run of sub-grids) and load the
 // next sub-grid's rows into registers while they transform the current one.
 // Every sub-grid is read before it is written, by the one workgroup that
 // transforms it, so in-place batches (in == out, injective indices) are
@@ -161,13 +161,21 @@ __global__ __launch_bounds__(FFT_NT) void k_fft2d_128(
   // share the parity of y + j2 (the (-1)^(x + y) centring sign).
   const int y = tid >> 3, j2 = tid & 7;
   float2 nxt[16];  // the thread's row elements of the workgroup's next sub-grid
-  int64_t b = blockIdx.x;
-  if (b < batch) {
+  // Each workgroup takes a contiguous run of the batch.  When consecutive
+  // sub-grids read the same input (in_idx equal, input and output distinct
+  // buffers: the per-plane transforms of one chunk image), the rows already
+  // in registers are reused instead of read again.
+  const int64_t per = (batch + gridDim.x - 1) / gridDim.x;
+  const int64_t bbeg = (int64_t)blockIdx.x * per;
+  const int64_t bend = bbeg + per < batch ? bbeg + per : batch;
+  const bool reuse = in_idx != nullptr && (const void *)in != (const void *)out;
+  int64_t b = bbeg;
+  if (b < bend) {
     const float2 *row = in + (int64_t)(in_idx ? in_idx[b] : b) * batch_stride + y * FN + j2;
 #pragma unroll
     for (int j1 = 0; j1 < 16; j1++) nxt[j1] = row[8 * j1];
   }
-  for (; b < batch; b += gridDim.x) {
+  for (; b < bend; ++b) {
     float2 *Y = out + (int64_t)(out_idx ? out_idx[b] : b) * batch_stride;
     // An offset the compiler cannot see through (an empty asm, no code)
     // keeps the twiddle and phase-table loads inside the loop: hoisted out,
@@ -201,8 +209,8 @@ __global__ __launch_bounds__(FFT_NT) void k_fft2d_128(
         p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
       }
       // the next sub-grid's rows, in flight during the remaining passes
-      const int64_t bn = b + gridDim.x;
-      if (bn < batch) {
+      const int64_t bn = b + 1;
+      if (bn < bend && !(reuse && in_idx[bn] == in_idx[b])) {
         const float2 *row =
             in + (int64_t)(in_idx ? in_idx[bn] : bn) * batch_stride + y * FN + j2;
 #pragma unroll
