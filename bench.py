@@ -344,12 +344,15 @@ def run_towers(args, torch, G, W, _lib, dev):
     # timed steps: 8 FP32 flop per kernel tap (one complex multiply-add),
     # S^2 taps per visibility and direction
     taps = float(nv) * S * S
-    # separable sets of support <= 16 run on the matrix-core job kernels
-    sfx = '_mfma' if S <= 16 else ''
+    # separable sets of support <= 16: gridding on the matrix cores
+    # (k_tw_grid_jobs_mfma), degridding one visibility per thread
+    # (k_tw_degrid_sep16)
+    gname = 'k_tw_grid_jobs_mfma' if S <= 16 else 'k_tw_grid_jobs'
+    dname = 'k_tw_degrid_sep16' if S <= 16 else 'k_tw_degrid_jobs'
     if res['grid_tap_ms'] >= res['degrid_tap_ms']:
-        kname, kms, kn = 'k_tw_grid_jobs' + sfx, res['grid_tap_ms'], res['grid_launches']
+        kname, kms, kn = gname, res['grid_tap_ms'], res['grid_launches']
     else:
-        kname, kms, kn = 'k_tw_degrid_jobs' + sfx, res['degrid_tap_ms'], res['degrid_launches']
+        kname, kms, kn = dname, res['degrid_tap_ms'], res['degrid_launches']
     achieved = 8.0 * taps / (kms * 1e-3) / 1e12
     # algorithmic HBM bytes of a job kernel: each visibility's 16-byte
     # record and 8-byte value once, each (chunk, w-plane) job's 128^2
@@ -359,13 +362,16 @@ def run_towers(args, torch, G, W, _lib, dev):
         'bound': 'valu', 'achieved': round(achieved, 3), 'peak': FP32_PEAK_TFLOPS,
         'unit': 'TFLOP/s', 'frac': round(achieved / FP32_PEAK_TFLOPS, 5),
         'bound_note': 'complex multiply-add per kernel tap, 8 FP32 flop/tap (the reference '
-                      'convention), against the FP32 vector peak: the job kernels form the '
-                      'products on the matrix cores (fp16 hi/lo split, FP32 accumulation) but '
-                      'are bound by the vector work around them (counters: MFMA busy ~5 %)',
+                      'convention), against the FP32 vector peak: the gridding job kernel '
+                      'forms the products on the matrix cores (fp16 hi/lo split, FP32 '
+                      'accumulation) and is bound by the vector work and memory latency '
+                      'around them (counters: MFMA busy ~6 %, waves waiting ~2/3 of their '
+                      'cycles); the degridding job kernel runs one visibility per thread on '
+                      'the vector ALUs from the sub-grid staged in LDS',
         'kernel': kname, 'launches_per_step': kn,
         'kernel_ms_per_launch': round(kms / max(kn, 1), 4),
-        'kernel_ms_per_step': {'k_tw_grid_jobs' + sfx: round(res['grid_tap_ms'], 3),
-                               'k_tw_degrid_jobs' + sfx: round(res['degrid_tap_ms'], 3)},
+        'kernel_ms_per_step': {gname: round(res['grid_tap_ms'], 3),
+                               dname: round(res['degrid_tap_ms'], 3)},
         'algorithmic_flops_per_launch': 8.0 * taps / max(kn, 1),
         'algorithmic_bytes_per_launch': round(alg_bytes),
         'traffic': None,

@@ -812,6 +812,76 @@ __global__ void k_tw_cut(const int32_t *__restrict__ chunks, const float2 *__res
   }
 }
 
+// Degrid job parts with a separable kernel of support S <= 16, one
+// visibility per thread: the part's sub-grid is staged in LDS (as in
+// k_tw_degrid_jobs); each thread loads its rows kx (sub-offset xf) and ky
+// (yf) of its w-kernel plane and sums, in FP32 and in a fixed order,
+//   v = sum_y ky[y] sum_x kx[x] G[wy + y][wx + x]
+// (the taps k = ky[y] kx[x] of synthesis.py:456 on the sub-grid window).
+// The window lies inside the sub-grid (S <= margin, checked at binning).
+__global__ __launch_bounds__(TW_NT) void k_tw_degrid_sep16(
+    const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
+    const float2 *__restrict__ sub, const float2 *__restrict__ kern, int S, int64_t tsize,
+    float2 *__restrict__ vis_out, uint32_t *__restrict__ qc) {
+  constexpr int DEG_PITCH = SG + 16;  // 128-byte row padding against bank conflicts
+  __shared__ float2 sg[SG * DEG_PITCH];
+  __shared__ int s_b;
+  const int tid = threadIdx.x;
+  for (;;) {  // parts taken from the launch's queue counter qc
+    if (tid == 0) s_b = (int)atomicAdd(qc, 1u);
+    __syncthreads();
+    const int b = s_b;
+    if (b >= nb) break;
+    const int4 job = parts[b];
+    const float2 *src = sub + (int64_t)job.x * SGN;
+    static_assert(SGN % (8 * TW_NT) == 0, "staging covers the sub-grid exactly");
+    for (int i0 = tid; i0 < SGN; i0 += 8 * TW_NT) {
+      float2 gv[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) gv[k] = load_stream(src + i0 + k * TW_NT);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int i = i0 + k * TW_NT;
+        sg[(i / SG) * DEG_PITCH + i % SG] = gv[k];
+      }
+    }
+    __syncthreads();
+    for (int j = job.y + tid; j < job.z; j += TW_NT) {
+      const uint4 r = rec[j];
+      const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+      const int wx = (int)(r.x & 0xffffu), wy = (int)(r.x >> 16);
+      const float2 *kt = kern + (int64_t)r.z * tsize;
+      const float2 *kx = kt + (int64_t)xf * S, *ky = kt + (int64_t)yf * S;
+      float2 kxv[16], kyv[16];
+#pragma unroll
+      for (int x = 0; x < 16; x++) {
+        kxv[x] = x < S ? kx[x] : make_float2(0.f, 0.f);
+        kyv[x] = x < S ? ky[x] : make_float2(0.f, 0.f);
+      }
+      const float2 *g = sg + wy * DEG_PITCH + wx;
+      float ar = 0.f, ai = 0.f;
+#pragma unroll
+      for (int y = 0; y < 16; y++) {
+        if (y < S) {
+          float rr = 0.f, ri = 0.f;
+#pragma unroll
+          for (int x = 0; x < 16; x++) {
+            if (x < S) {
+              const float2 gv = g[y * DEG_PITCH + x];
+              rr = fmaf(gv.x, kxv[x].x, fmaf(-gv.y, kxv[x].y, rr));
+              ri = fmaf(gv.x, kxv[x].y, fmaf(gv.y, kxv[x].x, ri));
+            }
+          }
+          ar = fmaf(rr, kyv[y].x, fmaf(-ri, kyv[y].y, ar));
+          ai = fmaf(rr, kyv[y].y, fmaf(ri, kyv[y].x, ai));
+        }
+      }
+      store_stream(vis_out + r.w, make_float2(ar, ai));
+    }
+    __syncthreads();  // the sub-grid is read; the next part rewrites it
+  }
+}
+
 // Degrid job parts from their sub-grids: parts[b] = (slot, first record,
 // end record), one workgroup per part staging sub[slot] in LDS.
 template <bool SEPARABLE>
@@ -880,241 +950,6 @@ __global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs(
       }
     }
     __syncthreads();
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Degridding of job parts with separable kernels on the matrix cores.
-//
-// v_k = sum_{r,c} ky_k[r - wy_k] G[r][c] kx_k[c - wx_k].  Per 16 x 16 block
-// (by, bx) of the part's sub-grid and 16 of the visibilities whose window
-// meets it, T = G_blk Kx (16 x 16 x 16, v_mfma_f32_16x16x16_f16 with both
-// operands split as fp16 hi + lo, FP32 accumulation; Kx[c][k] = kx_k at
-// column c, from the split kernel set) gives T[r][k] = sum_c G[r][c] kx_k,
-// and v_k's share of the block is sum_r ky_k[r] T[r][k] in FP32 (ky from
-// the FP32 kernel set), reduced across the lane groups holding the rows.
-// A visibility meets at most 2 x 2 blocks (S <= 16); each block's share is
-// stored in its own LDS slot and the shares are added in a fixed order, so
-// the result is deterministic.  Blocks are taken dynamically by the
-// wavefronts; the visibilities of a block are listed from the buckets of
-// their window's first block, as in k_tw_grid_jobs_mfma.
-typedef _Float16 tw_half4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ tw_floatx4 tw_mfma16(tw_half4 a, tw_half4 b, tw_floatx4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
-
-constexpr int MD_WIN = 1024;  // visibilities per window of a part
-
-__global__ __launch_bounds__(TW_NT) void k_tw_degrid_jobs_mfma(
-    const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
-    const float2 *__restrict__ sub, const float2 *__restrict__ kern,
-    const uint2 *__restrict__ ksplit, int S, int64_t tsize, const TwScale *__restrict__ fs,
-    float2 *__restrict__ vis_out, uint32_t *__restrict__ qc) {
-  __shared__ uint4 s_rec[MD_WIN];        // {wx | wy << 16, ky offset, kx offset, vis index}
-  __shared__ float2 s_part[MD_WIN][4];   // a visibility's share of each block it meets
-  __shared__ uint8_t s_key[MD_WIN];
-  __shared__ uint16_t s_idx[MD_WIN];
-  __shared__ uint4 s_ring[MG_NW][MG_RING];  // {wx | wy << 7 | j << 14, ky off, kx off, -}
-  __shared__ int s_cnt[MG_NW][MG_NB];
-  __shared__ int s_bstart[MG_NB + 1];
-  __shared__ int s_b, s_blk;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int g4 = 4 * (lane >> 4);   // first row (T, ky) / column (G, Kx) of the lane's group
-  const int k = lane & 15;          // visibility of the group / G row of the lane
-  const unsigned long long lt = (1ull << lane) - 1ull;
-  const float sk = unit_scale(__uint_as_float(fs->max_k_bits));
-  uint4 *ring = s_ring[wave];
-  ring[lane] = make_uint4(0u, 0u, 0u, 0u);
-  ring[lane + 64] = make_uint4(0u, 0u, 0u, 0u);
-  for (;;) {  // parts taken from the launch's queue counter qc
-    if (tid == 0) s_b = (int)atomicAdd(qc, 1u);
-    __syncthreads();
-    const int b = s_b;
-    if (b >= nb) break;
-    const int4 job = parts[b];
-    const float2 *G = sub + (int64_t)job.x * SGN;
-    for (int jb = job.y; jb < job.z; jb += MD_WIN) {
-      const int n = min(job.z - jb, MD_WIN);
-      const int per = (n + MG_NW - 1) / MG_NW;
-      const int lo = min(n, wave * per), hi = min(n, lo + per);
-      s_cnt[tid >> 6][tid & 63] = 0;
-      for (int e = tid; e < 4 * n; e += TW_NT) s_part[e >> 2][e & 3] = make_float2(0.f, 0.f);
-      __syncthreads();
-      for (int j = lo + lane; j < hi; j += 64) {
-        const uint4 r = rec[jb + j];
-        const uint32_t base = r.z * (uint32_t)tsize;
-        const uint32_t xf = r.y & 0xffffu, yf = r.y >> 16;
-        s_rec[j] = make_uint4(r.x, base + yf * (uint32_t)S, base + xf * (uint32_t)S, r.w);
-        const int key = (int)((r.x >> 20) << 3) + (int)((r.x & 0xffffu) >> 4);
-        s_key[j] = (uint8_t)key;
-        atomicAdd(&s_cnt[wave][key], 1);
-      }
-      barrier_after_lds_atomics();
-      if (tid < MG_NB) {
-        int tot = 0;
-        for (int w = 0; w < MG_NW; w++) tot += s_cnt[w][tid];
-        int inc = tot;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const int y = __shfl_up(inc, off, 64);
-          if (lane >= off) inc += y;
-        }
-        int run = inc - tot;
-        s_bstart[tid] = run;
-        if (tid == MG_NB - 1) s_bstart[MG_NB] = inc;
-        for (int w = 0; w < MG_NW; w++) {
-          const int c = s_cnt[w][tid];
-          s_cnt[w][tid] = run;
-          run += c;
-        }
-      }
-      __syncthreads();
-      for (int j0 = lo; j0 < hi; j0 += 64) {
-        const int j = j0 + lane;
-        bool todo = j < hi;
-        const int key = todo ? (int)s_key[j] : 0;
-        for (;;) {
-          const unsigned long long pend = __ballot(todo);
-          if (!pend) break;
-          const int kq =
-              __builtin_amdgcn_readfirstlane(__shfl(key, __ffsll((long long)pend) - 1, 64));
-          const unsigned long long mk = __ballot(todo && key == kq);
-          const int cur = __builtin_amdgcn_readfirstlane(s_cnt[wave][kq]);
-          if (todo && key == kq) {
-            s_idx[cur + __popcll(mk & lt)] = (uint16_t)j;
-            todo = false;
-          }
-          if (lane == 0) s_cnt[wave][kq] = cur + __popcll(mk);
-        }
-      }
-      if (tid == 0) s_blk = 0;
-      __syncthreads();
-      for (;;) {
-        int blk = 0;
-        if (lane == 0) blk = atomicAdd(&s_blk, 1);
-        blk = __builtin_amdgcn_readfirstlane(__shfl(blk, 0, 64));
-        if (blk >= MG_NB) break;
-        const int by = blk >> 3, bx = blk & 7;
-        const int r0 = 16 * by, c0 = 16 * bx;
-        // the block of G as the A operand: row k (= lane % 16), columns g4 .. g4 + 3
-        float2 gv[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) gv[q] = load_stream(G + (r0 + k) * SG + c0 + g4 + q);
-        float gm = 0.f;
-#pragma unroll
-        for (int q = 0; q < 4; q++) gm = fmaxf(gm, fmaxf(fabsf(gv[q].x), fabsf(gv[q].y)));
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) gm = fmaxf(gm, __shfl_xor(gm, off, 64));
-        const float sg = unit_scale(gm);
-        const float back = 1.f / (sg * sk);
-        tw_half4 grh, grl, gih, gil;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const float re = gv[q].x * sg, im = gv[q].y * sg;
-          grh[q] = (_Float16)re;
-          grl[q] = (_Float16)(re - (float)grh[q]);
-          gih[q] = (_Float16)im;
-          gil[q] = (_Float16)(im - (float)gih[q]);
-        }
-        const tw_half4 ngih = -gih, ngil = -gil;
-        uint32_t head = 0, tail = 0;
-        // 16 listed visibilities at ring position h (nvalid real)
-        auto group = [&](uint32_t h, int nvalid) {
-          const uint4 e = ring[(h + k) & (MG_RING - 1)];
-          const int wx = (int)(e.x & 127u), wy = (int)((e.x >> 7) & 127u);
-          const int j = (int)(e.x >> 14);
-          const bool ok = k < nvalid;
-          uint2 kb[4];
-          float2 ky[4];
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int cc = c0 + g4 + q - wx, rr = r0 + g4 + q - wy;
-            const uint2 bq = ksplit[e.z + (uint32_t)min(max(cc, 0), S - 1)];
-            const float2 yq = kern[e.y + (uint32_t)min(max(rr, 0), S - 1)];
-            const bool vb = ok && cc >= 0 && cc < S, vy = ok && rr >= 0 && rr < S;
-            kb[q] = make_uint2(vb ? bq.x : 0u, vb ? bq.y : 0u);
-            ky[q] = vy ? yq : make_float2(0.f, 0.f);
-          }
-          tw_half4 brh, brl, bih, bil;
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            union {
-              uint2 u;
-              _Float16 h[4];
-            } c;
-            c.u = kb[q];
-            brh[q] = c.h[0];
-            brl[q] = c.h[1];
-            bih[q] = c.h[2];
-            bil[q] = c.h[3];
-          }
-          tw_floatx4 tr = {0.f, 0.f, 0.f, 0.f}, ti = {0.f, 0.f, 0.f, 0.f};
-          tr = tw_mfma16(grh, brh, tr);
-          tr = tw_mfma16(grh, brl, tr);
-          tr = tw_mfma16(grl, brh, tr);
-          tr = tw_mfma16(ngih, bih, tr);
-          tr = tw_mfma16(ngih, bil, tr);
-          tr = tw_mfma16(ngil, bih, tr);
-          ti = tw_mfma16(grh, bih, ti);
-          ti = tw_mfma16(grh, bil, ti);
-          ti = tw_mfma16(grl, bih, ti);
-          ti = tw_mfma16(gih, brh, ti);
-          ti = tw_mfma16(gih, brl, ti);
-          ti = tw_mfma16(gil, brh, ti);
-          // T[g4 + q][k] ky_k[g4 + q], summed over q and the lane groups
-          float pr = 0.f, pi = 0.f;
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            pr = fmaf(tr[q], ky[q].x, fmaf(-ti[q], ky[q].y, pr));
-            pi = fmaf(tr[q], ky[q].y, fmaf(ti[q], ky[q].x, pi));
-          }
-          pr += __shfl_xor(pr, 16, 64);
-          pi += __shfl_xor(pi, 16, 64);
-          pr += __shfl_xor(pr, 32, 64);
-          pi += __shfl_xor(pi, 32, 64);
-          if (lane < 16 && ok) {
-            const int slot = 2 * (by - (wy >> 4)) + (bx - (wx >> 4));
-            s_part[j][slot] = make_float2(pr * back, pi * back);
-          }
-        };
-        for (int sy = max(0, by - 1); sy <= by; sy++) {
-          for (int sx = max(0, bx - 1); sx <= bx; sx++) {
-            const int key = 8 * sy + sx;
-            const int p0 = s_bstart[key], p1 = s_bstart[key + 1];
-            for (int q0 = p0; q0 < p1; q0 += 64) {
-              const int q = q0 + lane;
-              bool mem = false;
-              uint4 rj = make_uint4(0u, 0u, 0u, 0u);
-              if (q < p1) {
-                const int j = s_idx[q];
-                rj = s_rec[j];
-                const int wx = (int)(rj.x & 0xffffu), wy = (int)(rj.x >> 16);
-                mem = wy < r0 + 16 && wy + S > r0 && wx < c0 + 16 && wx + S > c0;
-                rj.x = (uint32_t)wx | ((uint32_t)wy << 7) | ((uint32_t)j << 14);
-              }
-              const unsigned long long bal = __ballot(mem);
-              if (mem) ring[(tail + __popcll(bal & lt)) & (MG_RING - 1)] = rj;
-              tail += (uint32_t)__popcll(bal);
-              while (tail - head >= 16) {
-                group(head, 16);
-                head += 16;
-              }
-            }
-          }
-        }
-        if (tail > head) group(head, (int)(tail - head));
-      }
-      __syncthreads();
-      // shares added in block order (top-left, top-right, bottom-left, bottom-right)
-      for (int j = tid; j < n; j += TW_NT) {
-        const float2 a = s_part[j][0], b2 = s_part[j][1], c = s_part[j][2], d = s_part[j][3];
-        store_stream(vis_out + s_rec[j].w,
-                     make_float2(((a.x + b2.x) + c.x) + d.x, ((a.y + b2.y) + c.y) + d.y));
-      }
-      __syncthreads();
-    }
   }
 }
 
@@ -1608,21 +1443,9 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
   uint32_t *qc = (uint32_t *)(phi + SGN);  // a part queue per batch
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
   const int64_t tsize = separable ? (int64_t)a.Q * a.S : (int64_t)a.Q * a.Q * a.S * a.S;
-  // Separable kernels of support <= 16 degrid on the matrix cores
-  // (k_tw_degrid_jobs_mfma), with an fp16 (hi, lo) copy of the kernel set.
-  TwScale *fs = (TwScale *)aux;
-  uint2 *ksplit = nullptr;
-  if (!a.nearest && separable && a.S <= 16) {
-    const int64_t nk = tsize * a.nwk;
-    CROC_REQUIRE(nk < ((int64_t)1 << 32), CROC_EINVAL, "kernel set beyond 2^32 elements");
-    ksplit = (uint2 *)ws_get(ctx, WS_KSPLIT, sizeof(uint2) * (size_t)nk);
-    if (!ksplit) return CROC_ENOMEM;
-    if ((rc = zero_u32_async(ctx, (uint32_t *)fs, sizeof(TwScale) / 4))) return rc;
-    k_tw_maxima<<<ctx->num_cu, 1024, 0, s>>>(nullptr, 0, (const float2 *)kern, nk, fs);
-    k_tw_split_tables<<<(unsigned)((nk + 255) / 256), 256, 0, s>>>((const float2 *)kern, nk, fs,
-                                                                  ksplit);
-    CROC_HIP(hipGetLastError());
-  }
+  // Separable kernels of support <= 16 degrid one visibility per thread
+  // (k_tw_degrid_sep16).
+  const bool sep16 = !a.nearest && separable && a.S <= 16;
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
   if ((rc = zero_u32_async(ctx, qc, (int64_t)ds.poff.size()))) return rc;
   dim3 g(16, (unsigned)p.chunks.size());
@@ -1643,10 +1466,10 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
     const int npart = (int)(ds.poff[bi + 1] - ds.poff[bi]);
     const int G = std::min(npart, ctx->num_cu);  // one per CU (LDS)
     if ((rc = tap_timer_begin(ctx, 1))) return rc;
-    if (ksplit)
-      k_tw_degrid_jobs_mfma<<<G, TW_NT, 0, s>>>(ds.parts + ds.poff[bi], npart, p.rec, sub,
-                                                (const float2 *)kern, ksplit, a.S, tsize, fs,
-                                                (float2 *)vis_out, qc + bi);
+    if (sep16)
+      k_tw_degrid_sep16<<<G, TW_NT, 0, s>>>(ds.parts + ds.poff[bi], npart, p.rec, sub,
+                                            (const float2 *)kern, a.S, tsize, (float2 *)vis_out,
+                                            qc + bi);
     else if (a.nearest || !separable)
       k_tw_degrid_jobs<false><<<G, TW_NT, 0, s>>>(ds.parts + ds.poff[bi], npart, p.rec, sub,
                                                   (const float2 *)kern, a.Q, a.S, tsize,
