@@ -47,7 +47,7 @@ def parse():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='towers8192',
                     choices=['wproj2048', 'plain512', 'towers4096', 'towers8192', 'diff1024',
-                             'dft128', 'stream32k'])
+                             'dft128', 'stream32k', 'wstack2048', 'awtowers1800'])
     ap.add_argument('--nv', type=int, default=0,
                     help='towers workloads: visibilities (0 = the workload\'s own count)')
     ap.add_argument('--cpu-sample', type=int, default=0,
@@ -741,6 +741,103 @@ def run_stream(args, torch, W, _lib, dev):
     print(json.dumps(line), flush=True)
 
 
+def run_imaging_extra(args, torch, G, W, _lib, dev):
+    """The notebook imaging drivers outside the w-towers headline, timed on
+    inputs resident in HBM: full-grid w-stacking (wstack2048,
+    wstacking.ipynb cell 10) and AW-towers with per-antenna A-screens
+    (awtowers1800, awtowers.ipynb cells 17-22).  One step is one imaging
+    call.  The CPU baseline is the FP64 oracle on a seeded sample, and the
+    device result on that sample is compared with it."""
+    import numpy
+    name = args.workload
+    if name == 'wstack2048':
+        c, uvw, vis = W.wstack2048()
+        theta, lam = c['theta'], c['lam']
+        uvw_d = torch.as_tensor(uvw, device=dev)
+        vis_d = torch.as_tensor(vis, device=dev)
+
+        def step():
+            return G.w_stacking_imaging(theta, lam, uvw_d, None, vis_d, c['wstep'])
+        nv = uvw.shape[0]
+        planes = len(numpy.unique(numpy.round(uvw[:, 2] / c['wstep'])))
+        config = {'workload': name, 'N': c['N'], 'theta': theta, 'nv': nv, 'w_planes': planes,
+                  'wstep': c['wstep'], 'step': 'w_stacking_imaging (croc_wstack_grid, batched '
+                  'rocFFT, croc_wstack_add)'}
+    else:
+        c, N, uvw, src, vis, k = W.awtowers1800()
+        theta, lam = c['theta'], c['lam']
+        uvw_d = torch.as_tensor(uvw, device=dev)
+        src_d = torch.as_tensor(src, device=dev)
+        vis_d = torch.as_tensor(vis, device=dev)
+        kd = {n: torch.as_tensor(numpy.asarray(k[n]), device=dev) for n in ('aa_gcf', 'aimage',
+                                                                           'wkern')}
+
+        def step():
+            return G.aw_towers_imaging(theta, lam, uvw_d, src_d, vis_d, c['wstep'],
+                                       c['subgrid'], c['margin'], kd['aa_gcf'], k['aa_sub'],
+                                       k['aa_sub2'], kd['aimage'], kd['wkern'])
+        nv = uvw.shape[0]
+        config = {'workload': name, 'N': N, 'theta': theta, 'nv': nv, 'antennas': c['nant'],
+                  'dumps': c['ndump'], 'subgrid': c['subgrid'], 'margin': c['margin'],
+                  'wstep': c['wstep'], 'aa_support': c['aa_support'], 'aa_over': c['aa_over'],
+                  'step': 'aw_towers_imaging (croc_awt_pregrid, batched rocFFT, '
+                          'croc_awt_apply, rocFFT, croc_awt_place)'}
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print('[bench] timing %d %s steps' % (args.steps, name), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+
+    cpu, parity = None, None
+    n_cpu = args.cpu_sample or (20000 if name == 'wstack2048' else 1500)
+    if n_cpu > 0:
+        from oracle import wstacking_ref as WS, awtowers_ref as AWT
+        print('[bench] cpu baseline: FP64 oracle on %d visibilities' % n_cpu, file=sys.stderr,
+              flush=True)
+        if name == 'wstack2048':
+            c_s, uvw_s, vis_s = W.wstack2048(nv=n_cpu, seed=c['seed'] + 1)
+            t0 = time.perf_counter()
+            ref = WS.w_stacking_imaging(theta, lam, uvw_s, None, vis_s, c['wstep'])
+            t_cpu = time.perf_counter() - t0
+            got = G.w_stacking_imaging(theta, lam, uvw_s, None, vis_s, c['wstep'])
+            what = ('%d visibilities of wstack2048 (seed %d) on the full 2048^2 grid: the '
+                    'planes\' FFTs dominate, so the rate does not extrapolate linearly'
+                    % (n_cpu, c['seed'] + 1))
+        else:
+            _, _, uvw_s, src_s, vis_s, ks = W.awtowers1800(nsample=n_cpu)
+            args_s = (theta, lam, uvw_s, src_s, vis_s, c['wstep'], c['subgrid'], c['margin'],
+                      numpy.asarray(ks['aa_gcf']), ks['aa_sub'], ks['aa_sub2'], ks['aimage'],
+                      ks['wkern'])
+            t0 = time.perf_counter()
+            ref = AWT.aw_towers_imaging(*args_s)
+            t_cpu = time.perf_counter() - t0
+            got = G.aw_towers_imaging(*args_s)
+            what = ('%d visibilities of awtowers1800 (a seeded subset): one (2S)^2 transform '
+                    'per (chunk, plane, antenna pair) group' % n_cpu)
+        cpu = {'value': n_cpu / t_cpu, 'unit': 'visibilities/s', 'cores': 1, 'kind': 'port',
+               'sample': what + '; %.2f s' % t_cpu}
+        d = numpy.abs(got - ref)
+        parity = {'sample_nv': n_cpu,
+                  'max_rel': float(d.max() / numpy.abs(ref).max()),
+                  'rms_rel': float(numpy.sqrt((d ** 2).mean() / (numpy.abs(ref) ** 2).mean()))}
+
+    line = {
+        'metric': 'visibilities/s imaging', 'value': round(nv / dt, 1),
+        'unit': 'visibilities/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(1e3 * dt, 3), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic', 'config': config,
+        'roofline': None,
+        'roofline_note': 'a notebook driver beside the towers8192 headline, reported for '
+                         'throughput and parity; its transforms are rocFFT\'s',
+        'cpu_baseline': cpu, 'parity_vs_fp64_oracle': parity,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def copy_probe(torch, dev, mib):
     """One device copy of ``mib`` MiB (a blit kernel of known traffic, to
     scale the HBM counters of a profiled run); nothing when mib <= 0."""
@@ -791,6 +888,8 @@ def main():
         return run_stream(args, torch, W, _lib, dev)
     if args.workload == 'diff1024':
         return run_diff(args, torch, G, W, _lib, dev)
+    if args.workload in ('wstack2048', 'awtowers1800'):
+        return run_imaging_extra(args, torch, G, W, _lib, dev)
     print('[bench] building workload %s' % args.workload, file=sys.stderr, flush=True)
     if args.workload == 'wproj2048':
         wl = build_wproj(G, W, torch, dev)

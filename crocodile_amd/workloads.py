@@ -31,6 +31,15 @@ Workload names:
                  of view = image / 210000); one GPU's share is one 64-channel
                  frequency chunk of every baseline, 1.07e9 visibilities;
                  separable PSWF kernel S=8, Q=1024.
+* ``wstack2048`` full-grid w-stacking (wstacking.ipynb cell 10): N=2048,
+                 theta=0.1, 2e6 vis in a disk of radius 0.4 lam, w uniform
+                 +-4000, 64 w-planes of 125 wavelengths, 20 point sources.
+* ``awtowers1800`` AW-towers at the geometry of awtowers.ipynb (cells 3-22):
+                 theta=0.1, lam=18000 (N=1800), sub-grids 128 with margin 30,
+                 wstep 50, 27 antennas (core and three arms to 18 km) over
+                 36 dumps at 5 m, dec 45 deg; per-antenna ionospheric phase
+                 screens as the notebook's ion_sample; anti-aliasing PSWF
+                 support 6 oversampled 4096 times.
 """
 
 import math
@@ -51,6 +60,11 @@ SPEC = {
     'towers8192': dict(N=8192, theta=0.05, lam=163840.0, nv=50_000_000, Q=32,
                        S=16, nant=256, hours=8.0, freq=150e6, dec=-45.0,
                        subgrid=128, nsrc=20, seed=1005),
+    'wstack2048': dict(N=2048, theta=0.1, lam=20480.0, nv=2_000_000, wmax=4000.0, wstep=125.0,
+                       nsrc=20, seed=1007),
+    'awtowers1800': dict(theta=0.1, lam=18000.0, subgrid=128, margin=30, wstep=50.0, nant=27,
+                         ndump=36, wvl=5.0, dec=45.0, hours=6.0, aa_support=6, aa_over=4096,
+                         nsrc=25, seed=1008),
     'stream32k': dict(image_size=32768, theta=32768 / 210000, xA=832, xM=1024, nant=512,
                       ndump=128, time_chunk=64, nchan=64, freq_chunk=64, freq0=260e6,
                       dfreq=40e6 / 2048, hours=4.0, dec=-45.0, S=8, Q=1024, seed=1006),
@@ -504,3 +518,62 @@ def stream_subgrids(name='stream32k', umax=None):
     if umax is not None:
         kmax = min(kmax, int(math.ceil((umax * th - xA / 2) / xA)))
     return [(iu * xA, iv * xA) for iv in range(-kmax, kmax + 1) for iu in range(0, kmax + 1)]
+
+
+def wstack2048(nv=None, seed=None):
+    """uvw [nv, 3] (wavelengths), visibilities of 20 point sources, and the
+    spec of the w-stacking workload."""
+    c = dict(SPEC['wstack2048'])
+    nv = nv or c['nv']
+    rng = numpy.random.default_rng(c['seed'] if seed is None else seed)
+    r = 0.4 * c['lam'] * numpy.sqrt(rng.uniform(0, 1, nv))
+    ph = rng.uniform(0, 2 * numpy.pi, nv)
+    uvw = numpy.stack([r * numpy.cos(ph), r * numpy.sin(ph),
+                       rng.uniform(-c['wmax'], c['wmax'], nv)], 1)
+    l, m, flux = point_sources(rng, c['nsrc'], 0.8 * c['theta'])
+    return c, uvw, dft_vis(uvw, l, m, flux)
+
+
+def awtowers1800(nsample=None, seed=None):
+    """The AW-towers workload: flipped uvw (v >= 0), antenna pairs, the
+    visibilities of point sources seen through per-antenna phase screens,
+    and the notebook's kernels and image-space tables (awtowers.ipynb cells
+    3, 11-22).  ``nsample``: a seeded subset of the visibilities."""
+    from . import synthesis as G
+    c = dict(SPEC['awtowers1800'])
+    rng = numpy.random.default_rng(c['seed'] if seed is None else seed)
+    xyz = antenna_layout(rng, c['nant'], sigma_m=2000.0, arms_to_m=18000.0)
+    uvw = baseline_tracks(xyz, c['hours'], C_LIGHT / c['wvl'], c['dec'], c['ndump'])
+    a1, a2 = numpy.triu_indices(c['nant'], 1)
+    src = numpy.tile(numpy.stack([a1, a2], 1), (c['ndump'], 1))
+    coef = rng.uniform(-0.5, 0.5, (c['nant'], 4))
+
+    def screen(a, l, m):
+        return numpy.exp(2j * numpy.pi * (coef[a, 0] + 3 * coef[a, 1] * l + 3 * coef[a, 2] * m
+                                          + 60 * coef[a, 3] * (l * l + m * m)))
+
+    theta = c['theta']
+    ls, ms, flux = point_sources(rng, c['nsrc'], 0.6 * theta)
+    vis = numpy.zeros(len(uvw), dtype=complex)
+    for l, m, f in zip(ls, ms, flux):
+        n1 = numpy.sqrt(1 - l * l - m * m) - 1
+        ratio = screen(src[:, 0], l, m) / screen(src[:, 1], l, m)
+        vis += f * ratio * numpy.exp(-2j * numpy.pi * (uvw[:, 0] * l + uvw[:, 1] * m
+                                                       + uvw[:, 2] * n1))
+    flip = uvw[:, 1] < 0
+    uvw[flip] = -uvw[flip]
+    vis[flip] = numpy.conj(vis[flip])
+    src[flip] = src[flip, ::-1]
+    if nsample:
+        pick = numpy.sort(rng.choice(len(uvw), nsample, replace=False))
+        uvw, src, vis = uvw[pick], src[pick], vis[pick]
+    N = int(math.ceil(theta * c['lam']))
+    S, sup = c['subgrid'], c['aa_support']
+    aa = G.anti_aliasing_function(N, 0, sup)
+    l, m = theta * G.coordinates2(S)
+    k = dict(aa=aa, aa_gcf=G.kernel_oversample(aa, c['aa_over'], sup),
+             aa_sub=G.anti_aliasing_function(S, 0, sup),
+             aa_sub2=G.anti_aliasing_function(2 * S, 0, sup),
+             aimage=numpy.array([screen(a, l, m) for a in range(c['nant'])]),
+             wkern=G.w_kernel_function(l, m, c['wstep']))
+    return c, N, uvw, src, vis, k
