@@ -259,3 +259,42 @@ def generate_w_kernels(directory, theta, wstep, wcount, NpixFF, NpixKern, Qpx=8,
         numpy.save(p + '.tmp.npy', t, allow_pickle=False)
         os.replace(p + '.tmp.npy', p)
     return todo
+
+
+def generate_w_kernels_h5(filename, theta, wstep, wcount, NpixFF, NpixKern, Qpx=8,
+                          overwrite=False):
+    """scripts/generate_wkernels.py itself: the w-kernels of w = iw * wstep,
+    iw in [-wcount, wcount], made in one launch and stored in the HDF5 file
+    ``filename`` as ``wkern/<theta>/<w>/kern`` [Qpx, Qpx, NpixKern, NpixKern]
+    complex128 (the names and layout that hdf5.c:594-700 and
+    image_dataset.py:114-129 read).  An existing file is extended, its
+    kernels kept unless ``overwrite`` (the script opens it in append mode);
+    the whole file is rewritten.  Returns the list of w written."""
+    from . import formats, h5
+    from .synthesis import w_kernel_bank
+    ws = [iw * wstep for iw in range(-wcount, wcount + 1)]
+    tree = {}
+    if os.path.exists(filename):
+        f = h5.File(filename)
+
+        def collect(g, path):
+            for k in g.keys():
+                o = g[k]
+                if isinstance(o, h5.Group):
+                    collect(o, path + k + '/')
+                else:
+                    tree[path + k] = o[()]
+        collect(f, '')
+        del f
+    name = lambda w: 'wkern/%s/%s/kern' % (formats._name(theta), formats._name(w))  # noqa: E731
+    todo = [w for w in ws if overwrite or name(w) not in tree]
+    if not todo:
+        return []
+    bank = w_kernel_bank(theta, todo, NpixFF, NpixKern, Qpx, True, double=True)
+    n = Qpx * Qpx * NpixKern * NpixKern
+    for k, w in enumerate(todo):
+        o = int(bank.plane_off_h[k])
+        tree[name(w)] = bank.kern[o:o + n].reshape(Qpx, Qpx, NpixKern, NpixKern).cpu().numpy()
+    h5.write(filename + '.tmp', tree)
+    os.replace(filename + '.tmp', filename)
+    return todo
