@@ -113,11 +113,13 @@ def load_vis(filename, min_len=-math.inf, max_len=math.inf, device=None):
     return VisData(bls, nant, device=device)
 
 
-def write_vis(filename, baselines, extra=None):
+def write_vis(filename, baselines, extra=None, chunks=None):
     """Write baselines (BlData) as a visibility file of baseline groups
     ``vis/<a1>/<a2>`` (the form hdf5.c:395-478 creates and :330-362 reads).
-    ``extra``: more {path: array} entries for the same file."""
-    tree = {}
+    ``chunks`` (time_chunk, freq_chunk): store each ``vis`` dataset in chunks
+    of [time_chunk, freq_chunk, 1], as hdf5.c:416-466 does.  ``extra``: more
+    {path: array} entries for the same file."""
+    tree, chunked = {}, {}
     for b in baselines:
         base = 'vis/%d/%d/' % (b.antenna1, b.antenna2)
         tree[base + 'frequency'] = numpy.asarray(b.freq, dtype=numpy.float64)
@@ -125,8 +127,10 @@ def write_vis(filename, baselines, extra=None):
         tree[base + 'uvw'] = numpy.asarray(b.uvw_m, dtype=numpy.float64)
         tree[base + 'vis'] = numpy.asarray(b.vis, dtype=numpy.complex128).reshape(
             b.time_count, b.freq_count, 1)
+        if chunks is not None:
+            chunked['/' + base + 'vis'] = (int(chunks[0]), int(chunks[1]), 1)
     tree.update(extra or {})
-    h5.write(filename, tree)
+    h5.write(filename, tree, chunks=chunked)
 
 
 # ---------------------------------------------------------------------------

@@ -544,6 +544,11 @@ def _pad8(b):
     return b + b'\0' * (-len(b) % 8)
 
 
+class VlenStr(str):
+    """An attribute value to be written as a variable-length string (as h5py
+    stores Python strings), its bytes in a global heap collection."""
+
+
 class _Writer:
     def __init__(self):
         self.buf = bytearray(b'\0' * 96)        # superblock, filled in last
@@ -571,7 +576,19 @@ class _Writer:
         null-terminated strings, everything else as a NumPy array."""
         msgs = []
         for name, v in (attrs or {}).items():
-            if isinstance(v, str):
+            if isinstance(v, VlenStr):
+                raw = v.encode('utf-8')
+                # one-object global heap collection, the rest a free block
+                obj = struct.pack('<HH4xQ', 1, 1, len(raw)) + _pad8(raw)
+                size = max(4096, 16 + len(obj) + 16)
+                free = struct.pack('<HH4xQ', 0, 0, size - 16 - len(obj))
+                coll = b'GCOL' + struct.pack('<B3xQ', 1, size) + obj + free
+                caddr = self.alloc(coll + b'\0' * (size - len(coll)))
+                base = bytes([0x10]) + (0).to_bytes(3, 'little') + struct.pack('<IHH', 1, 0, 8)
+                dt_msg = (bytes([0x19]) + (1 | (1 << 8)).to_bytes(3, 'little') +
+                          struct.pack('<I', 16) + base)
+                space, data = self._space(()), struct.pack('<IQI', len(raw), caddr, 1)
+            elif isinstance(v, str):
                 raw = v.encode('utf-8') + b'\0'
                 dt_msg = bytes([0x13]) + (0).to_bytes(3, 'little') + struct.pack('<I', len(raw))
                 space, data = self._space(()), raw
@@ -584,7 +601,46 @@ class _Writer:
             msgs.append((MSG_ATTRIBUTE, body))
         return msgs
 
-    def dataset(self, a, attrs=None):
+    def chunked(self, a, chunks):
+        """Layout message of ``a`` stored in chunks of shape ``chunks`` under
+        a version-1 B-tree of chunk nodes (at most 64 entries each), edge
+        chunks padded with zeros."""
+        nd = a.ndim + 1
+        grid = [range(0, max(n, 1), c) for n, c in zip(a.shape, chunks)]
+        entries = []
+        for off in numpy.ndindex(*[len(g) for g in grid]):
+            o = [grid[d][off[d]] for d in range(a.ndim)]
+            block = numpy.zeros(chunks, a.dtype)
+            part = a[tuple(slice(x, x + c) for x, c in zip(o, chunks))]
+            block[tuple(slice(0, n) for n in part.shape)] = part
+            raw = block.tobytes()
+            entries.append((o + [0], self.alloc(raw), len(raw)))
+
+        def key(offs, size):
+            return struct.pack('<II', size, 0) + b''.join(struct.pack('<Q', x) for x in offs)
+
+        end = [n for n in a.shape] + [0]
+        level, nodes = 0, [(e[0], e[1], e[2]) for e in entries]
+        while True:
+            new = []
+            for i in range(0, len(nodes), 64):
+                g = nodes[i:i + 64]
+                body = b''
+                for offs, child, size in g:
+                    body += key(offs, size) + struct.pack('<Q', child)
+                nxt = nodes[i + 64][0] if i + 64 < len(nodes) else end
+                body += key(nxt, 0)
+                addr = self.alloc(b'TREE' + struct.pack('<BBHQQ', 1, level, len(g), UNDEF,
+                                                        UNDEF) + body)
+                new.append((g[0][0], addr, 0))
+            if len(new) == 1:
+                root = new[0][1]
+                break
+            nodes, level = new, level + 1
+        dims = b''.join(struct.pack('<I', c) for c in chunks) + struct.pack('<I', a.dtype.itemsize)
+        return struct.pack('<BBBQ', 3, 2, nd, root) + dims
+
+    def dataset(self, a, attrs=None, chunks=None):
         a = numpy.asarray(a, order='C')       # keeps a 0-d array 0-d
         if a.dtype.byteorder == '>':
             a = a.astype(a.dtype.newbyteorder('<'))
@@ -592,17 +648,22 @@ class _Writer:
             a = a.astype(numpy.uint8)
         space = struct.pack('<BBB5x', 1, a.ndim, 0) + b''.join(struct.pack('<Q', s)
                                                                for s in a.shape)
-        addr = self.alloc(a.tobytes()) if a.nbytes else UNDEF
-        layout = struct.pack('<BBQQ', 3, 1, addr, a.nbytes)
+        if chunks is not None:
+            if len(chunks) != a.ndim or min(chunks, default=1) < 1:
+                raise ValueError('chunk shape %s does not fit the array' % (chunks,))
+            layout = self.chunked(a, tuple(int(c) for c in chunks))
+        else:
+            addr = self.alloc(a.tobytes()) if a.nbytes else UNDEF
+            layout = struct.pack('<BBQQ', 3, 1, addr, a.nbytes)
         fill = struct.pack('<BBBB', 2, 2, 1, 0)      # version 2, late allocation, never written, undefined
         return self.object_header([(MSG_DATASPACE, space), (MSG_DATATYPE, _datatype_message(a.dtype)),
                                    (MSG_FILL, fill), (MSG_LAYOUT, layout)] +
                                   self.attributes(attrs))
 
-    def group(self, tree, path='', attrs=None):
+    def group(self, tree, path='', attrs=None, chunks=None):
         """Symbol-table group of the (name -> dict | array) tree; ``attrs``
-        {object path: {name: value}}."""
-        attrs = attrs or {}
+        {object path: {name: value}}, ``chunks`` {dataset path: shape}."""
+        attrs, chunks = attrs or {}, chunks or {}
         names = sorted(tree)
         # local heap: empty string at 0, then the names
         heap_data = b'\0' * 8
@@ -614,8 +675,8 @@ class _Writer:
         children = {}
         for n in names:
             sub = path + '/' + n
-            children[n] = (self.group(tree[n], sub, attrs) if isinstance(tree[n], dict)
-                           else self.dataset(tree[n], attrs.get(sub)))
+            children[n] = (self.group(tree[n], sub, attrs, chunks) if isinstance(tree[n], dict)
+                           else self.dataset(tree[n], attrs.get(sub), chunks.get(sub)))
         data_addr = self.alloc(heap_data)
         heap = self.alloc(b'HEAP' + struct.pack('<B3xQQQ', 0, len(heap_data), UNDEF, data_addr))
         # symbol table nodes of at most 2 K = 8 entries (group leaf node K 4)
@@ -677,13 +738,17 @@ def _tree(flat):
     return out
 
 
-def write(path, tree, attrs=None):
+def write(path, tree, attrs=None, chunks=None):
     """Write ``tree`` ({name or path: array or nested dict}) as a classic
-    HDF5 file: groups as symbol tables, datasets contiguous; complex arrays
-    as the {r, i} compound.  ``attrs``: {object path ('/' for the root,
-    '/vis', '/vis/uvw', ...): {name: str or array}}."""
-    norm = {('/' + k.strip('/')) if k.strip('/') else '/': v for k, v in (attrs or {}).items()}
+    HDF5 file: groups as symbol tables, datasets contiguous unless ``chunks``
+    {dataset path: chunk shape} says otherwise (as hdf5.c:449-466 stores
+    ``vis``); complex arrays as the {r, i} compound.  ``attrs``: {object
+    path ('/' for the root, '/vis', '/vis/uvw', ...): {name: value}}, a
+    value a str (fixed-length string), :class:`VlenStr` (variable-length
+    string, as h5py writes str) or an array."""
+    def norm(d):
+        return {('/' + k.strip('/')) if k.strip('/') else '/': v for k, v in (d or {}).items()}
     w = _Writer()
-    root = w.group(_tree(tree), '', norm)
+    root = w.group(_tree(tree), '', norm(attrs), norm(chunks))
     with open(path, 'wb') as fh:
         fh.write(w.finish(root))

@@ -105,6 +105,28 @@ def baselines(rng, nant=5, nt=4, nf=3):
     return out
 
 
+@pytest.mark.parametrize('shape,chunks', [((37, 11, 1), (8, 4, 1)), ((70, 5), (1, 5)),
+                                          ((5,), (5,)), ((0, 3), (2, 3))])
+def test_chunked_datasets(tmp_path, shape, chunks):
+    """Chunked layout under version-1 chunk B-trees (70 chunks: two levels),
+    edge chunks cut at the dataset's extent."""
+    rng = numpy.random.default_rng(5)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    p = str(tmp_path / 'c.h5')
+    h5.write(p, {'d': a}, chunks={'/d': chunks})
+    ds = h5.File(p)['d']
+    assert ds.shape == shape
+    assert_array_equal(ds[()], a)
+
+
+def test_variable_length_string_attribute(tmp_path):
+    p = str(tmp_path / 's.h5')
+    h5.write(p, {'g/x': numpy.ones(1)}, attrs={'/g': {'type': h5.VlenStr('Visibility'),
+                                                      'name': h5.VlenStr('LOW-\u00e9')}})
+    a = h5.File(p)['g'].attrs
+    assert a['type'] == 'Visibility' and a['name'] == 'LOW-\u00e9'
+
+
 def test_visibility_file_round_trip_and_length_cut(tmp_path):
     """vis/<a1>/<a2> groups (hdf5.c:330-362): every baseline back, in
     (a1, a2) order; baselines whose first uvw is shorter than min_len or at
@@ -115,6 +137,10 @@ def test_visibility_file_round_trip_and_length_cut(tmp_path):
     formats.write_vis(p, bls)
     got, nant = formats.read_baselines(p)
     assert nant == 5 and len(got) == len(bls)
+    q = str(tmp_path / 'vis_chunked.h5')          # vis in [2, 2, 1] chunks (hdf5.c:416-466)
+    formats.write_vis(q, bls, chunks=(2, 2))
+    for a, b in zip(formats.read_baselines(q)[0], bls):
+        assert_array_equal(a.vis, b.vis)
     for a, b in zip(got, bls):
         assert (a.antenna1, a.antenna2) == (b.antenna1, b.antenna2)
         assert_array_equal(a.uvw_m, b.uvw_m)
@@ -140,7 +166,7 @@ def test_legacy_flat_visibility_group(tmp_path):
     p = str(tmp_path / 'flat.h5')
     h5.write(p, {'vis/frequency': numpy.array([1e8, 2e8]), 'vis/time': numpy.arange(n, dtype=float),
                  'vis/uvw': uvw, 'vis/vis': vis, 'vis/antenna1': a1, 'vis/antenna2': a2},
-             attrs={'/vis': {'type': 'Visibility'}})
+             attrs={'/vis': {'type': h5.VlenStr('Visibility')}})   # as h5py writes a str
     bls, nant = formats.read_baselines(p)
     assert nant == 4 and len(bls) == n
     for i, b in enumerate(bls):
