@@ -345,10 +345,10 @@ def run_towers(args, torch, G, W, _lib, dev):
     # S^2 taps per visibility and direction
     taps = float(nv) * S * S
     # separable sets of support <= 16: gridding on the matrix cores
-    # (k_tw_grid_jobs_mfma), degridding one visibility per thread
-    # (k_tw_degrid_sep16)
+    # (k_tw_grid_jobs_mfma); predict transforms each plane sub-grid and
+    # degrids from it in LDS (k_tw_predict_sep: FFT and taps in one launch)
     gname = 'k_tw_grid_jobs_mfma' if S <= 16 else 'k_tw_grid_jobs'
-    dname = 'k_tw_degrid_sep16' if S <= 16 else 'k_tw_degrid_jobs'
+    dname = 'k_tw_predict_sep' if S <= 16 else 'k_tw_degrid_jobs'
     if res['grid_tap_ms'] >= res['degrid_tap_ms']:
         kname, kms, kn = gname, res['grid_tap_ms'], res['grid_launches']
     else:

@@ -1,138 +1,18 @@
 // Batched centred 2D FFT of 128 x 128 complex64 sub-grids on the vector
 // ALUs, the sub-grid transform of the w-towers drivers.
 //
-// One workgroup per sub-grid holds it in LDS (rows padded to 129 elements
-// so that row- and column-wise accesses spread over the banks).  Each 1D
-// transform of 128 points is split as 128 = 16 x 8 (j = 8 j1 + j2,
-// k = k1 + 16 k2):
-//   pass A: for each j2, a 16-point DFT over j1 in registers, times the
-//           twiddle W128^(j2 k1), written back in place;
-//   pass B: for each k1, an 8-point DFT over j2 in registers, written to
-//           the natural position k1 + 16 k2.
-// Rows are transformed, then columns.  The centred transform of
+// One workgroup per sub-grid holds it in LDS; each 1D transform of 128
+// points is two passes, 16 x 8 (fft128.h).  Rows are transformed, then
+// columns.  The centred transform of
 // crocodile's fft / ifft (synthesis.py:83-106) is the plain DFT with input
 // and output multiplied by (-1)^(x + y) (n = 128: the constant factor
 // (-1)^(n/2) is 1); that sign, the per-pixel phase modulations of the
 // w-towers (exp(i s k phi[pixel]) on input and / or output) and the output
 // scale are applied while loading and storing.  Exact arithmetic is FP32,
 // twiddles are rounded from FP64 values: ~1e-7 relative error.
-#include "internal.h"
+#include "fft128.h"
 
 namespace {
-
-constexpr int FN = 128;
-constexpr int FP = FN + 1;  // padded LDS row
-constexpr int FFT_NT = 1024;
-constexpr int PASS_A = FN * 8 / FFT_NT;   // pass-A items (line, j2) per thread
-constexpr int PASS_B = FN * 16 / FFT_NT;  // pass-B items (line, k1) per thread
-static_assert(PASS_A * FFT_NT == FN * 8 && PASS_B * FFT_NT == FN * 16, "item split");
-
-struct FModArgs {
-  const double *phi;
-  const int32_t *kplane;
-  int sign;
-};
-
-// v * exp(i kf phi), kf = s k (the plane multiplier times the sign): the
-// angle in turns, reduced to [-1/2, 1/2] in FP64, then the hardware sine
-// and cosine of turns (v_sin_f32 / v_cos_f32: 1.24e-7 largest absolute
-// error on [-1/2, 1/2], measured against FP64 over 2^20 points; a library
-// sincosf of the FP32 angle in radians is 2.1e-7 there and several times
-// the instructions).
-__device__ __forceinline__ float2 rotate(float2 v, double kf, double phi) {
-  const double t = kf * phi * 0.15915494309189535;
-  const float f = (float)(t - rint(t));
-  const float s = __builtin_amdgcn_sinf(f), c = __builtin_amdgcn_cosf(f);
-  return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
-}
-
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-
-// cos / sin of 2 pi k / 16, k = 0..7 (rounded from FP64).
-__device__ constexpr float C16[8] = {1.0f, 0.92387953251128674f, 0.70710678118654757f,
-                                      0.38268343236508984f, 0.0f, -0.38268343236508973f,
-                                      -0.70710678118654746f, -0.92387953251128674f};
-__device__ constexpr float S16[8] = {0.0f, 0.38268343236508978f, 0.70710678118654746f,
-                                      0.92387953251128674f, 1.0f, 0.92387953251128674f,
-                                      0.70710678118654757f, 0.38268343236508989f};
-
-// In-register radix-2 DFT of N <= 16 points: v[k] = sum_j v[j] W_N^(jk),
-// W_N = exp(sgn 2 pi i / N).
-template <int N>
-__device__ __forceinline__ void fft_reg(float2 *v, float sgn) {
-  if constexpr (N == 1) {
-    return;
-  } else {
-    float2 e[N / 2], o[N / 2];
-#pragma unroll
-    for (int k = 0; k < N / 2; k++) {
-      e[k] = v[2 * k];
-      o[k] = v[2 * k + 1];
-    }
-    fft_reg<N / 2>(e, sgn);
-    fft_reg<N / 2>(o, sgn);
-#pragma unroll
-    for (int k = 0; k < N / 2; k++) {
-      const int m = k * (16 / N);  // W_N^k = W_16^(k * 16 / N)
-      const float2 w = make_float2(C16[m], sgn * S16[m]);
-      const float2 t = cmul(o[k], w);
-      v[k] = cadd(e[k], t);
-      v[k + N / 2] = csub(e[k], t);
-    }
-  }
-}
-
-// Pass A of the 128-point transforms of all 128 LDS lines (line l starts at
-// l * line_stride, elements at stride es) on items (line, j2), in place;
-// tw[m] = W128^m.  LINE_FAST puts consecutive lanes on consecutive lines
-// (columns: consecutive addresses), otherwise on consecutive items of one
-// line (rows).
-template <bool LINE_FAST>
-__device__ __forceinline__ void pass_a(float2 *lds, int line_stride, int es, const float2 *tw,
-                                       float sgn) {
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < PASS_A; r++) {
-    const int it = tid + r * FFT_NT;
-    const int line = LINE_FAST ? (it & 127) : (it >> 3);
-    const int j2 = LINE_FAST ? (it >> 7) : (it & 7);
-    float2 *p = lds + line * line_stride + j2 * es;
-    float2 v[16];
-#pragma unroll
-    for (int j1 = 0; j1 < 16; j1++) v[j1] = p[8 * j1 * es];
-    fft_reg<16>(v, sgn);
-#pragma unroll
-    for (int k1 = 0; k1 < 16; k1++) {
-      const float2 w = tw[(j2 * k1) & 127];
-      p[8 * k1 * es] = cmul(v[k1], make_float2(w.x, sgn * w.y));
-    }
-  }
-}
-
-// Pass-B item r of the thread: (line, k1).
-template <bool LINE_FAST>
-__device__ __forceinline__ void pass_b_item(int r, int &line, int &k1) {
-  const int it = threadIdx.x + r * FFT_NT;
-  line = LINE_FAST ? (it & 127) : (it >> 4);
-  k1 = LINE_FAST ? (it >> 7) : (it & 15);
-}
-
-// Pass B on items (line, k1): v[r][k2] = output k1 + 16 k2 of the thread's
-// r-th line, read from LDS; the caller stores it.
-template <bool LINE_FAST>
-__device__ __forceinline__ void pass_b(const float2 *lds, int line_stride, int es, float sgn,
-                                       float2 (&v)[PASS_B][8]) {
-#pragma unroll
-  for (int r = 0; r < PASS_B; r++) {
-    int line, k1;
-    pass_b_item<LINE_FAST>(r, line, k1);
-    const float2 *p = lds + line * line_stride + 8 * k1 * es;
-#pragma unroll
-    for (int j2 = 0; j2 < 8; j2++) v[r][j2] = p[j2 * es];
-    fft_reg<8>(v[r], sgn);
-  }
-}
 
 // Rows are transformed first, columns second.  The rows' pass A takes its
 // input straight from memory and the columns' pass B writes straight to
@@ -155,11 +35,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fft2d_128(
   // The phase tables are read in batches ahead of use so that their loads
   // overlap one another.
   constexpr bool mi = MI, mo = MO;  // input / output modulation
-  if (tid < FN) {
-    double s, c;
-    sincospi((double)tid / 64.0, &s, &c);  // W128^m = exp(sgn 2 pi i m / 128)
-    tw[tid] = make_float2((float)c, (float)s);
-  }
+  fft128_twiddles(tw);
   // Row pass A: the thread owns row y and its elements x = 8 j1 + j2, which
   // share the parity of y + j2 (the (-1)^(x + y) centring sign).
   const int y = tid >> 3, j2 = tid & 7;

@@ -28,6 +28,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include "fft128.h"
 #include "taps.h"
 
 namespace {
@@ -736,13 +737,13 @@ __global__ void k_tw_chunk_accum(const int4 *__restrict__ groups, const int32_t 
   }
 }
 
-// grid[window] += csub[chunk] (clipped); chunk ids of one parity class.
-__global__ void k_tw_add(const int32_t *__restrict__ chunks, const float2 *__restrict__ csub,
+// grid[window] += chunk_sub[chunk] (clipped); chunk ids of one parity class.
+__global__ void k_tw_add(const int32_t *__restrict__ chunks, const float2 *__restrict__ chunk_sub,
                          float2 *__restrict__ grid, int N, int chunk, int margin, int ncx) {
   const int c = chunks[blockIdx.y];
   const int cx = c % ncx, cy = c / ncx;
   const int x0 = cx * chunk - margin / 2, y0 = cy * chunk - margin / 2;
-  const float2 *s = csub + (int64_t)c * SGN;
+  const float2 *s = chunk_sub + (int64_t)c * SGN;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < SGN; i += gridDim.x * blockDim.x) {
     const int y = y0 + i / SG, x = x0 + i % SG;
     if (x < 0 || y < 0 || x >= N || y >= N) continue;
@@ -812,73 +813,168 @@ __global__ void k_tw_cut(const int32_t *__restrict__ chunks, const float2 *__res
   }
 }
 
-// Degrid job parts with a separable kernel of support S <= 16, one
-// visibility per thread: the part's sub-grid is staged in LDS (as in
-// k_tw_degrid_jobs); each thread loads its rows kx (sub-offset xf) and ky
-// (yf) of its w-kernel plane and sums, in FP32 and in a fixed order,
-//   v = sum_y ky[y] sum_x kx[x] G[wy + y][wx + x]
-// (the taps k = ky[y] kx[x] of synthesis.py:456 on the sub-grid window).
-// The window lies inside the sub-grid (S <= margin, checked at binning).
-__global__ __launch_bounds__(TW_NT) void k_tw_degrid_sep16(
-    const int4 *__restrict__ parts, int nb, const uint4 *__restrict__ rec,
-    const float2 *__restrict__ sub, const float2 *__restrict__ kern, int S, int64_t tsize,
-    float2 *__restrict__ vis_out, uint32_t *__restrict__ qc) {
-  constexpr int DEG_PITCH = SG + 16;  // 128-byte row padding against bank conflicts
-  __shared__ float2 sg[SG * DEG_PITCH];
-  __shared__ int s_b;
-  const int tid = threadIdx.x;
-  for (;;) {  // parts taken from the launch's queue counter qc
-    if (tid == 0) s_b = (int)atomicAdd(qc, 1u);
-    __syncthreads();
-    const int b = s_b;
-    if (b >= nb) break;
-    const int4 job = parts[b];
-    const float2 *src = sub + (int64_t)job.x * SGN;
-    static_assert(SGN % (8 * TW_NT) == 0, "staging covers the sub-grid exactly");
-    for (int i0 = tid; i0 < SGN; i0 += 8 * TW_NT) {
-      float2 gv[8];
+// One visibility from a sub-grid window in LDS (row pitch `pitch`) with
+// the separable kernel rows kx and ky of support S <= 16, summed in FP32
+// and in a fixed order:
+//   v = sum_y ky[y] sum_x kx[x] G[y][x]
+// (the taps k = ky[y] kx[x] of synthesis.py:456).  SF = 16 fixes S = 16 at
+// compile time (no per-tap branches, the rows loaded 16 bytes at a time:
+// they start at multiples of 16 elements); SF = 0 takes any S <= 16.  Both
+// sum in the same order, so they agree bit for bit at S = 16.
+template <int SF>
+__device__ __forceinline__ float2 degrid_sep_lds(const float2 *g, int pitch, const float2 *kx,
+                                                 const float2 *ky, int S) {
+  float2 kxv[16], kyv[16];
+  if constexpr (SF == 16) {
+    const float4 *kx4 = (const float4 *)kx, *ky4 = (const float4 *)ky;
 #pragma unroll
-      for (int k = 0; k < 8; k++) gv[k] = load_stream(src + i0 + k * TW_NT);
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int i = i0 + k * TW_NT;
-        sg[(i / SG) * DEG_PITCH + i % SG] = gv[k];
-      }
+    for (int x = 0; x < 8; x++) {
+      const float4 a = kx4[x], b = ky4[x];
+      kxv[2 * x] = make_float2(a.x, a.y);
+      kxv[2 * x + 1] = make_float2(a.z, a.w);
+      kyv[2 * x] = make_float2(b.x, b.y);
+      kyv[2 * x + 1] = make_float2(b.z, b.w);
     }
-    __syncthreads();
-    for (int j = job.y + tid; j < job.z; j += TW_NT) {
-      const uint4 r = rec[j];
-      const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
-      const int wx = (int)(r.x & 0xffffu), wy = (int)(r.x >> 16);
-      const float2 *kt = kern + (int64_t)r.z * tsize;
-      const float2 *kx = kt + (int64_t)xf * S, *ky = kt + (int64_t)yf * S;
-      float2 kxv[16], kyv[16];
+  } else {
+#pragma unroll
+    for (int x = 0; x < 16; x++) {
+      kxv[x] = x < S ? kx[x] : make_float2(0.f, 0.f);
+      kyv[x] = x < S ? ky[x] : make_float2(0.f, 0.f);
+    }
+  }
+  const int SS = SF == 16 ? 16 : S;
+  float ar = 0.f, ai = 0.f;
+#pragma unroll
+  for (int y = 0; y < 16; y++) {
+    if (y < SS) {
+      float rr = 0.f, ri = 0.f;
 #pragma unroll
       for (int x = 0; x < 16; x++) {
-        kxv[x] = x < S ? kx[x] : make_float2(0.f, 0.f);
-        kyv[x] = x < S ? ky[x] : make_float2(0.f, 0.f);
-      }
-      const float2 *g = sg + wy * DEG_PITCH + wx;
-      float ar = 0.f, ai = 0.f;
-#pragma unroll
-      for (int y = 0; y < 16; y++) {
-        if (y < S) {
-          float rr = 0.f, ri = 0.f;
-#pragma unroll
-          for (int x = 0; x < 16; x++) {
-            if (x < S) {
-              const float2 gv = g[y * DEG_PITCH + x];
-              rr = fmaf(gv.x, kxv[x].x, fmaf(-gv.y, kxv[x].y, rr));
-              ri = fmaf(gv.x, kxv[x].y, fmaf(gv.y, kxv[x].x, ri));
-            }
-          }
-          ar = fmaf(rr, kyv[y].x, fmaf(-ri, kyv[y].y, ar));
-          ai = fmaf(rr, kyv[y].y, fmaf(ri, kyv[y].x, ai));
+        if (x < SS) {
+          const float2 gv = g[y * pitch + x];
+          rr = fmaf(gv.x, kxv[x].x, fmaf(-gv.y, kxv[x].y, rr));
+          ri = fmaf(gv.x, kxv[x].y, fmaf(gv.y, kxv[x].x, ri));
         }
       }
-      store_stream(vis_out + r.w, make_float2(ar, ai));
+      ar = fmaf(rr, kyv[y].x, fmaf(-ri, kyv[y].y, ar));
+      ai = fmaf(rr, kyv[y].y, fmaf(ri, kyv[y].x, ai));
     }
-    __syncthreads();  // the sub-grid is read; the next part rewrites it
+  }
+  return make_float2(ar, ai);
+}
+
+// Predict with a separable kernel of support S <= 16 (SF as in
+// degrid_sep_lds), the plane sub-grids never leaving LDS.  items[i] =
+// (chunk, w-plane wp, first record, end record); a run is the items
+// roff[r] .. roff[r + 1] - 1, runs taken from the queue counter qc by
+// persistent workgroups.  Per item:
+//   1. the sub-grid fft(chunk image * exp(+i wp phi)) formed in LDS with the
+//      arithmetic of k_fft2d_128 (input sign and phase on load, rows then
+//      columns), its last column pass written back to LDS with the output
+//      sign instead of to memory.  The chunk image's rows are read at each
+//      item (consecutive items mostly share the image, which L2 then holds);
+//      carrying them in registers would take 32 registers from the degridding;
+//   2. the item's visibilities degridded from LDS one per thread
+//      (degrid_sep_lds); the window lies inside the sub-grid (S <= margin,
+//      checked at binning).
+template <int SF>
+__global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
+    const int4 *__restrict__ items, const int32_t *__restrict__ roff, int nruns,
+    const float2 *__restrict__ cimg, const double *__restrict__ phi,
+    const uint4 *__restrict__ rec, const float2 *__restrict__ kern, int S, int64_t tsize,
+    float2 *__restrict__ vis_out, uint32_t *__restrict__ qc) {
+  __shared__ float2 lds[FN * FP];
+  __shared__ float2 tw[FN];
+  __shared__ int s_run;
+  const int tid = threadIdx.x;
+  constexpr float sgn = -1.0f;  // forward transform
+  fft128_twiddles(tw);
+  // Row pass A: the thread owns row y and its elements x = 8 j1 + j2, which
+  // share the parity of y + j2 (the (-1)^(x + y) centring sign).
+  const int y = tid >> 3, j2 = tid & 7;
+  for (;;) {
+    if (tid == 0) s_run = (int)atomicAdd(qc, 1u);
+    __syncthreads();
+    const int run = s_run;
+    if (run >= nruns) break;
+    const int i1 = roff[run + 1];
+    for (int i = roff[run]; i < i1; i++) {
+      const int4 it = items[i];
+      // an offset the compiler cannot see through keeps the twiddle and
+      // phase-table loads inside the loop (as in k_fft2d_128)
+      int zero = 0;
+      asm volatile("" : "+s"(zero));
+      const float2 *twz = tw + zero;
+      {
+        float2 v[16];
+        double ph[16];
+        const float2 *row = cimg + (int64_t)it.x * SGN + y * FN + j2;
+        const double *prow = phi + zero + y * FN + j2;
+#pragma unroll
+        for (int j1 = 0; j1 < 16; j1++) {
+          v[j1] = row[8 * j1];
+          ph[j1] = prow[8 * j1];
+        }
+        __syncthreads();  // tw written; the previous item's LDS reads done
+        const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
+        const double kin = (double)it.y;
+#pragma unroll
+        for (int j1 = 0; j1 < 16; j1++) {
+          const float2 t = rotate(v[j1], kin, ph[j1]);
+          v[j1] = make_float2(neg * t.x, neg * t.y);
+        }
+        fft_reg<16>(v, sgn);
+        float2 *p = lds + y * FP + j2;
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) {
+          const float2 w = twz[(j2 * k1) & 127];
+          p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
+        }
+      }
+      __syncthreads();
+      {
+        // row pass B, written back to the natural positions once all are read
+        float2 v[PASS_B][8];
+        pass_b<false>(lds, FP, 1, sgn, v);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < PASS_B; r++) {
+          int line, k1;
+          pass_b_item<false>(r, line, k1);
+#pragma unroll
+          for (int k2 = 0; k2 < 8; k2++) lds[line * FP + k1 + 16 * k2] = v[r][k2];
+        }
+      }
+      __syncthreads();
+      pass_a<true>(lds, 1, FP, twz, sgn);  // columns
+      __syncthreads();
+      {
+        // column pass B back to LDS with the output sign: item (column x,
+        // k1) holds rows y = k1 + 16 k2, which share the parity of x + k1
+        float2 v[PASS_B][8];
+        pass_b<true>(lds, 1, FP, sgn, v);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < PASS_B; r++) {
+          int x, k1;
+          pass_b_item<true>(r, x, k1);
+          const float sc = ((x + k1) & 1) ? -1.0f : 1.0f;
+#pragma unroll
+          for (int k2 = 0; k2 < 8; k2++)
+            lds[(k1 + 16 * k2) * FP + x] = make_float2(v[r][k2].x * sc, v[r][k2].y * sc);
+        }
+      }
+      __syncthreads();
+      for (int j = it.z + tid; j < it.w; j += FFT_NT) {
+        const uint4 r = rec[j];
+        const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+        const int wx = (int)(r.x & 0xffffu), wy = (int)(r.x >> 16);
+        const float2 *kt = kern + (int64_t)r.z * tsize;
+        store_stream(vis_out + r.w, degrid_sep_lds<SF>(lds + wy * FP + wx, FP,
+                                                       kt + (int64_t)xf * S,
+                                                       kt + (int64_t)yf * S, S));
+      }
+    }
   }
 }
 
@@ -987,13 +1083,56 @@ struct GridSched {
 
 // Degridding schedule: batches of at most JOB_BATCH jobs (job b of a batch in
 // slot b), every job cut into parts of at most PART_VIS visibilities sharing
-// its slot; per job its w-plane and chunk.
+// its slot; per job its w-plane and chunk.  For k_tw_predict_sep: the jobs
+// as items (chunk, w-plane, first, end) in job order, cut into nruns runs
+// of about equal cost (items roff[r] .. roff[r + 1] - 1); a job that straddles
+// a run boundary becomes one item in each run, each transforming it.
 struct DegridSched {
   int jb = 0;
   std::vector<size_t> poff;  // first part of each batch, and the end
   int32_t *kw = nullptr, *jchunk = nullptr, *chunks = nullptr;
   int4 *parts = nullptr;
+  int4 *items = nullptr;
+  int32_t *roff = nullptr;
+  int nruns = 0;
 };
+
+// Cost of a predict item in visibility units: TW_FFT_COST for its sub-grid
+// transform plus one per visibility.  Runs of about 1 / TW_RUNS_PER_CU of a
+// compute unit's share: the queue evens out the model's error, and a job
+// larger than a run is cut over several runs (each transforming it).
+#ifndef TW_FFT_COST
+#define TW_FFT_COST 1024
+#endif
+#ifndef TW_RUNS_PER_CU
+#define TW_RUNS_PER_CU 8
+#endif
+constexpr int TW_MIN_SPLIT = 256;  // fewest visibilities of an item cut at a run's end
+
+void tw_predict_runs(const std::vector<int4> &hjobs, int nwp, int wp_min, int num_cu,
+                     std::vector<int4> &items, std::vector<int32_t> &roff) {
+  items.clear();
+  roff.assign(1, 0);
+  double total = 0.0;
+  for (const int4 &j : hjobs) total += TW_FFT_COST + (double)(j.z - j.y);
+  const double target = total / std::max(1, TW_RUNS_PER_CU * num_cu);
+  double acc = 0.0;  // cost of the open run
+  for (const int4 &j : hjobs) {
+    const int chunk = j.x / nwp, wp = j.x % nwp + wp_min;
+    for (int v = j.y; v < j.z;) {
+      if (acc > 0.0 && acc + TW_FFT_COST + TW_MIN_SPLIT > target) {
+        roff.push_back((int32_t)items.size());
+        acc = 0.0;
+      }
+      const double room = std::max((double)TW_MIN_SPLIT, target - acc - TW_FFT_COST);
+      const int e = (double)(j.z - v) > room ? v + (int)room : j.z;
+      items.push_back(make_int4(chunk, wp, v, e));
+      acc += TW_FFT_COST + (double)(e - v);
+      v = e;
+    }
+  }
+  if ((int)items.size() > roff.back()) roff.push_back((int32_t)items.size());
+}
 
 struct Plan {
   TowerArgs a;
@@ -1247,22 +1386,33 @@ int tw_degrid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
     hk[j] = p.hjobs[j].x % a.nwp + a.wp_min;
     hc[j] = p.hjobs[j].x / a.nwp;
   }
-  const size_t bytes = sizeof(int4) * (parts.size() + 16) +
-                       sizeof(int32_t) * (2 * (size_t)p.njobs + p.chunks.size()) + 256;
+  std::vector<int4> items;
+  std::vector<int32_t> roff;
+  tw_predict_runs(p.hjobs, a.nwp, a.wp_min, ctx->num_cu, items, roff);
+  g.nruns = (int)roff.size() - 1;
+  const size_t bytes = sizeof(int4) * (parts.size() + items.size() + 32) +
+                       sizeof(int32_t) * (2 * (size_t)p.njobs + p.chunks.size() + roff.size()) +
+                       256;
   char *d = (char *)ws_get_in(ctx, set, WS_GRIDQ, bytes);
   if (!d) return CROC_ENOMEM;
   g.parts = (int4 *)d;
-  g.kw = (int32_t *)(g.parts + parts.size() + 16);
+  g.items = g.parts + parts.size() + 16;
+  g.kw = (int32_t *)(g.items + items.size() + 16);
   g.jchunk = g.kw + p.njobs;
   g.chunks = g.jchunk + p.njobs;
+  g.roff = g.chunks + p.chunks.size();
   hipStream_t s = ctx->stream;
   if (!parts.empty())
     CROC_HIP(hipMemcpyAsync(g.parts, parts.data(), sizeof(int4) * parts.size(),
+                            hipMemcpyHostToDevice, s));
+  if (!items.empty())
+    CROC_HIP(hipMemcpyAsync(g.items, items.data(), sizeof(int4) * items.size(),
                             hipMemcpyHostToDevice, s));
   int rc;
   if ((rc = upload_i32(ctx, g.kw, hk))) return rc;
   if ((rc = upload_i32(ctx, g.jchunk, hc))) return rc;
   if ((rc = upload_i32(ctx, g.chunks, p.chunks))) return rc;
+  if ((rc = upload_i32(ctx, g.roff, roff))) return rc;
   CROC_HIP(hipStreamSynchronize(s));
   p.ds_ready = true;
   return CROC_OK;
@@ -1435,17 +1585,17 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
   hipStream_t s = ctx->stream;
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
                                                 sizeof(uint32_t) * (ds.poff.size() + 64));
+  const int64_t tsize = separable ? (int64_t)a.Q * a.S : (int64_t)a.Q * a.Q * a.S * a.S;
+  // Separable kernels of support <= 16 transform and degrid each plane
+  // sub-grid in LDS (k_tw_predict_sep): no sub-grid slots.
+  const bool sep16 = !a.nearest && separable && a.S <= 16;
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
-  float2 *stack =
-      (float2 *)ws_get(ctx, WS_SUB, cimg_bytes + sizeof(float2) * SGN * (size_t)ds.jb);
+  float2 *stack = (float2 *)ws_get(
+      ctx, WS_SUB, cimg_bytes + (sep16 ? 0 : sizeof(float2) * SGN * (size_t)ds.jb));
   if (!aux || !stack) return CROC_ENOMEM;
   double *phi = (double *)(aux + 256);
   uint32_t *qc = (uint32_t *)(phi + SGN);  // a part queue per batch
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
-  const int64_t tsize = separable ? (int64_t)a.Q * a.S : (int64_t)a.Q * a.Q * a.S * a.S;
-  // Separable kernels of support <= 16 degrid one visibility per thread
-  // (k_tw_degrid_sep16).
-  const bool sep16 = !a.nearest && separable && a.S <= 16;
   k_tw_phi<<<SGN / 256, 256, 0, s>>>(phi, a.theta, a.wstep);
   if ((rc = zero_u32_async(ctx, qc, (int64_t)ds.poff.size()))) return rc;
   dim3 g(16, (unsigned)p.chunks.size());
@@ -1457,7 +1607,21 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
   rc = fft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, +1, 1.0f / (float)SGN,
                     nullptr, nullptr, 0, nullptr, nullptr, 0, ds.chunks, ds.chunks);
   if (rc) return rc;
-  for (int j0 = 0, bi = 0; j0 < p.njobs; j0 += ds.jb, bi++) {
+  if (sep16) {
+    const int G = std::min(ds.nruns, ctx->num_cu);  // one per CU (LDS)
+    if ((rc = tap_timer_begin(ctx, 1))) return rc;
+    if (a.S == 16)
+      k_tw_predict_sep<16><<<G, FFT_NT, 0, s>>>(ds.items, ds.roff, ds.nruns, cimg, phi, p.rec,
+                                                (const float2 *)kern, a.S, tsize,
+                                                (float2 *)vis_out, qc);
+    else
+      k_tw_predict_sep<0><<<G, FFT_NT, 0, s>>>(ds.items, ds.roff, ds.nruns, cimg, phi, p.rec,
+                                               (const float2 *)kern, a.S, tsize,
+                                               (float2 *)vis_out, qc);
+    CROC_HIP(hipGetLastError());
+    if ((rc = tap_timer_end(ctx, 1))) return rc;
+  }
+  for (int j0 = 0, bi = 0; !sep16 && j0 < p.njobs; j0 += ds.jb, bi++) {
     const int nb = std::min(ds.jb, p.njobs - j0);
     // plane sub-grid = fft(chunk image * exp(+i wp phi))
     rc = fft2d_launch(ctx, cimg, sub, SG, nb, SGN, -1, 1.0f, phi, ds.kw + j0, +1, nullptr, nullptr,
@@ -1466,11 +1630,7 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
     const int npart = (int)(ds.poff[bi + 1] - ds.poff[bi]);
     const int G = std::min(npart, ctx->num_cu);  // one per CU (LDS)
     if ((rc = tap_timer_begin(ctx, 1))) return rc;
-    if (sep16)
-      k_tw_degrid_sep16<<<G, TW_NT, 0, s>>>(ds.parts + ds.poff[bi], npart, p.rec, sub,
-                                            (const float2 *)kern, a.S, tsize, (float2 *)vis_out,
-                                            qc + bi);
-    else if (a.nearest || !separable)
+    if (a.nearest || !separable)
       k_tw_degrid_jobs<false><<<G, TW_NT, 0, s>>>(ds.parts + ds.poff[bi], npart, p.rec, sub,
                                                   (const float2 *)kern, a.Q, a.S, tsize,
                                                   a.nearest, (float2 *)vis_out, qc + bi);
