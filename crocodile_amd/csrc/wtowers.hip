@@ -813,44 +813,89 @@ __global__ void k_tw_cut(const int32_t *__restrict__ chunks, const float2 *__res
   }
 }
 
-// One visibility from a sub-grid window in LDS (row pitch `pitch`) with
-// the separable kernel rows kx and ky of support S <= 16, summed in FP32
-// and in a fixed order:
-//   v = sum_y ky[y] sum_x kx[x] G[y][x]
-// (the taps k = ky[y] kx[x] of synthesis.py:456).  SF = 16 fixes S = 16 at
-// compile time (no per-tap branches, the rows loaded 16 bytes at a time:
-// they start at multiples of 16 elements); SF = 0 takes any S <= 16.  Both
-// sum in the same order, so they agree bit for bit at S = 16.
-template <int SF>
+// Separable degridding from a sub-grid window in LDS (row pitch `pitch`),
+// the taps k = ky[y] kx[x] of synthesis.py:456, in FP32:
+//   v = sum_y ky[y] r[y],  r[y] = sum_x kx[x] G[y][x].
+// Support 16: r[y] summed over x in order; the 16 products ky[y] r[y]
+// summed as a binary tree ((p0 + p1) + (p2 + p3)) + ...  degrid_rows16<R>
+// gives the subtree of R consecutive rows (R = 16 / lanes per visibility),
+// the lanes' subtrees then being added by shuffles, so every split gives
+// the same value bit for bit.  Rows kx and ky start at multiples of 16
+// elements: 16-byte loads.
+__device__ __forceinline__ float2 tap_row16(const float2 *g, const float2 (&kxv)[16]) {
+  float2 gv[16];  // the row's cells read before the products: reads in flight together
+#pragma unroll
+  for (int x = 0; x < 16; x++) gv[x] = g[x];
+  float rr = 0.f, ri = 0.f;
+#pragma unroll
+  for (int x = 0; x < 16; x++) {
+    rr = fmaf(gv[x].x, kxv[x].x, fmaf(-gv[x].y, kxv[x].y, rr));
+    ri = fmaf(gv[x].x, kxv[x].y, fmaf(gv[x].y, kxv[x].x, ri));
+  }
+  return make_float2(rr, ri);
+}
+
+template <int R>
+__device__ __forceinline__ float2 degrid_rows16(const float2 *g, int pitch, const float2 *kx,
+                                                const float2 *ky) {
+  float2 kxv[16], kyv[R];
+  const float4 *kx4 = (const float4 *)kx;
+#pragma unroll
+  for (int x = 0; x < 8; x++) {
+    const float4 a = kx4[x];
+    kxv[2 * x] = make_float2(a.x, a.y);
+    kxv[2 * x + 1] = make_float2(a.z, a.w);
+  }
+  if constexpr (R == 1) {
+    kyv[0] = ky[0];
+  } else {
+    const float4 *ky4 = (const float4 *)ky;
+#pragma unroll
+    for (int y = 0; y < R / 2; y++) {
+      const float4 b = ky4[y];
+      kyv[2 * y] = make_float2(b.x, b.y);
+      kyv[2 * y + 1] = make_float2(b.z, b.w);
+    }
+  }
+  // the tree built as a binary counter: acc[l] holds a finished subtree of
+  // 2^l rows waiting for its right neighbour
+  float2 acc[5];
+#pragma unroll
+  for (int y = 0; y < R; y++) {
+    const float2 t = tap_row16(g + y * pitch, kxv);
+    float2 p = make_float2(fmaf(t.x, kyv[y].x, -(t.y * kyv[y].y)),
+                           fmaf(t.x, kyv[y].y, t.y * kyv[y].x));
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      if ((y >> l) & 1) {
+        p = make_float2(acc[l].x + p.x, acc[l].y + p.y);
+      } else {
+        acc[l] = p;
+        break;
+      }
+    }
+  }
+  constexpr int top = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0;
+  return acc[top];
+}
+
+// Any support S <= 16: rows and cells in order, v accumulated row by row.
 __device__ __forceinline__ float2 degrid_sep_lds(const float2 *g, int pitch, const float2 *kx,
                                                  const float2 *ky, int S) {
   float2 kxv[16], kyv[16];
-  if constexpr (SF == 16) {
-    const float4 *kx4 = (const float4 *)kx, *ky4 = (const float4 *)ky;
 #pragma unroll
-    for (int x = 0; x < 8; x++) {
-      const float4 a = kx4[x], b = ky4[x];
-      kxv[2 * x] = make_float2(a.x, a.y);
-      kxv[2 * x + 1] = make_float2(a.z, a.w);
-      kyv[2 * x] = make_float2(b.x, b.y);
-      kyv[2 * x + 1] = make_float2(b.z, b.w);
-    }
-  } else {
-#pragma unroll
-    for (int x = 0; x < 16; x++) {
-      kxv[x] = x < S ? kx[x] : make_float2(0.f, 0.f);
-      kyv[x] = x < S ? ky[x] : make_float2(0.f, 0.f);
-    }
+  for (int x = 0; x < 16; x++) {
+    kxv[x] = x < S ? kx[x] : make_float2(0.f, 0.f);
+    kyv[x] = x < S ? ky[x] : make_float2(0.f, 0.f);
   }
-  const int SS = SF == 16 ? 16 : S;
   float ar = 0.f, ai = 0.f;
 #pragma unroll
   for (int y = 0; y < 16; y++) {
-    if (y < SS) {
+    if (y < S) {
       float rr = 0.f, ri = 0.f;
 #pragma unroll
       for (int x = 0; x < 16; x++) {
-        if (x < SS) {
+        if (x < S) {
           const float2 gv = g[y * pitch + x];
           rr = fmaf(gv.x, kxv[x].x, fmaf(-gv.y, kxv[x].y, rr));
           ri = fmaf(gv.x, kxv[x].y, fmaf(gv.y, kxv[x].x, ri));
@@ -863,8 +908,38 @@ __device__ __forceinline__ float2 degrid_sep_lds(const float2 *g, int pitch, con
   return make_float2(ar, ai);
 }
 
-// Predict with a separable kernel of support S <= 16 (SF as in
-// degrid_sep_lds), the plane sub-grids never leaving LDS.  items[i] =
+// One round of support-16 degridding with T lanes per visibility: the
+// visibilities j0 + tid / T (below j1), lane tid % T taking window rows
+// [R (tid % T), R (tid % T + 1)), R = 16 / T.  Every lane takes part in
+// the shuffles.
+template <int T>
+__device__ __forceinline__ void degrid_round16(const float2 *lds, int j0, int j1,
+                                               const uint4 *__restrict__ rec,
+                                               const float2 *__restrict__ kern, int64_t tsize,
+                                               float2 *__restrict__ vis_out) {
+  constexpr int R = 16 / T;
+  const int tid = threadIdx.x, j = j0 + tid / T, sub = tid % T;
+  float2 v = make_float2(0.f, 0.f);
+  uint4 r = make_uint4(0u, 0u, 0u, 0u);
+  if (j < j1) {
+    r = rec[j];
+    const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+    const int wx = (int)(r.x & 0xffffu), wy = (int)(r.x >> 16);
+    const float2 *kt = kern + (int64_t)r.z * tsize;
+    v = degrid_rows16<R>(lds + (wy + R * sub) * FP + wx, FP, kt + (int64_t)xf * 16,
+                         kt + (int64_t)yf * 16 + R * sub);
+  }
+#pragma unroll
+  for (int off = 1; off < T; off <<= 1) {
+    v.x += __shfl_xor(v.x, off, 64);
+    v.y += __shfl_xor(v.y, off, 64);
+  }
+  if (j < j1 && sub == 0) store_stream(vis_out + r.w, v);
+}
+
+// Predict with a separable kernel of support S <= 16 (SF = 16: S fixed at
+// 16, degrid_round16; SF = 0: any S, degrid_sep_lds), the plane sub-grids
+// never leaving LDS.  items[i] =
 // (chunk, w-plane wp, first record, end record); a run is the items
 // roff[r] .. roff[r + 1] - 1, runs taken from the queue counter qc by
 // persistent workgroups.  Per item:
@@ -874,9 +949,8 @@ __device__ __forceinline__ float2 degrid_sep_lds(const float2 *g, int pitch, con
 //      sign instead of to memory.  The chunk image's rows are read at each
 //      item (consecutive items mostly share the image, which L2 then holds);
 //      carrying them in registers would take 32 registers from the degridding;
-//   2. the item's visibilities degridded from LDS one per thread
-//      (degrid_sep_lds); the window lies inside the sub-grid (S <= margin,
-//      checked at binning).
+//   2. the item's visibilities degridded from LDS; the window lies inside
+//      the sub-grid (S <= margin, checked at binning).
 template <int SF>
 __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
     const int4 *__restrict__ items, const int32_t *__restrict__ roff, int nruns,
@@ -965,14 +1039,39 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
         }
       }
       __syncthreads();
-      for (int j = it.z + tid; j < it.w; j += FFT_NT) {
-        const uint4 r = rec[j];
-        const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
-        const int wx = (int)(r.x & 0xffffu), wy = (int)(r.x >> 16);
-        const float2 *kt = kern + (int64_t)r.z * tsize;
-        store_stream(vis_out + r.w, degrid_sep_lds<SF>(lds + wy * FP + wx, FP,
-                                                       kt + (int64_t)xf * S,
-                                                       kt + (int64_t)yf * S, S));
+      if constexpr (SF == 16) {
+        // rounds of the workgroup's width; a round with fewer visibilities
+        // left than threads gives each up to 16 lanes (one window row each
+        // at most), so that short items are not one long chain per thread
+        for (int j0 = it.z; j0 < it.w;) {
+          const int rem = it.w - j0;
+          if (rem > FFT_NT / 2) {
+            degrid_round16<1>(lds, j0, it.w, rec, kern, tsize, vis_out);
+            j0 += FFT_NT;
+          } else if (rem > FFT_NT / 4) {
+            degrid_round16<2>(lds, j0, it.w, rec, kern, tsize, vis_out);
+            j0 += FFT_NT / 2;
+          } else if (rem > FFT_NT / 8) {
+            degrid_round16<4>(lds, j0, it.w, rec, kern, tsize, vis_out);
+            j0 += FFT_NT / 4;
+          } else if (rem > FFT_NT / 16) {
+            degrid_round16<8>(lds, j0, it.w, rec, kern, tsize, vis_out);
+            j0 += FFT_NT / 8;
+          } else {
+            degrid_round16<16>(lds, j0, it.w, rec, kern, tsize, vis_out);
+            j0 += FFT_NT / 16;
+          }
+        }
+      } else {
+        for (int j = it.z + tid; j < it.w; j += FFT_NT) {
+          const uint4 r = rec[j];
+          const int xf = (int)(r.y & 0xffffu), yf = (int)(r.y >> 16);
+          const int wx = (int)(r.x & 0xffffu), wy = (int)(r.x >> 16);
+          const float2 *kt = kern + (int64_t)r.z * tsize;
+          store_stream(vis_out + r.w, degrid_sep_lds(lds + wy * FP + wx, FP,
+                                                     kt + (int64_t)xf * S,
+                                                     kt + (int64_t)yf * S, S));
+        }
       }
     }
   }
