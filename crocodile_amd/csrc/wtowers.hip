@@ -737,6 +737,133 @@ __global__ void k_tw_chunk_accum(const int4 *__restrict__ groups, const int32_t 
   }
 }
 
+// Job images of a batch summed per chunk without writing them out.  runs[r]
+// = (chunk, first job, end job, out) over the batch's jobs b (sub-grid slot
+// jslot[b], w-plane kw[b]).  Per job the image
+//   ifft(sub[jslot[b]]) * exp(-i kw[b] phi)
+// is formed with the arithmetic of k_fft2d_128 (rows then columns, the
+// 1 / 128^2 scale, centring sign and phase applied as the last column pass
+// leaves LDS) and added, in job order, to the thread's 16 output pixels in
+// registers (the last pass gives every thread the same pixels for every
+// job).  out < 0: cimg[chunk] += the run's sum; otherwise the sum goes to
+// sub[out], the slot of the run's first job, which only this run reads (a
+// later k_tw_chunk_accum adds those sums to cimg in run order).  The next
+// job's rows are loaded during the current transform.  Runs are taken from
+// the queue counter qc.
+__global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
+    const int4 *__restrict__ runs, int nruns, const int32_t *__restrict__ jslot,
+    const int32_t *__restrict__ kw, const double *__restrict__ phi, float2 *sub,
+    float2 *__restrict__ cimg, uint32_t *__restrict__ qc) {
+  __shared__ float2 lds[FN * FP];
+  __shared__ float2 tw[FN];
+  __shared__ int s_run;
+  const int tid = threadIdx.x;
+  constexpr float sgn = 1.0f;  // inverse transform
+  constexpr float out_scale = 1.0f / (float)SGN;
+  fft128_twiddles(tw);
+  const int y = tid >> 3, j2 = tid & 7;
+  for (;;) {
+    if (tid == 0) s_run = (int)atomicAdd(qc, 1u);
+    __syncthreads();
+    const int run = s_run;
+    if (run >= nruns) break;
+    const int4 rn = runs[run];
+    float2 acc[PASS_B][8];
+#pragma unroll
+    for (int r = 0; r < PASS_B; r++)
+#pragma unroll
+      for (int k2 = 0; k2 < 8; k2++) acc[r][k2] = make_float2(0.f, 0.f);
+    float2 nxt[16];  // the thread's row elements of the next job's sub-grid
+    {
+      const float2 *row = sub + (int64_t)jslot[rn.y] * SGN + y * FN + j2;
+#pragma unroll
+      for (int j1 = 0; j1 < 16; j1++) nxt[j1] = row[8 * j1];
+    }
+    for (int b = rn.y; b < rn.z; b++) {
+      int zero = 0;
+      asm volatile("" : "+s"(zero));
+      const float2 *twz = tw + zero;
+      {
+        float2 v[16];
+#pragma unroll
+        for (int j1 = 0; j1 < 16; j1++) v[j1] = nxt[j1];
+        __syncthreads();  // tw written; the previous job's LDS reads done
+        const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
+#pragma unroll
+        for (int j1 = 0; j1 < 16; j1++) v[j1] = make_float2(neg * v[j1].x, neg * v[j1].y);
+        fft_reg<16>(v, sgn);
+        float2 *p = lds + y * FP + j2;
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) {
+          const float2 w = twz[(j2 * k1) & 127];
+          p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
+        }
+        if (b + 1 < rn.z) {
+          const float2 *row = sub + (int64_t)jslot[b + 1] * SGN + y * FN + j2;
+#pragma unroll
+          for (int j1 = 0; j1 < 16; j1++) nxt[j1] = row[8 * j1];
+        }
+      }
+      __syncthreads();
+      {
+        float2 v[PASS_B][8];
+        pass_b<false>(lds, FP, 1, sgn, v);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < PASS_B; r++) {
+          int line, k1;
+          pass_b_item<false>(r, line, k1);
+#pragma unroll
+          for (int k2 = 0; k2 < 8; k2++) lds[line * FP + k1 + 16 * k2] = v[r][k2];
+        }
+      }
+      __syncthreads();
+      pass_a<true>(lds, 1, FP, twz, sgn);  // columns
+      __syncthreads();
+      {
+        // column pass B: item (column x, k1) holds rows y = k1 + 16 k2,
+        // which share the parity of x + k1
+        const double kout = -(double)kw[b];
+#pragma unroll
+        for (int r = 0; r < PASS_B; r++) {
+          int x, k1;
+          pass_b_item<true>(r, x, k1);
+          double ph[8];
+#pragma unroll
+          for (int k2 = 0; k2 < 8; k2++) ph[k2] = phi[zero + (k1 + 16 * k2) * FN + x];
+          float2 v[8];
+          const float2 *p = lds + x + 8 * k1 * FP;
+#pragma unroll
+          for (int jj = 0; jj < 8; jj++) v[jj] = p[jj * FP];
+          fft_reg<8>(v, sgn);
+          const float sc = ((x + k1) & 1) ? -out_scale : out_scale;
+#pragma unroll
+          for (int k2 = 0; k2 < 8; k2++) {
+            const float2 t = rotate(make_float2(v[k2].x * sc, v[k2].y * sc), kout, ph[k2]);
+            acc[r][k2] = make_float2(acc[r][k2].x + t.x, acc[r][k2].y + t.y);
+          }
+        }
+      }
+    }
+    float2 *dst = rn.w < 0 ? cimg + (int64_t)rn.x * SGN : sub + (int64_t)rn.w * SGN;
+#pragma unroll
+    for (int r = 0; r < PASS_B; r++) {
+      int x, k1;
+      pass_b_item<true>(r, x, k1);
+#pragma unroll
+      for (int k2 = 0; k2 < 8; k2++) {
+        const int pix = (k1 + 16 * k2) * FN + x;
+        float2 o = acc[r][k2];
+        if (rn.w < 0) {
+          const float2 c = dst[pix];
+          o = make_float2(c.x + o.x, c.y + o.y);
+        }
+        dst[pix] = o;
+      }
+    }
+  }
+}
+
 // grid[window] += chunk_sub[chunk] (clipped); chunk ids of one parity class.
 __global__ void k_tw_add(const int32_t *__restrict__ chunks, const float2 *__restrict__ chunk_sub,
                          float2 *__restrict__ grid, int N, int chunk, int margin, int ncx) {
@@ -1167,15 +1294,20 @@ __global__ void k_tw_phi(double *phi, double theta, double wstep) {
 // (slot, first, end), the multi-part jobs (first slot, parts) and the chunk
 // groups (chunk, first job, end job); per job its w-plane and slot; the
 // chunks in four parity classes for the final adds.
+//
+// The job images of a batch are summed per chunk by k_tw_image_runs over
+// runs of at most `per` consecutive jobs of one chunk, longest runs first;
+// a chunk's later runs leave their sums in pslot, added to the chunk image
+// in run order by k_tw_chunk_accum over rgroups (chunk, first, end).
 struct GridSched {
   struct Batch {
     int j0, nj;
-    size_t p0, np, m0, nm, g0, ng;
+    size_t p0, np, m0, nm, g0, ng, r0, nr, q0, nq;
   };
   std::vector<Batch> batches;
   int max_slots = 0;
-  int32_t *kw = nullptr, *chunks = nullptr, *jslot = nullptr, *par = nullptr;
-  int4 *parts = nullptr, *groups = nullptr;
+  int32_t *kw = nullptr, *chunks = nullptr, *jslot = nullptr, *par = nullptr, *pslot = nullptr;
+  int4 *parts = nullptr, *groups = nullptr, *runs = nullptr, *rgroups = nullptr;
   int2 *multi = nullptr;
   size_t npar[4] = {0, 0, 0, 0};
 };
@@ -1399,7 +1531,7 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
     const int np = std::min(std::max(1, (n + PART_VIS - 1) / PART_VIS), JOB_BATCH / 4);
     if (g.batches.empty() || slots + np > JOB_BATCH) {
       g.batches.push_back(GridSched::Batch{j, 0, parts.size(), 0, multi.size(), 0,
-                                           groups.size(), 0});
+                                           groups.size(), 0, 0, 0, 0, 0});
       slots = 0;
     }
     GridSched::Batch &bt = g.batches.back();
@@ -1423,6 +1555,29 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
     bt.np = parts.size() - bt.p0;
     bt.nm = multi.size() - bt.m0;
   }
+  // summing runs: a batch's chunk groups cut into runs of at most `per`
+  // jobs (about two runs per compute unit), queued longest first
+  std::vector<int4> runs, rgroups;
+  std::vector<int32_t> pslot;
+  for (GridSched::Batch &bt : g.batches) {
+    const int per = std::min(8, std::max(1, (bt.nj + 2 * ctx->num_cu - 1) / (2 * ctx->num_cu)));
+    bt.r0 = runs.size();
+    bt.q0 = rgroups.size();
+    for (size_t gi = bt.g0; gi < bt.g0 + bt.ng; gi++) {
+      const int4 gr = groups[gi];
+      const size_t q = pslot.size();
+      for (int b = gr.y; b < gr.z; b += per) {
+        const bool first = b == gr.y;
+        runs.push_back(make_int4(gr.x, b, std::min(gr.z, b + per), first ? -1 : jslot[bt.j0 + b]));
+        if (!first) pslot.push_back(jslot[bt.j0 + b]);
+      }
+      if (pslot.size() > q) rgroups.push_back(make_int4(gr.x, (int)q, (int)pslot.size(), 0));
+    }
+    bt.nr = runs.size() - bt.r0;
+    bt.nq = rgroups.size() - bt.q0;
+    std::stable_sort(runs.begin() + bt.r0, runs.end(),
+                     [](const int4 &u, const int4 &v) { return u.z - u.y > v.z - v.y; });
+  }
   // parity classes of the chunks: one add launch each, stream order keeps
   // overlapping margins of different classes apart
   std::vector<int32_t> par[4], allpar;
@@ -1431,18 +1586,22 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
     allpar.insert(allpar.end(), par[q].begin(), par[q].end());
     g.npar[q] = par[q].size();
   }
-  const size_t n32 = 2 * (size_t)p.njobs + 2 * p.chunks.size();
-  const size_t bytes = sizeof(int4) * (parts.size() + groups.size()) +
+  const size_t n32 = 2 * (size_t)p.njobs + 2 * p.chunks.size() + pslot.size();
+  const size_t bytes = sizeof(int4) * (parts.size() + groups.size() + runs.size() +
+                                       rgroups.size()) +
                        sizeof(int2) * multi.size() + sizeof(int32_t) * n32 + 256;
   char *d = (char *)ws_get_in(ctx, set, WS_TILES, bytes);
   if (!d) return CROC_ENOMEM;
   g.parts = (int4 *)d;
   g.groups = g.parts + parts.size();
-  g.multi = (int2 *)(g.groups + groups.size());
+  g.runs = g.groups + groups.size();
+  g.rgroups = g.runs + runs.size();
+  g.multi = (int2 *)(g.rgroups + rgroups.size());
   g.kw = (int32_t *)(g.multi + multi.size());
   g.jslot = g.kw + p.njobs;
   g.chunks = g.jslot + p.njobs;
   g.par = g.chunks + p.chunks.size();
+  g.pslot = g.par + p.chunks.size();
   hipStream_t s = ctx->stream;
   if (!parts.empty())
     CROC_HIP(hipMemcpyAsync(g.parts, parts.data(), sizeof(int4) * parts.size(),
@@ -1453,11 +1612,18 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
   if (!multi.empty())
     CROC_HIP(hipMemcpyAsync(g.multi, multi.data(), sizeof(int2) * multi.size(),
                             hipMemcpyHostToDevice, s));
+  if (!runs.empty())
+    CROC_HIP(hipMemcpyAsync(g.runs, runs.data(), sizeof(int4) * runs.size(),
+                            hipMemcpyHostToDevice, s));
+  if (!rgroups.empty())
+    CROC_HIP(hipMemcpyAsync(g.rgroups, rgroups.data(), sizeof(int4) * rgroups.size(),
+                            hipMemcpyHostToDevice, s));
   int rc;
   if ((rc = upload_i32(ctx, g.kw, hk))) return rc;
   if ((rc = upload_i32(ctx, g.jslot, jslot))) return rc;
   if ((rc = upload_i32(ctx, g.chunks, p.chunks))) return rc;
   if ((rc = upload_i32(ctx, g.par, allpar))) return rc;
+  if ((rc = upload_i32(ctx, g.pslot, pslot))) return rc;
   // the host vectors above end here: wait for their copies
   CROC_HIP(hipStreamSynchronize(s));
   p.gs_ready = true;
@@ -1561,7 +1727,7 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
   const int64_t tsize = separable ? Q * S : Q * Q * S * S;  // elements per w-kernel plane
   const int64_t nk = kern ? tsize * a.nwk : 0;
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
-                                                sizeof(uint32_t) * (gs.batches.size() + 64) +
+                                                sizeof(uint32_t) * (2 * gs.batches.size() + 64) +
                                                 sizeof(uint32_t) * (size_t)a.ncx * a.ncx);
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
   float2 *stack =
@@ -1569,8 +1735,8 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
   if (!aux || !stack) return CROC_ENOMEM;
   TwScale *fs = (TwScale *)aux;
   double *phi = (double *)(aux + 256);
-  uint32_t *qc = (uint32_t *)(phi + SGN);
-  uint32_t *cflag = qc + gs.batches.size() + 64;  // chunks to set non-finite
+  uint32_t *qc = (uint32_t *)(phi + SGN);  // part queues, then summing-run queues
+  uint32_t *cflag = qc + 2 * gs.batches.size() + 64;  // chunks to set non-finite
   float2 *cimg = stack, *sub = stack + (size_t)SGN * a.ncx * a.ncx;
   if ((rc = zero_u32_async(ctx, (uint32_t *)fs, sizeof(TwScale) / 4))) return rc;
   k_tw_maxima<<<ctx->num_cu, 1024, 0, s>>>(p.svis, a.nv, (const float2 *)kern, nk, fs);
@@ -1591,7 +1757,7 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
   k_tw_zero_chunks<<<dim3(16, (unsigned)p.chunks.size()), 256, 0, s>>>(gs.chunks, cimg);
   CROC_HIP(hipGetLastError());
 
-  if ((rc = zero_u32_async(ctx, qc, (int64_t)gs.batches.size()))) return rc;
+  if ((rc = zero_u32_async(ctx, qc, 2 * (int64_t)gs.batches.size()))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[0], s));
   tap_timer_reset(ctx, 0);
   for (size_t bi = 0; bi < gs.batches.size(); bi++) {
@@ -1616,13 +1782,16 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
       k_tw_sum_parts<<<dim3(ACC_SLICES, (unsigned)bt.nm), 256, 0, s>>>(gs.multi + bt.m0, sub);
       CROC_HIP(hipGetLastError());
     }
-    // image = ifft(sub[slot]) * exp(-i wp phi), in place in each job's slot
-    rc = fft2d_launch(ctx, sub, sub, SG, bt.nj, SGN, +1, 1.0f / (float)SGN, nullptr, nullptr, 0,
-                      phi, gs.kw + bt.j0, -1, gs.jslot + bt.j0, gs.jslot + bt.j0);
-    if (rc) return rc;
-    k_tw_chunk_accum<<<dim3(ACC_SLICES, (unsigned)bt.ng), 256, 0, s>>>(
-        gs.groups + bt.g0, gs.jslot + bt.j0, sub, cimg);
+    // images ifft(sub[slot]) * exp(-i wp phi) summed into the chunk images
+    k_tw_image_runs<<<(unsigned)std::min<size_t>(bt.nr, (size_t)ctx->num_cu), FFT_NT, 0, s>>>(
+        gs.runs + bt.r0, (int)bt.nr, gs.jslot + bt.j0, gs.kw + bt.j0, phi, sub, cimg,
+        qc + gs.batches.size() + bi);
     CROC_HIP(hipGetLastError());
+    if (bt.nq) {
+      k_tw_chunk_accum<<<dim3(ACC_SLICES, (unsigned)bt.nq), 256, 0, s>>>(gs.rgroups + bt.q0,
+                                                                         gs.pslot, sub, cimg);
+      CROC_HIP(hipGetLastError());
+    }
   }
   // chunk sub-grids = fft(chunk image), then parity-ordered adds
   rc = fft2d_launch(ctx, cimg, cimg, SG, (int64_t)p.chunks.size(), SGN, -1, 1.0f, nullptr,
