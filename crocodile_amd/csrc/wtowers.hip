@@ -737,25 +737,43 @@ __global__ void k_tw_chunk_accum(const int4 *__restrict__ groups, const int32_t 
   }
 }
 
-// Job images of a batch summed per chunk without writing them out.  runs[r]
-// = (chunk, first job, end job, out) over the batch's jobs b (sub-grid slot
-// jslot[b], w-plane kw[b]).  Per job the image
-//   ifft(sub[jslot[b]]) * exp(-i kw[b] phi)
+// Jobs of at most TW_SMALL_VIS visibilities are gridded by the summing
+// kernel itself (separable kernels of support S <= 16), never through a
+// sub-grid slot.
+constexpr int TW_SMALL_VIS = 64;
+
+// Job images summed per chunk without writing them out.  runs[r] = (chunk,
+// first, end, out).  Each job's image
+//   ifft(G) * exp(-i kw[job] phi)
 // is formed with the arithmetic of k_fft2d_128 (rows then columns, the
 // 1 / 128^2 scale, centring sign and phase applied as the last column pass
 // leaves LDS) and added, in job order, to the thread's 16 output pixels in
 // registers (the last pass gives every thread the same pixels for every
 // job).  out < 0: cimg[chunk] += the run's sum; otherwise the sum goes to
-// sub[out], the slot of the run's first job, which only this run reads (a
-// later k_tw_chunk_accum adds those sums to cimg in run order).  The next
-// job's rows are loaded during the current transform.  Runs are taken from
-// the queue counter qc.
+// sub[out], which only this run touches (a later k_tw_chunk_accum adds
+// those sums to cimg in run order).  Runs are taken from the queue counter
+// qc.  The run's jobs are jl[first .. end - 1]; the job sub-grid G:
+//   SMALL = false: G = sub[jslot[job]] as the gridding kernels left it; the
+//     next job's rows are loaded during the current transform;
+//   SMALL = true: (jobs[] the record ranges, at most TW_SMALL_VIS records
+//     each) the records, v kx
+//     and ky rows staged in LDS, then each thread sums the taps falling on
+//     its own row elements (row y, columns x = j2 mod 8: at most two per
+//     window) into the LDS sub-grid in record order,
+//       G[wy + dy][wx + dx] += (v kx[dx]) ky[dy]
+//     (the taps of synthesis.py:456), before the transform reads them.
+template <bool SMALL>
 __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
-    const int4 *__restrict__ runs, int nruns, const int32_t *__restrict__ jslot,
-    const int32_t *__restrict__ kw, const double *__restrict__ phi, float2 *sub,
-    float2 *__restrict__ cimg, uint32_t *__restrict__ qc) {
+    const int4 *__restrict__ runs, int nruns, const int32_t *__restrict__ jl,
+    const int32_t *__restrict__ jslot, const int32_t *__restrict__ kw,
+    const double *__restrict__ phi, float2 *sub, float2 *__restrict__ cimg,
+    uint32_t *__restrict__ qc, const int4 *__restrict__ jobs, const uint4 *__restrict__ rec,
+    const float2 *__restrict__ svis, const float2 *__restrict__ kern, int S, int64_t tsize) {
+  constexpr int NS = SMALL ? TW_SMALL_VIS : 1;
   __shared__ float2 lds[FN * FP];
   __shared__ float2 tw[FN];
+  __shared__ float2 s_vk[NS][16], s_ky[NS][16];  // v kx and ky rows (zero beyond S)
+  __shared__ uint32_t s_pos[NS];                 // window corners wx | wy << 16
   __shared__ int s_run;
   const int tid = threadIdx.x;
   constexpr float sgn = 1.0f;  // inverse transform
@@ -773,9 +791,9 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
     for (int r = 0; r < PASS_B; r++)
 #pragma unroll
       for (int k2 = 0; k2 < 8; k2++) acc[r][k2] = make_float2(0.f, 0.f);
-    float2 nxt[16];  // the thread's row elements of the next job's sub-grid
-    {
-      const float2 *row = sub + (int64_t)jslot[rn.y] * SGN + y * FN + j2;
+    float2 nxt[SMALL ? 1 : 16];  // the thread's row elements of the next job's sub-grid
+    if constexpr (!SMALL) {
+      const float2 *row = sub + (int64_t)jslot[jl[rn.y]] * SGN + y * FN + j2;
 #pragma unroll
       for (int j1 = 0; j1 < 16; j1++) nxt[j1] = row[8 * j1];
     }
@@ -783,11 +801,54 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
       int zero = 0;
       asm volatile("" : "+s"(zero));
       const float2 *twz = tw + zero;
+      const int job = jl[b];
       {
         float2 v[16];
+        if constexpr (SMALL) {
+          const int4 jr = jobs[job];
+          const int n = jr.z - jr.y;
+          __syncthreads();  // tw written; the previous job's LDS reads done
+          if (tid < 16 * n) {
+            const int i = tid >> 4, c = tid & 15;
+            const uint4 r = rec[jr.y + i];
+            const float2 w = svis[jr.y + i];
+            const float2 *kt = kern + (int64_t)r.z * tsize;
+            float2 kx = make_float2(0.f, 0.f), ky = make_float2(0.f, 0.f);
+            if (c < S) {
+              kx = kt[(int64_t)(r.y & 0xffffu) * S + c];
+              ky = kt[(int64_t)(r.y >> 16) * S + c];
+            }
+            s_vk[i][c] = cmul(w, kx);
+            s_ky[i][c] = ky;
+            if (c == 0) s_pos[i] = r.x;
+          }
 #pragma unroll
-        for (int j1 = 0; j1 < 16; j1++) v[j1] = nxt[j1];
-        __syncthreads();  // tw written; the previous job's LDS reads done
+          for (int j1 = 0; j1 < 16; j1++) lds[y * FP + 8 * j1 + j2] = make_float2(0.f, 0.f);
+          __syncthreads();
+          for (int i = 0; i < n; i++) {
+            const uint32_t pos = s_pos[i];
+            const int wx = (int)(pos & 0xffffu), dy = y - (int)(pos >> 16);
+            if ((unsigned)dy < (unsigned)S) {
+              const float2 kyv = s_ky[i][dy];
+              const int o = (j2 - wx) & 7;  // first window column of this thread
+              float2 *c0 = lds + y * FP + wx + o;
+              if (o < S) {
+                const float2 t = cmul(s_vk[i][o], kyv);
+                c0[0] = make_float2(c0[0].x + t.x, c0[0].y + t.y);
+              }
+              if (o + 8 < S) {
+                const float2 t = cmul(s_vk[i][o + 8], kyv);
+                c0[8] = make_float2(c0[8].x + t.x, c0[8].y + t.y);
+              }
+            }
+          }
+#pragma unroll
+          for (int j1 = 0; j1 < 16; j1++) v[j1] = lds[y * FP + 8 * j1 + j2];
+        } else {
+#pragma unroll
+          for (int j1 = 0; j1 < 16; j1++) v[j1] = nxt[j1];
+          __syncthreads();  // tw written; the previous job's LDS reads done
+        }
         const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
 #pragma unroll
         for (int j1 = 0; j1 < 16; j1++) v[j1] = make_float2(neg * v[j1].x, neg * v[j1].y);
@@ -798,10 +859,12 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
           const float2 w = twz[(j2 * k1) & 127];
           p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
         }
-        if (b + 1 < rn.z) {
-          const float2 *row = sub + (int64_t)jslot[b + 1] * SGN + y * FN + j2;
+        if constexpr (!SMALL) {
+          if (b + 1 < rn.z) {
+            const float2 *row = sub + (int64_t)jslot[jl[b + 1]] * SGN + y * FN + j2;
 #pragma unroll
-          for (int j1 = 0; j1 < 16; j1++) nxt[j1] = row[8 * j1];
+            for (int j1 = 0; j1 < 16; j1++) nxt[j1] = row[8 * j1];
+          }
         }
       }
       __syncthreads();
@@ -823,7 +886,7 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
       {
         // column pass B: item (column x, k1) holds rows y = k1 + 16 k2,
         // which share the parity of x + k1
-        const double kout = -(double)kw[b];
+        const double kout = -(double)kw[job];
 #pragma unroll
         for (int r = 0; r < PASS_B; r++) {
           int x, k1;
@@ -1291,23 +1354,29 @@ __global__ void k_tw_phi(double *phi, double theta, double wstep) {
 // Gridding schedule of a binning, made on the host once and kept on the
 // device: jobs cut into parts of at most PART_VIS visibilities, batches of
 // whole jobs holding at most JOB_BATCH part slots.  Per batch: its parts
-// (slot, first, end), the multi-part jobs (first slot, parts) and the chunk
-// groups (chunk, first job, end job); per job its w-plane and slot; the
-// chunks in four parity classes for the final adds.
+// (slot, first, end) and multi-part jobs (first slot, parts); per job its
+// w-plane and slot; the chunks in four parity classes for the final adds.
 //
-// The job images of a batch are summed per chunk by k_tw_image_runs over
-// runs of at most `per` consecutive jobs of one chunk, longest runs first;
-// a chunk's later runs leave their sums in pslot, added to the chunk image
-// in run order by k_tw_chunk_accum over rgroups (chunk, first, end).
+// A batch's job images are summed per chunk by k_tw_image_runs over runs
+// (chunk, first, end, out) of at most `per` of its jobs of one chunk
+// (entries first .. end - 1 of the job list jl), longest runs first; a
+// chunk's later runs leave their sums in slots listed in pslot, added to
+// the chunk image in run order by k_tw_chunk_accum over rgroups (chunk,
+// first, end).  With `split`, jobs of at most TW_SMALL_VIS visibilities are
+// left out of the batches and summed by the small-job form of
+// k_tw_image_runs before them: runs small.r0 .. + small.nr, partial sums in
+// slots 0, 1, ... (the batches' slots are not yet in use).
 struct GridSched {
   struct Batch {
-    int j0, nj;
-    size_t p0, np, m0, nm, g0, ng, r0, nr, q0, nq;
+    size_t p0, np, m0, nm, r0, nr, q0, nq;
   };
   std::vector<Batch> batches;
+  Batch small{};  // split: the small jobs' runs and partial-sum groups
+  bool split = false;
   int max_slots = 0;
   int32_t *kw = nullptr, *chunks = nullptr, *jslot = nullptr, *par = nullptr, *pslot = nullptr;
-  int4 *parts = nullptr, *groups = nullptr, *runs = nullptr, *rgroups = nullptr;
+  int32_t *jl = nullptr;
+  int4 *parts = nullptr, *runs = nullptr, *rgroups = nullptr;
   int2 *multi = nullptr;
   size_t npar[4] = {0, 0, 0, 0};
 };
@@ -1375,9 +1444,9 @@ struct Plan {
   int njobs = 0;
   std::vector<int4> hjobs;
   std::vector<int32_t> chunks;  // non-empty chunk ids, ascending
-  GridSched gs;
+  GridSched gs, gs_split;  // without / with the small jobs split off
   DegridSched ds;
-  bool gs_ready = false, ds_ready = false;
+  bool gs_ready = false, gss_ready = false, ds_ready = false;
 };
 
 int tw_sync_status(croc_ctx *ctx, uint32_t *st) {
@@ -1517,67 +1586,93 @@ int upload_i32(croc_ctx *ctx, int32_t *dst, const std::vector<int32_t> &v) {
 }
 
 // Build the gridding schedule of p and upload it into `set` (slot WS_TILES).
-int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
-  GridSched &g = p.gs;
+// Runs over the job-list entries e0 .. e1 - 1 (chunk[e] their chunks, in
+// order by chunk): at most `per` jobs each, the first of each chunk adding
+// to the chunk image, later ones to the next partial slot (*nslot).
+void tw_sum_runs(const std::vector<int32_t> &jl, const std::vector<int32_t> &jslot,
+                 const std::vector<int> &jchunk, size_t e0, size_t e1, int per, bool slot_of_job,
+                 int *nslot, std::vector<int4> &runs, std::vector<int4> &rgroups,
+                 std::vector<int32_t> &pslot, GridSched::Batch &bt) {
+  bt.r0 = runs.size();
+  bt.q0 = rgroups.size();
+  for (size_t e = e0; e < e1;) {
+    const int c = jchunk[jl[e]];
+    size_t f = e;
+    while (f < e1 && jchunk[jl[f]] == c) f++;
+    const size_t q = pslot.size();
+    for (size_t b = e; b < f; b += per) {
+      int out = -1;
+      if (b != e) {
+        out = slot_of_job ? jslot[jl[b]] : (*nslot)++;
+        pslot.push_back(out);
+      }
+      runs.push_back(make_int4(c, (int)b, (int)std::min(f, b + per), out));
+    }
+    if (pslot.size() > q) rgroups.push_back(make_int4(c, (int)q, (int)pslot.size(), 0));
+    e = f;
+  }
+  bt.nr = runs.size() - bt.r0;
+  bt.nq = rgroups.size() - bt.q0;
+  std::stable_sort(runs.begin() + bt.r0, runs.end(),
+                   [](const int4 &u, const int4 &v) { return u.z - u.y > v.z - v.y; });
+}
+
+int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool split) {
+  GridSched &g = split ? p.gs_split : p.gs;
   g = GridSched();
+  g.split = split;
   const TowerArgs &a = p.a;
-  std::vector<int4> parts, groups;
+  std::vector<int4> parts, runs, rgroups;
   std::vector<int2> multi;
-  std::vector<int32_t> jslot(p.njobs), hk(p.njobs);
+  std::vector<int32_t> jslot(p.njobs, -1), hk(p.njobs), jl, pslot;
+  std::vector<int> jchunk(p.njobs);
+  for (int j = 0; j < p.njobs; j++) {
+    hk[j] = p.hjobs[j].x % a.nwp + a.wp_min;
+    jchunk[j] = p.hjobs[j].x / a.nwp;
+  }
+  // small jobs first, in job order
+  if (split) {
+    for (int j = 0; j < p.njobs; j++)
+      if (p.hjobs[j].z - p.hjobs[j].y <= TW_SMALL_VIS) jl.push_back(j);
+    int nslot = 0;
+    const int per = std::max(1, (int)((jl.size() + 2 * ctx->num_cu - 1) / (2 * ctx->num_cu)));
+    tw_sum_runs(jl, jslot, jchunk, 0, jl.size(), per, false, &nslot, runs, rgroups, pslot,
+                g.small);
+    g.max_slots = nslot;
+  }
+  // batches of the other jobs
   int slots = 0;
+  size_t e0 = jl.size();
+  auto close_batch = [&]() {
+    GridSched::Batch &bt = g.batches.back();
+    bt.np = parts.size() - bt.p0;
+    bt.nm = multi.size() - bt.m0;
+    const int nj = (int)(jl.size() - e0);
+    const int per = std::min(8, std::max(1, (nj + 2 * ctx->num_cu - 1) / (2 * ctx->num_cu)));
+    tw_sum_runs(jl, jslot, jchunk, e0, jl.size(), per, true, nullptr, runs, rgroups, pslot, bt);
+    e0 = jl.size();
+  };
   for (int j = 0; j < p.njobs; j++) {
     const int4 job = p.hjobs[j];
     const int n = job.z - job.y;
+    if (split && n <= TW_SMALL_VIS) continue;
     const int np = std::min(std::max(1, (n + PART_VIS - 1) / PART_VIS), JOB_BATCH / 4);
     if (g.batches.empty() || slots + np > JOB_BATCH) {
-      g.batches.push_back(GridSched::Batch{j, 0, parts.size(), 0, multi.size(), 0,
-                                           groups.size(), 0, 0, 0, 0, 0});
+      if (!g.batches.empty()) close_batch();
+      g.batches.push_back(GridSched::Batch{parts.size(), 0, multi.size(), 0, 0, 0, 0, 0});
       slots = 0;
     }
-    GridSched::Batch &bt = g.batches.back();
     jslot[j] = slots;
-    hk[j] = job.x % a.nwp + a.wp_min;
+    jl.push_back(j);
     const int len = (n + np - 1) / np;
     for (int k = 0; k < np; k++)
       parts.push_back(make_int4(slots + k, job.y + k * len,
                                 std::min(job.z, job.y + (k + 1) * len), 0));
     if (np > 1) multi.push_back(make_int2(slots, np));
-    const int c = job.x / a.nwp, lj = j - bt.j0;
-    if (bt.ng == 0 || groups.back().x != c) {
-      groups.push_back(make_int4(c, lj, lj + 1, 0));
-      bt.ng++;
-    } else {
-      groups.back().z = lj + 1;
-    }
     slots += np;
     g.max_slots = std::max(g.max_slots, slots);
-    bt.nj++;
-    bt.np = parts.size() - bt.p0;
-    bt.nm = multi.size() - bt.m0;
   }
-  // summing runs: a batch's chunk groups cut into runs of at most `per`
-  // jobs (about two runs per compute unit), queued longest first
-  std::vector<int4> runs, rgroups;
-  std::vector<int32_t> pslot;
-  for (GridSched::Batch &bt : g.batches) {
-    const int per = std::min(8, std::max(1, (bt.nj + 2 * ctx->num_cu - 1) / (2 * ctx->num_cu)));
-    bt.r0 = runs.size();
-    bt.q0 = rgroups.size();
-    for (size_t gi = bt.g0; gi < bt.g0 + bt.ng; gi++) {
-      const int4 gr = groups[gi];
-      const size_t q = pslot.size();
-      for (int b = gr.y; b < gr.z; b += per) {
-        const bool first = b == gr.y;
-        runs.push_back(make_int4(gr.x, b, std::min(gr.z, b + per), first ? -1 : jslot[bt.j0 + b]));
-        if (!first) pslot.push_back(jslot[bt.j0 + b]);
-      }
-      if (pslot.size() > q) rgroups.push_back(make_int4(gr.x, (int)q, (int)pslot.size(), 0));
-    }
-    bt.nr = runs.size() - bt.r0;
-    bt.nq = rgroups.size() - bt.q0;
-    std::stable_sort(runs.begin() + bt.r0, runs.end(),
-                     [](const int4 &u, const int4 &v) { return u.z - u.y > v.z - v.y; });
-  }
+  if (!g.batches.empty()) close_batch();
   // parity classes of the chunks: one add launch each, stream order keeps
   // overlapping margins of different classes apart
   std::vector<int32_t> par[4], allpar;
@@ -1586,15 +1681,13 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
     allpar.insert(allpar.end(), par[q].begin(), par[q].end());
     g.npar[q] = par[q].size();
   }
-  const size_t n32 = 2 * (size_t)p.njobs + 2 * p.chunks.size() + pslot.size();
-  const size_t bytes = sizeof(int4) * (parts.size() + groups.size() + runs.size() +
-                                       rgroups.size()) +
+  const size_t n32 = 2 * (size_t)p.njobs + 2 * p.chunks.size() + pslot.size() + jl.size();
+  const size_t bytes = sizeof(int4) * (parts.size() + runs.size() + rgroups.size()) +
                        sizeof(int2) * multi.size() + sizeof(int32_t) * n32 + 256;
-  char *d = (char *)ws_get_in(ctx, set, WS_TILES, bytes);
+  char *d = (char *)ws_get_in(ctx, set, split ? WS_TOWER : WS_TILES, bytes);
   if (!d) return CROC_ENOMEM;
   g.parts = (int4 *)d;
-  g.groups = g.parts + parts.size();
-  g.runs = g.groups + groups.size();
+  g.runs = g.parts + parts.size();
   g.rgroups = g.runs + runs.size();
   g.multi = (int2 *)(g.rgroups + rgroups.size());
   g.kw = (int32_t *)(g.multi + multi.size());
@@ -1602,12 +1695,10 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
   g.chunks = g.jslot + p.njobs;
   g.par = g.chunks + p.chunks.size();
   g.pslot = g.par + p.chunks.size();
+  g.jl = g.pslot + pslot.size();
   hipStream_t s = ctx->stream;
   if (!parts.empty())
     CROC_HIP(hipMemcpyAsync(g.parts, parts.data(), sizeof(int4) * parts.size(),
-                            hipMemcpyHostToDevice, s));
-  if (!groups.empty())
-    CROC_HIP(hipMemcpyAsync(g.groups, groups.data(), sizeof(int4) * groups.size(),
                             hipMemcpyHostToDevice, s));
   if (!multi.empty())
     CROC_HIP(hipMemcpyAsync(g.multi, multi.data(), sizeof(int2) * multi.size(),
@@ -1624,9 +1715,10 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
   if ((rc = upload_i32(ctx, g.chunks, p.chunks))) return rc;
   if ((rc = upload_i32(ctx, g.par, allpar))) return rc;
   if ((rc = upload_i32(ctx, g.pslot, pslot))) return rc;
+  if ((rc = upload_i32(ctx, g.jl, jl))) return rc;
   // the host vectors above end here: wait for their copies
   CROC_HIP(hipStreamSynchronize(s));
-  p.gs_ready = true;
+  (split ? p.gss_ready : p.gs_ready) = true;
   return CROC_OK;
 }
 
@@ -1704,7 +1796,7 @@ extern "C" int croc_wtowers_grid(croc_ctx *ctx, void *grid, int N, const double 
                      kern != nullptr, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if ((rc = tw_bin(ctx, p, (const float2 *)vis))) return rc;
-  if ((rc = tw_grid_sched(ctx, p, ctx->ws))) return rc;
+  if ((rc = tw_grid_sched(ctx, p, ctx->ws, !p.a.nearest && separable && S <= 16))) return rc;
   return tw_grid_run(ctx, p, grid, kern, separable);
 }
 
@@ -1717,15 +1809,18 @@ namespace {
 int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int separable) {
   int rc;
   if (p.njobs == 0) return CROC_OK;
-  CROC_REQUIRE(p.gs_ready, CROC_EINVAL, "gridding schedule missing");
   const TowerArgs &a = p.a;
-  const GridSched &gs = p.gs;
   hipStream_t s = ctx->stream;
 
   // fixed-point scale, Fresnel phase table, a part queue per batch
   const int64_t Q = a.Q, S = a.S;
   const int64_t tsize = separable ? Q * S : Q * Q * S * S;  // elements per w-kernel plane
   const int64_t nk = kern ? tsize * a.nwk : 0;
+  // Separable kernels of support <= 16 grid on the matrix cores
+  // (k_tw_grid_jobs_mfma), the small jobs in the summing kernel.
+  const bool mfma = !a.nearest && separable && a.S <= 16;
+  CROC_REQUIRE(mfma ? p.gss_ready : p.gs_ready, CROC_EINVAL, "gridding schedule missing");
+  const GridSched &gs = mfma ? p.gs_split : p.gs;
   char *aux = (char *)ws_get(ctx, WS_SCALE, 256 + sizeof(double) * SGN +
                                                 sizeof(uint32_t) * (2 * gs.batches.size() + 64) +
                                                 sizeof(uint32_t) * (size_t)a.ncx * a.ncx);
@@ -1741,10 +1836,9 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
   if ((rc = zero_u32_async(ctx, (uint32_t *)fs, sizeof(TwScale) / 4))) return rc;
   k_tw_maxima<<<ctx->num_cu, 1024, 0, s>>>(p.svis, a.nv, (const float2 *)kern, nk, fs);
   k_tw_pick<<<1, 1, 0, s>>>(fs, a.nv, a.nearest, separable);
-  // Separable kernels of support <= 16 grid on the matrix cores
-  // (k_tw_grid_jobs_mfma), from an fp16 (hi, lo) copy of the kernel set.
+  // the matrix-core gridding reads an fp16 (hi, lo) copy of the kernel set
   uint2 *ksplit = nullptr;
-  if (!a.nearest && separable && a.S <= 16) {
+  if (mfma) {
     CROC_REQUIRE(nk < ((int64_t)1 << 32), CROC_EINVAL, "kernel set beyond 2^32 elements");
     ksplit = (uint2 *)ws_get(ctx, WS_KSPLIT, sizeof(uint2) * (size_t)nk);
     if (!ksplit) return CROC_ENOMEM;
@@ -1757,9 +1851,23 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
   k_tw_zero_chunks<<<dim3(16, (unsigned)p.chunks.size()), 256, 0, s>>>(gs.chunks, cimg);
   CROC_HIP(hipGetLastError());
 
-  if ((rc = zero_u32_async(ctx, qc, 2 * (int64_t)gs.batches.size()))) return rc;
+  if ((rc = zero_u32_async(ctx, qc, 2 * (int64_t)gs.batches.size() + 1))) return rc;
   CROC_HIP(hipEventRecord(ctx->ev[0], s));
   tap_timer_reset(ctx, 0);
+  if (gs.small.nr) {
+    // small jobs gridded, transformed and summed per chunk; partial sums in
+    // slots 0, 1, ... before the batches take the slots
+    k_tw_image_runs<true><<<(unsigned)std::min<size_t>(gs.small.nr, (size_t)ctx->num_cu), FFT_NT,
+                            0, s>>>(gs.runs + gs.small.r0, (int)gs.small.nr, gs.jl, gs.jslot,
+                                    gs.kw, phi, sub, cimg, qc + 2 * gs.batches.size(), p.jobs,
+                                    p.rec, p.svis, (const float2 *)kern, a.S, tsize);
+    CROC_HIP(hipGetLastError());
+    if (gs.small.nq) {
+      k_tw_chunk_accum<<<dim3(ACC_SLICES, (unsigned)gs.small.nq), 256, 0, s>>>(
+          gs.rgroups + gs.small.q0, gs.pslot, sub, cimg);
+      CROC_HIP(hipGetLastError());
+    }
+  }
   for (size_t bi = 0; bi < gs.batches.size(); bi++) {
     const GridSched::Batch &bt = gs.batches[bi];
     const int G = (int)std::min<size_t>(bt.np, (size_t)ctx->num_cu);  // one per CU (LDS)
@@ -1783,9 +1891,10 @@ int tw_grid_run(croc_ctx *ctx, const Plan &p, void *grid, const void *kern, int 
       CROC_HIP(hipGetLastError());
     }
     // images ifft(sub[slot]) * exp(-i wp phi) summed into the chunk images
-    k_tw_image_runs<<<(unsigned)std::min<size_t>(bt.nr, (size_t)ctx->num_cu), FFT_NT, 0, s>>>(
-        gs.runs + bt.r0, (int)bt.nr, gs.jslot + bt.j0, gs.kw + bt.j0, phi, sub, cimg,
-        qc + gs.batches.size() + bi);
+    k_tw_image_runs<false><<<(unsigned)std::min<size_t>(bt.nr, (size_t)ctx->num_cu), FFT_NT, 0,
+                             s>>>(gs.runs + bt.r0, (int)bt.nr, gs.jl, gs.jslot, gs.kw, phi, sub,
+                                  cimg, qc + gs.batches.size() + bi, nullptr, nullptr, nullptr,
+                                  nullptr, 0, 0);
     CROC_HIP(hipGetLastError());
     if (bt.nq) {
       k_tw_chunk_accum<<<dim3(ACC_SLICES, (unsigned)bt.nq), 256, 0, s>>>(gs.rgroups + bt.q0,
@@ -1989,7 +2098,10 @@ extern "C" int croc_wtowers_plan_bin(croc_ctx *ctx, croc_tw_plan *plan, int N,
                      has_kernel != 0, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if (nv > 0 && (rc = tw_bin(ctx, p, nullptr))) return rc;
-  if (nv > 0 && (rc = tw_grid_sched(ctx, p, plan->ws))) return rc;
+  // both gridding schedules: the kernel kind is known only at gridding
+  if (nv > 0 && (rc = tw_grid_sched(ctx, p, plan->ws, false))) return rc;
+  if (nv > 0 && !p.a.nearest && S <= 16 && (rc = tw_grid_sched(ctx, p, plan->ws, true)))
+    return rc;
   if (nv > 0 && (rc = tw_degrid_sched(ctx, p, plan->ws))) return rc;
   plan->ready = true;
   return CROC_OK;
