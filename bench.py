@@ -36,6 +36,9 @@ import time
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# w-towers imaging: jobs of at most this many visibilities are gridded by the
+# summing kernel (k_tw_image_runs), the rest by the job kernel (wtowers.hip)
+TW_SMALL_VIS = 64
 FP16_MFMA_PEAK_TFLOPS = 2500.0  # dense
 LDS_PEAK_TBS = 78.6  # 256 CUs x 128 B/clk x 2.4 GHz
 
@@ -309,9 +312,14 @@ def run_towers(args, torch, G, W, _lib, dev):
     key = key * 100000 + (torch.floor(uvw_d[:, 1] * theta + 0.5).to(torch.int64) + N // 2) \
         // (128 - margin)
     key = key * 4096 + (wb.to(torch.int64) - lo)
-    njobs = int(torch.unique(key).numel())
+    _, jcount = torch.unique(key, return_counts=True)
+    njobs = int(jcount.numel())
+    # jobs of at most TW_SMALL_VIS visibilities are gridded by the summing
+    # kernel (k_tw_image_runs), the rest by the job kernel
+    small = jcount <= TW_SMALL_VIS
+    nv_small, njobs_small = int(jcount[small].sum()), int(small.sum())
     nchunks = int(torch.unique(key // 4096).numel())
-    del key
+    del key, jcount, small
 
     # predict accuracy against the DFT, full size
     print('[bench] predict error against the DFT', file=sys.stderr, flush=True)
@@ -343,31 +351,40 @@ def run_towers(args, torch, G, W, _lib, dev):
     # roofline of the dominant tap kernel, from the live HIP events of the
     # timed steps: 8 FP32 flop per kernel tap (one complex multiply-add),
     # S^2 taps per visibility and direction
-    taps = float(nv) * S * S
-    # separable sets of support <= 16: gridding on the matrix cores
-    # (k_tw_grid_jobs_mfma); predict transforms each plane sub-grid and
-    # degrids from it in LDS (k_tw_predict_sep: FFT and taps in one launch)
-    gname = 'k_tw_grid_jobs_mfma' if S <= 16 else 'k_tw_grid_jobs'
-    dname = 'k_tw_predict_sep' if S <= 16 else 'k_tw_degrid_jobs'
+    # separable sets of support <= 16: gridding of the jobs above
+    # TW_SMALL_VIS visibilities on the matrix cores (k_tw_grid_jobs_mfma);
+    # predict transforms each plane sub-grid and degrids from it in LDS
+    # (k_tw_predict_sep: FFT and taps in one launch)
+    split = S <= 16
+    gname = 'k_tw_grid_jobs_mfma' if split else 'k_tw_grid_jobs'
+    dname = 'k_tw_predict_sep' if split else 'k_tw_degrid_jobs'
+    nv_g = nv - nv_small if split else nv
+    nj_g = njobs - njobs_small if split else njobs
     if res['grid_tap_ms'] >= res['degrid_tap_ms']:
         kname, kms, kn = gname, res['grid_tap_ms'], res['grid_launches']
+        taps = float(nv_g) * S * S
+        # each gridded visibility's 16-byte record and 8-byte value read
+        # once, each job's 128^2 complex64 sub-grid written once
+        alg_bytes = (24.0 * nv_g + 8.0 * 128 * 128 * nj_g) / max(kn, 1)
     else:
         kname, kms, kn = dname, res['degrid_tap_ms'], res['degrid_launches']
+        taps = float(nv) * S * S
+        # each visibility's record read and value written once, each chunk
+        # image read once (the plane sub-grids stay in LDS)
+        alg_bytes = (24.0 * nv + 8.0 * 128 * 128 * nchunks) / max(kn, 1)
     achieved = 8.0 * taps / (kms * 1e-3) / 1e12
-    # algorithmic HBM bytes of a job kernel: each visibility's 16-byte
-    # record and 8-byte value once, each (chunk, w-plane) job's 128^2
-    # complex64 sub-grid once (written by gridding, read by degridding)
-    alg_bytes = (24.0 * nv + 8.0 * 128 * 128 * njobs) / max(kn, 1)
     roofline = {
         'bound': 'valu', 'achieved': round(achieved, 3), 'peak': FP32_PEAK_TFLOPS,
         'unit': 'TFLOP/s', 'frac': round(achieved / FP32_PEAK_TFLOPS, 5),
         'bound_note': 'complex multiply-add per kernel tap, 8 FP32 flop/tap (the reference '
-                      'convention), against the FP32 vector peak: the gridding job kernel '
-                      'forms the products on the matrix cores (fp16 hi/lo split, FP32 '
-                      'accumulation) and is bound by the vector work and memory latency '
-                      'around them (counters: MFMA busy ~6 %, waves waiting ~2/3 of their '
-                      'cycles); the degridding job kernel runs one visibility per thread on '
-                      'the vector ALUs from the sub-grid staged in LDS',
+                      'convention), over the taps the kernel forms, against the FP32 vector '
+                      'peak: the gridding job kernel forms the products of the jobs above %d '
+                      'visibilities (%d of %d jobs, %.1f %% of the visibilities) on the matrix '
+                      'cores (fp16 hi/lo split, FP32 accumulation) and is bound by the vector '
+                      'work and memory latency around them; the predict kernel transforms each '
+                      'plane sub-grid in LDS and degrids from it one visibility per thread on '
+                      'the vector ALUs (its time includes the transforms)'
+                      % (TW_SMALL_VIS, nj_g, njobs, 100.0 * nv_g / nv),
         'kernel': kname, 'launches_per_step': kn,
         'kernel_ms_per_launch': round(kms / max(kn, 1), 4),
         'kernel_ms_per_step': {gname: round(res['grid_tap_ms'], 3),

@@ -1395,6 +1395,7 @@ struct DegridSched {
   int4 *items = nullptr;
   int32_t *roff = nullptr;
   int nruns = 0;
+  bool fused = false;
 };
 
 // Cost of a predict item in visibility units: TW_FFT_COST for its sub-grid
@@ -1447,6 +1448,7 @@ struct Plan {
   GridSched gs, gs_split;  // without / with the small jobs split off
   DegridSched ds;
   bool gs_ready = false, gss_ready = false, ds_ready = false;
+  int kind = 0;  // plans: 0 no kernel (nearest cell), 1 separable, 2 two-dimensional
 };
 
 int tw_sync_status(croc_ctx *ctx, uint32_t *st) {
@@ -1723,13 +1725,16 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool split) {
 }
 
 // Build the degridding schedule of p and upload it into `set` (slot WS_GRIDQ).
-int tw_degrid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
+// fused: the schedule of k_tw_predict_sep (items and runs), else that of
+// the batched transforms and k_tw_degrid_jobs (batches and parts).
+int tw_degrid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool fused) {
   DegridSched &g = p.ds;
   g = DegridSched();
+  g.fused = fused;
   const TowerArgs &a = p.a;
   g.jb = std::min(p.njobs, JOB_BATCH);
   std::vector<int4> parts;
-  for (int j0 = 0; j0 < p.njobs; j0 += g.jb) {
+  for (int j0 = 0; !fused && j0 < p.njobs; j0 += g.jb) {
     g.poff.push_back(parts.size());
     for (int b = 0; b < std::min(g.jb, p.njobs - j0); b++) {
       const int4 job = p.hjobs[j0 + b];
@@ -1744,8 +1749,8 @@ int tw_degrid_sched(croc_ctx *ctx, Plan &p, WsSet &set) {
     hc[j] = p.hjobs[j].x / a.nwp;
   }
   std::vector<int4> items;
-  std::vector<int32_t> roff;
-  tw_predict_runs(p.hjobs, a.nwp, a.wp_min, ctx->num_cu, items, roff);
+  std::vector<int32_t> roff(1, 0);
+  if (fused) tw_predict_runs(p.hjobs, a.nwp, a.wp_min, ctx->num_cu, items, roff);
   g.nruns = (int)roff.size() - 1;
   const size_t bytes = sizeof(int4) * (parts.size() + items.size() + 32) +
                        sizeof(int32_t) * (2 * (size_t)p.njobs + p.chunks.size() + roff.size()) +
@@ -1944,7 +1949,7 @@ extern "C" int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N, const
                      kern != nullptr, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if ((rc = tw_bin(ctx, p, nullptr))) return rc;
-  if ((rc = tw_degrid_sched(ctx, p, ctx->ws))) return rc;
+  if ((rc = tw_degrid_sched(ctx, p, ctx->ws, kern && separable && S <= 16))) return rc;
   return tw_degrid_run(ctx, p, grid, vis_out, kern, separable);
 }
 
@@ -1966,6 +1971,7 @@ int tw_degrid_run(croc_ctx *ctx, const Plan &p, const void *grid, void *vis_out,
   // Separable kernels of support <= 16 transform and degrid each plane
   // sub-grid in LDS (k_tw_predict_sep): no sub-grid slots.
   const bool sep16 = !a.nearest && separable && a.S <= 16;
+  CROC_REQUIRE(sep16 == ds.fused, CROC_EINVAL, "degridding schedule made for another kernel kind");
   const size_t cimg_bytes = sizeof(float2) * SGN * (size_t)a.ncx * a.ncx;
   float2 *stack = (float2 *)ws_get(
       ctx, WS_SUB, cimg_bytes + (sep16 ? 0 : sizeof(float2) * SGN * (size_t)ds.jb));
@@ -2045,7 +2051,7 @@ __global__ void k_tw_vis_order(int64_t nv, const uint4 *__restrict__ rec,
   if (pos < nv) svis[pos] = vis[rec[pos].w];
 }
 
-int tw_plan_check(croc_ctx *ctx, const croc_tw_plan *plan, const void *kern) {
+int tw_plan_check(croc_ctx *ctx, const croc_tw_plan *plan, const void *kern, int separable) {
   int rc = ctx_enter(ctx);
   if (rc) return rc;
   CROC_REQUIRE(plan != nullptr, CROC_EINVAL, "null plan");
@@ -2054,6 +2060,9 @@ int tw_plan_check(croc_ctx *ctx, const croc_tw_plan *plan, const void *kern) {
   CROC_REQUIRE((kern != nullptr) == !plan->p.a.nearest, CROC_EINVAL,
                plan->p.a.nearest ? "plan was binned without a kernel"
                                  : "plan was binned for a kernel, none given");
+  CROC_REQUIRE(!kern || (separable != 0) == (plan->p.kind == 1), CROC_EINVAL,
+               plan->p.kind == 1 ? "plan was binned for a separable kernel"
+                                 : "plan was binned for a two-dimensional kernel");
   return CROC_OK;
 }
 
@@ -2094,22 +2103,25 @@ extern "C" int croc_wtowers_plan_bin(croc_ctx *ctx, croc_tw_plan *plan, int N,
   Plan &p = plan->p;
   p = Plan();
   p.keep = &plan->ws;
+  CROC_REQUIRE(has_kernel >= 0 && has_kernel <= 2, CROC_EINVAL,
+               "has_kernel must be 0 (none), 1 (separable) or 2 (two-dimensional)");
   if ((rc = tw_setup(ctx, p, N, uvw, nv, stride, theta, wstep, subgrid, margin,
                      has_kernel != 0, Q, S, nwk, wk_min, wk_step)))
     return rc;
   if (nv > 0 && (rc = tw_bin(ctx, p, nullptr))) return rc;
-  // both gridding schedules: the kernel kind is known only at gridding
-  if (nv > 0 && (rc = tw_grid_sched(ctx, p, plan->ws, false))) return rc;
-  if (nv > 0 && !p.a.nearest && S <= 16 && (rc = tw_grid_sched(ctx, p, plan->ws, true)))
-    return rc;
-  if (nv > 0 && (rc = tw_degrid_sched(ctx, p, plan->ws))) return rc;
+  // the schedules of the kernel kind: separable kernels of support <= 16
+  // split the small jobs off and fuse the predict
+  p.kind = has_kernel;
+  const bool sep16 = has_kernel == 1 && S <= 16;
+  if (nv > 0 && (rc = tw_grid_sched(ctx, p, plan->ws, sep16))) return rc;
+  if (nv > 0 && (rc = tw_degrid_sched(ctx, p, plan->ws, sep16))) return rc;
   plan->ready = true;
   return CROC_OK;
 }
 
 extern "C" int croc_wtowers_grid_plan(croc_ctx *ctx, croc_tw_plan *plan, void *grid,
                                       const void *vis, const void *kern, int separable) {
-  int rc = tw_plan_check(ctx, plan, kern);
+  int rc = tw_plan_check(ctx, plan, kern, separable);
   if (rc) return rc;
   CROC_REQUIRE(grid, CROC_EINVAL, "null grid");
   Plan &p = plan->p;
@@ -2125,7 +2137,7 @@ extern "C" int croc_wtowers_grid_plan(croc_ctx *ctx, croc_tw_plan *plan, void *g
 
 extern "C" int croc_wtowers_degrid_plan(croc_ctx *ctx, croc_tw_plan *plan, const void *grid,
                                         void *vis_out, const void *kern, int separable) {
-  int rc = tw_plan_check(ctx, plan, kern);
+  int rc = tw_plan_check(ctx, plan, kern, separable);
   if (rc) return rc;
   CROC_REQUIRE(grid, CROC_EINVAL, "null grid");
   if (plan->p.njobs == 0) return CROC_OK;

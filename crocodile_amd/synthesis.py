@@ -1295,18 +1295,19 @@ class WTowersPlan:
         uv = _uvw(uvw)
         if uv.shape[1] < 3:
             raise ValueError('w-towers need uvw with a w column')
-        kern, Q, S, _, nwk, wk0, wks = _tower_kernel(kv)
+        kern, Q, S, sep, nwk, wk0, wks = _tower_kernel(kv)
         self.nv = 0
+        kind = 0 if kern is None else (1 if sep else 2)
         _lib.check(_lib.load().croc_wtowers_plan_bin(
             _lib.ctx(), self._h, self.N, _lib.ptr(uv), uv.shape[0], uv.shape[1], self.theta,
-            self.wstep, self.subgrid_size, self.margin, int(kern is not None), Q, S,
-            nwk, wk0, wks))
-        self._shape = (kern is not None, Q, S, nwk, wk0, wks)
+            self.wstep, self.subgrid_size, self.margin, kind, Q, S, nwk, wk0, wks))
+        self._shape = (kind, Q, S, nwk, wk0, wks)
         self.nv, self._device = int(uv.shape[0]), uv.device
 
     def _kernel(self, kv):
         kern, Q, S, sep, nwk, wk0, wks = _tower_kernel(kv)
-        if (kern is not None, Q, S, nwk, wk0, wks) != self._shape:
+        kind = 0 if kern is None else (1 if sep else 2)
+        if (kind, Q, S, nwk, wk0, wks) != self._shape:
             raise ValueError('kernel shape or w-kernel set differs from the planned one')
         return kern, sep
 

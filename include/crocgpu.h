@@ -339,10 +339,13 @@ int croc_wtowers_degrid(croc_ctx *ctx, const void *grid, int N,
  * reference, whose notebooks bin inside each imaging / predict cell: an
  * extension for callers that image and predict the same visibilities
  * repeatedly.  croc_wtowers_plan_bin takes the coordinate and geometry
- * arguments of the one-shot calls, has_kernel (their kern != NULL) and the
- * kernel shape Q, S; it validates every coordinate.  The plan calls then
- * take kernel values of that shape (kern NULL exactly when has_kernel was
- * 0).  uvw may change after binning.  Calls with one plan must not overlap. */
+ * arguments of the one-shot calls, the kernel kind has_kernel (0: none, the
+ * one-shot calls' kern == NULL; 1: separable [Q][S] tables; 2:
+ * two-dimensional [Q][Q][S][S] tables) and the kernel shape Q, S; it
+ * validates every coordinate and makes the job schedules of that kind.  The
+ * plan calls then take kernel values of that kind and shape (kern NULL
+ * exactly when has_kernel was 0, separable exactly when it was 1).  uvw may
+ * change after binning.  Calls with one plan must not overlap. */
 typedef struct croc_tw_plan croc_tw_plan;
 int croc_wtowers_plan_create(croc_ctx *ctx, croc_tw_plan **out);
 int croc_wtowers_plan_destroy(croc_ctx *ctx, croc_tw_plan *plan);

@@ -138,6 +138,14 @@ def test_wtowers_plan_errors_and_empty(gpu):
             _lib.ctx(), plan._h, _lib.ptr(torch.zeros((512, 512), dtype=torch.complex64,
                                                       device='cuda')),
             _lib.ptr(torch.zeros(1000, dtype=torch.complex64, device='cuda')), None, 0))
+    with pytest.raises(ValueError):        # binned for a two-dimensional kernel
+        plan.imaging(vis, kernels(rng)['separable'])
+    k2 = torch.as_tensor(kv.astype(numpy.complex64), device='cuda')
+    with pytest.raises(ValueError):        # the C ABI checks the kernel kind as well
+        _lib.check(_lib.load().croc_wtowers_grid_plan(
+            _lib.ctx(), plan._h, _lib.ptr(torch.zeros((512, 512), dtype=torch.complex64,
+                                                      device='cuda')),
+            _lib.ptr(torch.zeros(1000, dtype=torch.complex64, device='cuda')), _lib.ptr(k2), 1))
     bad = uvw.copy()
     bad[3, 0] = 2.0 * LAM                  # beyond the last chunk
     with pytest.raises((ValueError, AssertionError)):
