@@ -774,6 +774,7 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
   __shared__ float2 tw[FN];
   __shared__ float2 s_vk[NS][16], s_ky[NS][16];  // v kx and ky rows (zero beyond S)
   __shared__ uint32_t s_pos[NS];                 // window corners wx | wy << 16
+  __shared__ int s_hit[SMALL ? FN : 1];          // rows holding taps (small jobs)
   __shared__ int s_run;
   const int tid = threadIdx.x;
   constexpr float sgn = 1.0f;  // inverse transform
@@ -802,6 +803,7 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
       asm volatile("" : "+s"(zero));
       const float2 *twz = tw + zero;
       const int job = jl[b];
+      bool hit = true;  // row y holds taps (small jobs: else it stays zero through the row passes)
       {
         float2 v[16];
         if constexpr (SMALL) {
@@ -825,10 +827,12 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
 #pragma unroll
           for (int j1 = 0; j1 < 16; j1++) lds[y * FP + 8 * j1 + j2] = make_float2(0.f, 0.f);
           __syncthreads();
+          hit = false;
           for (int i = 0; i < n; i++) {
             const uint32_t pos = s_pos[i];
             const int wx = (int)(pos & 0xffffu), dy = y - (int)(pos >> 16);
             if ((unsigned)dy < (unsigned)S) {
+              hit = true;
               const float2 kyv = s_ky[i][dy];
               const int o = (j2 - wx) & 7;  // first window column of this thread
               float2 *c0 = lds + y * FP + wx + o;
@@ -842,6 +846,7 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
               }
             }
           }
+          if (j2 == 0) s_hit[y] = hit;
 #pragma unroll
           for (int j1 = 0; j1 < 16; j1++) v[j1] = lds[y * FP + 8 * j1 + j2];
         } else {
@@ -849,15 +854,17 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
           for (int j1 = 0; j1 < 16; j1++) v[j1] = nxt[j1];
           __syncthreads();  // tw written; the previous job's LDS reads done
         }
-        const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
+        if (hit) {
+          const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
 #pragma unroll
-        for (int j1 = 0; j1 < 16; j1++) v[j1] = make_float2(neg * v[j1].x, neg * v[j1].y);
-        fft_reg<16>(v, sgn);
-        float2 *p = lds + y * FP + j2;
+          for (int j1 = 0; j1 < 16; j1++) v[j1] = make_float2(neg * v[j1].x, neg * v[j1].y);
+          fft_reg<16>(v, sgn);
+          float2 *p = lds + y * FP + j2;
 #pragma unroll
-        for (int k1 = 0; k1 < 16; k1++) {
-          const float2 w = twz[(j2 * k1) & 127];
-          p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
+          for (int k1 = 0; k1 < 16; k1++) {
+            const float2 w = twz[(j2 * k1) & 127];
+            p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
+          }
         }
         if constexpr (!SMALL) {
           if (b + 1 < rn.z) {
@@ -869,15 +876,30 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
       }
       __syncthreads();
       {
+        // row pass B (rows without taps are zero and stay so)
         float2 v[PASS_B][8];
-        pass_b<false>(lds, FP, 1, sgn, v);
+        bool on[PASS_B];
+#pragma unroll
+        for (int r = 0; r < PASS_B; r++) {
+          int line, k1;
+          pass_b_item<false>(r, line, k1);
+          on[r] = !SMALL || s_hit[line];
+          if (on[r]) {
+            const float2 *p = lds + line * FP + 8 * k1;
+#pragma unroll
+            for (int jj = 0; jj < 8; jj++) v[r][jj] = p[jj];
+            fft_reg<8>(v[r], sgn);
+          }
+        }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < PASS_B; r++) {
           int line, k1;
           pass_b_item<false>(r, line, k1);
+          if (on[r]) {
 #pragma unroll
-          for (int k2 = 0; k2 < 8; k2++) lds[line * FP + k1 + 16 * k2] = v[r][k2];
+            for (int k2 = 0; k2 < 8; k2++) lds[line * FP + k1 + 16 * k2] = v[r][k2];
+          }
         }
       }
       __syncthreads();
