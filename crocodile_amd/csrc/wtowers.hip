@@ -1151,16 +1151,16 @@ __device__ __forceinline__ void degrid_round16(const float2 *lds, int j0, int j1
 
 // Predict with a separable kernel of support S <= 16 (SF = 16: S fixed at
 // 16, degrid_round16; SF = 0: any S, degrid_sep_lds), the plane sub-grids
-// never leaving LDS.  items[i] =
-// (chunk, w-plane wp, first record, end record); a run is the items
-// roff[r] .. roff[r + 1] - 1, runs taken from the queue counter qc by
-// persistent workgroups.  Per item:
+// never leaving LDS.  items[i] = (chunk, w-plane wp, first record, end
+// record); a run is the items roff[r] .. roff[r + 1] - 1, runs taken from
+// the queue counter qc by persistent workgroups.  Per item:
 //   1. the sub-grid fft(chunk image * exp(+i wp phi)) formed in LDS with the
-//      arithmetic of k_fft2d_128 (input sign and phase on load, rows then
-//      columns), its last column pass written back to LDS with the output
-//      sign instead of to memory.  The chunk image's rows are read at each
-//      item (consecutive items mostly share the image, which L2 then holds);
-//      carrying them in registers would take 32 registers from the degridding;
+//      arithmetic of k_fft2d_128 (centring sign and phase applied to the
+//      input as it is read, the output sign as the last pass writes back),
+//      columns first, then the rows the item's windows cover (all rows
+//      when it holds more than FFT_NT visibilities); the chunk image's
+//      columns are read at each item (consecutive items mostly share it,
+//      and L2 holds it);
 //   2. the item's visibilities degridded from LDS; the window lies inside
 //      the sub-grid (S <= margin, checked at binning).
 template <int SF>
@@ -1171,13 +1171,16 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
     float2 *__restrict__ vis_out, uint32_t *__restrict__ qc) {
   __shared__ float2 lds[FN * FP];
   __shared__ float2 tw[FN];
-  __shared__ int s_run;
+  __shared__ int s_need[FN];  // rows some window of the item covers
+  __shared__ int s_rows[FN];  // those rows, ascending
+  __shared__ int s_nrows, s_run;
   const int tid = threadIdx.x;
   constexpr float sgn = -1.0f;  // forward transform
   fft128_twiddles(tw);
-  // Row pass A: the thread owns row y and its elements x = 8 j1 + j2, which
-  // share the parity of y + j2 (the (-1)^(x + y) centring sign).
-  const int y = tid >> 3, j2 = tid & 7;
+  if (tid < FN) s_need[tid] = 0;
+  // Column pass A: the thread owns column x and its elements y = 8 j1 + c2,
+  // which share the parity of x + c2 (the (-1)^(x + y) centring sign).
+  const int x = tid & (FN - 1), c2 = tid >> 7;
   for (;;) {
     if (tid == 0) s_run = (int)atomicAdd(qc, 1u);
     __syncthreads();
@@ -1194,15 +1197,24 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
       {
         float2 v[16];
         double ph[16];
-        const float2 *row = cimg + (int64_t)it.x * SGN + y * FN + j2;
-        const double *prow = phi + zero + y * FN + j2;
+        const float2 *col = cimg + (int64_t)it.x * SGN + c2 * FN + x;
+        const double *pcol = phi + zero + c2 * FN + x;
 #pragma unroll
         for (int j1 = 0; j1 < 16; j1++) {
-          v[j1] = row[8 * j1];
-          ph[j1] = prow[8 * j1];
+          v[j1] = col[8 * j1 * FN];
+          ph[j1] = pcol[8 * j1 * FN];
         }
         __syncthreads();  // tw written; the previous item's LDS reads done
-        const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
+        // the rows the item's windows cover (s_need was cleared after the
+        // previous item's row passes)
+        const int n = it.w - it.z;
+        if (n > FFT_NT) {
+          if (tid < FN) s_need[tid] = 1;
+        } else if (tid < n) {
+          const int wy = (int)(rec[it.z + tid].x >> 16);
+          for (int k = 0; k < S; k++) s_need[wy + k] = 1;
+        }
+        const float neg = ((x + c2) & 1) ? -1.0f : 1.0f;
         const double kin = (double)it.y;
 #pragma unroll
         for (int j1 = 0; j1 < 16; j1++) {
@@ -1210,47 +1222,90 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
           v[j1] = make_float2(neg * t.x, neg * t.y);
         }
         fft_reg<16>(v, sgn);
-        float2 *p = lds + y * FP + j2;
+        float2 *p = lds + x + c2 * FP;
 #pragma unroll
         for (int k1 = 0; k1 < 16; k1++) {
-          const float2 w = twz[(j2 * k1) & 127];
-          p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
+          const float2 w = twz[(c2 * k1) & 127];
+          p[8 * k1 * FP] = cmul(v[k1], make_float2(w.x, sgn * w.y));
         }
       }
       __syncthreads();
-      {
-        // row pass B, written back to the natural positions once all are read
-        float2 v[PASS_B][8];
-        pass_b<false>(lds, FP, 1, sgn, v);
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < PASS_B; r++) {
-          int line, k1;
-          pass_b_item<false>(r, line, k1);
-#pragma unroll
-          for (int k2 = 0; k2 < 8; k2++) lds[line * FP + k1 + 16 * k2] = v[r][k2];
-        }
+      if (tid < 64) {
+        // the covered rows listed in order (ballots over two rows per lane)
+        const int f0 = s_need[2 * tid], f1 = s_need[2 * tid + 1];
+        const unsigned long long m0 = __ballot(f0 != 0), m1 = __ballot(f1 != 0);
+        const unsigned long long lt = (1ull << tid) - 1ull;
+        const int pos = __popcll(m0 & lt) + __popcll(m1 & lt);
+        if (f0) s_rows[pos] = 2 * tid;
+        if (f1) s_rows[pos + (f0 ? 1 : 0)] = 2 * tid + 1;
+        if (tid == 0) s_nrows = __popcll(m0) + __popcll(m1);
       }
-      __syncthreads();
-      pass_a<true>(lds, 1, FP, twz, sgn);  // columns
-      __syncthreads();
       {
-        // column pass B back to LDS with the output sign: item (column x,
-        // k1) holds rows y = k1 + 16 k2, which share the parity of x + k1
+        // column pass B, written back to the natural positions once all are read
         float2 v[PASS_B][8];
         pass_b<true>(lds, 1, FP, sgn, v);
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < PASS_B; r++) {
-          int x, k1;
-          pass_b_item<true>(r, x, k1);
-          const float sc = ((x + k1) & 1) ? -1.0f : 1.0f;
+          int line, k1;
+          pass_b_item<true>(r, line, k1);
 #pragma unroll
-          for (int k2 = 0; k2 < 8; k2++)
-            lds[(k1 + 16 * k2) * FP + x] = make_float2(v[r][k2].x * sc, v[r][k2].y * sc);
+          for (int k2 = 0; k2 < 8; k2++) lds[(k1 + 16 * k2) * FP + line] = v[r][k2];
         }
       }
       __syncthreads();
+      {
+        // row pass A on the listed rows: item (k-th row, j2) as pass_a
+        const int nrows = s_nrows;
+        const int k = tid >> 3, jj = tid & 7;
+        if (k < nrows) {
+          float2 *p = lds + s_rows[k] * FP + jj;
+          float2 v[16];
+#pragma unroll
+          for (int j1 = 0; j1 < 16; j1++) v[j1] = p[8 * j1];
+          fft_reg<16>(v, sgn);
+#pragma unroll
+          for (int k1 = 0; k1 < 16; k1++) {
+            const float2 w = twz[(jj * k1) & 127];
+            p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
+          }
+        }
+      }
+      __syncthreads();
+      {
+        // row pass B on the listed rows, back to LDS with the output sign:
+        // item (row y, k1) holds columns k1 + 16 k2, which share the parity
+        // of y + k1.  Wavefronts past the list hold zeros and write nothing.
+        const int nrows = s_nrows;
+        float2 v[PASS_B][8];
+#pragma unroll
+        for (int r = 0; r < PASS_B; r++) {
+          const int itr = tid + r * FFT_NT, k = itr >> 4, k1 = itr & 15;
+          if (__builtin_amdgcn_readfirstlane((itr & ~63) >> 4) < nrows) {
+            const float2 *p = lds + s_rows[min(k, nrows - 1)] * FP + 8 * k1;
+#pragma unroll
+            for (int jj = 0; jj < 8; jj++) v[r][jj] = p[jj];
+            fft_reg<8>(v[r], sgn);
+          } else {
+#pragma unroll
+            for (int jj = 0; jj < 8; jj++) v[r][jj] = make_float2(0.f, 0.f);
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < PASS_B; r++) {
+          const int itr = tid + r * FFT_NT, k = itr >> 4, k1 = itr & 15;
+          if (k < nrows) {
+            const int line = s_rows[k];
+            const float sc = ((line + k1) & 1) ? -1.0f : 1.0f;
+#pragma unroll
+            for (int k2 = 0; k2 < 8; k2++)
+              lds[line * FP + k1 + 16 * k2] = make_float2(v[r][k2].x * sc, v[r][k2].y * sc);
+          }
+        }
+      }
+      __syncthreads();
+      if (tid < FN) s_need[tid] = 0;  // read again only after the next item's first barrier
       if constexpr (SF == 16) {
         // rounds of the workgroup's width; a round with fewer visibilities
         // left than threads gives each up to 16 lanes (one window row each
