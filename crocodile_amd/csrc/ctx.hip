@@ -100,6 +100,21 @@ void ws_free(WsSet &set) {
   }
 }
 
+void *host_stage(croc_ctx *ctx, size_t bytes) {
+  if (ctx->stage_size >= bytes) return ctx->stage;
+  if (ctx->stage) (void)hipHostFree(ctx->stage);
+  ctx->stage = nullptr;
+  ctx->stage_size = 0;
+  const size_t want = bytes + bytes / 4;
+  if (hipHostMalloc(&ctx->stage, want, hipHostMallocDefault) != hipSuccess) {
+    ctx->stage = nullptr;
+    croc_set_error("hipHostMalloc of %zu bytes failed", want);
+    return nullptr;
+  }
+  ctx->stage_size = want;
+  return ctx->stage;
+}
+
 extern "C" int croc_ctx_create(int device, croc_ctx **out) {
   CROC_REQUIRE(out != nullptr, CROC_EINVAL, "null output pointer");
   int ndev = 0;
@@ -140,6 +155,7 @@ extern "C" int croc_ctx_destroy(croc_ctx *ctx) {
   for (int d = 0; d < 2; d++)
     for (hipEvent_t e : ctx->tev[d]) (void)hipEventDestroy(e);
   if (ctx->host_status) (void)hipHostFree(ctx->host_status);
+  if (ctx->stage) (void)hipHostFree(ctx->stage);
   delete ctx;
   return CROC_OK;
 }

@@ -97,6 +97,10 @@ struct croc_ctx {
   // tev[dir][2k + 1] bracket launch k of tev_n[dir].
   std::vector<hipEvent_t> tev[2];
   int tev_n[2] = {0, 0};
+  // pinned host staging of uploads (grow-only; reused once the copy that
+  // read it has been waited for)
+  void *stage = nullptr;
+  size_t stage_size = 0;
   int num_cu = 256;
   int tile = 0;     // 0: automatic
   int chunk = 512;  // visibilities per work item
@@ -111,6 +115,11 @@ void ws_free(WsSet &set);
 
 // Make the context's device current.
 int ctx_enter(croc_ctx *ctx);
+
+// The context's pinned host staging buffer of at least `bytes` (nullptr on
+// failure, error set); the caller waits for the copy that reads it before
+// the next use.
+void *host_stage(croc_ctx *ctx, size_t bytes);
 
 // Tap-kernel launch timers: tap_timer_reset at the start of a gridding
 // (dir 0) or degridding (dir 1) call, tap_timer_begin / tap_timer_end

@@ -24,6 +24,7 @@
 // exp(+2 pi i wp wstep (1 - n)) applied to the input per job, degrid.
 #include <algorithm>
 #include <climits>
+#include <cstring>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -1657,14 +1658,27 @@ int tw_setup(croc_ctx *ctx, Plan &p, int N, const double *uvw, int64_t nv, int64
   return CROC_OK;
 }
 
-int upload_i32(croc_ctx *ctx, int32_t *dst, const std::vector<int32_t> &v) {
-  if (v.empty()) return CROC_OK;
-  CROC_HIP(hipMemcpyAsync(dst, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice,
-                          ctx->stream));
-  return CROC_OK;
-}
+// A schedule's device block filled from its host image: the arrays are
+// copied into the context's pinned staging buffer at their device offsets
+// and the block goes up in one copy, waited for before send() returns (the
+// host vectors and the staging buffer are then free).
+struct Upload {
+  croc_ctx *ctx;
+  char *d, *h;
+  size_t n;
+  Upload(croc_ctx *c, char *dev, size_t bytes)
+      : ctx(c), d(dev), h((char *)host_stage(c, bytes)), n(bytes) {}
+  template <class T>
+  void put(T *dst, const std::vector<T> &v) {
+    if (!v.empty()) memcpy(h + ((char *)dst - d), v.data(), sizeof(T) * v.size());
+  }
+  int send() {
+    CROC_HIP(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, ctx->stream));
+    CROC_HIP(hipStreamSynchronize(ctx->stream));
+    return CROC_OK;
+  }
+};
 
-// Build the gridding schedule of p and upload it into `set` (slot WS_TILES).
 // Runs over the job-list entries e0 .. e1 - 1 (chunk[e] their chunks, in
 // order by chunk): at most `per` jobs each, the first of each chunk adding
 // to the chunk image, later ones to the next partial slot (*nslot).
@@ -1696,6 +1710,8 @@ void tw_sum_runs(const std::vector<int32_t> &jl, const std::vector<int32_t> &jsl
                    [](const int4 &u, const int4 &v) { return u.z - u.y > v.z - v.y; });
 }
 
+// Build the gridding schedule of p (split: the small jobs apart, p.gs_split
+// in slot WS_TOWER; else p.gs in slot WS_TILES) and upload it into `set`.
 int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool split) {
   GridSched &g = split ? p.gs_split : p.gs;
   g = GridSched();
@@ -1775,28 +1791,20 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool split) {
   g.par = g.chunks + p.chunks.size();
   g.pslot = g.par + p.chunks.size();
   g.jl = g.pslot + pslot.size();
-  hipStream_t s = ctx->stream;
-  if (!parts.empty())
-    CROC_HIP(hipMemcpyAsync(g.parts, parts.data(), sizeof(int4) * parts.size(),
-                            hipMemcpyHostToDevice, s));
-  if (!multi.empty())
-    CROC_HIP(hipMemcpyAsync(g.multi, multi.data(), sizeof(int2) * multi.size(),
-                            hipMemcpyHostToDevice, s));
-  if (!runs.empty())
-    CROC_HIP(hipMemcpyAsync(g.runs, runs.data(), sizeof(int4) * runs.size(),
-                            hipMemcpyHostToDevice, s));
-  if (!rgroups.empty())
-    CROC_HIP(hipMemcpyAsync(g.rgroups, rgroups.data(), sizeof(int4) * rgroups.size(),
-                            hipMemcpyHostToDevice, s));
-  int rc;
-  if ((rc = upload_i32(ctx, g.kw, hk))) return rc;
-  if ((rc = upload_i32(ctx, g.jslot, jslot))) return rc;
-  if ((rc = upload_i32(ctx, g.chunks, p.chunks))) return rc;
-  if ((rc = upload_i32(ctx, g.par, allpar))) return rc;
-  if ((rc = upload_i32(ctx, g.pslot, pslot))) return rc;
-  if ((rc = upload_i32(ctx, g.jl, jl))) return rc;
-  // the host vectors above end here: wait for their copies
-  CROC_HIP(hipStreamSynchronize(s));
+  Upload up(ctx, d, bytes);
+  if (!up.h) return CROC_ENOMEM;
+  up.put(g.parts, parts);
+  up.put(g.runs, runs);
+  up.put(g.rgroups, rgroups);
+  up.put(g.multi, multi);
+  up.put(g.kw, hk);
+  up.put(g.jslot, jslot);
+  up.put(g.chunks, p.chunks);
+  up.put(g.par, allpar);
+  up.put(g.pslot, pslot);
+  up.put(g.jl, jl);
+  int rc = up.send();
+  if (rc) return rc;
   (split ? p.gss_ready : p.gs_ready) = true;
   return CROC_OK;
 }
@@ -1840,19 +1848,16 @@ int tw_degrid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool fused) {
   g.jchunk = g.kw + p.njobs;
   g.chunks = g.jchunk + p.njobs;
   g.roff = g.chunks + p.chunks.size();
-  hipStream_t s = ctx->stream;
-  if (!parts.empty())
-    CROC_HIP(hipMemcpyAsync(g.parts, parts.data(), sizeof(int4) * parts.size(),
-                            hipMemcpyHostToDevice, s));
-  if (!items.empty())
-    CROC_HIP(hipMemcpyAsync(g.items, items.data(), sizeof(int4) * items.size(),
-                            hipMemcpyHostToDevice, s));
-  int rc;
-  if ((rc = upload_i32(ctx, g.kw, hk))) return rc;
-  if ((rc = upload_i32(ctx, g.jchunk, hc))) return rc;
-  if ((rc = upload_i32(ctx, g.chunks, p.chunks))) return rc;
-  if ((rc = upload_i32(ctx, g.roff, roff))) return rc;
-  CROC_HIP(hipStreamSynchronize(s));
+  Upload up(ctx, d, bytes);
+  if (!up.h) return CROC_ENOMEM;
+  up.put(g.parts, parts);
+  up.put(g.items, items);
+  up.put(g.kw, hk);
+  up.put(g.jchunk, hc);
+  up.put(g.chunks, p.chunks);
+  up.put(g.roff, roff);
+  int rc = up.send();
+  if (rc) return rc;
   p.ds_ready = true;
   return CROC_OK;
 }
