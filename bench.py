@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 # w-towers imaging: jobs of at most this many visibilities are gridded by the
 # summing kernel (k_tw_image_runs), the rest by the job kernel (wtowers.hip)
-TW_SMALL_VIS = 64
+TW_SMALL_VIS = 128
 FP16_MFMA_PEAK_TFLOPS = 2500.0  # dense
 LDS_PEAK_TBS = 78.6  # 256 CUs x 128 B/clk x 2.4 GHz
 

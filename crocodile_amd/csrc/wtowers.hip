@@ -740,8 +740,11 @@ __global__ void k_tw_chunk_accum(const int4 *__restrict__ groups, const int32_t 
 
 // Jobs of at most TW_SMALL_VIS visibilities are gridded by the summing
 // kernel itself (separable kernels of support S <= 16), never through a
-// sub-grid slot.
-constexpr int TW_SMALL_VIS = 64;
+// sub-grid slot; their records are staged TW_STAGE at a time.
+#ifndef TW_SMALL_VIS
+#define TW_SMALL_VIS 128
+#endif
+constexpr int TW_STAGE = 64;
 
 // Job images summed per chunk without writing them out.  runs[r] = (chunk,
 // first, end, out).  Each job's image
@@ -757,7 +760,7 @@ constexpr int TW_SMALL_VIS = 64;
 //   SMALL = false: G = sub[jslot[job]] as the gridding kernels left it; the
 //     next job's rows are loaded during the current transform;
 //   SMALL = true: (jobs[] the record ranges, at most TW_SMALL_VIS records
-//     each) the records, v kx
+//     each) TW_STAGE records at a time: their v kx
 //     and ky rows staged in LDS, then each thread sums the taps falling on
 //     its own row elements (row y, columns x = j2 mod 8: at most two per
 //     window) into the LDS sub-grid in record order,
@@ -770,7 +773,7 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
     const double *__restrict__ phi, float2 *sub, float2 *__restrict__ cimg,
     uint32_t *__restrict__ qc, const int4 *__restrict__ jobs, const uint4 *__restrict__ rec,
     const float2 *__restrict__ svis, const float2 *__restrict__ kern, int S, int64_t tsize) {
-  constexpr int NS = SMALL ? TW_SMALL_VIS : 1;
+  constexpr int NS = SMALL ? TW_STAGE : 1;
   __shared__ float2 lds[FN * FP];
   __shared__ float2 tw[FN];
   __shared__ float2 s_vk[NS][16], s_ky[NS][16];  // v kx and ky rows (zero beyond S)
@@ -811,39 +814,43 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
           const int4 jr = jobs[job];
           const int n = jr.z - jr.y;
           __syncthreads();  // tw written; the previous job's LDS reads done
-          if (tid < 16 * n) {
-            const int i = tid >> 4, c = tid & 15;
-            const uint4 r = rec[jr.y + i];
-            const float2 w = svis[jr.y + i];
-            const float2 *kt = kern + (int64_t)r.z * tsize;
-            float2 kx = make_float2(0.f, 0.f), ky = make_float2(0.f, 0.f);
-            if (c < S) {
-              kx = kt[(int64_t)(r.y & 0xffffu) * S + c];
-              ky = kt[(int64_t)(r.y >> 16) * S + c];
-            }
-            s_vk[i][c] = cmul(w, kx);
-            s_ky[i][c] = ky;
-            if (c == 0) s_pos[i] = r.x;
-          }
 #pragma unroll
           for (int j1 = 0; j1 < 16; j1++) lds[y * FP + 8 * j1 + j2] = make_float2(0.f, 0.f);
-          __syncthreads();
           hit = false;
-          for (int i = 0; i < n; i++) {
-            const uint32_t pos = s_pos[i];
-            const int wx = (int)(pos & 0xffffu), dy = y - (int)(pos >> 16);
-            if ((unsigned)dy < (unsigned)S) {
-              hit = true;
-              const float2 kyv = s_ky[i][dy];
-              const int o = (j2 - wx) & 7;  // first window column of this thread
-              float2 *c0 = lds + y * FP + wx + o;
-              if (o < S) {
-                const float2 t = cmul(s_vk[i][o], kyv);
-                c0[0] = make_float2(c0[0].x + t.x, c0[0].y + t.y);
+          for (int c0 = 0; c0 < n; c0 += TW_STAGE) {
+            const int m = min(TW_STAGE, n - c0);
+            if (c0 > 0) __syncthreads();  // the previous records' reads done
+            if (tid < 16 * m) {
+              const int i = tid >> 4, c = tid & 15;
+              const uint4 r = rec[jr.y + c0 + i];
+              const float2 w = svis[jr.y + c0 + i];
+              const float2 *kt = kern + (int64_t)r.z * tsize;
+              float2 kx = make_float2(0.f, 0.f), ky = make_float2(0.f, 0.f);
+              if (c < S) {
+                kx = kt[(int64_t)(r.y & 0xffffu) * S + c];
+                ky = kt[(int64_t)(r.y >> 16) * S + c];
               }
-              if (o + 8 < S) {
-                const float2 t = cmul(s_vk[i][o + 8], kyv);
-                c0[8] = make_float2(c0[8].x + t.x, c0[8].y + t.y);
+              s_vk[i][c] = cmul(w, kx);
+              s_ky[i][c] = ky;
+              if (c == 0) s_pos[i] = r.x;
+            }
+            __syncthreads();
+            for (int i = 0; i < m; i++) {
+              const uint32_t pos = s_pos[i];
+              const int wx = (int)(pos & 0xffffu), dy = y - (int)(pos >> 16);
+              if ((unsigned)dy < (unsigned)S) {
+                hit = true;
+                const float2 kyv = s_ky[i][dy];
+                const int o = (j2 - wx) & 7;  // first window column of this thread
+                float2 *cell = lds + y * FP + wx + o;
+                if (o < S) {
+                  const float2 t = cmul(s_vk[i][o], kyv);
+                  cell[0] = make_float2(cell[0].x + t.x, cell[0].y + t.y);
+                }
+                if (o + 8 < S) {
+                  const float2 t = cmul(s_vk[i][o + 8], kyv);
+                  cell[8] = make_float2(cell[8].x + t.x, cell[8].y + t.y);
+                }
               }
             }
           }
