@@ -1495,25 +1495,31 @@ struct DegridSched {
 #endif
 constexpr int TW_MIN_SPLIT = 256;  // fewest visibilities of an item cut at a run's end
 
-void tw_predict_runs(const std::vector<int4> &hjobs, int nwp, int wp_min, int num_cu,
-                     std::vector<int4> &items, std::vector<int32_t> &roff) {
+void tw_predict_runs(const std::vector<int4> &hjobs, const std::vector<int32_t> &jchunk,
+                     const std::vector<int32_t> &jwp, int num_cu, std::vector<int4> &items,
+                     std::vector<int32_t> &roff) {
+  const int nruns = std::max(1, TW_RUNS_PER_CU * num_cu);
   items.clear();
+  items.reserve(hjobs.size() + 2 * (size_t)nruns + 16);
   roff.assign(1, 0);
-  double total = 0.0;
-  for (const int4 &j : hjobs) total += TW_FFT_COST + (double)(j.z - j.y);
-  const double target = total / std::max(1, TW_RUNS_PER_CU * num_cu);
-  double acc = 0.0;  // cost of the open run
-  for (const int4 &j : hjobs) {
-    const int chunk = j.x / nwp, wp = j.x % nwp + wp_min;
+  roff.reserve(2 * (size_t)nruns + 16);
+  // integer costs: the loop is a chain through acc
+  int64_t total = 0;
+  for (const int4 &j : hjobs) total += TW_FFT_COST + (int64_t)(j.z - j.y);
+  const int64_t target = total / nruns;
+  int64_t acc = 0;  // cost of the open run
+  for (size_t k = 0; k < hjobs.size(); k++) {
+    const int4 j = hjobs[k];
+    const int chunk = jchunk[k], wp = jwp[k];
     for (int v = j.y; v < j.z;) {
-      if (acc > 0.0 && acc + TW_FFT_COST + TW_MIN_SPLIT > target) {
+      if (acc > 0 && acc + TW_FFT_COST + TW_MIN_SPLIT > target) {
         roff.push_back((int32_t)items.size());
-        acc = 0.0;
+        acc = 0;
       }
-      const double room = std::max((double)TW_MIN_SPLIT, target - acc - TW_FFT_COST);
-      const int e = (double)(j.z - v) > room ? v + (int)room : j.z;
+      const int64_t room = std::max<int64_t>(TW_MIN_SPLIT, target - acc - TW_FFT_COST);
+      const int e = (int64_t)(j.z - v) > room ? v + (int)room : j.z;
       items.push_back(make_int4(chunk, wp, v, e));
-      acc += TW_FFT_COST + (double)(e - v);
+      acc += TW_FFT_COST + (int64_t)(e - v);
       v = e;
     }
   }
@@ -1529,6 +1535,7 @@ struct Plan {
   int4 *jobs = nullptr;
   int njobs = 0;
   std::vector<int4> hjobs;
+  std::vector<int32_t> jchunk, jwp;  // per job its chunk and w-plane
   std::vector<int32_t> chunks;  // non-empty chunk ids, ascending
   GridSched gs, gs_split;  // without / with the small jobs split off
   DegridSched ds;
@@ -1621,8 +1628,12 @@ int tw_bin(croc_ctx *ctx, Plan &p, const float2 *vis) {
     CROC_HIP(hipStreamSynchronize(s));
   }
   p.chunks.clear();
-  for (const int4 &j : p.hjobs) {
-    const int c = j.x / a.nwp;
+  p.jchunk.resize(p.njobs);
+  p.jwp.resize(p.njobs);
+  for (int j = 0; j < p.njobs; j++) {
+    const int c = p.hjobs[j].x / a.nwp;
+    p.jchunk[j] = c;
+    p.jwp[j] = p.hjobs[j].x - c * a.nwp + a.wp_min;
     if (p.chunks.empty() || p.chunks.back() != c) p.chunks.push_back(c);
   }
   return CROC_OK;
@@ -1690,7 +1701,7 @@ struct Upload {
 // order by chunk): at most `per` jobs each, the first of each chunk adding
 // to the chunk image, later ones to the next partial slot (*nslot).
 void tw_sum_runs(const std::vector<int32_t> &jl, const std::vector<int32_t> &jslot,
-                 const std::vector<int> &jchunk, size_t e0, size_t e1, int per, bool slot_of_job,
+                 const std::vector<int32_t> &jchunk, size_t e0, size_t e1, int per, bool slot_of_job,
                  int *nslot, std::vector<int4> &runs, std::vector<int4> &rgroups,
                  std::vector<int32_t> &pslot, GridSched::Batch &bt) {
   bt.r0 = runs.size();
@@ -1713,8 +1724,19 @@ void tw_sum_runs(const std::vector<int32_t> &jl, const std::vector<int32_t> &jsl
   }
   bt.nr = runs.size() - bt.r0;
   bt.nq = rgroups.size() - bt.q0;
-  std::stable_sort(runs.begin() + bt.r0, runs.end(),
-                   [](const int4 &u, const int4 &v) { return u.z - u.y > v.z - v.y; });
+  // longest first, stable: a counting sort by length (at most `per`)
+  std::vector<size_t> cnt(per + 2, 0);
+  for (size_t r = bt.r0; r < runs.size(); r++) cnt[per - (runs[r].z - runs[r].y)]++;
+  size_t at = bt.r0;
+  for (int l = 0; l <= per; l++) {
+    const size_t c = cnt[l];
+    cnt[l] = at;
+    at += c;
+  }
+  std::vector<int4> sorted(bt.nr);
+  for (size_t r = bt.r0; r < runs.size(); r++)
+    sorted[cnt[per - (runs[r].z - runs[r].y)]++ - bt.r0] = runs[r];
+  std::copy(sorted.begin(), sorted.end(), runs.begin() + bt.r0);
 }
 
 // Build the gridding schedule of p (split: the small jobs apart, p.gs_split
@@ -1726,12 +1748,13 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool split) {
   const TowerArgs &a = p.a;
   std::vector<int4> parts, runs, rgroups;
   std::vector<int2> multi;
-  std::vector<int32_t> jslot(p.njobs, -1), hk(p.njobs), jl, pslot;
-  std::vector<int> jchunk(p.njobs);
-  for (int j = 0; j < p.njobs; j++) {
-    hk[j] = p.hjobs[j].x % a.nwp + a.wp_min;
-    jchunk[j] = p.hjobs[j].x / a.nwp;
-  }
+  std::vector<int32_t> jslot(p.njobs, -1), jl, pslot;
+  const std::vector<int32_t> &jchunk = p.jchunk;
+  parts.reserve(p.njobs + 64);
+  runs.reserve(p.njobs + 64);
+  rgroups.reserve(p.chunks.size() + 64);
+  jl.reserve(p.njobs);
+  pslot.reserve(p.njobs);
   // small jobs first, in job order
   if (split) {
     for (int j = 0; j < p.njobs; j++)
@@ -1804,7 +1827,7 @@ int tw_grid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool split) {
   up.put(g.runs, runs);
   up.put(g.rgroups, rgroups);
   up.put(g.multi, multi);
-  up.put(g.kw, hk);
+  up.put(g.kw, p.jwp);
   up.put(g.jslot, jslot);
   up.put(g.chunks, p.chunks);
   up.put(g.par, allpar);
@@ -1823,7 +1846,6 @@ int tw_degrid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool fused) {
   DegridSched &g = p.ds;
   g = DegridSched();
   g.fused = fused;
-  const TowerArgs &a = p.a;
   g.jb = std::min(p.njobs, JOB_BATCH);
   std::vector<int4> parts;
   for (int j0 = 0; !fused && j0 < p.njobs; j0 += g.jb) {
@@ -1835,14 +1857,9 @@ int tw_degrid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool fused) {
     }
   }
   g.poff.push_back(parts.size());
-  std::vector<int32_t> hk(p.njobs), hc(p.njobs);
-  for (int j = 0; j < p.njobs; j++) {
-    hk[j] = p.hjobs[j].x % a.nwp + a.wp_min;
-    hc[j] = p.hjobs[j].x / a.nwp;
-  }
   std::vector<int4> items;
   std::vector<int32_t> roff(1, 0);
-  if (fused) tw_predict_runs(p.hjobs, a.nwp, a.wp_min, ctx->num_cu, items, roff);
+  if (fused) tw_predict_runs(p.hjobs, p.jchunk, p.jwp, ctx->num_cu, items, roff);
   g.nruns = (int)roff.size() - 1;
   const size_t bytes = sizeof(int4) * (parts.size() + items.size() + 32) +
                        sizeof(int32_t) * (2 * (size_t)p.njobs + p.chunks.size() + roff.size()) +
@@ -1859,8 +1876,8 @@ int tw_degrid_sched(croc_ctx *ctx, Plan &p, WsSet &set, bool fused) {
   if (!up.h) return CROC_ENOMEM;
   up.put(g.parts, parts);
   up.put(g.items, items);
-  up.put(g.kw, hk);
-  up.put(g.jchunk, hc);
+  up.put(g.kw, p.jwp);
+  up.put(g.jchunk, p.jchunk);
   up.put(g.chunks, p.chunks);
   up.put(g.roff, roff);
   int rc = up.send();
