@@ -37,7 +37,9 @@ namespace {
 constexpr int SG = 128;
 constexpr int SGN = SG * SG;
 constexpr int TW_NT = 1024;
-constexpr int JOB_BATCH = 2048;    // sub-grid slots per batch
+#ifndef JOB_BATCH
+#define JOB_BATCH 8192  // sub-grid slots per batch (1 GiB of 128^2 complex64)
+#endif
 constexpr int PART_VIS = 4096;     // visibilities per work part of a job
 constexpr uint32_t INVALID = 0xffffffffu;
 
