@@ -1493,7 +1493,7 @@ struct DegridSched {
 #define TW_FFT_COST 1024
 #endif
 #ifndef TW_RUNS_PER_CU
-#define TW_RUNS_PER_CU 8
+#define TW_RUNS_PER_CU 16
 #endif
 constexpr int TW_MIN_SPLIT = 256;  // fewest visibilities of an item cut at a run's end
 
