@@ -128,6 +128,62 @@ __device__ __forceinline__ void pass_b(const float2 *lds, int line_stride, int e
   }
 }
 
+// Row passes over the rows rows[0 .. nrows - 1] (rows == nullptr: row k
+// is k) of a sub-grid in LDS, with 32 consecutive lanes on 32 rows at the
+// same element offset: with the odd pitch FP they fall on distinct banks
+// (a lane per row segment, the layout of pass_a<false>, puts 16 lanes of a
+// pass-B read on 4 banks).  Items of pass A: (k, j2), k = 32 (t >> 8) +
+// (t & 31), j2 = (t >> 5) & 7; of pass B: (k, k1) for it = t + r FFT_NT,
+// k = 32 (it >> 9) + (it & 31), k1 = (it >> 5) & 15.  Wavefronts whose 32
+// rows lie past nrows skip as a whole.
+__device__ __forceinline__ int rowlist_line(const int *rows, int k) { return rows ? rows[k] : k; }
+
+__device__ __forceinline__ void rows_pass_a(float2 *lds, const int *rows, int nrows,
+                                            const float2 *tw, float sgn) {
+  const int t = threadIdx.x;
+  const int k = 32 * (t >> 8) + (t & 31), j2 = (t >> 5) & 7;
+  if (__builtin_amdgcn_readfirstlane(32 * (t >> 8)) >= nrows || k >= nrows) return;
+  float2 *p = lds + rowlist_line(rows, k) * FP + j2;
+  float2 v[16];
+#pragma unroll
+  for (int j1 = 0; j1 < 16; j1++) v[j1] = p[8 * j1];
+  fft_reg<16>(v, sgn);
+#pragma unroll
+  for (int k1 = 0; k1 < 16; k1++) {
+    const float2 w = tw[(j2 * k1) & 127];
+    p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
+  }
+}
+
+// Pass B item r of the thread: list entry k and k1 (k past nrows: none).
+__device__ __forceinline__ void rows_pass_b_item(int r, int &k, int &k1) {
+  const int it = threadIdx.x + r * FFT_NT;
+  k = 32 * (it >> 9) + (it & 31);
+  k1 = (it >> 5) & 15;
+}
+
+// Pass B reads: v[r][k2] = output k1 + 16 k2 of the thread's r-th item
+// (zeros for wavefronts past the list); the caller writes them back after a
+// barrier.
+__device__ __forceinline__ void rows_pass_b(const float2 *lds, const int *rows, int nrows,
+                                            float sgn, float2 (&v)[PASS_B][8]) {
+#pragma unroll
+  for (int r = 0; r < PASS_B; r++) {
+    int k, k1;
+    rows_pass_b_item(r, k, k1);
+    const int kw = __builtin_amdgcn_readfirstlane(32 * ((threadIdx.x + r * FFT_NT) >> 9));
+    if (kw < nrows) {
+      const float2 *p = lds + rowlist_line(rows, min(k, nrows - 1)) * FP + 8 * k1;
+#pragma unroll
+      for (int j2 = 0; j2 < 8; j2++) v[r][j2] = p[j2];
+      fft_reg<8>(v[r], sgn);
+    } else {
+#pragma unroll
+      for (int j2 = 0; j2 < 8; j2++) v[r][j2] = make_float2(0.f, 0.f);
+    }
+  }
+}
+
 // W128^m = exp(2 pi i m / 128), m = 0..127, rounded from FP64 (the caller
 // applies the sign of the transform to the imaginary part).
 __device__ __forceinline__ void fft128_twiddles(float2 *tw) {

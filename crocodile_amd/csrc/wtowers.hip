@@ -1265,46 +1265,21 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
       }
       __syncthreads();
       {
-        // row pass A on the listed rows: item (k-th row, j2) as pass_a
-        const int nrows = s_nrows;
-        const int k = tid >> 3, jj = tid & 7;
-        if (k < nrows) {
-          float2 *p = lds + s_rows[k] * FP + jj;
-          float2 v[16];
-#pragma unroll
-          for (int j1 = 0; j1 < 16; j1++) v[j1] = p[8 * j1];
-          fft_reg<16>(v, sgn);
-#pragma unroll
-          for (int k1 = 0; k1 < 16; k1++) {
-            const float2 w = twz[(jj * k1) & 127];
-            p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
-          }
-        }
+        // row passes on the listed rows, 32 rows per half-wavefront
+        rows_pass_a(lds, s_rows, s_nrows, twz, sgn);
       }
       __syncthreads();
       {
-        // row pass B on the listed rows, back to LDS with the output sign:
-        // item (row y, k1) holds columns k1 + 16 k2, which share the parity
-        // of y + k1.  Wavefronts past the list hold zeros and write nothing.
+        // row pass B back to LDS with the output sign: item (row y, k1)
+        // holds columns k1 + 16 k2, which share the parity of y + k1
         const int nrows = s_nrows;
         float2 v[PASS_B][8];
-#pragma unroll
-        for (int r = 0; r < PASS_B; r++) {
-          const int itr = tid + r * FFT_NT, k = itr >> 4, k1 = itr & 15;
-          if (__builtin_amdgcn_readfirstlane((itr & ~63) >> 4) < nrows) {
-            const float2 *p = lds + s_rows[min(k, nrows - 1)] * FP + 8 * k1;
-#pragma unroll
-            for (int jj = 0; jj < 8; jj++) v[r][jj] = p[jj];
-            fft_reg<8>(v[r], sgn);
-          } else {
-#pragma unroll
-            for (int jj = 0; jj < 8; jj++) v[r][jj] = make_float2(0.f, 0.f);
-          }
-        }
+        rows_pass_b(lds, s_rows, nrows, sgn, v);
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < PASS_B; r++) {
-          const int itr = tid + r * FFT_NT, k = itr >> 4, k1 = itr & 15;
+          int k, k1;
+          rows_pass_b_item(r, k, k1);
           if (k < nrows) {
             const int line = s_rows[k];
             const float sc = ((line + k1) & 1) ? -1.0f : 1.0f;
