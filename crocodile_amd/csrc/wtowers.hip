@@ -780,12 +780,14 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
   __shared__ float2 tw[FN];
   __shared__ float2 s_vk[NS][16], s_ky[NS][16];  // v kx and ky rows (zero beyond S)
   __shared__ uint32_t s_pos[NS];                 // window corners wx | wy << 16
-  __shared__ int s_hit[SMALL ? FN : 1];          // rows holding taps (small jobs)
-  __shared__ int s_run;
+  __shared__ int s_need[SMALL ? FN : 1];         // small jobs: rows holding taps,
+  __shared__ int s_rows[SMALL ? FN : 1];         // listed in order
+  __shared__ int s_nrows, s_run;
   const int tid = threadIdx.x;
   constexpr float sgn = 1.0f;  // inverse transform
   constexpr float out_scale = 1.0f / (float)SGN;
   fft128_twiddles(tw);
+  if (SMALL && tid < FN) s_need[tid] = 0;
   const int y = tid >> 3, j2 = tid & 7;
   for (;;) {
     if (tid == 0) s_run = (int)atomicAdd(qc, 1u);
@@ -809,104 +811,102 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
       asm volatile("" : "+s"(zero));
       const float2 *twz = tw + zero;
       const int job = jl[b];
-      bool hit = true;  // row y holds taps (small jobs: else it stays zero through the row passes)
-      {
+      int nrows = FN;  // rows through the row passes (small jobs: those holding taps)
+      if constexpr (SMALL) {
+        const int4 jr = jobs[job];
+        const int n = jr.z - jr.y;
+        __syncthreads();  // tw written; the previous job's LDS reads done
+        // the thread's own row elements (row y, columns 8 j1 + j2, centring
+        // sign (-1)^(y + j2)) cleared, then summed in record order
+#pragma unroll
+        for (int j1 = 0; j1 < 16; j1++) lds[y * FP + 8 * j1 + j2] = make_float2(0.f, 0.f);
+        const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
+        for (int c0 = 0; c0 < n; c0 += TW_STAGE) {
+          const int m = min(TW_STAGE, n - c0);
+          if (c0 > 0) __syncthreads();  // the previous records' reads done
+          if (tid < 16 * m) {
+            const int i = tid >> 4, c = tid & 15;
+            const uint4 r = rec[jr.y + c0 + i];
+            const float2 w = svis[jr.y + c0 + i];
+            const float2 *kt = kern + (int64_t)r.z * tsize;
+            float2 kx = make_float2(0.f, 0.f), ky = make_float2(0.f, 0.f);
+            if (c < S) {
+              kx = kt[(int64_t)(r.y & 0xffffu) * S + c];
+              ky = kt[(int64_t)(r.y >> 16) * S + c];
+              s_need[(int)(r.x >> 16) + c] = 1;  // the window's rows
+            }
+            s_vk[i][c] = cmul(w, kx);
+            s_ky[i][c] = ky;
+            if (c == 0) s_pos[i] = r.x;
+          }
+          __syncthreads();
+          for (int i = 0; i < m; i++) {
+            const uint32_t pos = s_pos[i];
+            const int wx = (int)(pos & 0xffffu), dy = y - (int)(pos >> 16);
+            if ((unsigned)dy < (unsigned)S) {
+              const float2 kyv = s_ky[i][dy];
+              const int o = (j2 - wx) & 7;  // first window column of this thread
+              float2 *cell = lds + y * FP + wx + o;
+              if (o < S) {
+                const float2 t = cmul(s_vk[i][o], kyv);
+                cell[0] = make_float2(cell[0].x + neg * t.x, cell[0].y + neg * t.y);
+              }
+              if (o + 8 < S) {
+                const float2 t = cmul(s_vk[i][o + 8], kyv);
+                cell[8] = make_float2(cell[8].x + neg * t.x, cell[8].y + neg * t.y);
+              }
+            }
+          }
+        }
+        if (tid < 64) {
+          // rows holding taps listed in order (ballots over two rows per
+          // lane); this wavefront alone reads the marks, and clears them
+          const int f0 = s_need[2 * tid], f1 = s_need[2 * tid + 1];
+          const unsigned long long m0 = __ballot(f0 != 0), m1 = __ballot(f1 != 0);
+          const unsigned long long lt = (1ull << tid) - 1ull;
+          const int pos = __popcll(m0 & lt) + __popcll(m1 & lt);
+          if (f0) s_rows[pos] = 2 * tid;
+          if (f1) s_rows[pos + (f0 ? 1 : 0)] = 2 * tid + 1;
+          if (tid == 0) s_nrows = __popcll(m0) + __popcll(m1);
+          s_need[2 * tid] = 0;
+          s_need[2 * tid + 1] = 0;
+        }
+        __syncthreads();
+        nrows = s_nrows;
+        rows_pass_a(lds, s_rows, nrows, twz, sgn);  // rows without taps stay zero
+      } else {
         float2 v[16];
-        if constexpr (SMALL) {
-          const int4 jr = jobs[job];
-          const int n = jr.z - jr.y;
-          __syncthreads();  // tw written; the previous job's LDS reads done
 #pragma unroll
-          for (int j1 = 0; j1 < 16; j1++) lds[y * FP + 8 * j1 + j2] = make_float2(0.f, 0.f);
-          hit = false;
-          for (int c0 = 0; c0 < n; c0 += TW_STAGE) {
-            const int m = min(TW_STAGE, n - c0);
-            if (c0 > 0) __syncthreads();  // the previous records' reads done
-            if (tid < 16 * m) {
-              const int i = tid >> 4, c = tid & 15;
-              const uint4 r = rec[jr.y + c0 + i];
-              const float2 w = svis[jr.y + c0 + i];
-              const float2 *kt = kern + (int64_t)r.z * tsize;
-              float2 kx = make_float2(0.f, 0.f), ky = make_float2(0.f, 0.f);
-              if (c < S) {
-                kx = kt[(int64_t)(r.y & 0xffffu) * S + c];
-                ky = kt[(int64_t)(r.y >> 16) * S + c];
-              }
-              s_vk[i][c] = cmul(w, kx);
-              s_ky[i][c] = ky;
-              if (c == 0) s_pos[i] = r.x;
-            }
-            __syncthreads();
-            for (int i = 0; i < m; i++) {
-              const uint32_t pos = s_pos[i];
-              const int wx = (int)(pos & 0xffffu), dy = y - (int)(pos >> 16);
-              if ((unsigned)dy < (unsigned)S) {
-                hit = true;
-                const float2 kyv = s_ky[i][dy];
-                const int o = (j2 - wx) & 7;  // first window column of this thread
-                float2 *cell = lds + y * FP + wx + o;
-                if (o < S) {
-                  const float2 t = cmul(s_vk[i][o], kyv);
-                  cell[0] = make_float2(cell[0].x + t.x, cell[0].y + t.y);
-                }
-                if (o + 8 < S) {
-                  const float2 t = cmul(s_vk[i][o + 8], kyv);
-                  cell[8] = make_float2(cell[8].x + t.x, cell[8].y + t.y);
-                }
-              }
-            }
-          }
-          if (j2 == 0) s_hit[y] = hit;
+        for (int j1 = 0; j1 < 16; j1++) v[j1] = nxt[j1];
+        __syncthreads();  // tw written; the previous job's LDS reads done
+        const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
 #pragma unroll
-          for (int j1 = 0; j1 < 16; j1++) v[j1] = lds[y * FP + 8 * j1 + j2];
-        } else {
+        for (int j1 = 0; j1 < 16; j1++) v[j1] = make_float2(neg * v[j1].x, neg * v[j1].y);
+        fft_reg<16>(v, sgn);
+        float2 *p = lds + y * FP + j2;
 #pragma unroll
-          for (int j1 = 0; j1 < 16; j1++) v[j1] = nxt[j1];
-          __syncthreads();  // tw written; the previous job's LDS reads done
+        for (int k1 = 0; k1 < 16; k1++) {
+          const float2 w = twz[(j2 * k1) & 127];
+          p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
         }
-        if (hit) {
-          const float neg = ((y + j2) & 1) ? -1.0f : 1.0f;
+        if (b + 1 < rn.z) {
+          const float2 *row = sub + (int64_t)jslot[jl[b + 1]] * SGN + y * FN + j2;
 #pragma unroll
-          for (int j1 = 0; j1 < 16; j1++) v[j1] = make_float2(neg * v[j1].x, neg * v[j1].y);
-          fft_reg<16>(v, sgn);
-          float2 *p = lds + y * FP + j2;
-#pragma unroll
-          for (int k1 = 0; k1 < 16; k1++) {
-            const float2 w = twz[(j2 * k1) & 127];
-            p[8 * k1] = cmul(v[k1], make_float2(w.x, sgn * w.y));
-          }
-        }
-        if constexpr (!SMALL) {
-          if (b + 1 < rn.z) {
-            const float2 *row = sub + (int64_t)jslot[jl[b + 1]] * SGN + y * FN + j2;
-#pragma unroll
-            for (int j1 = 0; j1 < 16; j1++) nxt[j1] = row[8 * j1];
-          }
+          for (int j1 = 0; j1 < 16; j1++) nxt[j1] = row[8 * j1];
         }
       }
       __syncthreads();
       {
-        // row pass B (rows without taps are zero and stay so)
+        // row pass B, written back to the natural positions once all are read
         float2 v[PASS_B][8];
-        bool on[PASS_B];
-#pragma unroll
-        for (int r = 0; r < PASS_B; r++) {
-          int line, k1;
-          pass_b_item<false>(r, line, k1);
-          on[r] = !SMALL || s_hit[line];
-          if (on[r]) {
-            const float2 *p = lds + line * FP + 8 * k1;
-#pragma unroll
-            for (int jj = 0; jj < 8; jj++) v[r][jj] = p[jj];
-            fft_reg<8>(v[r], sgn);
-          }
-        }
+        rows_pass_b(lds, SMALL ? s_rows : nullptr, nrows, sgn, v);
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < PASS_B; r++) {
-          int line, k1;
-          pass_b_item<false>(r, line, k1);
-          if (on[r]) {
+          int k, k1;
+          rows_pass_b_item(r, k, k1);
+          if (k < nrows) {
+            const int line = SMALL ? s_rows[k] : k;
 #pragma unroll
             for (int k2 = 0; k2 < 8; k2++) lds[line * FP + k1 + 16 * k2] = v[r][k2];
           }
