@@ -98,14 +98,15 @@ __global__ __launch_bounds__(FFT_NT) void k_fft2d_128(
     }
     __syncthreads();
     {
-      // Row pass B, written back to the natural positions once all are read.
+      // Row pass B (32 rows per half-wavefront), written back to the
+      // natural positions once all are read.
       float2 v[PASS_B][8];
-      pass_b<false>(lds, FP, 1, sgn, v);
+      rows_pass_b(lds, nullptr, FN, sgn, v);
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < PASS_B; r++) {
         int line, k1;
-        pass_b_item<false>(r, line, k1);
+        rows_pass_b_item(r, line, k1);
 #pragma unroll
         for (int k2 = 0; k2 < 8; k2++) lds[line * FP + k1 + 16 * k2] = v[r][k2];
       }
