@@ -747,6 +747,10 @@ __global__ void k_tw_chunk_accum(const int4 *__restrict__ groups, const int32_t 
 #define TW_SMALL_VIS 128
 #endif
 constexpr int TW_STAGE = 64;
+// fewest lanes per visibility in the fused predict's degridding rounds
+#ifndef TW_DEG_TMIN
+#define TW_DEG_TMIN 8
+#endif
 
 // Job images summed per chunk without writing them out.  runs[r] = (chunk,
 // first, end, out).  Each job's image
@@ -1297,16 +1301,16 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
         // at most), so that short items are not one long chain per thread
         for (int j0 = it.z; j0 < it.w;) {
           const int rem = it.w - j0;
-          if (rem > FFT_NT / 2) {
+          if (TW_DEG_TMIN <= 1 && rem > FFT_NT / 2) {
             degrid_round16<1>(lds, j0, it.w, rec, kern, tsize, vis_out);
             j0 += FFT_NT;
-          } else if (rem > FFT_NT / 4) {
+          } else if (TW_DEG_TMIN <= 2 && rem > FFT_NT / 4) {
             degrid_round16<2>(lds, j0, it.w, rec, kern, tsize, vis_out);
             j0 += FFT_NT / 2;
-          } else if (rem > FFT_NT / 8) {
+          } else if (TW_DEG_TMIN <= 4 && rem > FFT_NT / 8) {
             degrid_round16<4>(lds, j0, it.w, rec, kern, tsize, vis_out);
             j0 += FFT_NT / 4;
-          } else if (rem > FFT_NT / 16) {
+          } else if (TW_DEG_TMIN <= 8 && rem > FFT_NT / 16) {
             degrid_round16<8>(lds, j0, it.w, rec, kern, tsize, vis_out);
             j0 += FFT_NT / 8;
           } else {
