@@ -9,19 +9,25 @@
 // Imaging, per call:
 //   1. bin visibilities by job = (chunk, w-plane): stable radix sort of
 //      (job key, visibility index), run-length encoding of the sorted keys;
-//   2. grid each job on its own sg x sg sub-grid, one workgroup per job, in
-//      two passes of sg/2 rows: the pass's rows are held in LDS as exact
-//      64-bit fixed point (the int64 arithmetic of the tile gridder), every
-//      visibility adds the taps of its window rows that fall in the pass;
-//   3. inverse FFT of every job sub-grid (fft.hip), with the Fresnel factor
-//      exp(-2 pi i wp wstep (1 - n)) applied to the output;
-//   4. sum the job images of each chunk (one workgroup per chunk and batch,
-//      in job order: deterministic);
-//   5. forward FFT of every chunk image;
-//   6. add the chunk sub-grids into the grid in four passes by chunk parity,
+//      the job schedules are made on the host from the job table;
+//   2. grid the jobs, in batches of sub-grid slots, each part of a job on
+//      its own sg x sg slot: separable kernels of support <= 16 on the
+//      matrix cores (k_tw_grid_jobs_mfma), others in two passes of sg/2
+//      rows held in LDS as exact 64-bit fixed point (k_tw_grid_jobs);
+//      multi-part jobs summed (k_tw_sum_parts);
+//   3. the job images ifft(sub-grid) * exp(-2 pi i wp wstep (1 - n)) formed
+//      in LDS and summed per chunk in registers, in job order
+//      (k_tw_image_runs); with separable kernels of support <= 16 the jobs
+//      of at most TW_SMALL_VIS visibilities skip step 2 and are gridded
+//      inside this kernel;
+//   4. forward FFT of every chunk image;
+//   5. add the chunk sub-grids into the grid in four passes by chunk parity,
 //      so overlapping margins are never written concurrently (deterministic).
-// Predict reverses it: cut + inverse FFT per chunk, forward FFT with
-// exp(+2 pi i wp wstep (1 - n)) applied to the input per job, degrid.
+// Predict reverses it: cut + inverse FFT per chunk; per job the forward FFT
+// with exp(+2 pi i wp wstep (1 - n)) applied to the input, then degrid:
+// with separable kernels of support <= 16 both in one kernel from LDS
+// (k_tw_predict_sep), otherwise batched transforms into slots and
+// k_tw_degrid_jobs.
 #include <algorithm>
 #include <climits>
 #include <cstring>
@@ -1173,10 +1179,10 @@ __device__ __forceinline__ void degrid_round16(const float2 *lds, int j0, int j1
 //      input as it is read, the output sign as the last pass writes back),
 //      columns first, then the rows the item's windows cover (all rows
 //      when it holds more than FFT_NT visibilities); the chunk image's
-//      columns are read at each item (consecutive items mostly share it,
-//      and L2 holds it);
-//   2. the item's visibilities degridded from LDS; the window lies inside
-//      the sub-grid (S <= margin, checked at binning).
+//      columns are read at each item (consecutive items mostly share it);
+//   2. the item's visibilities degridded from LDS, several lanes per
+//      visibility; the window lies inside the sub-grid (S <= margin,
+//      checked at binning).
 template <int SF>
 __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
     const int4 *__restrict__ items, const int32_t *__restrict__ roff, int nruns,
@@ -1296,9 +1302,9 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
       __syncthreads();
       if (tid < FN) s_need[tid] = 0;  // read again only after the next item's first barrier
       if constexpr (SF == 16) {
-        // rounds of the workgroup's width; a round with fewer visibilities
-        // left than threads gives each up to 16 lanes (one window row each
-        // at most), so that short items are not one long chain per thread
+        // rounds of FFT_NT / T visibilities with T lanes each: at least
+        // TW_DEG_TMIN (the lanes of one visibility read different window
+        // rows, on different LDS banks), more when few visibilities are left
         for (int j0 = it.z; j0 < it.w;) {
           const int rem = it.w - j0;
           if (TW_DEG_TMIN <= 1 && rem > FFT_NT / 2) {
