@@ -214,3 +214,45 @@ def test_wtowers_kernel_set_must_cover_increment(gpu):
     narrow = G.WKernelSet(kset.tables, kset.w_min / 2, kset.w_step / 2)
     with pytest.raises(ValueError):
         G.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, narrow)
+
+
+def boundary_jobs(seed):
+    """One chunk (cells around 240 of 512: chunk 2 of 96) whose w-planes
+    hold jobs of exactly 64, 65, 128, 129, 1025 and 4097 visibilities: the
+    small-job staging (64 records at a time), the small / matrix-core split
+    (128), the gridding window (1024) and the part size (4096) on both
+    sides of each bound; plus a one-visibility job and a second chunk."""
+    rng = numpy.random.default_rng(seed)
+    sizes = [64, 65, 128, 129, 1025, 4097, 1]
+    cells, ws = [], []
+    for plane, n in zip(range(-3, 4), sizes):
+        cells.append(rng.uniform(-30, 30, (n, 2)) + 240.0)
+        ws.append(plane * 40.0 + rng.uniform(-15, 15, n))
+    cells.append(rng.uniform(-10, 10, (3, 2)) + numpy.array([330.0, 240.0]))  # chunk 3
+    ws.append(rng.uniform(-15, 15, 3))
+    cell = numpy.concatenate(cells)
+    uvw = numpy.concatenate([(cell - 256.0) / THETA, numpy.concatenate(ws)[:, None]], 1)
+    vis = rng.standard_normal(len(uvw)) + 1j * rng.standard_normal(len(uvw))
+    return rng, uvw, vis
+
+
+@pytest.mark.parametrize('support', [8, 16])
+def test_wtowers_job_size_bounds(gpu, support):
+    """Imaging and predict at the job-size bounds of the gridding and summing
+    kernels and the predict's run splitting (this set's runs cut every
+    job), against the oracle, with separable kernels of support 8 (the
+    generic-support code) and 16 (the fixed-support code)."""
+    print('case job size bounds S=%d' % support, flush=True)
+    rng, uvw, vis = boundary_jobs(30 + support)
+    aaf = R.anti_aliasing_function(512, 0, support)
+    kv = R.kernel_oversample(aaf, 8, support)
+    kv /= kv[0].sum()
+    g = G.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, kv)
+    ref = WT.w_towers_imaging(THETA, LAM, uvw, None, vis, 40.0, SG, MARGIN, kv)
+    assert_close(g, ref, what='w_towers_imaging job size bounds S=%d' % support)
+    grid = rng.standard_normal((512, 512)) + 1j * rng.standard_normal((512, 512))
+    v = G.w_towers_predict(THETA, LAM, uvw, None, grid, 40.0, SG, MARGIN, kv)
+    vref = WT.w_towers_predict(THETA, LAM, uvw, None, grid, 40.0, SG, MARGIN, kv)
+    assert_close(v, vref, what='w_towers_predict job size bounds S=%d' % support)
+    plan = G.WTowersPlan(THETA, LAM, uvw, 40.0, SG, MARGIN, kv)
+    assert (plan.imaging(vis, kv) == g).all() and (plan.predict(grid, kv) == v).all()
