@@ -44,7 +44,7 @@ constexpr int SG = 128;
 constexpr int SGN = SG * SG;
 constexpr int TW_NT = 1024;
 #ifndef JOB_BATCH
-#define JOB_BATCH 8192  // sub-grid slots per batch (1 GiB of 128^2 complex64)
+#define JOB_BATCH 16384  // sub-grid slots per batch (2 GiB of 128^2 complex64)
 #endif
 constexpr int PART_VIS = 4096;     // visibilities per work part of a job
 constexpr uint32_t INVALID = 0xffffffffu;
@@ -1475,7 +1475,7 @@ struct DegridSched {
 // compute unit's share: the queue evens out the model's error, and a job
 // larger than a run is cut over several runs (each transforming it).
 #ifndef TW_FFT_COST
-#define TW_FFT_COST 1024
+#define TW_FFT_COST 2048
 #endif
 #ifndef TW_RUNS_PER_CU
 #define TW_RUNS_PER_CU 16
