@@ -112,7 +112,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fft2d_128(
       }
     }
     __syncthreads();
-    pass_a<true>(lds, 1, FP, twz, sgn);  // columns
+    cols_pass_a(lds, twz, sgn);  // columns
     __syncthreads();
     {
       // Column pass B to memory, one item at a time (the next sub-grid's
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fft2d_128(
 #pragma unroll
       for (int r = 0; r < PASS_B; r++) {
         int x, k1;
-        pass_b_item<true>(r, x, k1);
+        cols_pass_b_item(r, x, k1);
         double ph[8];
         if (mo) {
 #pragma unroll

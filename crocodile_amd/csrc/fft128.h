@@ -77,53 +77,47 @@ __device__ __forceinline__ void fft_reg(float2 *v, float sgn) {
   }
 }
 
-// Pass A of the 128-point transforms of all 128 LDS lines (line l starts at
-// l * line_stride, elements at stride es) on items (line, j2), in place;
-// tw[m] = W128^m.  LINE_FAST puts consecutive lanes on consecutive lines
-// (columns: consecutive addresses), otherwise on consecutive items of one
-// line (rows).
-template <bool LINE_FAST>
-__device__ __forceinline__ void pass_a(float2 *lds, int line_stride, int es, const float2 *tw,
-                                       float sgn) {
+// Column passes of all 128 columns of a sub-grid in LDS (element (y, x) at
+// y FP + x), consecutive lanes on consecutive columns (consecutive
+// addresses: distinct banks).  Pass A on items (column x, j2), in place;
+// tw[m] = W128^m.
+__device__ __forceinline__ void cols_pass_a(float2 *lds, const float2 *tw, float sgn) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < PASS_A; r++) {
     const int it = tid + r * FFT_NT;
-    const int line = LINE_FAST ? (it & 127) : (it >> 3);
-    const int j2 = LINE_FAST ? (it >> 7) : (it & 7);
-    float2 *p = lds + line * line_stride + j2 * es;
+    const int x = it & 127, j2 = it >> 7;
+    float2 *p = lds + x + j2 * FP;
     float2 v[16];
 #pragma unroll
-    for (int j1 = 0; j1 < 16; j1++) v[j1] = p[8 * j1 * es];
+    for (int j1 = 0; j1 < 16; j1++) v[j1] = p[8 * j1 * FP];
     fft_reg<16>(v, sgn);
 #pragma unroll
     for (int k1 = 0; k1 < 16; k1++) {
       const float2 w = tw[(j2 * k1) & 127];
-      p[8 * k1 * es] = cmul(v[k1], make_float2(w.x, sgn * w.y));
+      p[8 * k1 * FP] = cmul(v[k1], make_float2(w.x, sgn * w.y));
     }
   }
 }
 
-// Pass-B item r of the thread: (line, k1).
-template <bool LINE_FAST>
-__device__ __forceinline__ void pass_b_item(int r, int &line, int &k1) {
+// Column pass-B item r of the thread: (column x, k1).
+__device__ __forceinline__ void cols_pass_b_item(int r, int &x, int &k1) {
   const int it = threadIdx.x + r * FFT_NT;
-  line = LINE_FAST ? (it & 127) : (it >> 4);
-  k1 = LINE_FAST ? (it >> 7) : (it & 15);
+  x = it & 127;
+  k1 = it >> 7;
 }
 
-// Pass B on items (line, k1): v[r][k2] = output k1 + 16 k2 of the thread's
-// r-th line, read from LDS; the caller stores it.
-template <bool LINE_FAST>
-__device__ __forceinline__ void pass_b(const float2 *lds, int line_stride, int es, float sgn,
-                                       float2 (&v)[PASS_B][8]) {
+// Column pass B on items (x, k1): v[r][k2] = output row k1 + 16 k2 of the
+// thread's r-th column, read from LDS; the caller stores it.
+__device__ __forceinline__ void cols_pass_b(const float2 *lds, float sgn,
+                                            float2 (&v)[PASS_B][8]) {
 #pragma unroll
   for (int r = 0; r < PASS_B; r++) {
-    int line, k1;
-    pass_b_item<LINE_FAST>(r, line, k1);
-    const float2 *p = lds + line * line_stride + 8 * k1 * es;
+    int x, k1;
+    cols_pass_b_item(r, x, k1);
+    const float2 *p = lds + x + 8 * k1 * FP;
 #pragma unroll
-    for (int j2 = 0; j2 < 8; j2++) v[r][j2] = p[j2 * es];
+    for (int j2 = 0; j2 < 8; j2++) v[r][j2] = p[j2 * FP];
     fft_reg<8>(v[r], sgn);
   }
 }
@@ -131,8 +125,8 @@ __device__ __forceinline__ void pass_b(const float2 *lds, int line_stride, int e
 // Row passes over the rows rows[0 .. nrows - 1] (rows == nullptr: row k
 // is k) of a sub-grid in LDS, with 32 consecutive lanes on 32 rows at the
 // same element offset: with the odd pitch FP they fall on distinct banks
-// (a lane per row segment, the layout of pass_a<false>, puts 16 lanes of a
-// pass-B read on 4 banks).  Items of pass A: (k, j2), k = 32 (t >> 8) +
+// (a lane per row segment, 8 lanes to a row, puts 16 lanes of a pass-B
+// read on 4 banks).  Items of pass A: (k, j2), k = 32 (t >> 8) +
 // (t & 31), j2 = (t >> 5) & 7; of pass B: (k, k1) for it = t + r FFT_NT,
 // k = 32 (it >> 9) + (it & 31), k1 = (it >> 5) & 15.  Wavefronts whose 32
 // rows lie past nrows skip as a whole.

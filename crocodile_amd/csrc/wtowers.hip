@@ -923,7 +923,7 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
         }
       }
       __syncthreads();
-      pass_a<true>(lds, 1, FP, twz, sgn);  // columns
+      cols_pass_a(lds, twz, sgn);  // columns
       __syncthreads();
       {
         // column pass B: item (column x, k1) holds rows y = k1 + 16 k2,
@@ -932,7 +932,7 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
 #pragma unroll
         for (int r = 0; r < PASS_B; r++) {
           int x, k1;
-          pass_b_item<true>(r, x, k1);
+          cols_pass_b_item(r, x, k1);
           double ph[8];
 #pragma unroll
           for (int k2 = 0; k2 < 8; k2++) ph[k2] = phi[zero + (k1 + 16 * k2) * FN + x];
@@ -954,7 +954,7 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_image_runs(
 #pragma unroll
     for (int r = 0; r < PASS_B; r++) {
       int x, k1;
-      pass_b_item<true>(r, x, k1);
+      cols_pass_b_item(r, x, k1);
 #pragma unroll
       for (int k2 = 0; k2 < 8; k2++) {
         const int pix = (k1 + 16 * k2) * FN + x;
@@ -1263,12 +1263,12 @@ __global__ __launch_bounds__(FFT_NT) void k_tw_predict_sep(
       {
         // column pass B, written back to the natural positions once all are read
         float2 v[PASS_B][8];
-        pass_b<true>(lds, 1, FP, sgn, v);
+        cols_pass_b(lds, sgn, v);
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < PASS_B; r++) {
           int line, k1;
-          pass_b_item<true>(r, line, k1);
+          cols_pass_b_item(r, line, k1);
 #pragma unroll
           for (int k2 = 0; k2 < 8; k2++) lds[(k1 + 16 * k2) * FP + line] = v[r][k2];
         }
